@@ -1,0 +1,132 @@
+"""Throughput of the plain path tracer on one MI355X, as one JSON line.
+
+A step is one full frame of the workload (all samples of every pixel) through the megakernel and
+the film gather, with the scene and BVH already resident in HBM. Workloads are generated from
+fixed seeds in the reference's own file formats (lighttransportsimulation_amd/scenegen.py).
+
+Extra fields:
+  roofline      the integrator kernel against HBM bandwidth: algorithmic bytes per launch are the
+                BVH node tests (32 B each) and triangle tests (36 B each) that the reference's
+                traversal performs for the same rays, measured per sample by the CPU oracle on
+                the bounded sample below, times the samples of one launch, over the launch's mean
+                duration from HIP events.
+  cpu_baseline  the oracle (the reference integrator restated in C++) on all host cores over a
+                bounded band of rows of the same frame.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Msamples/s & equal-time relMSE: cached vs plain path tracer, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "cornell_256": dict(desc="seeded icosphere (20,480 tris) in a Cornell box, 256x256, 16 spp stratified, "
+                             "5 bounces, RR from 3", gen="cornell", width=256, height=256, spp=16, bounces=5, rr=3,
+                        stratified=1),
+}
+
+
+def make_workload(name: str, d: str) -> dict:
+    from lighttransportsimulation_amd import scenegen
+    w = WORKLOADS[name]
+    if w["gen"] == "cornell":
+        cfg = scenegen.gen_cornell_sphere(d, width=w["width"], height=w["height"], spp=w["spp"], bounces=w["bounces"],
+                                          rr=w["rr"])
+    else:
+        raise ValueError(name)
+    cfg.update(width=w["width"], height=w["height"], spp=w["spp"], max_bounce=w["bounces"], rr_start=w["rr"],
+               stratified=w["stratified"])
+    return cfg
+
+
+def cpu_baseline(d: str, cfg: dict, budget_s: float) -> dict:
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    sc = oracle_lib.OracleScene(d, cfg["scene"], cfg["subject"], cfg["width"], cfg["height"])
+    H = cfg["height"]
+    r0 = H // 2
+    _, _, st = sc.render(cfg["spp"], cfg["max_bounce"], cfg["rr_start"], cfg["seed"], cfg["stratified"],
+                         rows=(r0, min(H, r0 + 2)))
+    per_row = max(st["seconds"] / 2, 1e-4)
+    rows = int(max(2, min(H - r0, budget_s / per_row)))
+    rows = min(rows, H)
+    r0 = max(0, (H - rows) // 2)
+    _, _, st = sc.render(cfg["spp"], cfg["max_bounce"], cfg["rr_start"], cfg["seed"], cfg["stratified"],
+                         rows=(r0, r0 + rows))
+    return dict(value=st["samples"] / st["seconds"] / 1e6, unit="Msamples/s", cores=st["threads"], kind="port",
+                sample=f"rows {r0}-{r0 + rows - 1} of {cfg['width']}x{H} at {cfg['spp']} spp "
+                       f"({st['samples']} samples, {st['seconds']:.1f} s)",
+                node_visits_per_sample=st["node_visits"] / st["samples"],
+                tri_tests_per_sample=st["tri_tests"] / st["samples"])
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cornell_256", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle work for cpu_baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        return 2
+    from lighttransportsimulation_amd import Context
+    with tempfile.TemporaryDirectory() as d:
+        cfg = make_workload(args.workload, d)
+        W, H, spp = cfg["width"], cfg["height"], cfg["spp"]
+        with Context(0) as ctx:
+            ctx.load_scene(d, cfg["scene"], cfg["subject"], W, H)
+            info = ctx.scene_info()
+            kw = dict(spp=spp, max_bounce=cfg["max_bounce"], rr_start=cfg["rr_start"], seed=cfg["seed"],
+                      stratified=cfg["stratified"], want_image=False)
+            for _ in range(args.warmup):
+                ctx.render(**kw)
+            t0 = time.perf_counter()
+            dev_ms, kern_ms, launches = 0.0, 0.0, 0
+            for _ in range(args.steps):
+                _, _, st = ctx.render(**kw)
+                dev_ms += st["render_ms"] + st["resolve_ms"]
+                kern_ms += st["render_ms"]
+                launches += st["launches"]
+            wall = time.perf_counter() - t0
+        samples = W * H * spp
+        ms_per_step = wall * 1e3 / args.steps
+        value = samples * args.steps / wall / 1e6
+        out = {
+            "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": WORKLOADS[args.workload]["desc"], "width": W, "height": H, "spp": spp,
+                       "max_bounce": cfg["max_bounce"], "rr_start": cfg["rr_start"], "triangles": info["tris"],
+                       "integrator": "megakernel", "device_ms_per_step": round(dev_ms / args.steps, 3)},
+        }
+        if not args.no_cpu:
+            cb = cpu_baseline(d, cfg, args.cpu_budget)
+            launch_s = kern_ms / launches / 1e3
+            bytes_per_sample = 32.0 * cb["node_visits_per_sample"] + 36.0 * cb["tri_tests_per_sample"]
+            samples_per_launch = samples * args.steps / launches
+            achieved = bytes_per_sample * samples_per_launch / launch_s / 1e9
+            out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                               "kernel": "k_megakernel", "kernel_avg_ms": round(kern_ms / launches, 3),
+                               "algorithmic_bytes_per_sample": round(bytes_per_sample, 1)}
+            out["cpu_baseline"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cb.items()
+                                   if k in ("value", "unit", "cores", "kind", "sample")}
+        print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

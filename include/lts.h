@@ -1,0 +1,122 @@
+/*
+ * lts.h — C ABI of the MI355X path tracer (liblts.so).
+ *
+ * Plain C types and host pointers only. Every entry point returns 0 on success and a negative
+ * LTS_E_* code on failure (the message is available from lts_last_error); none of them exits
+ * the process. Calls are synchronous on return. One context per host thread; a context owns all
+ * of its device memory and one HIP stream on the device it was created for.
+ *
+ * Each declaration names the reference routine it replaces (paths inside
+ * tierChampion/LightTransportSimulation/src).
+ */
+#ifndef LTS_H
+#define LTS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LTS_OK 0
+#define LTS_E_ARG -1      /* invalid argument */
+#define LTS_E_IO -2       /* file missing or malformed */
+#define LTS_E_HIP -3      /* HIP runtime error */
+#define LTS_E_STATE -4    /* call out of order (e.g. render before load) */
+#define LTS_E_NOMEM -5    /* device allocation failed */
+
+typedef struct lts_ctx lts_ctx;
+
+/* Canonical LBVH arrays (caller-allocated, n primitives). Children use the unified index space
+ * of the reference's flattened tree: interior i in [0, n-2], leaf k at n-1+k
+ * (acceleration/BVH.cuh:311-329). Bounds are {min x, y, z, max x, y, z}. */
+typedef struct lts_bvh_export {
+    uint32_t* codes_sorted; /* n     Morton codes after the stable sort           */
+    int32_t* perm;          /* n     sorted position -> primitive number          */
+    int32_t* left;          /* n-1   left child of interior i                     */
+    int32_t* right;         /* n-1   right child of interior i                    */
+    float* int_bounds;      /* 6(n-1)                                             */
+    int32_t* leaf_prim;     /* n     primitive referenced by leaf k               */
+    float* leaf_bounds;     /* 6n                                                 */
+    int32_t* parent;        /* 2n-1  parent in the unified index space, -1 at root */
+    float scene_bounds[6];  /* global bounds used for the Morton quantisation     */
+} lts_bvh_export;
+
+typedef struct lts_bvh_timing {
+    double total_ms;   /* bounds -> pack, device time of the whole build           */
+    double sort_ms;    /* the four radix-sort passes                               */
+    double bounds_ms;  /* primitive bounds + scene-bound reduction + Morton codes  */
+    double tree_ms;    /* Karras hierarchy + bottom-up refit + node packing        */
+} lts_bvh_timing;
+
+/* Integrators. */
+#define LTS_INTEGRATOR_MEGAKERNEL 0
+#define LTS_INTEGRATOR_WAVEFRONT 1
+#define LTS_INTEGRATOR_CACHED 2
+
+typedef struct lts_render_params {
+    int32_t spp;          /* samples per pixel                                     */
+    int32_t max_bounce;   /* PathTracingIntegrator MAX_BOUNCE                      */
+    int32_t rr_start;     /* Russian roulette from this bounce                     */
+    uint32_t seed;        /* sampler seed (replaces time(0))                       */
+    int32_t stratified;   /* 1: film sample per Sampler::stratifiedSample          */
+    int32_t integrator;   /* LTS_INTEGRATOR_*                                      */
+    int32_t spp_per_pass; /* samples per launch; 0 = automatic                     */
+    int32_t reserved[9];
+} lts_render_params;
+
+typedef struct lts_stats {
+    double render_ms;       /* device time of the integrator kernels (HIP events)  */
+    double resolve_ms;      /* device time of the film gather                       */
+    double kernel_avg_ms;   /* mean duration of one integrator launch               */
+    int64_t launches;       /* integrator launches                                  */
+    int64_t samples;        /* camera samples traced                                */
+    double msamples_per_s;  /* samples / (render_ms + resolve_ms)                   */
+    int64_t node_visits;    /* filled only when counting is enabled                 */
+    int64_t tri_tests;
+    double reserved[6];
+} lts_stats;
+
+/* Context. */
+int lts_create(lts_ctx** out, int hip_device);
+void lts_destroy(lts_ctx* ctx);
+const char* lts_last_error(const lts_ctx* ctx);
+const char* lts_version(void);
+
+/* Replaces readApplicationParameters (core/FileIO.cuh:234-257): the positional app.params. */
+int lts_read_app_params(const char* path, int32_t* width, int32_t* height, int32_t* spp, int32_t* max_bounce,
+                        int32_t* rr_start, int32_t* format, char* output, char* scene, char* subject, int32_t cap);
+
+/* Replaces parseScene (core/SceneLoader.cuh:195-360): reads <dir><scene>.scene and
+ * <dir><subject>.subject (subject may be "" or NULL), meshes, materials, images and lights,
+ * builds the LBVH on the device (sorted-leaf layout) and keeps everything resident. */
+int lts_load_scene(lts_ctx* ctx, const char* project_dir, const char* scene, const char* subject,
+                   int32_t width, int32_t height);
+int lts_scene_info(lts_ctx* ctx, int32_t* n_tris, int32_t* n_lights, int32_t* n_meshes);
+
+/* Replaces CreateBVHTree (acceleration/BVH.cuh:411-452) for a triangle soup of n triangles
+ * (9 floats each, host memory). mode 0 reproduces the reference's leaf assignment (leaf k holds
+ * primitive k, BVH.cuh:242); mode 1 assigns leaf k to the k-th primitive in Morton order.
+ * Results stay on the device; `out` (may be NULL) receives the canonical arrays. */
+int lts_bvh_build(lts_ctx* ctx, const float* tri_xyz, int32_t n, int32_t mode, lts_bvh_export* out,
+                  lts_bvh_timing* timing);
+/* Same build on a soup already resident in device memory (d_tri_xyz is a device pointer);
+ * `repeat` builds are timed back to back and timing holds the per-build mean. */
+int lts_bvh_build_device(lts_ctx* ctx, const float* d_tri_xyz, int32_t n, int32_t mode, int32_t repeat,
+                         lts_bvh_timing* timing);
+/* Copies the arrays of the most recent build to host memory. */
+int lts_bvh_export_last(lts_ctx* ctx, lts_bvh_export* out);
+
+/* Replaces PathTracingKernel + Camera::getRender (intergrators/IntegratorKernels.cu:17-23,
+ * rendering/Camera.cu:256-289). rgb_out (may be NULL) receives W*H*3 floats in getRender row
+ * order; sample_rgb (may be NULL) receives every sample's radiance, [spp][H][W][3]. */
+int lts_render(lts_ctx* ctx, const lts_render_params* params, float* rgb_out, float* sample_rgb, lts_stats* stats);
+
+/* Replaces writeToPPM / writeToPFM (core/ImageO.cuh:16-78). format 0: P3 PPM, 1: PFM. */
+int lts_write_image(const char* path, const float* rgb, int32_t width, int32_t height, int32_t format);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LTS_H */

@@ -1,0 +1,203 @@
+"""MI355X-native path tracer with the inputs and stages of tierChampion/LightTransportSimulation.
+
+Python host mirror of the C ABI in include/lts.h (liblts.so, HIP for gfx950). The library is
+required: importing works without it, but every call that needs it raises when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+ROOT_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "liblts.so")
+
+INTEGRATOR_MEGAKERNEL = 0
+INTEGRATOR_WAVEFRONT = 1
+INTEGRATOR_CACHED = 2
+
+# every symbol declared in include/lts.h
+EXPORTED_SYMBOLS = (
+    "lts_create", "lts_destroy", "lts_last_error", "lts_version", "lts_read_app_params", "lts_load_scene",
+    "lts_scene_info", "lts_bvh_build", "lts_bvh_build_device", "lts_bvh_export_last", "lts_render",
+    "lts_write_image",
+)
+
+
+class LtsError(RuntimeError):
+    pass
+
+
+class BvhExport(C.Structure):
+    _fields_ = [("codes_sorted", C.c_void_p), ("perm", C.c_void_p), ("left", C.c_void_p), ("right", C.c_void_p),
+                ("int_bounds", C.c_void_p), ("leaf_prim", C.c_void_p), ("leaf_bounds", C.c_void_p),
+                ("parent", C.c_void_p), ("scene_bounds", C.c_float * 6)]
+
+
+class BvhTiming(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("sort_ms", C.c_double), ("bounds_ms", C.c_double), ("tree_ms", C.c_double)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("spp", C.c_int32), ("max_bounce", C.c_int32), ("rr_start", C.c_int32), ("seed", C.c_uint32),
+                ("stratified", C.c_int32), ("integrator", C.c_int32), ("spp_per_pass", C.c_int32),
+                ("reserved", C.c_int32 * 9)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("render_ms", C.c_double), ("resolve_ms", C.c_double), ("kernel_avg_ms", C.c_double),
+                ("launches", C.c_int64), ("samples", C.c_int64), ("msamples_per_s", C.c_double),
+                ("node_visits", C.c_int64), ("tri_tests", C.c_int64), ("reserved", C.c_double * 6)]
+
+
+_lib = None
+
+
+def load_library():
+    """Loads liblts.so; raises LtsError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LtsError(f"{LIB_PATH} is missing: build it with lighttransportsimulation_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, u32 = C.c_void_p, C.c_int32, C.c_uint32
+    L.lts_create.argtypes = [C.POINTER(vp), C.c_int]
+    L.lts_destroy.argtypes = [vp]
+    L.lts_destroy.restype = None
+    L.lts_last_error.argtypes = [vp]
+    L.lts_last_error.restype = C.c_char_p
+    L.lts_version.restype = C.c_char_p
+    L.lts_read_app_params.argtypes = [C.c_char_p] + [C.POINTER(i32)] * 6 + [C.c_char_p] * 3 + [i32]
+    L.lts_load_scene.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, i32, i32]
+    L.lts_scene_info.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
+    L.lts_bvh_build.argtypes = [vp, vp, i32, i32, C.POINTER(BvhExport), C.POINTER(BvhTiming)]
+    L.lts_bvh_build_device.argtypes = [vp, vp, i32, i32, i32, C.POINTER(BvhTiming)]
+    L.lts_bvh_export_last.argtypes = [vp, C.POINTER(BvhExport)]
+    L.lts_render.argtypes = [vp, C.POINTER(RenderParams), vp, vp, C.POINTER(Stats)]
+    L.lts_write_image.argtypes = [C.c_char_p, vp, i32, i32, i32]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def bvh_arrays(n: int) -> dict:
+    m = max(n - 1, 0)
+    return dict(codes_sorted=np.zeros(n, np.uint32), perm=np.zeros(n, np.int32), left=np.zeros(m, np.int32),
+                right=np.zeros(m, np.int32), int_bounds=np.zeros((m, 6), np.float32), leaf_prim=np.zeros(n, np.int32),
+                leaf_bounds=np.zeros((n, 6), np.float32), parent=np.zeros(2 * n - 1, np.int32))
+
+
+def _export_struct(arrs: dict) -> BvhExport:
+    e = BvhExport()
+    for k in ("codes_sorted", "perm", "left", "right", "int_bounds", "leaf_prim", "leaf_bounds", "parent"):
+        setattr(e, k, _ptr(arrs[k]))
+    return e
+
+
+class Context:
+    """One HIP device context (include/lts.h: lts_create)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.lts_create(C.byref(h), device)
+        if rc != 0:
+            raise LtsError(f"lts_create({device}) failed with {rc}")
+        self.h = h
+        self.width = self.height = 0
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            raise LtsError(f"{what} failed ({rc}): {self.lib.lts_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lts_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------------ scene
+    def load_scene(self, project_dir: str, scene: str, subject: str | None, width: int, height: int):
+        d = project_dir if project_dir.endswith("/") else project_dir + "/"
+        self._check(self.lib.lts_load_scene(self.h, d.encode(), scene.encode(), (subject or "").encode(), width, height),
+                    "lts_load_scene")
+        self.width, self.height = width, height
+
+    def scene_info(self) -> dict:
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.lts_scene_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "lts_scene_info")
+        return dict(tris=a.value, lights=b.value, meshes=c.value)
+
+    # ------------------------------------------------------------------ LBVH
+    def bvh_build(self, tri: np.ndarray, mode: int = 0, export: bool = True):
+        tri = np.ascontiguousarray(tri, dtype=np.float32).reshape(-1, 9)
+        n = tri.shape[0]
+        arrs = bvh_arrays(n) if export else None
+        e = _export_struct(arrs) if export else None
+        t = BvhTiming()
+        self._check(self.lib.lts_bvh_build(self.h, _ptr(tri), n, mode, C.byref(e) if export else None, C.byref(t)),
+                    "lts_bvh_build")
+        if export:
+            arrs["scene_bounds"] = np.array(list(e.scene_bounds), np.float32)
+        return arrs, dict(total_ms=t.total_ms, sort_ms=t.sort_ms, bounds_ms=t.bounds_ms, tree_ms=t.tree_ms)
+
+    def bvh_build_device(self, dev_ptr: int, n: int, mode: int, repeat: int = 1) -> dict:
+        t = BvhTiming()
+        self._check(self.lib.lts_bvh_build_device(self.h, C.c_void_p(dev_ptr), n, mode, repeat, C.byref(t)),
+                    "lts_bvh_build_device")
+        return dict(total_ms=t.total_ms, sort_ms=t.sort_ms, bounds_ms=t.bounds_ms, tree_ms=t.tree_ms)
+
+    def bvh_export_last(self, n: int) -> dict:
+        arrs = bvh_arrays(n)
+        e = _export_struct(arrs)
+        self._check(self.lib.lts_bvh_export_last(self.h, C.byref(e)), "lts_bvh_export_last")
+        arrs["scene_bounds"] = np.array(list(e.scene_bounds), np.float32)
+        return arrs
+
+    # ------------------------------------------------------------------ render
+    def render(self, spp: int, max_bounce: int, rr_start: int, seed: int, stratified: int = 0,
+               integrator: int = INTEGRATOR_MEGAKERNEL, spp_per_pass: int = 0, want_image: bool = True,
+               want_samples: bool = False):
+        p = RenderParams(spp, max_bounce, rr_start, seed, stratified, integrator, spp_per_pass)
+        rgb = np.zeros((self.height, self.width, 3), np.float32) if want_image else None
+        smp = np.zeros((spp, self.height, self.width, 3), np.float32) if want_samples else None
+        st = Stats()
+        self._check(self.lib.lts_render(self.h, C.byref(p), _ptr(rgb), _ptr(smp), C.byref(st)), "lts_render")
+        stats = dict(render_ms=st.render_ms, resolve_ms=st.resolve_ms, kernel_avg_ms=st.kernel_avg_ms,
+                     launches=st.launches, samples=st.samples, msamples_per_s=st.msamples_per_s,
+                     node_visits=st.node_visits, tri_tests=st.tri_tests)
+        return rgb, smp, stats
+
+
+def read_app_params(path: str) -> dict:
+    L = load_library()
+    v = [C.c_int32() for _ in range(6)]
+    bufs = [C.create_string_buffer(1024) for _ in range(3)]
+    rc = L.lts_read_app_params(path.encode(), *[C.byref(x) for x in v], *bufs, 1024)
+    if rc != 0:
+        raise LtsError(f"lts_read_app_params({path}) failed ({rc})")
+    w, h, spp, b, rr, fmt = (x.value for x in v)
+    return dict(width=w, height=h, spp=spp, max_bounce=b, rr_start=rr, format=fmt, output=bufs[0].value.decode(),
+                scene=bufs[1].value.decode(), subject=bufs[2].value.decode())
+
+
+def write_image(path: str, rgb: np.ndarray, fmt: int) -> None:
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w, _ = rgb.shape
+    rc = load_library().lts_write_image(path.encode(), _ptr(rgb), w, h, fmt)
+    if rc != 0:
+        raise LtsError(f"lts_write_image({path}) failed ({rc})")

@@ -1,0 +1,443 @@
+// Karras-2012 LBVH build on gfx950, bit-exact with the reference's construction
+// (acceleration/BVH.cuh:77-452, acceleration/BVH_kernels.cu:8-50):
+//   k_prim_bounds  primitive AABB + centroid inputs, block-reduced scene bounds     (BVH.cuh:92-98)
+//   k_morton       30-bit Morton codes of the normalised centroids                   (BVH.cuh:101-137)
+//   radix sort     stable LSD sort of (code, primitive) pairs, 8-bit digits, 4 passes (BVH.cuh:140-146)
+//                  count -> per-digit scan over tiles -> digit bases -> LDS-ranked scatter
+//   k_karras       interior node i: direction, range end, split                     (BVH.cuh:173-236)
+//   k_refit        leaf-to-root climb, second arriver unions its children            (BVH.cuh:238-262)
+//   k_pack_*       traversal layout: both child boxes per node, triangles in leaf order
+// The scene-bound reduction uses a total order on floats (-0 below +0); it equals the reference's
+// float atomics (GeneralHelper.cuh:150-160) on every input without -0.0.
+#include "lts_device.h"
+#include "lts_internal.h"
+
+namespace lts {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kBlock * kItems;  // keys per radix tile
+
+__device__ __forceinline__ uint32_t order_key(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float order_val(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+__global__ void k_init_scene_keys(uint32_t* keys) {
+    int t = threadIdx.x;
+    if (t < 3) keys[t] = 0xffffffffu;       // running minimum
+    else if (t < 6) keys[t] = 0u;           // running maximum
+}
+
+// Triangle::worldBound = bUnion(Bounds3f(p0, p1), p2) with the Bounds3 constructor re-applied.
+__device__ __forceinline__ void tri_box(const float* t, float* mn, float* mx) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float lo = fminf(t[a], t[3 + a]), hi = fmaxf(t[a], t[3 + a]);
+        float lo2 = fminf(lo, t[6 + a]), hi2 = fmaxf(hi, t[6 + a]);
+        mn[a] = fminf(lo2, hi2);
+        mx[a] = fmaxf(lo2, hi2);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_prim_bounds(const float* __restrict__ soup, int n, float* __restrict__ primBox,
+                                                        uint32_t* __restrict__ sceneKeys) {
+    __shared__ uint32_t red[6][kBlock / 64];
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t k[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    if (i < n) {
+        float t[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) t[j] = soup[9 * (size_t)i + j];
+        float mn[3], mx[3];
+        tri_box(t, mn, mx);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            primBox[6 * (size_t)i + a] = mn[a];
+            primBox[6 * (size_t)i + 3 + a] = mx[a];
+            k[a] = order_key(mn[a]);
+            k[3 + a] = order_key(mx[a]);
+        }
+    }
+    // wave reduction, then one atomic per component per block
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            k[a] = min(k[a], (uint32_t)__shfl_xor((int)k[a], off));
+            k[3 + a] = max(k[3 + a], (uint32_t)__shfl_xor((int)k[3 + a], off));
+        }
+    }
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int a = 0; a < 6; a++) red[a][wave] = k[a];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int a = threadIdx.x;
+        uint32_t v = red[a][0];
+        for (int w = 1; w < kBlock / 64; w++) v = a < 3 ? min(v, red[a][w]) : max(v, red[a][w]);
+        if (a < 3) atomicMin(&sceneKeys[a], v); else atomicMax(&sceneKeys[a], v);
+    }
+}
+
+// leftShift3 (BVH.cuh:101-109)
+__device__ __forceinline__ uint32_t left_shift3(uint32_t x) {
+    if (x == (1u << 10)) --x;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ primBox, int n, const uint32_t* __restrict__ sceneKeys,
+                                                   uint32_t* __restrict__ codes, int32_t* __restrict__ ids) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float gmn[3], gmx[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) { gmn[a] = order_val(sceneKeys[a]); gmx[a] = order_val(sceneKeys[3 + a]); }
+    uint32_t code = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float c = primBox[6 * (size_t)i + a] * 0.5f + primBox[6 * (size_t)i + 3 + a] * 0.5f;  // centroid
+        float o = c - gmn[a];
+        if (gmx[a] > gmn[a]) o /= gmx[a] - gmn[a];                                         // Bounds3::offset
+        float s = (float)(1 << 10) * o;
+        code |= left_shift3((uint32_t)s) << a;
+    }
+    codes[i] = code;
+    ids[i] = i;
+}
+
+// ---------------------------------------------------------------- radix sort
+__global__ __launch_bounds__(kBlock) void k_radix_count(const uint32_t* __restrict__ keys, int n, int shift, int tiles,
+                                                        uint32_t* __restrict__ counts) {
+    __shared__ uint32_t hist[256];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    size_t base = (size_t)blockIdx.x * kTile;
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+        size_t idx = base + (size_t)j * kBlock + threadIdx.x;
+        if (idx < (size_t)n) atomicAdd(&hist[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[(size_t)threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads); returns the total.
+__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* out, uint32_t* scratch) {
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = (uint32_t)__shfl_up((int)incl, off);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) scratch[wave] = incl;
+    __syncthreads();
+    uint32_t wbase = 0, total = 0;
+    for (int w = 0; w < kBlock / 64; w++) {
+        if (w < wave) wbase += scratch[w];
+        total += scratch[w];
+    }
+    *out = wbase + incl - v;
+    __syncthreads();
+    return total;
+}
+
+// One block per digit: exclusive scan of that digit's per-tile counts, total into digitBase.
+__global__ __launch_bounds__(kBlock) void k_radix_scan_tiles(uint32_t* __restrict__ counts, int tiles, uint32_t* __restrict__ digitBase) {
+    __shared__ uint32_t scratch[kBlock / 64];
+    uint32_t* row = counts + (size_t)blockIdx.x * tiles;
+    int per = (tiles + kBlock - 1) / kBlock;
+    int beg = threadIdx.x * per, end = min(beg + per, tiles);
+    uint32_t sum = 0;
+    for (int t = beg; t < end; t++) sum += row[t];
+    uint32_t excl;
+    uint32_t total = block_exclusive_scan(sum, &excl, scratch);
+    for (int t = beg; t < end; t++) { uint32_t c = row[t]; row[t] = excl; excl += c; }
+    if (threadIdx.x == 0) digitBase[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_radix_scan_digits(uint32_t* __restrict__ digitBase) {
+    __shared__ uint32_t scratch[kBlock / 64];
+    uint32_t v = digitBase[threadIdx.x], excl;
+    block_exclusive_scan(v, &excl, scratch);
+    digitBase[threadIdx.x] = excl;
+}
+
+// Stable scatter of one 4096-key tile: ranks from wave ballots, tile-local order staged in LDS,
+// then written out in runs per digit.
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __restrict__ keysIn, const int32_t* __restrict__ valsIn,
+                                                          uint32_t* __restrict__ keysOut, int32_t* __restrict__ valsOut, int n,
+                                                          int shift, int tiles, const uint32_t* __restrict__ counts,
+                                                          const uint32_t* __restrict__ digitBase) {
+    __shared__ uint32_t sKeys[kTile];
+    __shared__ int32_t sVals[kTile];
+    __shared__ uint32_t waveCnt[kBlock / 64][256];
+    __shared__ uint32_t runTotal[256];
+    __shared__ uint32_t tileStart[256];
+    __shared__ uint32_t globalBase[256];
+    __shared__ uint32_t scratch[kBlock / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const size_t base = (size_t)blockIdx.x * kTile;
+    const int valid = (int)min((size_t)kTile, (size_t)n - base);
+    for (int w = 0; w < kBlock / 64; w++) waveCnt[w][t] = 0;
+    runTotal[t] = 0;
+    globalBase[t] = digitBase[t] + counts[(size_t)t * tiles + blockIdx.x];
+    uint32_t key[kItems];
+    int32_t val[kItems];
+    uint32_t rank[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+        size_t idx = base + (size_t)j * kBlock + t;
+        bool ok = idx < (size_t)n;
+        key[j] = ok ? keysIn[idx] : 0xffffffffu;  // tail keys rank last in digit 255 and are never written
+        val[j] = ok ? valsIn[idx] : 0;
+    }
+    __syncthreads();
+    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+        uint32_t d = (key[j] >> shift) & 255u;
+        uint64_t peers = ~0ull;
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        uint32_t inWave = (uint32_t)__popcll(peers & ltMask);
+        int leader = __ffsll((long long)peers) - 1;
+        if (lane == leader) waveCnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        uint32_t pre = runTotal[d];
+        for (int w = 0; w < wave; w++) pre += waveCnt[w][d];
+        rank[j] = pre + inWave;
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < kBlock / 64; w++) { add += waveCnt[w][t]; waveCnt[w][t] = 0; }
+        runTotal[t] += add;
+        __syncthreads();
+    }
+    uint32_t start;
+    block_exclusive_scan(runTotal[t], &start, scratch);
+    tileStart[t] = start;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+        uint32_t d = (key[j] >> shift) & 255u;
+        uint32_t pos = tileStart[d] + rank[j];
+        sKeys[pos] = key[j];
+        sVals[pos] = val[j];
+    }
+    __syncthreads();
+    for (int i = t; i < valid; i += kBlock) {
+        uint32_t k = sKeys[i];
+        uint32_t d = (k >> shift) & 255u;
+        size_t dst = (size_t)globalBase[d] + (i - tileStart[d]);
+        keysOut[dst] = k;
+        valsOut[dst] = sVals[i];
+    }
+}
+
+// ---------------------------------------------------------------- hierarchy
+__device__ __forceinline__ int delta_at(const uint32_t* __restrict__ codes, int n, int i, int j) {
+    if (i < 0 || i >= n || j < 0 || j >= n) return -1;
+    uint32_t x = codes[i] ^ codes[j];
+    int extra = 0;
+    if (x == 0) { extra = 40; x = (uint32_t)(i ^ j); }
+    return (x == 0 ? 32 : __clz((int)x)) + extra;
+}
+
+__global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ codes, int n, int32_t* __restrict__ left,
+                                                   int32_t* __restrict__ right, int32_t* __restrict__ parent) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = delta_at(codes, n, i, i + 1) - delta_at(codes, n, i, i - 1);
+    d = d > 0 ? 1 : -1;
+    int deltaMin = delta_at(codes, n, i, i - d);
+    int lMax = 128;
+    while (delta_at(codes, n, i, i + lMax * d) > deltaMin) lMax *= 4;
+    int l = 0;
+    for (int t = lMax / 2; t >= 1; t /= 2)
+        if (delta_at(codes, n, i, i + (l + t) * d) > deltaMin) l += t;
+    int j = i + l * d;
+    int deltaNode = delta_at(codes, n, i, j);
+    int s = 0;
+    for (int t = (l + 1) >> 1; t >= 1; t = (t == 1) ? 0 : ((t + 1) >> 1))  // ceil-halving
+        if (delta_at(codes, n, i, i + (s + t) * d) > deltaNode) s += t;
+    int gamma = i + s * d + min(d, 0);
+    int lo = min(i, j), hi = max(i, j);
+    int lc = (lo == gamma) ? (n - 1) + gamma : gamma;
+    int rc = (hi == gamma + 1) ? (n - 1) + gamma + 1 : gamma + 1;
+    left[i] = lc;
+    right[i] = rc;
+    parent[lc] = i;
+    parent[rc] = i;
+    if (i == 0) parent[0] = -1;
+}
+
+__global__ void k_fill_i32(int32_t* p, int n, int32_t v) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// bUnion(Bounds3, Bounds3) followed by the Bounds3(p1, p2) constructor
+__device__ __forceinline__ void union_box(const float* a, const float* b, float* o) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float lo = fminf(a[k], b[k]), hi = fmaxf(a[3 + k], b[3 + k]);
+        o[k] = fminf(lo, hi);
+        o[3 + k] = fmaxf(lo, hi);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit(const float* __restrict__ primBox, const int32_t* __restrict__ perm, int n, int mode,
+                                                  const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+                                                  const int32_t* __restrict__ parent, float* box, int32_t* flags) {
+    int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    int leaf = (n - 1) + k;
+    int prim = mode == 0 ? k : perm[k];
+#pragma unroll
+    for (int a = 0; a < 6; a++) box[6 * (size_t)leaf + a] = primBox[6 * (size_t)prim + a];
+    if (n == 1) return;
+    int node = parent[leaf];
+    // a Karras tree over n <= 2^26 primitives is at most 57 levels deep; the bound only guards
+    // against a malformed parent array
+    for (int step = 0; step < 128 && node >= 0 && node < n - 1; step++) {
+        __threadfence();
+        int old = __hip_atomic_fetch_add(&flags[node], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;  // first arrival: the sibling subtree is not finished yet
+        __threadfence();
+        float a[6], b[6], o[6];
+        int l = left[node], r = right[node];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            a[q] = __hip_atomic_load(&box[6 * (size_t)l + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            b[q] = __hip_atomic_load(&box[6 * (size_t)r + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        union_box(a, b, o);
+#pragma unroll
+        for (int q = 0; q < 6; q++) __hip_atomic_store(&box[6 * (size_t)node + q], o[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (node == 0) return;
+        node = parent[node];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_nodes(const float* __restrict__ box, const int32_t* __restrict__ left,
+                                                       const int32_t* __restrict__ right, int n, BvhNode* __restrict__ nodes) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    BvhNode nd;
+    int l = left[i], r = right[i];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        nd.lmin[a] = box[6 * (size_t)l + a]; nd.lmax[a] = box[6 * (size_t)l + 3 + a];
+        nd.rmin[a] = box[6 * (size_t)r + a]; nd.rmax[a] = box[6 * (size_t)r + 3 + a];
+    }
+    nd.left = l; nd.right = r; nd.pad0 = nd.pad1 = 0;
+    nodes[i] = nd;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_tris(const float* __restrict__ soup, const int32_t* __restrict__ perm, int n,
+                                                      TriLeaf* __restrict__ tris) {
+    int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    int p = perm[k];
+    const float* t = soup + 9 * (size_t)p;
+    TriLeaf o;
+    o.p0x = t[0]; o.p0y = t[1]; o.p0z = t[2]; o.p1x = t[3];
+    o.p1y = t[4]; o.p1z = t[5]; o.p2x = t[6]; o.p2y = t[7];
+    o.p2z = t[8]; o.prim = p; o.pad0 = o.pad1 = 0;
+    tris[k] = o;
+}
+
+inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t count) {
+    return hipMalloc((void**)p, (count ? count : 1) * sizeof(T));
+}
+
+hipError_t bvh_reserve(BvhBuffers& b, int n) {
+    if (n <= b.cap) return hipSuccess;
+    bvh_release(b);
+    int tiles = (n + kTile - 1) / kTile;
+    hipError_t e = hipSuccess;
+#define TRY(x) do { e = (x); if (e != hipSuccess) { bvh_release(b); return e; } } while (0)
+    TRY(dalloc(&b.primBox, 6 * (size_t)n));
+    TRY(dalloc(&b.box, 6 * (size_t)(2 * n - 1)));
+    TRY(dalloc(&b.keysA, n)); TRY(dalloc(&b.keysB, n));
+    TRY(dalloc(&b.valsA, n)); TRY(dalloc(&b.valsB, n));
+    TRY(dalloc(&b.counts, 256 * (size_t)tiles));
+    TRY(dalloc(&b.digitBase, 256));
+    TRY(dalloc(&b.left, n)); TRY(dalloc(&b.right, n));
+    TRY(dalloc(&b.parent, 2 * (size_t)n));
+    TRY(dalloc(&b.flags, n));
+    TRY(dalloc(&b.sceneKeys, 8));
+    TRY(dalloc(&b.nodes, n));
+    TRY(dalloc(&b.tris, n));
+#undef TRY
+    b.cap = n;
+    return hipSuccess;
+}
+
+void bvh_release(BvhBuffers& b) {
+    void* ps[] = {b.primBox, b.box, b.keysA, b.keysB, b.valsA, b.valsB, b.counts, b.digitBase, b.left, b.right,
+                  b.parent, b.flags, b.sceneKeys, b.nodes, b.tris};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    b = BvhBuffers();
+}
+
+hipError_t bvh_enqueue(BvhBuffers& b, const float* soup, int n, int mode, bool pack, hipStream_t s, BvhStageEvents* ev) {
+    if (n < 1 || n > b.cap) return hipErrorInvalidValue;
+    b.n = n;
+    b.tiles = (n + kTile - 1) / kTile;
+    if (ev) (void)hipEventRecord(ev->e[0], s);
+    k_init_scene_keys<<<1, 64, 0, s>>>(b.sceneKeys);
+    k_prim_bounds<<<blocks_for(n), kBlock, 0, s>>>(soup, n, b.primBox, b.sceneKeys);
+    k_morton<<<blocks_for(n), kBlock, 0, s>>>(b.primBox, n, b.sceneKeys, b.keysA, b.valsA);
+    if (ev) (void)hipEventRecord(ev->e[1], s);
+    uint32_t* kin = b.keysA; uint32_t* kout = b.keysB;
+    int32_t* vin = b.valsA; int32_t* vout = b.valsB;
+    for (int pass = 0; pass < 4; pass++) {
+        int shift = 8 * pass;
+        k_radix_count<<<b.tiles, kBlock, 0, s>>>(kin, n, shift, b.tiles, b.counts);
+        k_radix_scan_tiles<<<256, kBlock, 0, s>>>(b.counts, b.tiles, b.digitBase);
+        k_radix_scan_digits<<<1, kBlock, 0, s>>>(b.digitBase);
+        k_radix_scatter<<<b.tiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, shift, b.tiles, b.counts, b.digitBase);
+        uint32_t* tk = kin; kin = kout; kout = tk;
+        int32_t* tv = vin; vin = vout; vout = tv;
+    }
+    // after 4 passes the sorted arrays are back in keysA / valsA
+    if (ev) (void)hipEventRecord(ev->e[2], s);
+    if (n > 1) {
+        k_fill_i32<<<blocks_for(n - 1), kBlock, 0, s>>>(b.flags, n - 1, 0);
+        k_fill_i32<<<blocks_for(2 * n - 1), kBlock, 0, s>>>(b.parent, 2 * n - 1, -1);
+        k_karras<<<blocks_for(n - 1), kBlock, 0, s>>>(b.keysA, n, b.left, b.right, b.parent);
+    } else {
+        k_fill_i32<<<1, kBlock, 0, s>>>(b.parent, 1, -1);
+    }
+    k_refit<<<blocks_for(n), kBlock, 0, s>>>(b.primBox, b.valsA, n, mode, b.left, b.right, b.parent, b.box, b.flags);
+    if (ev) (void)hipEventRecord(ev->e[3], s);
+    if (pack) {
+        if (n > 1) k_pack_nodes<<<blocks_for(n - 1), kBlock, 0, s>>>(b.box, b.left, b.right, n, b.nodes);
+        k_pack_tris<<<blocks_for(n), kBlock, 0, s>>>(soup, b.valsA, n, b.tris);
+    }
+    if (ev) (void)hipEventRecord(ev->e[4], s);
+    return hipGetLastError();
+}
+
+}  // namespace lts

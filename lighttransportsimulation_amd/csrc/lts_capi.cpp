@@ -1,0 +1,399 @@
+// C ABI of liblts.so (include/lts.h): context, scene upload, LBVH build and export, rendering.
+#include "lts.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "image_io.h"
+#include "lts_internal.h"
+#include "scene_host.h"
+
+namespace lts {
+hipError_t upload_perlin_table(const int* table512);
+}
+
+using namespace lts;
+
+namespace {
+
+// Ken Perlin's permutation (improved noise reference implementation), as tabulated by the
+// reference at materials/textures/Procedural.cuh:79-107; stored twice so index + 255 stays in range.
+const int kPerm256[256] = {
+    151, 160, 137, 91,  90,  15,  131, 13,  201, 95,  96,  53,  194, 233, 7,   225, 140, 36,  103, 30,  69,  142,
+    8,   99,  37,  240, 21,  10,  23,  190, 6,   148, 247, 120, 234, 75,  0,   26,  197, 62,  94,  252, 219, 203,
+    117, 35,  11,  32,  57,  177, 33,  88,  237, 149, 56,  87,  174, 20,  125, 136, 171, 168, 68,  175, 74,  165,
+    71,  134, 139, 48,  27,  166, 77,  146, 158, 231, 83,  111, 229, 122, 60,  211, 133, 230, 220, 105, 92,  41,
+    55,  46,  245, 40,  244, 102, 143, 54,  65,  25,  63,  161, 1,   216, 80,  73,  209, 76,  132, 187, 208, 89,
+    18,  169, 200, 196, 135, 130, 116, 188, 159, 86,  164, 100, 109, 198, 173, 186, 3,   64,  52,  217, 226, 250,
+    124, 123, 5,   202, 38,  147, 118, 126, 255, 82,  85,  212, 207, 206, 59,  227, 47,  16,  58,  17,  182, 189,
+    28,  42,  223, 183, 170, 213, 119, 248, 152, 2,   44,  154, 163, 70,  221, 153, 101, 155, 167, 43,  172, 9,
+    129, 22,  39,  253, 19,  98,  108, 110, 79,  113, 224, 232, 178, 185, 112, 104, 218, 246, 97,  228, 251, 34,
+    242, 193, 238, 210, 144, 12,  191, 179, 162, 241, 81,  51,  145, 235, 249, 14,  239, 107, 49,  192, 214, 31,
+    181, 199, 106, 157, 184, 84,  204, 176, 115, 121, 50,  45,  127, 4,   150, 254, 138, 236, 205, 93,  222, 114,
+    67,  29,  24,  72,  243, 141, 128, 195, 78,  66,  215, 61,  156, 180};
+
+constexpr int kMaxPrims = 1 << 26;  // keeps the Karras depth (and the traversal stack) below 64
+
+struct DeviceScene {
+    bool loaded = false;
+    HostScene host;
+    float* soup = nullptr;
+    ShadeTri* shade = nullptr;
+    MatDesc* mats = nullptr;
+    TexDesc* texs = nullptr;
+    MipDesc* mips = nullptr;
+    float* mipData = nullptr;
+    LightDesc* lights = nullptr;
+    float* lfunc = nullptr;
+    float* lcdf = nullptr;
+    BvhBuffers bvh;
+    SceneView view{};
+};
+
+}  // namespace
+
+struct lts_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    BvhBuffers build;         // standalone builds (lts_bvh_build*)
+    float* buildSoup = nullptr;
+    size_t buildSoupCap = 0;
+    int lastMode = 0;
+    DeviceScene scene;
+    float* sampleL = nullptr;
+    size_t sampleCap = 0;
+    float* film = nullptr;
+    float* rgb = nullptr;
+    size_t filmCap = 0;
+    unsigned long long* counters = nullptr;
+    hipEvent_t ev[8] = {};
+};
+
+namespace {
+
+int fail(lts_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? LTS_E_NOMEM : LTS_E_HIP,                     \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                            \
+    } while (0)
+
+template <typename T>
+hipError_t upload(T** dst, const T* src, size_t count, hipStream_t s) {
+    hipError_t e = hipMalloc((void**)dst, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess || count == 0) return e;
+    e = hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+void free_scene(DeviceScene& s) {
+    void* ps[] = {s.soup, s.shade, s.mats, s.texs, s.mips, s.mipData, s.lights, s.lfunc, s.lcdf};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    bvh_release(s.bvh);
+    s = DeviceScene();
+}
+
+float ms_between(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+int copy_export(lts_ctx* c, const BvhBuffers& b, int mode, lts_bvh_export* out) {
+    const int n = b.n;
+    hipStream_t s = c->stream;
+    if (out->codes_sorted) HIPCHK(c, hipMemcpyAsync(out->codes_sorted, b.keysA, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    if (out->perm) HIPCHK(c, hipMemcpyAsync(out->perm, b.valsA, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (n > 1) {
+        if (out->left) HIPCHK(c, hipMemcpyAsync(out->left, b.left, (n - 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        if (out->right) HIPCHK(c, hipMemcpyAsync(out->right, b.right, (n - 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        if (out->int_bounds) HIPCHK(c, hipMemcpyAsync(out->int_bounds, b.box, 6 * (size_t)(n - 1) * sizeof(float), hipMemcpyDeviceToHost, s));
+    }
+    if (out->leaf_bounds) HIPCHK(c, hipMemcpyAsync(out->leaf_bounds, b.box + 6 * (size_t)(n - 1), 6 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (out->parent) HIPCHK(c, hipMemcpyAsync(out->parent, b.parent, (2 * (size_t)n - 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    uint32_t keys[6];
+    HIPCHK(c, hipMemcpyAsync(keys, b.sceneKeys, sizeof(keys), hipMemcpyDeviceToHost, s));
+    if (out->leaf_prim) {
+        if (mode == 0) {
+            for (int k = 0; k < n; k++) out->leaf_prim[k] = k;
+        } else {
+            HIPCHK(c, hipMemcpyAsync(out->leaf_prim, b.valsA, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(s));
+    for (int a = 0; a < 6; a++) {
+        uint32_t k = keys[a];
+        uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+        std::memcpy(&out->scene_bounds[a], &u, 4);
+    }
+    return LTS_OK;
+}
+
+int run_build(lts_ctx* c, BvhBuffers& b, const float* dsoup, int n, int mode, bool pack, lts_bvh_timing* t) {
+    BvhStageEvents ev;
+    for (int i = 0; i < 5; i++) ev.e[i] = c->ev[i];
+    HIPCHK(c, bvh_enqueue(b, dsoup, n, mode, pack, c->stream, &ev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (t) {
+        t->bounds_ms = ms_between(ev.e[0], ev.e[1]);
+        t->sort_ms = ms_between(ev.e[1], ev.e[2]);
+        t->tree_ms = ms_between(ev.e[2], ev.e[4]);
+        t->total_ms = ms_between(ev.e[0], ev.e[4]);
+    }
+    return LTS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lts_version(void) { return "lts-mi355x 0.1 (gfx950)"; }
+
+int lts_create(lts_ctx** out, int dev) {
+    if (!out) return LTS_E_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || dev < 0 || dev >= count) return LTS_E_HIP;
+    lts_ctx* c = new lts_ctx();
+    c->device = dev;
+    if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return LTS_E_HIP;
+    }
+    for (hipEvent_t& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) { delete c; return LTS_E_HIP; }
+    int table[512];
+    for (int i = 0; i < 512; i++) table[i] = kPerm256[i & 255];
+    if (upload_perlin_table(table) != hipSuccess) { delete c; return LTS_E_HIP; }
+    if (hipMalloc((void**)&c->counters, 2 * sizeof(unsigned long long)) != hipSuccess) { delete c; return LTS_E_HIP; }
+    *out = c;
+    return LTS_OK;
+}
+
+void lts_destroy(lts_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_scene(c->scene);
+    bvh_release(c->build);
+    void* ps[] = {c->buildSoup, c->sampleL, c->film, c->rgb, c->counters};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* lts_last_error(const lts_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int lts_read_app_params(const char* path, int32_t* w, int32_t* h, int32_t* spp, int32_t* bounce, int32_t* rr,
+                        int32_t* fmt, char* output, char* scene, char* subject, int32_t cap) {
+    if (!path || !w || !h || !spp || !bounce || !rr || !fmt || !output || !scene || !subject || cap < 2) return LTS_E_ARG;
+    std::ifstream in(path);
+    if (!in) return LTS_E_IO;
+    std::string t[9];
+    for (int i = 0; i < 9; i++)
+        if (!(in >> t[i])) return LTS_E_IO;
+    try {
+        *w = std::stoi(t[0]); *h = std::stoi(t[1]); *spp = std::stoi(t[2]);
+        *bounce = std::stoi(t[3]); *rr = std::stoi(t[4]); *fmt = std::stoi(t[5]);
+    } catch (...) {
+        return LTS_E_IO;
+    }
+    std::string outName = t[6] + (*fmt ? ".pfm" : ".hdr");  // FileIO.cuh:251
+    if ((int)outName.size() >= cap || (int)t[7].size() >= cap || (int)t[8].size() >= cap) return LTS_E_ARG;
+    std::strcpy(output, outName.c_str());
+    std::strcpy(scene, t[7].c_str());
+    std::strcpy(subject, t[8].c_str());
+    return LTS_OK;
+}
+
+int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* subject, int32_t width, int32_t height) {
+    if (!c || !dir || !scene) return fail(c, LTS_E_ARG, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    free_scene(c->scene);
+    DeviceScene& S = c->scene;
+    std::string err;
+    std::string d(dir);
+    if (!d.empty() && d.back() != '/') d += '/';
+    if (!load_scene_files(d, scene, subject ? subject : "", width, height, S.host, err)) return fail(c, LTS_E_IO, err);
+    const HostScene& H = S.host;
+    const int n = (int)H.shade.size();
+    if (n > kMaxPrims) return fail(c, LTS_E_ARG, "scene has more than 2^26 triangles");
+    hipStream_t s = c->stream;
+    HIPCHK(c, upload(&S.soup, H.soup.data(), H.soup.size(), s));
+    HIPCHK(c, upload(&S.shade, H.shade.data(), H.shade.size(), s));
+    HIPCHK(c, upload(&S.mats, H.mats.data(), H.mats.size(), s));
+    HIPCHK(c, upload(&S.texs, H.texs.data(), H.texs.size(), s));
+    HIPCHK(c, upload(&S.mips, H.mips.data(), H.mips.size(), s));
+    HIPCHK(c, upload(&S.mipData, H.mipData.data(), H.mipData.size(), s));
+    HIPCHK(c, upload(&S.lfunc, H.lfunc.data(), H.lfunc.size(), s));
+    HIPCHK(c, upload(&S.lcdf, H.lcdf.data(), H.lcdf.size(), s));
+    HIPCHK(c, bvh_reserve(S.bvh, n));
+    int rc = run_build(c, S.bvh, S.soup, n, 1, true, nullptr);
+    if (rc != LTS_OK) return rc;
+    // leaf index of every area light's triangle (inverse of the Morton permutation)
+    std::vector<int32_t> perm(n);
+    HIPCHK(c, hipMemcpy(perm.data(), S.bvh.valsA, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int32_t> leafOf(n);
+    for (int k = 0; k < n; k++) leafOf[perm[k]] = k;
+    std::vector<LightDesc> lights = H.lights;
+    for (LightDesc& L : lights)
+        if (L.kind == 0) L.leaf = leafOf[L.prim];
+    HIPCHK(c, upload(&S.lights, lights.data(), lights.size(), s));
+    SceneView& v = S.view;
+    v.tris = S.bvh.tris; v.nodes = S.bvh.nodes; v.shade = S.shade; v.mats = S.mats; v.texs = S.texs;
+    v.mips = S.mips; v.mipData = S.mipData; v.lights = S.lights; v.lfunc = S.lfunc; v.lcdf = S.lcdf;
+    v.funcInt = H.funcInt; v.nLights = (int)H.lights.size(); v.nPrim = n; v.infiniteLight = H.infiniteLight;
+    v.cam = H.cam;
+    std::memcpy(v.filter, H.filter, sizeof(v.filter));
+    S.loaded = true;
+    return LTS_OK;
+}
+
+int lts_scene_info(lts_ctx* c, int32_t* nt, int32_t* nl, int32_t* nm) {
+    if (!c) return LTS_E_ARG;
+    if (!c->scene.loaded) return fail(c, LTS_E_STATE, "no scene loaded");
+    if (nt) *nt = c->scene.view.nPrim;
+    if (nl) *nl = c->scene.view.nLights;
+    if (nm) *nm = c->scene.host.nMeshes;
+    return LTS_OK;
+}
+
+int lts_bvh_build(lts_ctx* c, const float* tri, int32_t n, int32_t mode, lts_bvh_export* out, lts_bvh_timing* t) {
+    if (!c || !tri || n < 1 || n > kMaxPrims || (mode != 0 && mode != 1)) return fail(c, LTS_E_ARG, "bad build arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    size_t bytes = 9 * (size_t)n * sizeof(float);
+    if (bytes > c->buildSoupCap) {
+        if (c->buildSoup) (void)hipFree(c->buildSoup);
+        c->buildSoup = nullptr;
+        c->buildSoupCap = 0;
+        HIPCHK(c, hipMalloc((void**)&c->buildSoup, bytes));
+        c->buildSoupCap = bytes;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->buildSoup, tri, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, bvh_reserve(c->build, n));
+    int rc = run_build(c, c->build, c->buildSoup, n, mode, true, t);
+    if (rc != LTS_OK) return rc;
+    c->lastMode = mode;
+    return out ? copy_export(c, c->build, mode, out) : LTS_OK;
+}
+
+int lts_bvh_build_device(lts_ctx* c, const float* dtri, int32_t n, int32_t mode, int32_t repeat, lts_bvh_timing* t) {
+    if (!c || !dtri || n < 1 || n > kMaxPrims || (mode != 0 && mode != 1) || repeat < 1) return fail(c, LTS_E_ARG, "bad build arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, bvh_reserve(c->build, n));
+    lts_bvh_timing acc{};
+    for (int r = 0; r < repeat; r++) {
+        lts_bvh_timing one{};
+        int rc = run_build(c, c->build, dtri, n, mode, false, &one);
+        if (rc != LTS_OK) return rc;
+        acc.total_ms += one.total_ms; acc.sort_ms += one.sort_ms; acc.bounds_ms += one.bounds_ms; acc.tree_ms += one.tree_ms;
+    }
+    if (t) {
+        t->total_ms = acc.total_ms / repeat; t->sort_ms = acc.sort_ms / repeat;
+        t->bounds_ms = acc.bounds_ms / repeat; t->tree_ms = acc.tree_ms / repeat;
+    }
+    c->lastMode = mode;
+    return LTS_OK;
+}
+
+int lts_bvh_export_last(lts_ctx* c, lts_bvh_export* out) {
+    if (!c || !out) return LTS_E_ARG;
+    if (c->build.n < 1) return fail(c, LTS_E_STATE, "no build to export");
+    return copy_export(c, c->build, c->lastMode, out);
+}
+
+int lts_render(lts_ctx* c, const lts_render_params* p, float* rgb_out, float* sample_rgb, lts_stats* st) {
+    if (!c || !p) return fail(c, LTS_E_ARG, "null argument");
+    if (!c->scene.loaded) return fail(c, LTS_E_STATE, "no scene loaded");
+    if (p->spp < 1 || p->max_bounce < 0 || p->integrator != LTS_INTEGRATOR_MEGAKERNEL)
+        return fail(c, LTS_E_ARG, "unsupported render parameters");
+    if (p->stratified) {
+        int s = (int)std::floor(std::sqrt((float)p->spp));
+        if (s * s != p->spp) return fail(c, LTS_E_ARG, "stratified film sampling needs a square sample count");
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    const SceneView& sv = c->scene.view;
+    const int W = sv.cam.width, H = sv.cam.height;
+    const size_t npix = (size_t)W * H;
+    int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, std::min<size_t>((size_t)p->spp, (size_t)(1u << 28) / (npix * 12)));
+    if (sample_rgb) pass = p->spp;  // all samples are returned: one pass
+    pass = std::min(pass, p->spp);
+    size_t need = npix * (size_t)pass * 3;
+    if (need > c->sampleCap) {
+        if (c->sampleL) (void)hipFree(c->sampleL);
+        c->sampleL = nullptr;
+        c->sampleCap = 0;
+        HIPCHK(c, hipMalloc((void**)&c->sampleL, need * sizeof(float)));
+        c->sampleCap = need;
+    }
+    if (npix > c->filmCap) {
+        if (c->film) (void)hipFree(c->film);
+        if (c->rgb) (void)hipFree(c->rgb);
+        c->film = c->rgb = nullptr;
+        c->filmCap = 0;
+        HIPCHK(c, hipMalloc((void**)&c->film, npix * 4 * sizeof(float)));
+        HIPCHK(c, hipMalloc((void**)&c->rgb, npix * 3 * sizeof(float)));
+        c->filmCap = npix;
+    }
+    hipStream_t s = c->stream;
+    HIPCHK(c, launch_film_clear(c->film, (int)npix, s));
+    RenderLaunch rl{};
+    rl.spp = p->spp; rl.maxBounce = p->max_bounce; rl.rrStart = p->rr_start; rl.seed = p->seed;
+    rl.stratified = p->stratified; rl.sampleL = c->sampleL; rl.film = c->film; rl.counters = nullptr;
+    double renderMs = 0, resolveMs = 0;
+    int64_t launches = 0;
+    for (int s0 = 0; s0 < p->spp; s0 += pass) {
+        rl.s0 = s0;
+        rl.s1 = std::min(p->spp, s0 + pass);
+        HIPCHK(c, hipEventRecord(c->ev[5], s));
+        HIPCHK(c, launch_megakernel(sv, rl, s));
+        HIPCHK(c, hipEventRecord(c->ev[6], s));
+        HIPCHK(c, launch_film_gather(sv, rl, s));
+        HIPCHK(c, hipEventRecord(c->ev[7], s));
+        HIPCHK(c, hipEventSynchronize(c->ev[7]));
+        renderMs += ms_between(c->ev[5], c->ev[6]);
+        resolveMs += ms_between(c->ev[6], c->ev[7]);
+        launches++;
+        if (sample_rgb)
+            HIPCHK(c, hipMemcpy(sample_rgb, c->sampleL, npix * (size_t)(rl.s1 - rl.s0) * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    HIPCHK(c, launch_get_render(sv, c->film, c->rgb, s));
+    if (rgb_out) HIPCHK(c, hipMemcpyAsync(rgb_out, c->rgb, npix * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->render_ms = renderMs;
+        st->resolve_ms = resolveMs;
+        st->launches = launches;
+        st->kernel_avg_ms = launches ? renderMs / launches : 0;
+        st->samples = (int64_t)npix * p->spp;
+        st->msamples_per_s = (renderMs + resolveMs) > 0 ? st->samples / ((renderMs + resolveMs) * 1e3) : 0;
+    }
+    return LTS_OK;
+}
+
+int lts_write_image(const char* path, const float* rgb, int32_t w, int32_t h, int32_t format) {
+    if (!path || !rgb || w < 1 || h < 1) return LTS_E_ARG;
+    bool ok = format == 1 ? write_pfm(path, rgb, w, h) : write_ppm(path, rgb, w, h);
+    return ok ? LTS_OK : LTS_E_IO;
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// Device-resident scene layout (plain structs shared by the host loader and the kernels).
+//
+// HBM layout of one loaded scene:
+//   tris     TriLeaf[n]   48 B  vertices in BVH leaf order (leaf k = k-th primitive in Morton order)
+//   nodes    BvhNode[n-1] 64 B  both child boxes + child indices (unified index space)
+//   shade    ShadeTri[n]  80 B  per primitive: normals, uvs, material, light (read once per hit)
+//   lights   LightDesc[L]       area lights in global triangle order, infinite light last
+//   mats/texs/mips              material, texture and mip-pyramid descriptors
+//   mipData  float[3*]          all pyramid levels, in the reference's blocked-array slot order
+#pragma once
+#include <stdint.h>
+
+namespace lts {
+
+constexpr int kMaxMipLevels = 16;
+
+struct alignas(16) TriLeaf {
+    float p0x, p0y, p0z, p1x;
+    float p1y, p1z, p2x, p2y;
+    float p2z;
+    int32_t prim;  // primitive number (global triangle order of the scene)
+    int32_t pad0, pad1;
+};
+
+struct alignas(16) BvhNode {
+    float lmin[3], lmax[3];
+    float rmin[3], rmax[3];
+    int32_t left, right;  // unified index: < n-1 interior, >= n-1 leaf (k = idx - (n-1))
+    int32_t pad0, pad1;
+};
+
+struct alignas(16) ShadeTri {
+    float n[9];    // vertex normals (valid when hasN)
+    float uv[6];   // vertex uvs (defaults (0,0),(1,0),(1,1) when the mesh has none)
+    int32_t mat;   // material index (= mesh index)
+    int32_t light; // area light index or -1
+    int32_t hasN;
+    int32_t pad;
+};
+
+enum TexType : int32_t { TEX_CONST = 0, TEX_IMAGE = 1, TEX_FBM = 2, TEX_WRINKLED = 3, TEX_MARBLE = 4 };
+enum MatType : int32_t { MAT_MATTE = 0, MAT_GLASS = 1, MAT_MIRROR = 2, MAT_MICROFACET = 3 };
+
+struct TexDesc {
+    int32_t type;
+    int32_t img;
+    int32_t octaves;
+    float c[3];
+    float omega, scale, variation;
+    int32_t pad;
+};
+
+struct MatDesc {
+    int32_t type;
+    int32_t tex0, tex1;
+    float extra;     // roughness (matte, microfacet) or eta (glass)
+    float onA, onB;  // Oren-Nayar A, B (matte with roughness != 0)
+    float alpha;     // Trowbridge-Reitz alpha (microfacet)
+    int32_t pad;
+};
+
+struct MipLevelDesc {
+    int32_t ures, vres, ublocks;
+    int64_t offset;  // first slot of the level in mipData (slots are RGB float triples)
+};
+
+struct MipDesc {
+    int32_t wrap;    // 0 repeat, 1 black, 2 clamp
+    int32_t levels;
+    float lookupLevel;  // (levels - 1) + log2(max(width, 1e-8)) for the fixed width 0.01
+    int32_t pad;
+    MipLevelDesc lv[kMaxMipLevels];
+};
+
+struct LightDesc {
+    int32_t kind;  // 0 area, 1 infinite
+    int32_t prim;  // area: primitive number
+    int32_t leaf;  // area: leaf index of that primitive
+    float area;
+    float le[3];
+    float center[3];
+    float radius;
+    int32_t pad;
+};
+
+struct CameraDesc {
+    float o[3], right[3], up[3], llc[3], u[3], v[3];
+    float lensRadius;
+    int32_t width, height;
+};
+
+// Everything the integrator kernels read, by device pointer.
+struct SceneView {
+    const TriLeaf* tris;
+    const BvhNode* nodes;
+    const ShadeTri* shade;
+    const MatDesc* mats;
+    const TexDesc* texs;
+    const MipDesc* mips;
+    const float* mipData;
+    const LightDesc* lights;
+    const float* lfunc;
+    const float* lcdf;
+    float funcInt;
+    int32_t nLights;
+    int32_t nPrim;
+    int32_t infiniteLight;
+    CameraDesc cam;
+    float filter[256];
+};
+
+}  // namespace lts

@@ -1,0 +1,793 @@
+// Path-tracer device routines shared by the megakernel and the wavefront kernels.
+// Each routine restates a reference routine (tierChampion/LightTransportSimulation/src) with
+// the same fp32 evaluation order; see lts_device.h for the arithmetic conventions.
+#pragma once
+#include "lts_device.h"
+#include "lts_scene.h"
+
+namespace lts {
+
+__constant__ int c_perm[512];  // Perlin permutation table (materials/textures/Procedural.cuh:79-107), twice
+
+struct Ray {
+    V3 o, d;
+    float tMax;
+};
+
+struct Hit {
+    int leaf;   // leaf index of the closest triangle, -1 for a miss
+    float t, u, v, invDet;
+};
+
+struct SurfacePoint {
+    V3 p, pError, wo, n;
+    V2 uv;
+    V3 dpdu;
+    V3 sn, sdpdu;
+    V3 dpdx, dpdy;
+    int prim;
+};
+
+LTS_D void load_tri(const TriLeaf* __restrict__ tris, int k, V3& p0, V3& p1, V3& p2, int& prim) {
+    const float4* q = reinterpret_cast<const float4*>(tris + k);
+    float4 a = q[0], b = q[1], c = q[2];
+    p0 = v3(a.x, a.y, a.z);
+    p1 = v3(a.w, b.x, b.y);
+    p2 = v3(b.z, b.w, c.x);
+    prim = __float_as_int(c.y);
+}
+
+// Triangle::intersect hit test (shapes/Triangle.cu:94-128)
+LTS_D bool tri_hit(V3 p0, V3 p1, V3 p2, const Ray& r, float& t, float& u, float& v, float& invDetOut) {
+    V3 edge1 = p1 - p0, edge2 = p2 - p0;
+    V3 pVec = cross(r.d, edge2);
+    float det = dot(edge1, pVec);
+    if (det == 0) return false;
+    float invDet = (float)(1.0 / (double)det);
+    V3 tVec = r.o - p0;
+    u = invDet * dot(tVec, pVec);
+    if (u < SHADOW_EPS || (double)u > 1.0 - (double)SHADOW_EPS) return false;
+    V3 qVec = cross(tVec, edge1);
+    v = invDet * dot(r.d, qVec);
+    if (v < SHADOW_EPS || (double)(u + v) > 1.0 - (double)SHADOW_EPS) return false;
+    t = invDet * dot(edge2, qVec);
+    invDetOut = invDet;
+    return t > 0 && t < r.tMax;
+}
+
+// Triangle::simpleIntersect (shapes/Triangle.cu:50-84)
+LTS_D bool tri_occludes(V3 p0, V3 p1, V3 p2, const Ray& r) {
+    V3 edge1 = p1 - p0, edge2 = p2 - p0;
+    V3 pVec = cross(r.d, edge1);
+    float det = dot(edge2, pVec);
+    if (det == 0) return false;
+    float invDet = (float)(1.0 / (double)det);
+    V3 tVec = r.o - p0;
+    float u = invDet * dot(tVec, pVec);
+    if (u < SHADOW_EPS || (double)u > 1.0 + (double)SHADOW_EPS) return false;
+    V3 qVec = cross(tVec, edge2);
+    float v = invDet * dot(r.d, qVec);
+    if (v < SHADOW_EPS || (double)(u + v) > 1.0 + (double)SHADOW_EPS) return false;
+    float t = invDet * dot(edge1, qVec);
+    return t > 0 && t < r.tMax;
+}
+
+// Bounds3::simpleIntersect (geometry/Bound.cuh:200-227); also returns the entry distance.
+LTS_D bool box_hit(const float* mn, const float* mx, const Ray& r, V3 inv, int nx, int ny, int nz, float& tEnter) {
+    const float g3 = 1 + 2 * gammaN(3);
+    float tMin = ((nx ? mx[0] : mn[0]) - r.o.x) * inv.x;
+    float tMax = ((nx ? mn[0] : mx[0]) - r.o.x) * inv.x;
+    float tyMin = ((ny ? mx[1] : mn[1]) - r.o.y) * inv.y;
+    float tyMax = ((ny ? mn[1] : mx[1]) - r.o.y) * inv.y;
+    tMax *= g3;
+    tyMax *= g3;
+    if (tMin > tyMax || tyMin > tMax) return false;
+    if (tyMin > tMin) tMin = tyMin;
+    if (tyMax < tMax) tMax = tyMax;
+    float tzMin = ((nz ? mx[2] : mn[2]) - r.o.z) * inv.z;
+    float tzMax = ((nz ? mn[2] : mx[2]) - r.o.z) * inv.z;
+    tzMax *= g3;
+    if (tMin > tzMax || tzMin > tMax) return false;
+    if (tzMin > tMin) tMin = tzMin;
+    if (tzMax < tMax) tMax = tzMax;
+    tEnter = tMin;
+    return (tMin < r.tMax) && (tMax > 0);
+}
+
+constexpr int kStack = 64;
+
+// Closest hit (BVH::intersect, acceleration/BVH.cuh:365-398) over the same Karras tree, visiting
+// the nearer child first. The reported hit is order independent (the box test is conservative and
+// prunes with the shrinking tMax), except for exactly equal distances.
+LTS_D Hit trace_closest(const SceneView& sv, Ray& r, unsigned long long* nodeCount, unsigned long long* triCount) {
+    Hit h; h.leaf = -1; h.t = h.u = h.v = h.invDet = 0;
+    const int n = sv.nPrim;
+    V3 inv = v3((float)1 / r.d.x, (float)1 / r.d.y, (float)1 / r.d.z);
+    int nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+    unsigned long long nodes = 0, tests = 0;
+    if (n == 1) {
+        V3 p0, p1, p2; int prim;
+        load_tri(sv.tris, 0, p0, p1, p2, prim);
+        float t, u, v, id;
+        tests++;
+        if (tri_hit(p0, p1, p2, r, t, u, v, id)) { r.tMax = t; h.leaf = 0; h.t = t; h.u = u; h.v = v; h.invDet = id; }
+    } else {
+        int stack[kStack];
+        int sp = 0;
+        // child boxes are tested directly: a ray that passes a child box passes its parent's box,
+        // so the root test of the reference never removes a candidate
+        int cur = 0;
+        const unsigned total = 2u * (unsigned)n - 1u;
+        // every node is entered at most once in a tree; the step bound and the index checks keep a
+        // malformed node array from running away
+        for (unsigned steps = 0; steps < total + 64u; steps++) {
+            if ((unsigned)cur >= total) break;
+            if (cur < n - 1) {
+                const float4* q = reinterpret_cast<const float4*>(sv.nodes + cur);
+                float4 a = q[0], b = q[1], c = q[2], d = q[3];
+                float lmn[3] = {a.x, a.y, a.z}, lmx[3] = {a.w, b.x, b.y};
+                float rmn[3] = {b.z, b.w, c.x}, rmx[3] = {c.y, c.z, c.w};
+                int lc = __float_as_int(d.x), rc = __float_as_int(d.y);
+                float tl, tr;
+                nodes += 2;
+                bool hl = box_hit(lmn, lmx, r, inv, nx, ny, nz, tl);
+                bool hr = box_hit(rmn, rmx, r, inv, nx, ny, nz, tr);
+                if (hl && hr && sp < kStack) {
+                    int nearC = tl <= tr ? lc : rc, farC = tl <= tr ? rc : lc;
+                    stack[sp++] = farC;
+                    cur = nearC;
+                    continue;
+                } else if (hl) { cur = lc; continue; }
+                else if (hr) { cur = rc; continue; }
+            } else {
+                int k = cur - (n - 1);
+                V3 p0, p1, p2; int prim;
+                load_tri(sv.tris, k, p0, p1, p2, prim);
+                float t, u, v, id;
+                tests++;
+                if (tri_hit(p0, p1, p2, r, t, u, v, id)) { r.tMax = t; h.leaf = k; h.t = t; h.u = u; h.v = v; h.invDet = id; }
+            }
+            if (sp == 0) break;
+            cur = stack[--sp];
+        }
+    }
+    if (nodeCount) { atomicAdd(nodeCount, nodes); atomicAdd(triCount, tests); }
+    return h;
+}
+
+// Any hit (BVH::simpleIntersect, acceleration/BVH.cuh:331-363).
+LTS_D bool trace_any(const SceneView& sv, const Ray& r) {
+    const int n = sv.nPrim;
+    V3 inv = v3((float)1 / r.d.x, (float)1 / r.d.y, (float)1 / r.d.z);
+    int nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+    if (n == 1) {
+        V3 p0, p1, p2; int prim;
+        load_tri(sv.tris, 0, p0, p1, p2, prim);
+        return tri_occludes(p0, p1, p2, r);
+    }
+    int stack[kStack];
+    int sp = 0;
+    int cur = 0;
+    const unsigned total = 2u * (unsigned)n - 1u;
+    for (unsigned steps = 0; steps < total + 64u; steps++) {
+        if ((unsigned)cur >= total) return false;
+        if (cur < n - 1) {
+            const float4* q = reinterpret_cast<const float4*>(sv.nodes + cur);
+            float4 a = q[0], b = q[1], c = q[2], d = q[3];
+            float lmn[3] = {a.x, a.y, a.z}, lmx[3] = {a.w, b.x, b.y};
+            float rmn[3] = {b.z, b.w, c.x}, rmx[3] = {c.y, c.z, c.w};
+            int lc = __float_as_int(d.x), rc = __float_as_int(d.y);
+            float tl, tr;
+            bool hl = box_hit(lmn, lmx, r, inv, nx, ny, nz, tl);
+            bool hr = box_hit(rmn, rmx, r, inv, nx, ny, nz, tr);
+            if (hl && hr && sp < kStack) { stack[sp++] = rc; cur = lc; continue; }
+            else if (hl) { cur = lc; continue; }
+            else if (hr) { cur = rc; continue; }
+        } else {
+            V3 p0, p1, p2; int prim;
+            load_tri(sv.tris, cur - (n - 1), p0, p1, p2, prim);
+            if (tri_occludes(p0, p1, p2, r)) return true;
+        }
+        if (sp == 0) return false;
+        cur = stack[--sp];
+    }
+    return false;
+}
+
+// Triangle::intersect interaction part + SurfaceInteraction constructor / setShaddingGeometry
+// (shapes/Triangle.cu:130-200, geometry/Interaction.cuh:85-116)
+LTS_D SurfacePoint make_surface(const SceneView& sv, int leaf, const Ray& r, float u, float v, float invDet) {
+    V3 p0, p1, p2; int prim;
+    load_tri(sv.tris, leaf, p0, p1, p2, prim);
+    const ShadeTri& st = sv.shade[prim];
+    V3 edge1 = p1 - p0, edge2 = p2 - p0;
+    float w = 1 - u - v;
+    V2 uv0 = v2(st.uv[0], st.uv[1]), uv1 = v2(st.uv[2], st.uv[3]), uv2 = v2(st.uv[4], st.uv[5]);
+    V2 duv02 = uv0 - uv2, duv12 = uv1 - uv2;
+    V3 dp02 = p0 - p2, dp12 = p1 - p2;
+    float determinant = duv02.x * duv12.y - duv02.y * duv12.x;
+    V3 dpdu, dpdv;
+    if (determinant == 0) coordinateSystem(normalize(cross(edge2, edge1)), &dpdu, &dpdv);
+    else {
+        dpdu = (dp02 * duv12.y - dp12 * duv02.y) * invDet;
+        dpdv = (dp02 * -duv12.x + dp12 * duv02.x) * invDet;
+    }
+    float xs = fabsf(w * p0.x) + fabsf(u * p1.x) + fabsf(v * p2.x);
+    float ys = fabsf(w * p0.y) + fabsf(u * p1.y) + fabsf(v * p2.y);
+    float zs = fabsf(w * p0.z) + fabsf(u * p1.z) + fabsf(v * p2.z);
+    SurfacePoint s;
+    s.pError = v3(xs, ys, zs) * gammaN(7);
+    s.p = p0 * w + p1 * u + p2 * v;
+    s.uv = uv0 * w + uv1 * u + uv2 * v;
+    s.wo = -r.d;
+    s.dpdu = dpdu;
+    s.n = normalize(cross(dpdu, dpdv));
+    s.sn = s.n;
+    s.sdpdu = dpdu;
+    s.prim = prim;
+    s.dpdx = s.dpdy = v3(0, 0, 0);
+    if (st.hasN) {
+        V3 n0 = v3(st.n[0], st.n[1], st.n[2]), n1 = v3(st.n[3], st.n[4], st.n[5]), n2 = v3(st.n[6], st.n[7], st.n[8]);
+        V3 ns = normalize(n0 * w + n1 * u + n2 * v);
+        V3 tg = s.dpdu;
+        V3 bt = cross(ns, tg);
+        if (lengthSq(bt) > 0.0f) { bt = normalize(bt); tg = cross(bt, ns); }
+        else coordinateSystem(ns, &tg, &bt);
+        s.sn = normalize(cross(tg, bt));
+        s.n = faceTowards(s.n, s.sn);
+        s.sdpdu = tg;
+        s.n = faceTowards(s.n, s.sn);
+    }
+    return s;
+}
+
+LTS_D Ray spawn_ray(const SurfacePoint& it, V3 d) {
+    Ray r;
+    r.o = offsetRayOrigin(it.p, it.pError, it.n, d);
+    r.d = d;
+    r.tMax = INFINITY;
+    return r;
+}
+
+// ---------------------------------------------------------------- sampling (sampling/Sampling.cuh)
+LTS_D V2 uniform_disk(V2 u) {
+    float r = sqrtf(u.x);
+    float theta = 2 * PI_F * u.y;
+    return v2(r * f_cos(theta), r * f_sin(theta));
+}
+LTS_D V3 cosine_hemisphere(V2 u) {
+    V2 d = uniform_disk(u);
+    float z = sqrtf(fmaxf(0.0f, 1 - d.x * d.x - d.y * d.y));
+    return v3(d.x, d.y, z);
+}
+LTS_D V3 uniform_hemisphere(V2 u) {
+    float z = u.x;
+    float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float phi = 2 * PI_F * u.y;
+    return v3(r * f_cos(phi), r * f_sin(phi), z);
+}
+LTS_D float power_heuristic(float fPdf, float gPdf) {
+    float f = 1 * fPdf, g = 1 * gPdf;
+    return (f * f) / (f * f + g * g);
+}
+
+// ---------------------------------------------------------------- BxDFs (materials/models/*.cuh)
+enum { B_REFL = 1, B_TRANS = 2, B_DIFF = 4, B_GLOSSY = 8, B_SPEC = 16, B_ALL = 31 };
+
+LTS_D float cosT(V3 w) { return w.z; }
+LTS_D float absCosT(V3 w) { return fabsf(w.z); }
+LTS_D float cos2T(V3 w) { return w.z * w.z; }
+LTS_D float sin2T(V3 w) { return fmaxf(1 - cos2T(w), 0.0f); }
+LTS_D float sinT(V3 w) { return sqrtf(sin2T(w)); }
+LTS_D float tanT(V3 w) { return sinT(w) / cosT(w); }
+LTS_D float tan2T(V3 w) { return sin2T(w) / cos2T(w); }
+LTS_D float cosPhi(V3 w) { float s = sinT(w); return clampf(w.x / s, -1, 1); }  // BxDF.cuh:144-147
+LTS_D float sinPhi(V3 w) { float s = sinT(w); return clampf(w.y / s, -1, 1); }
+LTS_D float cos2Phi(V3 w) { return cosPhi(w) * cosPhi(w); }
+LTS_D float sin2Phi(V3 w) { return sinPhi(w) * sinPhi(w); }
+LTS_D bool sameHemi(V3 a, V3 b) { return a.z * b.z > 0; }
+
+// fresnelReflectionDielectric (BxDF.cuh:16-37), including its clamp(cos, 1, -1)
+LTS_D float fr_dielectric(float cosThetaI, float etaI, float etaT) {
+    cosThetaI = clampf(cosThetaI, 1, -1);
+    bool entering = cosThetaI > 0.0f;
+    if (!entering) { float tmp = etaI; etaI = etaT; etaT = tmp; cosThetaI = fabsf(cosThetaI); }
+    float sinThetaI = sqrtf(fmaxf(0.0f, 1 - cosThetaI * cosThetaI));
+    float sinThetaT = etaI / etaT * sinThetaI;
+    if (sinThetaT >= 1) return 1;
+    float cosThetaT = sqrtf(fmaxf(0.0f, 1 - sinThetaT * sinThetaT));
+    float rpar = ((etaT * cosThetaI) - (etaI * cosThetaT)) / ((etaT * cosThetaI) + (etaI * cosThetaT));
+    float rperp = ((etaI * cosThetaI) - (etaT * cosThetaT)) / ((etaI * cosThetaI) + (etaT * cosThetaT));
+    return (rpar * rpar + rperp * rperp) / 2;
+}
+LTS_D bool refract_dir(V3 wi, V3 n, float eta, V3* wt) {
+    float cosThetaI = dot(n, wi);
+    float sin2ThetaI = fmaxf(0.0f, 1.0f - cosThetaI * cosThetaI);
+    float sin2ThetaT = eta * eta * sin2ThetaI;
+    if (sin2ThetaT >= 1) return false;
+    float cosThetaT = sqrtf(1 - sin2ThetaT);
+    *wt = -wi * eta + n * (eta * cosThetaI - cosThetaT);
+    return true;
+}
+
+// Trowbridge-Reitz distribution (Microfacet.cuh:124-181)
+LTS_D float tr_lambda(V3 w, float ax, float ay) {
+    float absTan = fabsf(tanT(w));
+    if (isinf(absTan)) return 0.0f;
+    float alpha = sqrtf(cos2Phi(w) * ax * ax + sin2Phi(w) * ay * ay);
+    float a2t2 = (alpha * absTan) * (alpha * absTan);
+    return (-1 + sqrtf(1.0f + a2t2)) / 2;
+}
+LTS_D float tr_D(V3 wh, float ax, float ay) {
+    float tan2 = tan2T(wh);
+    if (isinf(tan2)) return 0;
+    const float cos4 = cos2T(wh) * cos2T(wh);
+    float e = (cos2Phi(wh) / (ax * ax) + sin2Phi(wh) / (ay * ay)) * tan2;
+    return 1 / (PI_F * ax * ay * cos4 * (1 + e) * (1 + e));
+}
+LTS_D float tr_G1(V3 w, float ax, float ay) { return 1 / (1 + tr_lambda(w, ax, ay)); }
+LTS_D float tr_G(V3 wo, V3 wi, float ax, float ay) { return 1 / (1 + tr_lambda(wo, ax, ay) + tr_lambda(wi, ax, ay)); }
+LTS_D float tr_pdf(V3 wo, V3 wh, float ax, float ay) { return tr_D(wh, ax, ay) * tr_G1(wo, ax, ay) * absDot(wo, wh) / absCosT(wo); }
+
+// TrowbridgeReitzSample11 / TrowbridgeReitzSample (Microfacet.cuh:53-121)
+LTS_D void tr_sample11(float cosTheta, float U1, float U2, float* sx, float* sy) {
+    if ((double)cosTheta > .9999) {
+        float r = sqrtf(U1 / (1 - U1));
+        float phi = (float)(6.28318530718 * (double)U2);
+        *sx = r * f_cos(phi);
+        *sy = r * f_sin(phi);
+        return;
+    }
+    float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+    float tanTheta = sinTheta / cosTheta;
+    float a = 1 / tanTheta;
+    float G1 = 2 / (1 + sqrtf(1.f + 1.f / (a * a)));
+    float A = 2 * U1 / G1 - 1;
+    float tmp = 1.f / (A * A - 1.f);
+    if ((double)tmp > 1e10) tmp = (float)1e10;
+    float B = tanTheta;
+    float D = sqrtf(fmaxf(float(B * B * tmp * tmp - (A * A - B * B) * tmp), 0.0f));
+    float sx1 = B * tmp - D, sx2 = B * tmp + D;
+    *sx = (A < 0 || sx2 > 1.f / tanTheta) ? sx1 : sx2;
+    float S;
+    if (U2 > 0.5f) { S = 1.f; U2 = 2.f * (U2 - .5f); }
+    else { S = -1.f; U2 = 2.f * (.5f - U2); }
+    float z = (U2 * (U2 * (U2 * 0.27385f - 0.73369f) + 0.46341f)) /
+              (U2 * (U2 * (U2 * 0.093073f + 0.309420f) - 1.000000f) + 0.597999f);
+    *sy = S * z * sqrtf(1.f + *sx * *sx);
+}
+LTS_D V3 tr_sample(V3 wi, float ax, float ay, float U1, float U2) {
+    V3 ws = normalize(v3(ax * wi.x, ay * wi.y, wi.z));
+    float sx, sy;
+    tr_sample11(cosT(ws), U1, U2, &sx, &sy);
+    float tmp = cosPhi(ws) * sx - sinPhi(ws) * sy;
+    sy = sinPhi(ws) * sx + cosPhi(ws) * sy;
+    sx = tmp;
+    sx = ax * sx;
+    sy = ay * sy;
+    return normalize(v3(-sx, -sy, (float)1.));
+}
+
+// The BSDF of one hit: at most one BxDF (Material.cuh:31-130 and the microfacet mirror extension).
+struct Bsdf {
+    V3 ns, ng, ss, ts;
+    int kind;   // -1 none, 0 Lambert, 1 OrenNayar, 2 SpecularReflection, 3 FresnelSpecular, 4 MicrofacetReflection
+    int type;
+    Spec R, T;
+    float A, B, eta, alpha;
+
+    LTS_D V3 toLocal(V3 v) const { return v3(dot(v, ss), dot(v, ts), dot(v, ns)); }
+    LTS_D V3 toWorld(V3 v) const {
+        return v3(ss.x * v.x + ts.x * v.y + ns.x * v.z, ss.y * v.x + ts.y * v.y + ns.y * v.z, ss.z * v.x + ts.z * v.y + ns.z * v.z);
+    }
+    LTS_D bool matches(int flags) const { return kind >= 0 && (type & flags) == type; }
+
+    LTS_D Spec fLocal(V3 wo, V3 wi) const {
+        if (kind == 0) return R * INV_PI_F;
+        if (kind == 1) {
+            float sinThetaO = sinT(wo), sinThetaI = sinT(wi);
+            float maxCos = 0.0f;
+            if ((double)sinThetaI > 1e-4 && (double)sinThetaO > 1e-4) {
+                float sinPhiO = sinPhi(wo), cosPhiO = cosPhi(wo);
+                float sinPhiI = sinPhi(wi), cosPhiI = cosPhi(wi);
+                float dCos = cosPhiI * cosPhiO + sinPhiI * sinPhiO;
+                maxCos = fmaxf(0.0f, dCos);
+            }
+            float sinAlpha, tanBeta;
+            if (absCosT(wi) > absCosT(wo)) { sinAlpha = sinThetaO; tanBeta = sinThetaI / absCosT(wi); }
+            else { sinAlpha = sinThetaI; tanBeta = sinThetaO / absCosT(wo); }
+            return R * INV_PI_F * (A + B * maxCos * sinAlpha * tanBeta);
+        }
+        if (kind == 4) {
+            float cosThetaO = absCosT(wo), cosThetaI = absCosT(wi);
+            V3 wh = wi + wo;
+            if (cosThetaI == 0 || cosThetaO == 0) return spec(0.0f);
+            if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return spec(0.0f);
+            wh = normalize(wh);
+            Spec F = spec(1.0f);
+            return R * tr_D(wh, alpha, alpha) * tr_G(wo, wi, alpha, alpha) * F / (4 * cosThetaI * cosThetaO);
+        }
+        return spec(0.0f);
+    }
+    LTS_D float pdfLocal(V3 wo, V3 wi) const {
+        if (kind == 0 || kind == 1) return sameHemi(wo, wi) ? absCosT(wi) * INV_PI_F : 0;
+        if (kind == 4) {
+            if (!sameHemi(wo, wi)) return 0;
+            V3 wh = normalize(wo + wi);
+            return tr_pdf(wo, wh, alpha, alpha) / (4 * dot(wo, wh));
+        }
+        return 0.0f;
+    }
+    // BSDF::f (BSDF.cuh:60-72)
+    LTS_D Spec f(V3 woW, V3 wiW, int flags) const {
+        V3 wi = toLocal(wiW), wo = toLocal(woW);
+        bool reflect = dot(wiW, ng) * dot(woW, ng) > 0;
+        Spec r = spec(0.0f);
+        if (matches(flags) && ((reflect && (type & B_REFL)) || (!reflect && (type & B_TRANS)))) r = r + fLocal(wo, wi);
+        return r;
+    }
+    // BSDF::Pdf (BSDF.cuh:129-144)
+    LTS_D float pdf(V3 woW, V3 wiW, int flags) const {
+        if (kind < 0) return 0.0f;
+        V3 wo = toLocal(woW), wi = toLocal(wiW);
+        if (wo.z == 0) return 0.0f;
+        float p = 0.0f;
+        int m = 0;
+        if (matches(flags)) { m++; p = p + pdfLocal(wo, wi); }
+        return m > 0 ? p / m : 0.0f;
+    }
+    // BSDF::sampleF (BSDF.cuh:74-127) with one component
+    LTS_D Spec sampleF(V3 woW, V3* wiW, V2 u, float* pdfOut, int flags, int* sampledType) const {
+        if (!matches(flags)) { *pdfOut = 0; return spec(0.0f); }
+        V3 wo = toLocal(woW), wi;
+        *pdfOut = 0;
+        *sampledType = type;
+        Spec f = spec(0.0f);
+        switch (kind) {
+            case 0: case 1:
+                wi = cosine_hemisphere(u);
+                if (wo.z < 0) wi.z *= -1;
+                *pdfOut = pdfLocal(wo, wi);
+                f = fLocal(wo, wi);
+                break;
+            case 2:
+                wi = v3(-wo.x, -wo.y, wo.z);
+                *pdfOut = 1.0f;
+                f = (spec(1.0f) * R) / absCosT(wi);
+                break;
+            case 3: {
+                float F = fr_dielectric(cosT(wo), 1.0f, eta);
+                if (u.x < F) {
+                    wi = v3(-wo.x, -wo.y, -wo.z);
+                    *sampledType = B_SPEC | B_REFL;
+                    *pdfOut = F;
+                    f = F * R / absCosT(wi);
+                } else {
+                    bool entering = cosT(wo) > 0;
+                    float etaI = entering ? 1.0f : eta, etaT = entering ? eta : 1.0f;
+                    if (!refract_dir(wo, faceTowards(v3(0, 0, 1), wo), etaI / etaT, &wi)) return spec(0.0f);
+                    Spec ft = T * (1 - F);
+                    ft = ft * ((etaI * etaI) / (etaT * etaT));
+                    *sampledType = B_SPEC | B_TRANS;
+                    *pdfOut = 1.0f - F;
+                    f = ft / absCosT(wi);
+                }
+                break;
+            }
+            case 4: {
+                if (wo.z == 0) return spec(0.0f);
+                bool flip = wo.z < 0;
+                V3 wh = tr_sample(flip ? -wo : wo, alpha, alpha, u.x, u.y);
+                if (flip) wh = -wh;
+                if (dot(wo, wh) < 0) return spec(0.0f);
+                wi = -wo + wh * (2 * dot(wo, wh));
+                if (!sameHemi(wo, wi)) return spec(0.0f);
+                *pdfOut = tr_pdf(wo, wh, alpha, alpha) / (4 * dot(wo, wh));
+                f = fLocal(wo, wi);
+                break;
+            }
+        }
+        if (*pdfOut == 0) return spec(0.0f);
+        *wiW = toWorld(wi);
+        return f;
+    }
+};
+
+// ---------------------------------------------------------------- textures (Procedural.cuh)
+LTS_D float noise_weight(float t) { float t3 = t * t * t; float t4 = t3 * t; return 6 * t4 * t - 15 * t4 + 10 * t3; }
+LTS_D float smooth_step(float mn, float mx, float val) {
+    float v = clampf((val - mn) / (mx - mn), 0, 1);
+    return v * v * (-2 * v + 3);
+}
+LTS_D float grad(int x, int y, int z, float dx, float dy, float dz) {
+    int h = c_perm[c_perm[c_perm[x] + y] + z];
+    h &= 15;
+    float u = h < 8 || h == 12 || h == 13 ? dx : dy;
+    float v = h < 4 || h == 12 || h == 13 ? dy : dz;
+    return ((h & 1) ? -u : u) + ((h & 2) ? -v : v);
+}
+LTS_D float perlin(V3 p) {
+    int ix = (int)floorf(p.x), iy = (int)floorf(p.y), iz = (int)floorf(p.z);
+    float dx = p.x - ix, dy = p.y - iy, dz = p.z - iz;
+    ix &= 255; iy &= 255; iz &= 255;
+    float w000 = grad(ix, iy, iz, dx, dy, dz);
+    float w100 = grad(ix + 1, iy, iz, dx - 1, dy, dz);
+    float w010 = grad(ix, iy + 1, iz, dx, dy - 1, dz);
+    float w001 = grad(ix, iy, iz + 1, dx, dy, dz - 1);
+    float w110 = grad(ix + 1, iy + 1, iz, dx - 1, dy - 1, dz);
+    float w011 = grad(ix, iy + 1, iz + 1, dx, dy - 1, dz - 1);
+    float w101 = grad(ix + 1, iy, iz + 1, dx - 1, dy, dz - 1);
+    float w111 = grad(ix + 1, iy + 1, iz + 1, dx - 1, dy - 1, dz - 1);
+    float wx = noise_weight(dx), wy = noise_weight(dy), wz = noise_weight(dz);
+    float x00 = lerpf(wx, w000, w100), x10 = lerpf(wx, w010, w110);
+    float x01 = lerpf(wx, w001, w101), x11 = lerpf(wx, w011, w111);
+    float y0 = lerpf(wy, x00, x10), y1 = lerpf(wy, x01, x11);
+    return lerpf(wz, y0, y1);
+}
+LTS_D float fbm(V3 p, V3 dpdx, V3 dpdy, float omega, int maxOct) {
+    float len2 = fmaxf(lengthSq(dpdx), lengthSq(dpdy));
+    float n = clampf(-1 - 0.5f * f_log2(len2), 0, (float)maxOct);
+    int nInt = (int)floorf(n);
+    float sum = 0, lambda = 1, o = 1;
+    for (int i = 0; i < nInt; i++) { sum += o * perlin(p * lambda); lambda *= 1.99f; o *= omega; }
+    float nPartial = n - nInt;
+    sum += o * smooth_step(0.3f, 0.7f, nPartial) * perlin(p * lambda);
+    return sum;
+}
+LTS_D float turbulence(V3 p, V3 dpdx, V3 dpdy, float omega, int maxOct) {
+    float len2 = fmaxf(lengthSq(dpdx), lengthSq(dpdy));
+    float n = clampf(-1 - 0.5f * f_log2(len2), 0, (float)maxOct);
+    int nInt = (int)floorf(n);
+    float sum = 0, lambda = 1, o = 1;
+    for (int i = 0; i < nInt; i++) { sum += o * fabsf(perlin(p * lambda)); lambda *= 1.99f; o *= omega; }
+    float nPartial = n - nInt;
+    sum += o * lerpf(smooth_step(0.3f, 0.7f, nPartial), 0.2f, fabsf(perlin(p * lambda)));
+    for (int i = nInt; i < maxOct; i++) { sum += o * 0.2f; o *= omega; }
+    return sum;
+}
+LTS_D Spec marble_tex(const TexDesc& X, V3 p, V3 dpdx, V3 dpdy) {
+    // MarbleTexture colours (Procedural.cuh:278-282); the zero row stands in for the read past the
+    // table at t == 1, where its weight is exactly 0
+    const float C[10][3] = {{.58f, .58f, .6f}, {.58f, .58f, .6f}, {.58f, .58f, .6f}, {.5f, .5f, .5f},
+                            {.6f, .59f, .58f}, {.58f, .58f, .6f}, {.58f, .58f, .6f}, {.2f, .2f, .33f},
+                            {.58f, .58f, .6f}, {0, 0, 0}};
+    p = p * X.scale;
+    float m = p.y + X.variation * fbm(p, dpdx * X.scale, dpdy * X.scale, X.omega, X.octaves);
+    float t = 0.5f + 0.5f * f_sin(m);
+    int first = (int)floorf(t * 6.0f);
+    first = first < 0 ? 0 : (first > 6 ? 6 : first);  // t is in [0, 1]; the clamp only fences NaN
+    t = (t * 6.0f - first);
+    Spec c0 = spec(C[first][0], C[first][1], C[first][2]);
+    Spec c1 = spec(C[first + 1][0], C[first + 1][1], C[first + 1][2]);
+    Spec c2 = spec(C[first + 2][0], C[first + 2][1], C[first + 2][2]);
+    Spec c3 = spec(C[first + 3][0], C[first + 3][1], C[first + 3][2]);
+    Spec s0 = (1.0f - t) * c0 + t * c1;
+    Spec s1 = (1.0f - t) * c1 + t * c2;
+    Spec s2 = (1.0f - t) * c2 + t * c3;
+    s0 = (1.0f - t) * s0 + t * s1;
+    s1 = (1.0f - t) * s1 + t * s2;
+    return 1.5f * ((1.0f - t) * s0 + t * s1);
+}
+
+// Mipmap texel / triangle / lookUp (materials/textures/Mipmap.cuh:14-72)
+LTS_D Spec mip_texel(const SceneView& sv, const MipDesc& m, int s, int t, int l) {
+    const MipLevelDesc& L = m.lv[l];
+    if (m.wrap == 0) { s = (int)fmodf((float)s, (float)L.ures); t = (int)fmodf((float)t, (float)L.vres); }
+    else if (m.wrap == 2) { s = (int)clampf((float)s, 0, (float)(L.ures - 1)); t = (int)clampf((float)t, 0, (float)(L.vres - 1)); }
+    else if (s < 0 || s >= L.ures || t < 0 || t >= L.vres) return spec(0.0f);
+    if (s < 0 || t < 0 || s >= L.ures || t >= L.vres) return spec(0.0f);
+    long long slot = L.offset + 16LL * ((long long)L.ublocks * (t >> 2) + (s >> 2)) + 4LL * (t >> 2) + (s & 3);
+    const float* px = sv.mipData + 3 * slot;
+    return spec(px[0], px[1], px[2]);
+}
+LTS_D Spec mip_triangle(const SceneView& sv, const MipDesc& m, int l, V2 st) {
+    l = (int)clampf((float)l, 0, (float)(m.levels - 1));
+    float s = st.x * m.lv[l].ures - 0.5f;
+    float t = st.y * m.lv[l].vres - 0.5f;
+    int s0 = (int)floorf(s), t0 = (int)floorf(t);
+    float ds = s - s0, dt = t - t0;
+    return (1 - ds) * (1 - dt) * mip_texel(sv, m, s0, t0, l) + (1 - ds) * dt * mip_texel(sv, m, s0, t0 + 1, l) +
+           ds * (1 - dt) * mip_texel(sv, m, s0 + 1, t0, l) + ds * dt * mip_texel(sv, m, s0 + 1, t0 + 1, l);
+}
+LTS_D Spec mip_lookup(const SceneView& sv, const MipDesc& m, V2 st) {
+    float levels = m.lookupLevel;
+    if (levels < 0) return mip_triangle(sv, m, 0, st);
+    if (levels >= m.levels - 1) return mip_texel(sv, m, m.levels - 1, 0, 0);
+    int iLevel = (int)floorf(levels);
+    float delta = levels - iLevel;
+    return lerpS(delta, mip_triangle(sv, m, iLevel, st), mip_triangle(sv, m, iLevel + 1, st));
+}
+
+LTS_D Spec eval_tex(const SceneView& sv, int ti, const SurfacePoint& s) {
+    const TexDesc& X = sv.texs[ti];
+    switch (X.type) {
+        case TEX_CONST: return spec(X.c[0], X.c[1], X.c[2]);
+        case TEX_IMAGE: return mip_lookup(sv, sv.mips[X.img], v2(1 * s.uv.x + 0, 1 * s.uv.y + 0));
+        case TEX_FBM: return spec(fbm(identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy), X.omega, X.octaves));
+        case TEX_WRINKLED: return spec(turbulence(identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy), X.omega, X.octaves));
+        case TEX_MARBLE: return marble_tex(X, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy));
+    }
+    return spec(0.0f);
+}
+
+// Material::computeScatteringFunctions (materials/Material.cuh:31-130)
+LTS_D Bsdf make_bsdf(const SceneView& sv, const SurfacePoint& s) {
+    Bsdf b;
+    b.ns = s.sn; b.ng = s.n;
+    b.ss = normalize(s.sdpdu);
+    b.ts = cross(b.ns, b.ss);
+    b.kind = -1; b.type = 0;
+    b.R = b.T = spec(0.0f);
+    b.A = b.B = b.eta = b.alpha = 0;
+    const MatDesc& M = sv.mats[sv.shade[s.prim].mat];
+    switch (M.type) {
+        case MAT_MATTE: {
+            Spec kd = eval_tex(sv, M.tex0, s);
+            if (!isBlack(kd)) { b.R = kd; b.type = B_REFL | B_DIFF; b.kind = M.extra == 0 ? 0 : 1; b.A = M.onA; b.B = M.onB; }
+            break;
+        }
+        case MAT_GLASS: {
+            Spec r = eval_tex(sv, M.tex0, s), t = eval_tex(sv, M.tex1, s);
+            if (isBlack(r) && isBlack(t)) break;
+            b.kind = 3; b.type = B_REFL | B_TRANS | B_SPEC; b.R = r; b.T = t; b.eta = M.extra;
+            break;
+        }
+        case MAT_MIRROR: {
+            Spec r = eval_tex(sv, M.tex0, s);
+            if (isBlack(r)) break;
+            b.kind = 2; b.type = B_REFL | B_SPEC; b.R = r;
+            break;
+        }
+        case MAT_MICROFACET: {
+            Spec r = eval_tex(sv, M.tex0, s);
+            if (isBlack(r)) break;
+            b.kind = 4; b.type = B_REFL | B_GLOSSY; b.R = r; b.alpha = M.alpha;
+            break;
+        }
+    }
+    return b;
+}
+
+// ---------------------------------------------------------------- lights (lights/Light.cu)
+struct LightPoint { V3 p, n, pError; };
+
+LTS_D LightPoint tri_sample(const SceneView& sv, const LightDesc& L, V2 u) {
+    float t = sqrtf(u.x);
+    V2 b = v2(1 - t, u.y * t);
+    V3 p0, p1, p2; int prim;
+    load_tri(sv.tris, L.leaf, p0, p1, p2, prim);
+    const ShadeTri& st = sv.shade[L.prim];
+    LightPoint lp;
+    lp.p = p0 * b.x + p1 * b.y + p2 * (1 - b.x - b.y);
+    if (st.hasN)
+        lp.n = normalize(v3(st.n[0], st.n[1], st.n[2]) * b.x + v3(st.n[3], st.n[4], st.n[5]) * b.y +
+                         v3(st.n[6], st.n[7], st.n[8]) * (1 - b.x - b.y));
+    else
+        lp.n = normalize(cross(p1 - p0, p2 - p0));
+    V3 pAbs = vabs(p0 * b.x) + vabs(p1 * b.y) + vabs(p2 * (1 - b.x - b.y));
+    lp.pError = pAbs * gammaN(6);
+    return lp;
+}
+
+// Triangle::Pdf(it, wi) (shapes/Triangle.cu:211-225)
+LTS_D float tri_pdf(const SceneView& sv, const LightDesc& L, const SurfacePoint& it, V3 wi) {
+    Ray r = spawn_ray(it, wi);
+    V3 p0, p1, p2; int prim;
+    load_tri(sv.tris, L.leaf, p0, p1, p2, prim);
+    float t, u, v, id;
+    if (!tri_hit(p0, p1, p2, r, t, u, v, id)) return 0;
+    SurfacePoint is = make_surface(sv, L.leaf, r, u, v, id);
+    return lengthSq(it.p - is.p) / (absDot(is.n, -wi) * L.area);
+}
+
+LTS_D float inf_pdf(V3 wi) {
+    float sinTheta = f_sin(f_acos(clampf(wi.z, -1.0f, 1.0f)));
+    return (0.5f * INV_PI_F) / (2 * PI_F * PI_F * sinTheta);
+}
+
+LTS_D Spec light_le(const LightDesc& L) { return spec(L.le[0], L.le[1], L.le[2]); }
+
+LTS_D Spec sample_li(const SceneView& sv, const LightDesc& L, const SurfacePoint& it, V2 u, V3* wi, float* pdf, LightPoint* target) {
+    if (L.kind == 0) {
+        LightPoint ps = tri_sample(sv, L, u);
+        *wi = normalize(ps.p - it.p);
+        *pdf = tri_pdf(sv, L, it, *wi);
+        *target = ps;
+        return dot(ps.n, -*wi) > 0.f ? light_le(L) : spec(0.0f);
+    }
+    V3 w = uniform_hemisphere(u);
+    V3 ref = v3(0, 1, 0), a, b;
+    coordinateSystem(ref, &a, &b);
+    w = a * w.x + b * w.y + ref * w.z;
+    V3 pL = v3(L.center[0], L.center[1], L.center[2]) + w * L.radius;
+    *wi = normalize(pL - it.p);
+    *pdf = inf_pdf(*wi);
+    target->p = pL; target->n = v3(0, 0, 0); target->pError = v3(0, 0, 0);
+    return light_le(L);
+}
+
+LTS_D bool unoccluded(const SceneView& sv, const SurfacePoint& it, const LightPoint& p1) {
+    V3 origin = offsetRayOrigin(it.p, it.pError, it.n, p1.p - it.p);
+    V3 target = offsetRayOrigin(p1.p, p1.pError, p1.n, origin - p1.p);
+    Ray r; r.o = origin; r.d = target - origin; r.tMax = 1 - SHADOW_EPS;
+    return !trace_any(sv, r);
+}
+
+// Distribution1D::sampleDiscrete with the reference's search length (Sampling.cuh:155-188)
+LTS_D int sample_light(const SceneView& sv, float u, float* pdf) {
+    int size = sv.nLights, off;
+    if (size == 1) off = 0;
+    else {
+        int first = 0, len = size;
+        while (len > 0) {
+            int half = len >> 1, middle = first + half;
+            if (sv.lcdf[middle] <= u) { first = middle + 1; len -= half + 1; }
+            else len = half;
+        }
+        off = (int)clampf((float)(first - 1), 0, (float)(size - 2));
+    }
+    *pdf = sv.lfunc[off] / (sv.funcInt * size);
+    return off;
+}
+
+// estimateDirect (intergrators/IntegratorHelper.cuh:51-125)
+LTS_D Spec estimate_direct(const SceneView& sv, const SurfacePoint& si, const Bsdf& bsdf, V2 uScatter, int li, V2 uLight, bool specular) {
+    const LightDesc& L = sv.lights[li];
+    Spec Ld = spec(0.0f);
+    int flags = specular ? B_ALL : (B_ALL & ~B_SPEC);
+    V3 wi;
+    float lightPdf = 0.0f, scatteringPdf = 0.0f;
+    LightPoint vis;
+    Spec Li = sample_li(sv, L, si, uLight, &wi, &lightPdf, &vis);
+    if (lightPdf > 0 && !isBlack(Li)) {
+        Spec f = bsdf.f(si.wo, wi, flags) * absDot(wi, si.sn);
+        scatteringPdf = bsdf.pdf(si.wo, wi, flags);
+        if (!isBlack(f)) {
+            if (!unoccluded(sv, si, vis)) Li = spec(0.0f);
+            if (!isBlack(Li)) {
+                float weight = power_heuristic(lightPdf, scatteringPdf);
+                Ld = Ld + f * Li * weight / lightPdf;
+            }
+        }
+    }
+    int sampledType = 0;
+    Spec f = bsdf.sampleF(si.wo, &wi, uScatter, &scatteringPdf, flags, &sampledType);
+    f = f * absDot(wi, si.sn);
+    bool sampledSpecular = (sampledType & B_SPEC) != 0;
+    if (!isBlack(f) && scatteringPdf > 0) {
+        float weight = 1;
+        if (!sampledSpecular) {
+            lightPdf = L.kind == 0 ? tri_pdf(sv, L, si, wi) : inf_pdf(wi);
+            if (lightPdf == 0) return Ld;
+            weight = power_heuristic(scatteringPdf, lightPdf);
+        }
+        Ray r = spawn_ray(si, wi);
+        Hit h = trace_closest(sv, r, nullptr, nullptr);
+        Spec Li2 = spec(0.f);
+        if (h.leaf >= 0) {
+            SurfacePoint lt = make_surface(sv, h.leaf, r, h.u, h.v, h.invDet);
+            if (sv.shade[lt.prim].light == li) Li2 = dot(lt.n, -wi) > 0.f ? light_le(L) : spec(0.0f);
+        } else if (L.kind == 1) {
+            Li2 = light_le(L);
+        }
+        if (!isBlack(Li2)) Ld = Ld + f * Li2 * spec(1.0f) * weight / scatteringPdf;
+    }
+    return Ld;
+}
+
+// Film sample position of (pixel, sample): dims 0-1, stratified per Sampler::stratifiedSample
+// (sampling/Sampler.cuh:49-64) when requested.
+LTS_D V2 film_sample(int px, int py, int sample, int spp, int stratified, const float d0[4]) {
+    V2 film = v2(d0[0], d0[1]);
+    if (stratified) {
+        int strata = (int)floorf(sqrtf((float)spp));
+        if (sample - 1 < strata * strata) {
+            float inv = (float)1 / strata;
+            film = v2(film.x * inv, film.y * inv);
+            film = film + v2((sample % strata) / (float)strata, (sample / strata) / (float)strata);
+        }
+    }
+    return v2((float)(px + film.x), (float)(py + film.y));
+}
+
+}  // namespace lts

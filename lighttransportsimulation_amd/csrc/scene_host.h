@@ -1,0 +1,36 @@
+// Host-side scene ingest: reads the reference's input files and produces the flat arrays of
+// lts_scene.h, ready to upload.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "lts_scene.h"
+
+namespace lts {
+
+struct HostScene {
+    int width = 0, height = 0;
+    CameraDesc cam{};
+    float filter[256];
+    int nMeshes = 0;
+    std::vector<float> soup;        // 9 floats per primitive, global triangle order
+    std::vector<ShadeTri> shade;    // per primitive
+    std::vector<MatDesc> mats;
+    std::vector<TexDesc> texs;
+    std::vector<MipDesc> mips;
+    std::vector<float> mipData;
+    std::vector<LightDesc> lights;
+    std::vector<float> lfunc, lcdf;
+    float funcInt = 0;
+    int infiniteLight = -1;
+};
+
+// parseScene (core/SceneLoader.cuh:195-360). Returns false with `err` set on failure.
+bool load_scene_files(const std::string& dir, const std::string& scene, const std::string& subject, int width,
+                      int height, HostScene& out, std::string& err);
+
+// Gaussian filter table, radius (1, 1), alpha 1 (rendering/Filter.cuh:32-50, :82-99;
+// LightTransportSimulation.cu:64).
+void gaussian_filter_table(float* out256);
+
+}  // namespace lts
