@@ -29,6 +29,10 @@ METRIC = "Msamples/s & equal-time relMSE: cached vs plain path tracer, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
+    "displaced_1080p": dict(desc="seeded 1M-triangle FBM-displaced sphere on a ground quad, matte/glass/microfacet "
+                                 "sectors, FBM/wrinkled/marble/2048^2 image textures, infinite light 0.5 + 2 area "
+                                 "lights, 1920x1080, 64 spp, 8 bounces, RR from 3", gen="displaced", width=1920,
+                            height=1080, spp=64, bounces=8, rr=3, stratified=0),
     "cornell_256": dict(desc="seeded icosphere (20,480 tris) in a Cornell box, 256x256, 16 spp stratified, "
                              "5 bounces, RR from 3", gen="cornell", width=256, height=256, spp=16, bounces=5, rr=3,
                         stratified=1),
@@ -38,7 +42,10 @@ WORKLOADS = {
 def make_workload(name: str, d: str) -> dict:
     from lighttransportsimulation_amd import scenegen
     w = WORKLOADS[name]
-    if w["gen"] == "cornell":
+    if w["gen"] == "displaced":
+        cfg = scenegen.gen_displaced_sphere(d, width=w["width"], height=w["height"], spp=w["spp"], bounces=w["bounces"],
+                                            rr=w["rr"])
+    elif w["gen"] == "cornell":
         cfg = scenegen.gen_cornell_sphere(d, width=w["width"], height=w["height"], spp=w["spp"], bounces=w["bounces"],
                                           rr=w["rr"])
     else:
@@ -74,7 +81,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cornell_256", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="displaced_1080p", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle work for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()

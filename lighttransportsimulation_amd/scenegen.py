@@ -192,3 +192,144 @@ def triangle_soup(n: int, seed: int = 3) -> np.ndarray:
     base = np.array([[0.0, 0.57735027, 0.0], [-0.5, -0.28867513, 0.0], [0.5, -0.28867513, 0.0]])
     verts = c[:, None, :] + s[:, None, None] * np.einsum("nij,vj->nvi", R, base)
     return _nonneg_zero(verts.reshape(n, 9))
+
+
+# ----------------------------------------------------------------------------- C3 scene
+def _value_noise3(p: np.ndarray, seed: int) -> np.ndarray:
+    """Seeded trilinear value noise in [-1, 1] on the integer lattice (hash table of 256^3 cells)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    table = rng.uniform(-1.0, 1.0, 4096)
+    perm = rng.permutation(4096)
+    i = np.floor(p).astype(np.int64)
+    f = p - i
+    f = f * f * (3 - 2 * f)
+
+    def h(ix, iy, iz):
+        return table[perm[(perm[(perm[ix & 4095] + iy) & 4095] + iz) & 4095]]
+
+    out = 0.0
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                w = (f[:, 0] if dx else 1 - f[:, 0]) * (f[:, 1] if dy else 1 - f[:, 1]) * (f[:, 2] if dz else 1 - f[:, 2])
+                out = out + w * h(i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz)
+    return out
+
+
+def fbm_displacement(p: np.ndarray, octaves=6, omega=0.5, seed=7) -> np.ndarray:
+    s, amp, freq = 0.0, 1.0, 2.0
+    for o in range(octaves):
+        s = s + amp * _value_noise3(p * freq, seed + o)
+        amp *= omega
+        freq *= 2.0
+    return s
+
+
+def value_noise_image(size: int, seed: int) -> np.ndarray:
+    """size x size RGB8 image of seeded multi-octave value noise (smoothstep-interpolated lattices)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    img = np.zeros((size, size, 3))
+    amp = 1.0
+    for o in range(5):
+        g = 4 * 2 ** o
+        grid = rng.random((g + 1, g + 1, 3))
+        x = np.linspace(0, g, size, endpoint=False)
+        i = np.floor(x).astype(int)
+        f = x - i
+        f = f * f * (3 - 2 * f)
+        M = np.zeros((size, g + 1))  # interpolation weights, one row per output coordinate
+        M[np.arange(size), i] = 1 - f
+        M[np.arange(size), i + 1] += f
+        for c in range(3):
+            img[:, :, c] += amp * (M @ grid[:, :, c] @ M.T)
+        amp *= 0.5
+    img /= img.max()
+    return (img * 255 + 0.5).astype(np.uint8)
+
+
+def uv_sphere(n_lat: int, n_lon: int):
+    """Vertices on a (n_lat + 1) x (n_lon + 1) latitude/longitude grid and 2 n_lat n_lon triangles."""
+    th = np.linspace(0.0, np.pi, n_lat + 1)
+    ph = np.linspace(0.0, 2 * np.pi, n_lon + 1)
+    T, Pg = np.meshgrid(th, ph, indexing="ij")
+    dirs = np.stack([np.sin(T) * np.cos(Pg), np.cos(T), np.sin(T) * np.sin(Pg)], -1).reshape(-1, 3)
+    uv = np.stack([Pg / (2 * np.pi), 1 - T / np.pi], -1).reshape(-1, 2)
+    idx = np.arange((n_lat + 1) * (n_lon + 1)).reshape(n_lat + 1, n_lon + 1)
+    a, b = idx[:-1, :-1], idx[:-1, 1:]
+    c, d = idx[1:, :-1], idx[1:, 1:]
+    tris = np.stack([np.stack([a, c, b], -1), np.stack([b, c, d], -1)], 2)  # (n_lat, n_lon, 2, 3)
+    return dirs, uv, tris
+
+
+def vertex_normals(P: np.ndarray, F: np.ndarray) -> np.ndarray:
+    fn = np.cross(P[F[:, 1]] - P[F[:, 0]], P[F[:, 2]] - P[F[:, 0]])
+    N = np.zeros_like(P)
+    for k in range(3):
+        np.add.at(N, F[:, k], fn)
+    ln = np.linalg.norm(N, axis=1, keepdims=True)
+    return N / np.where(ln > 0, ln, 1.0)
+
+
+def gen_displaced_sphere(dir_: str, width=1920, height=1080, spp=64, bounces=8, rr=3, n_lat=500, n_lon=1000,
+                         image_size=2048, submeshes=8, seed=11) -> dict:
+    """Seeded displaced sphere (radius 1 + 0.1 fbm, 6 octaves, omega 0.5; 2 n_lat n_lon triangles,
+    1,000,000 at the defaults) split into longitude sectors with materials drawn from matte /
+    glass (IOR 1.5) / microfacet mirror (roughness U[0.05, 0.5]) and matte textures from FBM /
+    wrinkled / marble (octaves 4-8, omega U[0.3, 0.7]) / a value-noise image; a ground quad with
+    the image texture; infinite light of radiance 0.5 and two quad area lights."""
+    sub = os.path.join(dir_, "c3")
+    os.makedirs(sub, exist_ok=True)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dirs, uv, tris = uv_sphere(n_lat, n_lon)
+    r = 1.0 + 0.1 * fbm_displacement(dirs * 1.5, octaves=6, omega=0.5, seed=seed)
+    P = dirs * r[:, None]
+    F_all = tris.reshape(-1, 3)
+    N = vertex_normals(P, F_all)
+    img = value_noise_image(image_size, seed + 1)
+    write_ppm(os.path.join(sub, "noise.ppm"), img)
+    st = SceneText(camera=(0.0, 1.1, -4.2, 0.0, 0.05, 0.0, 0, 1, 0, 40, 4.2))
+    st.images = [(0, "c3/noise.ppm")]
+    per = n_lon // submeshes
+    for k in range(submeshes):
+        cols = slice(k * per, n_lon if k == submeshes - 1 else (k + 1) * per)
+        F = tris[:, cols].reshape(-1, 3)
+        used = np.unique(F)
+        remap = np.full(P.shape[0], -1, np.int64)
+        remap[used] = np.arange(used.size)
+        Fi = remap[F]
+        write_bin(os.path.join(sub, f"sphere{k}"), _nonneg_zero(P[used]), Fi, N=_nonneg_zero(N[used]), idx_n=Fi,
+                  UV=_nonneg_zero(uv[used]), idx_uv=Fi)
+        st.subj_meshes.append(f"c3/sphere{k}")
+        kind = ["matte", "glass", "microfacet"][int(rng.integers(0, 3))] if k > 0 else "matte"
+        if kind == "glass":
+            st.subj_mats.append("G c 1 1 1 c 1 1 1 1.5")
+        elif kind == "microfacet":
+            col = rng.uniform(0.5, 0.95, 3)
+            st.subj_mats.append(f"F c {_f(col[0])} {_f(col[1])} {_f(col[2])} {_f(rng.uniform(0.05, 0.5))}")
+        else:
+            tex = ["f", "w", "m", "i"][k % 4]
+            octv, om = int(rng.integers(4, 9)), rng.uniform(0.3, 0.7)
+            if tex == "i":
+                spec_ = "i 0"
+            elif tex == "m":
+                spec_ = f"m {octv} {_f(om)} {_f(rng.uniform(1.5, 4.0))} {_f(rng.uniform(0.5, 2.0))}"
+            else:
+                spec_ = f"{tex} {octv} {_f(om)}"
+            st.subj_mats.append(f"L {spec_} {_f(0.0 if k % 2 == 0 else rng.uniform(5.0, 30.0))}")
+    # the subject file carries no images: the image index stays 0 in the scene stream
+    Pq, Fq, Nq = quad((0, -1.15, 0), (6, 0, 0), (0, 0, -6))
+    write_bin(os.path.join(sub, "ground"), _nonneg_zero(Pq), Fq, N=_nonneg_zero(Nq), idx_n=Fq,
+              UV=np.array([[0, 0], [4, 0], [4, 4], [0, 4]], np.float32), idx_uv=Fq)
+    st.meshes.append("c3/ground")
+    st.mats.append("L w 6 0.5 0")
+    for j, (c, s) in enumerate((((-2.0, 3.0, -1.0), 1.0), ((2.5, 2.5, 1.5), 0.7))):
+        Pl, Fl, Nl = quad(c, (s, 0, 0), (0, 0, s))  # normal cross(x, z) = -y: emits downwards
+        write_bin(os.path.join(sub, f"light{j}"), _nonneg_zero(Pl), Fl, N=_nonneg_zero(Nl), idx_n=Fl)
+        st.meshes.append(f"c3/light{j}")
+        st.mats.append("L c 0 0 0 0")
+    st.area_les = [(40.0, 36.0, 30.0), (12.0, 18.0, 28.0)]
+    st.infinite = (0.5, 0.5, 0.5)
+    st.write(dir_, "c3/displaced")
+    write_app_params(dir_, width, height, spp, bounces, rr, 1, "c3_out", "c3/displaced", "c3/displaced")
+    return dict(scene="c3/displaced", subject="c3/displaced", width=width, height=height, spp=spp,
+                max_bounce=bounces, rr_start=rr, stratified=0, seed=0x5EED)

@@ -907,8 +907,9 @@ extern "C" int oracle_render(oracle_scene* sc, const oracle_render_params* p, fl
     int W = s.W, H = s.H, spp = p->spp;
     int r0 = std::max(0, p->row_begin), r1 = p->row_end > 0 ? std::min(H, p->row_end) : H;
     size_t np = (size_t)W * H;
-    std::vector<float> Ls(sample_rgb ? 0 : np * spp * 3);
-    float* L = sample_rgb ? sample_rgb : Ls.data();
+    // per-sample radiance is kept only when the film gather or the caller needs it
+    std::vector<float> Ls((sample_rgb || !rgb_out) ? 0 : np * spp * 3);
+    float* L = sample_rgb ? sample_rgb : (rgb_out ? Ls.data() : nullptr);
     int nth = p->threads > 0 ? p->threads : (int)std::max(1u, std::thread::hardware_concurrency());
     std::atomic<int> nextTile(0);
     const int TILE = 16;
@@ -925,8 +926,10 @@ extern "C" int oracle_render(oracle_scene* sc, const oracle_render_params* p, fl
                     for (int smp = 0; smp < spp; smp++) {
                         V2 pf;
                         Spec v = Li(s, x, y, smp, *p, &pf, cnt[tid]);
-                        float* o = &L[3 * ((size_t)smp * np + (size_t)y * W + x)];
-                        o[0] = v.r; o[1] = v.g; o[2] = v.b;
+                        if (L) {
+                            float* o = &L[3 * ((size_t)smp * np + (size_t)y * W + x)];
+                            o[0] = v.r; o[1] = v.g; o[2] = v.b;
+                        }
                     }
         }
     };

@@ -35,3 +35,18 @@ def test_cornell_megakernel_matches_oracle(gpu_ctx, tmp_path):
     o_rgb, o_smp, _ = o.render(spp, 5, 3, SEED, stratified=1, want_samples=True)
     assert np.isfinite(g_rgb).all() and g_rgb.mean() > 0
     _compare(g_smp, o_smp, g_rgb, o_rgb, "cornell 64x48x4")
+
+
+@pytest.mark.gpu
+def test_displaced_sphere_megakernel_matches_oracle(gpu_ctx, tmp_path):
+    """Small version of the 1080p workload: Oren-Nayar and Lambert matte with FBM, wrinkled, marble
+    and image (mipmap) textures, glass (FresnelSpecular), microfacet mirror, two area-light meshes
+    and the infinite light."""
+    W, H, spp = 96, 54, 4
+    cfg = scenegen.gen_displaced_sphere(str(tmp_path), width=W, height=H, spp=spp, n_lat=40, n_lon=80, image_size=64)
+    gpu_ctx.load_scene(str(tmp_path), cfg["scene"], cfg["subject"], W, H)
+    g_rgb, g_smp, _ = gpu_ctx.render(spp, 8, 3, SEED, want_samples=True)
+    o = oracle_lib.OracleScene(str(tmp_path), cfg["scene"], cfg["subject"], W, H)
+    o_rgb, o_smp, _ = o.render(spp, 8, 3, SEED, want_samples=True)
+    assert np.isfinite(g_rgb).all() and g_rgb.mean() > 0
+    _compare(g_smp, o_smp, g_rgb, o_rgb, "displaced 96x54x4")

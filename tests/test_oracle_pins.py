@@ -1,0 +1,483 @@
+"""Pins of the CPU oracle: known answers, closed forms, invariants and independent restatements.
+
+The reference ships no tests, golden vectors or fixtures for this path, so every check here comes
+from an independent source: published known-answer vectors (Philox), exhaustive bit arithmetic
+(Morton), a textbook top-down formulation (Karras hierarchy, refit, stable sort), closed forms
+(Gaussian filter, Fresnel, Lambert, Oren-Nayar), brute force (closest hit) and analytic scenes
+(directly viewed emitter, mirror, Lambert plane under a triangle emitter).
+"""
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from lighttransportsimulation_amd import scenegen
+
+
+# ----------------------------------------------------------------------------- sampler
+# Known-answer vectors of Philox-4x32-10 (Random123 kat_vectors, Salmon et al. SC'11).
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, expect):
+    assert tuple(int(x) for x in O.philox(ctr, key)) == expect
+
+
+def _philox_py(ctr, key):
+    c, k = list(ctr), list(key)
+    for r in range(10):
+        if r:
+            k = [(k[0] + 0x9E3779B9) & 0xFFFFFFFF, (k[1] + 0xBB67AE85) & 0xFFFFFFFF]
+        p0, p1 = 0xD2511F53 * c[0], 0xCD9E8D57 * c[2]
+        c = [((p1 >> 32) ^ c[1] ^ k[0]) & 0xFFFFFFFF, p1 & 0xFFFFFFFF, ((p0 >> 32) ^ c[3] ^ k[1]) & 0xFFFFFFFF,
+             p0 & 0xFFFFFFFF]
+    return c
+
+
+def test_philox_matches_python_restatement():
+    rng = np.random.Generator(np.random.PCG64(5))
+    for _ in range(200):
+        ctr = [int(x) for x in rng.integers(0, 2 ** 32, 4)]
+        key = [int(x) for x in rng.integers(0, 2 ** 32, 2)]
+        assert list(O.philox(ctr, key)) == _philox_py(ctr, key)
+
+
+def test_sample_values_in_half_open_unit_interval():
+    v = np.array([O.lib().oracle_sample(7, p, s, d) for p in range(3) for s in range(4) for d in range(40)])
+    assert (v > 0).all() and (v <= 1).all()
+    assert np.all(v * 2 ** 24 == np.round(v * 2 ** 24))
+    # draws are addressed by (seed, pixel, sample, dimension): dimension d comes from block d // 4
+    ctr = [2, 3, 0, 0]
+    raw = O.philox(ctr, [7, 1])
+    for lane in range(4):
+        assert O.lib().oracle_sample(7, 1, 2, 12 + lane) == np.float32(((int(raw[lane]) >> 8) + 1) / 2 ** 24)
+
+
+# ----------------------------------------------------------------------------- Morton
+def _spread(x):
+    x = min(int(x), 1023)
+    out = 0
+    for b in range(10):
+        out |= ((x >> b) & 1) << (3 * b)
+    return out
+
+
+def test_morton_exhaustive_per_axis():
+    m = O.lib().oracle_morton3
+    for x in range(1025):
+        s = _spread(x)
+        assert m(float(x), 0.0, 0.0) == s
+        assert m(0.0, float(x), 0.0) == s << 1
+        assert m(0.0, 0.0, float(x)) == s << 2
+
+
+def test_morton_truncates_and_interleaves():
+    rng = np.random.Generator(np.random.PCG64(9))
+    for v in rng.uniform(0, 1024, (500, 3)).astype(np.float32):
+        ix = [int(np.floor(a)) for a in v]
+        assert O.lib().oracle_morton3(*[float(a) for a in v]) == _spread(ix[0]) | _spread(ix[1]) << 1 | _spread(ix[2]) << 2
+
+
+# ----------------------------------------------------------------------------- LBVH
+def _delta(codes, i, j):
+    n = len(codes)
+    if j < 0 or j >= n:
+        return -1
+    x = int(codes[i]) ^ int(codes[j])
+    if x == 0:
+        return 40 + (32 - (i ^ j).bit_length())
+    return 32 - x.bit_length()
+
+
+def _topdown(codes):
+    """Textbook top-down construction: a range splits after the last position that shares more
+    leading bits with its first element than the range's two ends do (equal codes compare by
+    index). Interior nodes take the index of the range end that touches the split."""
+    n = len(codes)
+    left, right = np.zeros(max(n - 1, 0), np.int32), np.zeros(max(n - 1, 0), np.int32)
+    if n == 1:
+        return left, right
+    stack = [(0, n - 1, 0)]
+    while stack:
+        first, last, node = stack.pop()
+        common = _delta(codes, first, last)
+        split = first
+        for k in range(first, last):  # linear scan: the largest k with delta(first, k) > common
+            if _delta(codes, first, k) > common:
+                split = k
+        lc = (n - 1) + split if split == first else split
+        rc = (n - 1) + split + 1 if split + 1 == last else split + 1
+        left[node], right[node] = lc, rc
+        if split != first:
+            stack.append((first, split, split))
+        if split + 1 != last:
+            stack.append((split + 1, last, split + 1))
+    return left, right
+
+
+def _reference_codes(tri):
+    t = tri.reshape(-1, 3, 3).astype(np.float32)
+    mn = np.minimum(np.minimum(t[:, 0], t[:, 1]), t[:, 2])
+    mx = np.maximum(np.maximum(t[:, 0], t[:, 1]), t[:, 2])
+    c = mn * np.float32(0.5) + mx * np.float32(0.5)
+    gmn, gmx = mn.min(0), mx.max(0)
+    o = c - gmn
+    span = gmx - gmn
+    o = np.where(gmx > gmn, o / np.where(span > 0, span, np.float32(1)), o).astype(np.float32)
+    s = (np.float32(1024) * o).astype(np.float32)
+    q = np.minimum(np.floor(s).astype(np.int64), 1023)
+    codes = np.array([_spread(a) | _spread(b) << 1 | _spread(cc) << 2 for a, b, cc in q], np.uint32)
+    return codes, np.concatenate([mn, mx], 1), np.concatenate([gmn, gmx])
+
+
+def _subtree_boxes(left, right, leafbox):
+    n = leafbox.shape[0]
+    ib = np.zeros((max(n - 1, 0), 6), np.float32)
+
+    def box(u):
+        return leafbox[u - (n - 1)] if u >= n - 1 else ib[u]
+
+    order = []
+    stack = [0] if n > 1 else []
+    while stack:
+        i = stack.pop()
+        order.append(i)
+        for c in (left[i], right[i]):
+            if c < n - 1:
+                stack.append(c)
+    for i in reversed(order):
+        a, b = box(left[i]), box(right[i])
+        ib[i] = np.concatenate([np.minimum(a[:3], b[:3]), np.maximum(a[3:], b[3:])])
+    return ib
+
+
+def _dup(n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    c = (rng.integers(0, 4, (n, 3)) + 0.5) / 4.0
+    base = np.array([[0.0, 0.01, 0.0], [-0.01, -0.005, 0.0], [0.01, -0.005, 0.0]])
+    return scenegen._nonneg_zero((c[:, None] + base[None]).reshape(n, 9))
+
+
+@pytest.mark.parametrize("n,kind", [(1, "soup"), (2, "soup"), (3, "soup"), (7, "soup"), (64, "soup"), (300, "soup"),
+                                    (1500, "soup"), (40, "dup"), (600, "dup")])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_lbvh_matches_textbook_construction(n, kind, mode):
+    tri = scenegen.triangle_soup(n, seed=n) if kind == "soup" else _dup(n, n)
+    out = O.lbvh_build(tri, mode)
+    codes, pbox, gb = _reference_codes(tri)
+    perm = np.argsort(codes, kind="stable")
+    assert np.array_equal(out["scene_bounds"], gb)
+    assert np.array_equal(out["perm"], perm)
+    assert np.array_equal(out["codes_sorted"], codes[perm])
+    left, right = _topdown(codes[perm])
+    assert np.array_equal(out["left"], left)
+    assert np.array_equal(out["right"], right)
+    leaf_prim = np.arange(n) if mode == 0 else perm
+    assert np.array_equal(out["leaf_prim"], leaf_prim)
+    assert np.array_equal(out["leaf_bounds"], pbox[leaf_prim])
+    assert np.array_equal(out["int_bounds"], _subtree_boxes(left, right, pbox[leaf_prim]))
+    # every node except the root has exactly one parent, consistent with the child arrays
+    par = out["parent"]
+    assert par[0] == -1 if n > 1 else par[0] == -1
+    for i in range(n - 1):
+        assert par[left[i]] == i and par[right[i]] == i
+
+
+# ----------------------------------------------------------------------------- filter, BSDFs, textures
+def test_gaussian_filter_table_closed_form():
+    t = np.zeros(256, np.float32)
+    O.lib().oracle_filter_table(O._p(t))
+    x = (np.arange(16) + 0.5) / 16
+    g = np.maximum(0, np.exp(-x * x) - np.exp(-1.0))
+    ref = np.outer(g, g).reshape(-1)
+    assert np.allclose(t, ref, rtol=2e-6, atol=1e-9)
+
+
+def _bxdf(kind, param, mode, wo, wi=(0, 0, 1), u=(0.5, 0.5)):
+    out = np.zeros(6, np.float32)
+    a, b, c = (np.asarray(x, np.float32) for x in (wo, wi, u))
+    O.lib().oracle_bxdf.argtypes = [O.C.c_int32, O.C.c_float, O.C.c_int32] + [O.C.c_void_p] * 4
+    O.lib().oracle_bxdf(kind, param, mode, O._p(a), O._p(b), O._p(c), O._p(out))
+    return out
+
+
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return (v / np.linalg.norm(v)).astype(np.float32)
+
+
+def test_fresnel_dielectric_clamp_gives_normal_incidence():
+    fr = O.lib().oracle_fr_dielectric
+    fr.restype = O.C.c_float
+    fr.argtypes = [O.C.c_float] * 3
+    for eta in (1.33, 1.5, 2.419):
+        ref = ((eta - 1) / (eta + 1)) ** 2
+        for c in (-0.9, -0.2, 0.0, 0.3, 0.99):
+            assert abs(fr(c, 1.0, eta) - ref) < 1e-6
+
+
+def test_lambert_sampling_is_a_perfect_furnace():
+    rng = np.random.Generator(np.random.PCG64(3))
+    for _ in range(200):
+        wo = _unit([rng.normal(), rng.normal(), abs(rng.normal()) + 0.05])
+        r = _bxdf(0, 0, 2, wo, u=rng.random(2))
+        wi, pdf, f = r[:3], r[3], r[4]
+        assert pdf > 0 and abs(f * abs(wi[2]) / pdf - 1) < 1e-5
+        assert abs(f - 1 / math.pi) < 1e-7
+
+
+def test_oren_nayar_closed_form():
+    rng = np.random.Generator(np.random.PCG64(4))
+    sig_deg = 20.0
+    s = math.radians(sig_deg)
+    A, B = 1 - s * s / (2 * (s * s + 0.33)), 0.45 * s * s / (s * s + 0.09)
+    for _ in range(200):
+        wo = _unit([rng.normal(), rng.normal(), abs(rng.normal()) + 0.05])
+        wi = _unit([rng.normal(), rng.normal(), abs(rng.normal()) + 0.05])
+        st_o, st_i = math.sqrt(1 - wo[2] ** 2), math.sqrt(1 - wi[2] ** 2)
+        dcos = (wi[0] * wo[0] + wi[1] * wo[1]) / (st_o * st_i)
+        if abs(wi[2]) > abs(wo[2]):
+            sa, tb = st_o, st_i / abs(wi[2])
+        else:
+            sa, tb = st_i, st_o / abs(wo[2])
+        ref = (A + B * max(0.0, dcos) * sa * tb) / math.pi
+        assert abs(_bxdf(1, sig_deg, 0, wo, wi)[0] - ref) < 2e-5 * max(1, ref)
+    assert abs(_bxdf(1, 0.0, 0, [0.3, 0.2, 0.93], [0.1, -0.5, 0.86])[0] - 1 / math.pi) < 1e-7
+
+
+def test_microfacet_sampling_agrees_with_its_pdf():
+    """Visible-normal sampling: the returned pdf equals the BxDF's pdf at the sampled direction, and
+    the pdf integrates (over the hemisphere) to the fraction of samples that stay above it."""
+    rng = np.random.Generator(np.random.PCG64(8))
+    for rough in (0.1, 0.4):
+        wo = _unit([0.4, -0.2, 0.8])
+        accepted = 0
+        N = 4000
+        for _ in range(N):
+            r = _bxdf(4, rough, 2, wo, u=rng.random(2))
+            if r[3] > 0:
+                accepted += 1
+                p = _bxdf(4, rough, 1, wo, r[:3])[0]
+                assert abs(p - r[3]) <= 1e-3 * r[3]
+        # Monte Carlo integral of pdf over the upper hemisphere (uniform directions)
+        M = 20000
+        z = rng.random(M)
+        phi = 2 * math.pi * rng.random(M)
+        rr = np.sqrt(1 - z * z)
+        total = 0.0
+        for k in range(M):
+            total += _bxdf(4, rough, 1, wo, [rr[k] * math.cos(phi[k]), rr[k] * math.sin(phi[k]), z[k]])[0]
+        integral = total * 2 * math.pi / M
+        assert abs(integral - accepted / N) < 0.05
+
+
+def _perlin_ref(p, perm):
+    def fade(t):
+        return t * t * t * (t * (t * 6 - 15) + 10)
+
+    def grad(h, x, y, z):
+        h &= 15
+        u = x if (h < 8 or h == 12 or h == 13) else y
+        v = y if (h < 4 or h == 12 or h == 13) else z
+        return (-u if h & 1 else u) + (-v if h & 2 else v)
+
+    x, y, z = p
+    X, Y, Z = int(math.floor(x)) & 255, int(math.floor(y)) & 255, int(math.floor(z)) & 255
+    x, y, z = x - math.floor(x), y - math.floor(y), z - math.floor(z)
+    u, v, w = fade(x), fade(y), fade(z)
+
+    def h(a, b, c):
+        return perm[perm[perm[a] + b] + c]
+
+    def lerp(t, a, b):
+        return a + t * (b - a)
+
+    return lerp(w, lerp(v, lerp(u, grad(h(X, Y, Z), x, y, z), grad(h(X + 1, Y, Z), x - 1, y, z)),
+                        lerp(u, grad(h(X, Y + 1, Z), x, y - 1, z), grad(h(X + 1, Y + 1, Z), x - 1, y - 1, z))),
+                lerp(v, lerp(u, grad(h(X, Y, Z + 1), x, y, z - 1), grad(h(X + 1, Y, Z + 1), x - 1, y, z - 1)),
+                     lerp(u, grad(h(X, Y + 1, Z + 1), x, y - 1, z - 1), grad(h(X + 1, Y + 1, Z + 1), x - 1, y - 1, z - 1))))
+
+
+def _perm_table():
+    path = os.path.join(os.path.dirname(__file__), "golden", "perlin_permutation.txt")
+    p = [int(x) for x in open(path).read().split()]
+    assert sorted(p) == list(range(256))
+    return p + p
+
+
+def test_perlin_matches_improved_noise():
+    L = O.lib()
+    L.oracle_perlin.restype = O.C.c_float
+    L.oracle_perlin.argtypes = [O.C.c_float] * 3
+    perm = _perm_table()
+    rng = np.random.Generator(np.random.PCG64(12))
+    for p in rng.uniform(-20, 20, (400, 3)).astype(np.float32):
+        assert abs(L.oracle_perlin(*map(float, p)) - _perlin_ref([float(a) for a in p], perm)) < 2e-6
+    for p in rng.integers(-30, 30, (50, 3)):
+        assert L.oracle_perlin(*map(float, p)) == 0.0
+
+
+def test_fbm_octave_count_and_sum():
+    L = O.lib()
+    L.oracle_fbm.restype = O.C.c_float
+    L.oracle_fbm.argtypes = [O.C.c_void_p] * 3 + [O.C.c_float, O.C.c_int32]
+    perm = _perm_table()
+    p = np.array([0.3, 1.7, -2.2], np.float32)
+    z = np.zeros(3, np.float32)
+    # without differentials every octave is used and the partial octave has weight 0
+    ref = sum(0.5 ** i * _perlin_ref([float(a) * 1.99 ** i for a in p], perm) for i in range(5))
+    assert abs(L.oracle_fbm(O._p(p), O._p(z), O._p(z), 0.5, 5) - ref) < 1e-4
+    # |dpdx|^2 = 2^-6 -> n = -1 + 3 = 2 octaves exactly
+    dx = np.array([2 ** -3, 0, 0], np.float32)
+    ref2 = sum(0.5 ** i * _perlin_ref([float(a) * 1.99 ** i for a in p], perm) for i in range(2))
+    assert abs(L.oracle_fbm(O._p(p), O._p(dx), O._p(z), 0.5, 5) - ref2) < 1e-4
+
+
+def test_marble_spline_endpoints():
+    L = O.lib()
+    L.oracle_marble.argtypes = [O.C.c_void_p, O.C.c_int32, O.C.c_float, O.C.c_float, O.C.c_float, O.C.c_void_p]
+    out = np.zeros(3, np.float32)
+    # variation 0: marble value = p.y; t = 0.5 + 0.5 sin(y): sin = -1 -> t = 0 -> 1.5 * colour 0
+    p = np.array([0.1, -math.pi / 2, 0.2], np.float32)
+    L.oracle_marble(O._p(p), 4, 0.5, 1.0, 0.0, O._p(out))
+    assert np.allclose(out, 1.5 * np.array([0.58, 0.58, 0.6]), atol=1e-5)
+
+
+# ----------------------------------------------------------------------------- intersection
+def _read_bin(path):
+    b = open(path + ".bin", "rb").read()
+    nt, npnt, nuv, nn = struct.unpack("<4i", b[:16])
+    o = 16
+    P = np.frombuffer(b, np.float32, 3 * npnt, o).reshape(-1, 3); o += 12 * npnt + 8 * nuv + 12 * nn
+    idx = np.frombuffer(b, np.int32, 9 * nt, o).reshape(nt, 9)
+    return P[idx[:, [0, 3, 6]]]
+
+
+def _mt_brute(tris, o, d):
+    """Moller-Trumbore in fp64 with the reference's barycentric margins (Triangle.cu:94-128)."""
+    p0, p1, p2 = tris[:, 0].astype(np.float64), tris[:, 1].astype(np.float64), tris[:, 2].astype(np.float64)
+    e1, e2 = p1 - p0, p2 - p0
+    pv = np.cross(d, e2)
+    det = np.einsum("ij,ij->i", e1, pv)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = 1.0 / det
+        tv = o - p0
+        u = inv * np.einsum("ij,ij->i", tv, pv)
+        qv = np.cross(tv, e1)
+        v = inv * (qv @ d)
+        t = inv * np.einsum("ij,ij->i", e2, qv)
+    ok = (det != 0) & (u >= 1e-4) & (u <= 1 - 1e-4) & (v >= 1e-4) & (u + v <= 1 - 1e-4) & (t > 0)
+    margin = np.minimum.reduce([u - 1e-4, 1 - 1e-4 - u, v - 1e-4, 1 - 1e-4 - u - v])
+    return ok, t, margin
+
+
+def test_closest_hit_matches_brute_force(tmp_path):
+    cfg = scenegen.gen_cornell_sphere(str(tmp_path), width=8, height=8, spp=1, subdiv=3)
+    sc = O.OracleScene(str(tmp_path), cfg["scene"], cfg["subject"], 8, 8)
+    names = ["sphere", "floor", "ceiling", "back", "left", "right", "light"]
+    tris = np.concatenate([_read_bin(os.path.join(str(tmp_path), "c1", n)) for n in names])
+    rng = np.random.Generator(np.random.PCG64(21))
+    checked = 0
+    for _ in range(400):
+        o = rng.uniform(-1.5, 1.5, 3).astype(np.float32)
+        d = _unit(rng.normal(size=3))
+        ok, t, margin = _mt_brute(tris, o.astype(np.float64), d.astype(np.float64))
+        prim, th = sc.intersect(o, d)
+        if not ok.any():
+            assert prim == -1
+            continue
+        tt = np.where(ok, t, np.inf)
+        k = int(np.argmin(tt))
+        s = np.sort(tt)
+        if margin[k] < 1e-6 or (len(s) > 1 and s[1] - s[0] < 1e-6):
+            continue  # too close to an edge margin or a tie for an fp64 vs fp32 comparison
+        assert prim == k and abs(th - tt[k]) <= 1e-5 * max(1.0, tt[k])
+        occ = sc.occluded(o, d, float(tt[k]) * 0.999)
+        assert occ == bool(((tt < tt[k] * 0.999)).any())
+        checked += 1
+    assert checked > 200
+
+
+# ----------------------------------------------------------------------------- integrator, analytic scenes
+def _analytic_scene(d, meshes, mats, les, camera, infinite=None):
+    st = scenegen.SceneText(camera=camera)
+    for name, (P, F, N) in meshes:
+        scenegen.write_bin(os.path.join(d, name), scenegen._nonneg_zero(P), F, N=scenegen._nonneg_zero(N), idx_n=F)
+        st.meshes.append(name)
+    st.mats = mats
+    st.area_les = les
+    st.infinite = infinite
+    st.write(d, "scene")
+    return "scene"
+
+
+def test_directly_viewed_emitter_is_exact(tmp_path):
+    d = str(tmp_path)
+    # a large emitting quad facing the camera fills the view; its material is black
+    P, F, N = scenegen.quad((0, 0, 3), (5, 0, 0), (0, -5, 0))  # normal -z, towards the camera
+    sc = _analytic_scene(d, [("light", (P, F, N))], ["L c 0 0 0 0"], [(2.5, 1.5, 0.5)], (0, 0, 0, 0, 0, 1, 0, 1, 0, 30, 1))
+    s = O.OracleScene(d, sc, "", 8, 6)
+    rgb, smp, _ = s.render(4, 3, 3, 1, want_samples=True)
+    assert np.all(smp == np.array([2.5, 1.5, 0.5], np.float32))
+    assert np.allclose(rgb, [2.5, 1.5, 0.5], rtol=1e-6)
+
+
+def test_mirror_reflects_emitter(tmp_path):
+    d = str(tmp_path)
+    mirror = scenegen.quad((0, 0, 3), (5, 0, 0), (0, -5, 0))       # faces the camera
+    light = scenegen.quad((0, 0, -3), (50, 0, 0), (0, 50, 0))       # behind the camera, faces +z
+    sc = _analytic_scene(d, [("mirror", mirror), ("light", light)], ["R c 0.7 0.5 0.3 0", "L c 0 0 0 0"],
+                         [(4.0, 4.0, 4.0)], (0, 0, 0, 0, 0, 1, 0, 1, 0, 20, 1))
+    s = O.OracleScene(d, sc, "", 6, 6)
+    _, smp, _ = s.render(4, 3, 3, 2, want_samples=True)
+    # camera -> mirror (specular, beta = R / |cos| * |cos|) -> emitter seen after a specular bounce
+    assert np.allclose(smp, np.array([2.8, 2.0, 1.2], np.float32), rtol=1e-5)
+
+
+def _polygon_irradiance(x, n, verts):
+    """Lambert's formula: E / L = 0.5 * sum_i theta_i (n . gamma_i) for a uniformly emitting polygon."""
+    total = 0.0
+    k = len(verts)
+    for i in range(k):
+        a, b = verts[i] - x, verts[(i + 1) % k] - x
+        a, b = a / np.linalg.norm(a), b / np.linalg.norm(b)
+        g = np.cross(a, b)
+        gn = np.linalg.norm(g)
+        theta = math.atan2(gn, np.dot(a, b))
+        total += theta * np.dot(n, g / gn)
+    return 0.5 * abs(total)
+
+
+def test_lambert_plane_under_triangle_emitter(tmp_path):
+    """Direct lighting only (one bounce): the pixel converges to albedo/pi * Le * E/L with the
+    irradiance of the emitting triangle from Lambert's polygon formula."""
+    d = str(tmp_path)
+    plane = scenegen.quad((0, 0, 0), (50, 0, 0), (0, 0, -50))           # normal +y
+    tri_v = np.array([[-0.4, 1.0, -0.3], [0.5, 1.2, -0.2], [0.0, 0.9, 0.6]])
+    # winding chosen so that the geometric normal points down (towards the plane)
+    Pl = tri_v[[0, 1, 2]]
+    nl = np.cross(Pl[1] - Pl[0], Pl[2] - Pl[0]); nl /= np.linalg.norm(nl)
+    assert nl[1] < 0
+    light = (Pl, np.array([[0, 1, 2]]), np.tile(nl, (3, 1)))
+    albedo, Le = 0.6, 5.0
+    # below the emitter, looking straight down at x0 (away from the quad's diagonal, where the
+    # barycentric margins of Triangle::intersect reject hits in both halves)
+    x0 = np.array([0.3, 0.0, 0.2])
+    cam = (0.3, 0.5, 0.2 - 0.0001, 0.3, 0.0, 0.2, 0, 1, 0, 0.05, 0.5)
+    sc = _analytic_scene(d, [("plane", plane), ("light", light)], [f"L c {albedo} {albedo} {albedo} 0", "L c 0 0 0 0"],
+                         [(Le, Le, Le)], cam)
+    s = O.OracleScene(d, sc, "", 2, 2)
+    _, smp, _ = s.render(4096, 1, 5, 3, want_samples=True)
+    est = smp[..., 0].reshape(-1)
+    expect = albedo / math.pi * Le * _polygon_irradiance(x0, np.array([0, 1.0, 0]), tri_v)
+    se = est.std() / math.sqrt(est.size)
+    assert abs(est.mean() - expect) < 4 * se + 1e-4 * expect, (est.mean(), expect, se)
