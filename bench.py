@@ -33,6 +33,10 @@ WORKLOADS = {
                                  "sectors, FBM/wrinkled/marble/2048^2 image textures, infinite light 0.5 + 2 area "
                                  "lights, 1920x1080, 64 spp, 8 bounces, RR from 3", gen="displaced", width=1920,
                             height=1080, spp=64, bounces=8, rr=3, stratified=0),
+    "lbvh_1m": dict(desc="LBVH build only: 1,048,576 fp32 triangles, centroids U[0,1)^3, log-normal(-5, 0.7) edges, "
+                         "random orientations; reference leaf layout", gen="soup", n=1 << 20),
+    "lbvh_8m": dict(desc="LBVH build only: 8,388,608 fp32 triangles, centroids U[0,1)^3, log-normal(-5, 0.7) edges, "
+                         "random orientations; reference leaf layout", gen="soup", n=1 << 23),
     "cornell_256": dict(desc="seeded icosphere (20,480 tris) in a Cornell box, 256x256, 16 spp stratified, "
                              "5 bounces, RR from 3", gen="cornell", width=256, height=256, spp=16, bounces=5, rr=3,
                         stratified=1),
@@ -76,6 +80,50 @@ def cpu_baseline(d: str, cfg: dict, budget_s: float) -> dict:
                 tri_tests_per_sample=st["tri_tests"] / st["samples"])
 
 
+# Algorithmic HBM bytes of one LBVH build per primitive: read the soup (36), write code + id (8),
+# four 8-bit radix passes over (code, id) pairs (16 each), write the node and leaf records (60).
+LBVH_BYTES_PER_PRIM = 36 + 8 + 16 * 4 + 60
+
+
+def run_lbvh(args) -> dict:
+    import ctypes
+    from lighttransportsimulation_amd import Context, scenegen
+    w = WORKLOADS[args.workload]
+    n = w["n"]
+    tri = scenegen.triangle_soup(n, seed=3)
+    with Context(0) as ctx:
+        ctx.bvh_build(tri, mode=0, export=False)  # uploads the soup once; it stays resident
+        if args.warmup:
+            ctx.bvh_build_device(None, n, 0, repeat=args.warmup)
+        t0 = time.perf_counter()
+        t = ctx.bvh_build_device(None, n, 0, repeat=args.steps)
+        wall = time.perf_counter() - t0
+        copy_bytes = 1 << 30
+        copy_ms = ctx.copy_benchmark(copy_bytes, 10)
+    value = n / (t["total_ms"] * 1e3)
+    achieved = LBVH_BYTES_PER_PRIM * n / (t["total_ms"] * 1e-3) / 1e9
+    out = {"metric": "LBVH build Mprims/s, 1 MI355X", "value": round(value, 2), "unit": "Mprims/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t["total_ms"], 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": w["desc"], "primitives": n, "wall_ms_per_step": round(wall * 1e3 / args.steps, 3),
+                      "stage_ms": {k: round(v, 4) for k, v in t.items()}},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "whole build",
+                        "algorithmic_bytes_per_prim": LBVH_BYTES_PER_PRIM,
+                        "copy_kernel_GBps": round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1)}}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        m = min(n, 1 << 20)
+        sub = tri[:m]
+        t1 = time.perf_counter()
+        oracle_lib.lbvh_build(sub, 0)
+        sec = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": round(m / sec / 1e6, 3), "unit": "Mprims/s", "cores": 1, "kind": "port",
+                               "sample": f"first {m} triangles of the same soup ({sec:.2f} s, std::stable_sort)"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,12 +132,27 @@ def main() -> int:
     ap.add_argument("--workload", default="displaced_1080p", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle work for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--integrator", default="megakernel", choices=("megakernel", "wavefront"))
+    ap.add_argument("--autotune", type=int, default=1, help="wavefront: tune block/grid/sort during warm-up")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="run only the plain 1 GiB device copy (traffic-counter calibration)")
     args = ap.parse_args()
+    if args.copy_only:
+        from lighttransportsimulation_amd import Context
+        with Context(0) as ctx:
+            ms = ctx.copy_benchmark(1 << 30, 3)
+        print(json.dumps({"copy_bytes": 1 << 30, "launches": 4, "ms_per_copy": round(ms, 4),
+                          "GBps_read_plus_write": round(2 * (1 << 30) / (ms * 1e-3) / 1e9, 1)}), flush=True)
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         return 2
-    from lighttransportsimulation_amd import Context
+    if WORKLOADS[args.workload]["gen"] == "soup":
+        print(json.dumps(run_lbvh(args)), flush=True)
+        return 0
+    from lighttransportsimulation_amd import INTEGRATOR_MEGAKERNEL, INTEGRATOR_WAVEFRONT, Context
+    wavefront = args.integrator == "wavefront"
     with tempfile.TemporaryDirectory() as d:
         cfg = make_workload(args.workload, d)
         W, H, spp = cfg["width"], cfg["height"], cfg["spp"]
@@ -97,9 +160,15 @@ def main() -> int:
             ctx.load_scene(d, cfg["scene"], cfg["subject"], W, H)
             info = ctx.scene_info()
             kw = dict(spp=spp, max_bounce=cfg["max_bounce"], rr_start=cfg["rr_start"], seed=cfg["seed"],
-                      stratified=cfg["stratified"], want_image=False)
-            for _ in range(args.warmup):
-                ctx.render(**kw)
+                      stratified=cfg["stratified"], want_image=False,
+                      integrator=INTEGRATOR_WAVEFRONT if wavefront else INTEGRATOR_MEGAKERNEL,
+                      autotune=int(wavefront and args.autotune))
+            tuning = None
+            for _ in range(max(args.warmup, 1 if wavefront and args.autotune else 0)):
+                _, _, st = ctx.render(**kw)
+                if wavefront:
+                    tuning = {k: st[k] for k in ("trace_block", "persistent_blocks", "sort_materials", "use_graph")}
+                    tuning["autotune_ms"] = round(st["autotune_ms"], 1)
             t0 = time.perf_counter()
             dev_ms, kern_ms, launches = 0.0, 0.0, 0
             for _ in range(args.steps):
@@ -117,8 +186,10 @@ def main() -> int:
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": WORKLOADS[args.workload]["desc"], "width": W, "height": H, "spp": spp,
                        "max_bounce": cfg["max_bounce"], "rr_start": cfg["rr_start"], "triangles": info["tris"],
-                       "integrator": "megakernel", "device_ms_per_step": round(dev_ms / args.steps, 3)},
+                       "integrator": args.integrator, "device_ms_per_step": round(dev_ms / args.steps, 3)},
         }
+        if tuning:
+            out["config"]["tuning"] = tuning
         if not args.no_cpu:
             cb = cpu_baseline(d, cfg, args.cpu_budget)
             launch_s = kern_ms / launches / 1e3
@@ -127,7 +198,8 @@ def main() -> int:
             achieved = bytes_per_sample * samples_per_launch / launch_s / 1e9
             out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                               "kernel": "k_megakernel", "kernel_avg_ms": round(kern_ms / launches, 3),
+                               "kernel": "wavefront pass (graph)" if wavefront else "k_megakernel",
+                               "kernel_avg_ms": round(kern_ms / launches, 3),
                                "algorithmic_bytes_per_sample": round(bytes_per_sample, 1)}
             out["cpu_baseline"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cb.items()
                                    if k in ("value", "unit", "cores", "kind", "sample")}

@@ -61,8 +61,16 @@ typedef struct lts_render_params {
     uint32_t seed;        /* sampler seed (replaces time(0))                       */
     int32_t stratified;   /* 1: film sample per Sampler::stratifiedSample          */
     int32_t integrator;   /* LTS_INTEGRATOR_*                                      */
-    int32_t spp_per_pass; /* samples per launch; 0 = automatic                     */
-    int32_t reserved[9];
+    int32_t spp_per_pass; /* samples per pass; 0 = automatic                       */
+    /* wavefront only (0 / -1 = default or the autotuned value) */
+    int32_t trace_block;        /* block size of the persistent traversal kernels: 64, 128, 256 */
+    int32_t persistent_blocks;  /* grid of the persistent traversal kernels                     */
+    int32_t sort_materials;     /* -1 default, 0 off, 1 counting-sort shading work by material  */
+    int32_t use_graph;          /* -1 default (on), 0 off: capture each pass in a hipGraph       */
+    int32_t autotune;           /* 1: time candidate tunings on this scene before rendering;    */
+                                /*    the choice is kept per scene, film and pass size in the   */
+                                /*    context, and in the file named by $LTS_TUNE_CACHE if set  */
+    int32_t reserved[4];
 } lts_render_params;
 
 typedef struct lts_stats {
@@ -74,7 +82,12 @@ typedef struct lts_stats {
     double msamples_per_s;  /* samples / (render_ms + resolve_ms)                   */
     int64_t node_visits;    /* filled only when counting is enabled                 */
     int64_t tri_tests;
-    double reserved[6];
+    int32_t trace_block;    /* wavefront tuning that was used                      */
+    int32_t persistent_blocks;
+    int32_t sort_materials;
+    int32_t use_graph;
+    double autotune_ms;     /* time spent in the autotuner by this call             */
+    double reserved[3];
 } lts_stats;
 
 /* Context. */
@@ -100,8 +113,9 @@ int lts_scene_info(lts_ctx* ctx, int32_t* n_tris, int32_t* n_lights, int32_t* n_
  * Results stay on the device; `out` (may be NULL) receives the canonical arrays. */
 int lts_bvh_build(lts_ctx* ctx, const float* tri_xyz, int32_t n, int32_t mode, lts_bvh_export* out,
                   lts_bvh_timing* timing);
-/* Same build on a soup already resident in device memory (d_tri_xyz is a device pointer);
- * `repeat` builds are timed back to back and timing holds the per-build mean. */
+/* Same build on a soup already resident in device memory (d_tri_xyz is a device pointer, or NULL
+ * for the soup uploaded by the last lts_bvh_build with the same n); `repeat` builds are timed back
+ * to back and timing holds the per-build mean. */
 int lts_bvh_build_device(lts_ctx* ctx, const float* d_tri_xyz, int32_t n, int32_t mode, int32_t repeat,
                          lts_bvh_timing* timing);
 /* Copies the arrays of the most recent build to host memory. */
@@ -111,6 +125,19 @@ int lts_bvh_export_last(lts_ctx* ctx, lts_bvh_export* out);
  * rendering/Camera.cu:256-289). rgb_out (may be NULL) receives W*H*3 floats in getRender row
  * order; sample_rgb (may be NULL) receives every sample's radiance, [spp][H][W][3]. */
 int lts_render(lts_ctx* ctx, const lts_render_params* params, float* rgb_out, float* sample_rgb, lts_stats* stats);
+
+/* Batched form of lts_render with the wavefront integrator (the reference renders one scene per
+ * process, LightTransportSimulation.cu:23-146): the scenes loaded in ctxs[0..n-1] (contexts of one
+ * device) are rendered together, every queue kernel of a pass working on the paths of all of
+ * them, on ctxs[0]'s stream and buffers. Each scene's image equals what lts_render gives for it
+ * alone; rgb_out[i] (entries may be NULL) receives scene i in getRender row order. */
+int lts_render_batch(lts_ctx* const* ctxs, int32_t n, const lts_render_params* params, float* const* rgb_out,
+                     lts_stats* stats);
+
+/* No reference counterpart: a plain device-to-device copy kernel over `bytes` bytes, timed with
+ * HIP events (mean of `repeat` launches). Used to calibrate HBM traffic counters and to measure
+ * the achievable copy bandwidth next to the kernels' roofline figures. */
+int lts_device_copy_benchmark(lts_ctx* ctx, uint64_t bytes, int32_t repeat, double* ms_per_copy);
 
 /* Replaces writeToPPM / writeToPFM (core/ImageO.cuh:16-78). format 0: P3 PPM, 1: PFM. */
 int lts_write_image(const char* path, const float* rgb, int32_t width, int32_t height, int32_t format);

@@ -22,7 +22,7 @@ INTEGRATOR_CACHED = 2
 EXPORTED_SYMBOLS = (
     "lts_create", "lts_destroy", "lts_last_error", "lts_version", "lts_read_app_params", "lts_load_scene",
     "lts_scene_info", "lts_bvh_build", "lts_bvh_build_device", "lts_bvh_export_last", "lts_render",
-    "lts_write_image",
+    "lts_write_image", "lts_device_copy_benchmark", "lts_render_batch",
 )
 
 
@@ -43,13 +43,16 @@ class BvhTiming(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_int32), ("max_bounce", C.c_int32), ("rr_start", C.c_int32), ("seed", C.c_uint32),
                 ("stratified", C.c_int32), ("integrator", C.c_int32), ("spp_per_pass", C.c_int32),
-                ("reserved", C.c_int32 * 9)]
+                ("trace_block", C.c_int32), ("persistent_blocks", C.c_int32), ("sort_materials", C.c_int32),
+                ("use_graph", C.c_int32), ("autotune", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class Stats(C.Structure):
     _fields_ = [("render_ms", C.c_double), ("resolve_ms", C.c_double), ("kernel_avg_ms", C.c_double),
                 ("launches", C.c_int64), ("samples", C.c_int64), ("msamples_per_s", C.c_double),
-                ("node_visits", C.c_int64), ("tri_tests", C.c_int64), ("reserved", C.c_double * 6)]
+                ("node_visits", C.c_int64), ("tri_tests", C.c_int64), ("trace_block", C.c_int32),
+                ("persistent_blocks", C.c_int32), ("sort_materials", C.c_int32), ("use_graph", C.c_int32),
+                ("autotune_ms", C.c_double), ("reserved", C.c_double * 3)]
 
 
 _lib = None
@@ -77,7 +80,9 @@ def load_library():
     L.lts_bvh_build_device.argtypes = [vp, vp, i32, i32, i32, C.POINTER(BvhTiming)]
     L.lts_bvh_export_last.argtypes = [vp, C.POINTER(BvhExport)]
     L.lts_render.argtypes = [vp, C.POINTER(RenderParams), vp, vp, C.POINTER(Stats)]
+    L.lts_render_batch.argtypes = [C.POINTER(vp), i32, C.POINTER(RenderParams), C.POINTER(vp), C.POINTER(Stats)]
     L.lts_write_image.argtypes = [C.c_char_p, vp, i32, i32, i32]
+    L.lts_device_copy_benchmark.argtypes = [vp, C.c_uint64, i32, C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -155,9 +160,11 @@ class Context:
             arrs["scene_bounds"] = np.array(list(e.scene_bounds), np.float32)
         return arrs, dict(total_ms=t.total_ms, sort_ms=t.sort_ms, bounds_ms=t.bounds_ms, tree_ms=t.tree_ms)
 
-    def bvh_build_device(self, dev_ptr: int, n: int, mode: int, repeat: int = 1) -> dict:
+    def bvh_build_device(self, dev_ptr, n: int, mode: int, repeat: int = 1) -> dict:
+        """Builds from a device-resident soup (dev_ptr None: the soup of the last bvh_build)."""
         t = BvhTiming()
-        self._check(self.lib.lts_bvh_build_device(self.h, C.c_void_p(dev_ptr), n, mode, repeat, C.byref(t)),
+        self._check(self.lib.lts_bvh_build_device(self.h, None if dev_ptr is None else C.c_void_p(dev_ptr), n, mode,
+                                                  repeat, C.byref(t)),
                     "lts_bvh_build_device")
         return dict(total_ms=t.total_ms, sort_ms=t.sort_ms, bounds_ms=t.bounds_ms, tree_ms=t.tree_ms)
 
@@ -168,19 +175,59 @@ class Context:
         arrs["scene_bounds"] = np.array(list(e.scene_bounds), np.float32)
         return arrs
 
+    def copy_benchmark(self, nbytes: int, repeat: int = 10) -> float:
+        """Mean milliseconds of a plain device copy of nbytes (HIP events)."""
+        ms = C.c_double()
+        self._check(self.lib.lts_device_copy_benchmark(self.h, nbytes, repeat, C.byref(ms)), "lts_device_copy_benchmark")
+        return ms.value
+
     # ------------------------------------------------------------------ render
     def render(self, spp: int, max_bounce: int, rr_start: int, seed: int, stratified: int = 0,
                integrator: int = INTEGRATOR_MEGAKERNEL, spp_per_pass: int = 0, want_image: bool = True,
-               want_samples: bool = False):
-        p = RenderParams(spp, max_bounce, rr_start, seed, stratified, integrator, spp_per_pass)
+               want_samples: bool = False, trace_block: int = 0, persistent_blocks: int = 0, sort_materials: int = -1,
+               use_graph: int = -1, autotune: int = 0):
+        """Renders the loaded scene (include/lts.h: lts_render). Returns (rgb, samples, stats): rgb is
+        [H][W][3] in getRender row order, samples [spp][H][W][3] per-sample radiance when asked for."""
+        p = render_params(spp, max_bounce, rr_start, seed, stratified, integrator, spp_per_pass, trace_block,
+                          persistent_blocks, sort_materials, use_graph, autotune)
         rgb = np.zeros((self.height, self.width, 3), np.float32) if want_image else None
         smp = np.zeros((spp, self.height, self.width, 3), np.float32) if want_samples else None
         st = Stats()
         self._check(self.lib.lts_render(self.h, C.byref(p), _ptr(rgb), _ptr(smp), C.byref(st)), "lts_render")
-        stats = dict(render_ms=st.render_ms, resolve_ms=st.resolve_ms, kernel_avg_ms=st.kernel_avg_ms,
-                     launches=st.launches, samples=st.samples, msamples_per_s=st.msamples_per_s,
-                     node_visits=st.node_visits, tri_tests=st.tri_tests)
-        return rgb, smp, stats
+        return rgb, smp, _stats_dict(st)
+
+
+def render_params(spp, max_bounce, rr_start, seed, stratified=0, integrator=INTEGRATOR_MEGAKERNEL, spp_per_pass=0,
+                  trace_block=0, persistent_blocks=0, sort_materials=-1, use_graph=-1, autotune=0) -> RenderParams:
+    return RenderParams(spp, max_bounce, rr_start, seed, stratified, integrator, spp_per_pass, trace_block,
+                        persistent_blocks, sort_materials, use_graph, autotune)
+
+
+def _stats_dict(st: Stats) -> dict:
+    return dict(render_ms=st.render_ms, resolve_ms=st.resolve_ms, kernel_avg_ms=st.kernel_avg_ms,
+                launches=st.launches, samples=st.samples, msamples_per_s=st.msamples_per_s,
+                node_visits=st.node_visits, tri_tests=st.tri_tests, trace_block=st.trace_block,
+                persistent_blocks=st.persistent_blocks, sort_materials=st.sort_materials, use_graph=st.use_graph,
+                autotune_ms=st.autotune_ms)
+
+
+def render_batch(contexts, spp: int, max_bounce: int, rr_start: int, seed: int, stratified: int = 0,
+                 spp_per_pass: int = 0, trace_block: int = 0, persistent_blocks: int = 0, sort_materials: int = -1,
+                 use_graph: int = -1, autotune: int = 0, want_images: bool = True):
+    """Renders the scenes of several contexts together with the wavefront integrator
+    (include/lts.h: lts_render_batch). Returns ([rgb per scene], stats)."""
+    ctxs = list(contexts)
+    if not ctxs:
+        raise LtsError("render_batch needs at least one context")
+    L = ctxs[0].lib
+    p = render_params(spp, max_bounce, rr_start, seed, stratified, INTEGRATOR_WAVEFRONT, spp_per_pass, trace_block,
+                      persistent_blocks, sort_materials, use_graph, autotune)
+    imgs = [np.zeros((c.height, c.width, 3), np.float32) if want_images else None for c in ctxs]
+    handles = (C.c_void_p * len(ctxs))(*[c.h.value for c in ctxs])
+    outs = (C.c_void_p * len(ctxs))(*[None if a is None else a.ctypes.data for a in imgs])
+    st = Stats()
+    ctxs[0]._check(L.lts_render_batch(handles, len(ctxs), C.byref(p), outs, C.byref(st)), "lts_render_batch")
+    return imgs, _stats_dict(st)
 
 
 def read_app_params(path: str) -> dict:

@@ -5,10 +5,14 @@
 //   radix sort     stable LSD sort of (code, primitive) pairs, 8-bit digits, 4 passes (BVH.cuh:140-146)
 //                  count -> per-digit scan over tiles -> digit bases -> LDS-ranked scatter
 //   k_karras       interior node i: direction, range end, split                     (BVH.cuh:173-236)
-//   k_refit        leaf-to-root climb, second arriver unions its children            (BVH.cuh:238-262)
+//   k_refit_l1/l2  leaf-to-root climb, second arriver unions its children            (BVH.cuh:238-262):
+//   /global        subtrees inside 1024-leaf chunks in LDS, then 64K-leaf chunks with LDS arrival
+//                  bits, then the few nodes crossing 64K boundaries with device-scope ordering
 //   k_pack_*       traversal layout: both child boxes per node, triangles in leaf order
 // The scene-bound reduction uses a total order on floats (-0 below +0); it equals the reference's
 // float atomics (GeneralHelper.cuh:150-160) on every input without -0.0.
+#include <algorithm>
+
 #include "lts_device.h"
 #include "lts_internal.h"
 
@@ -28,12 +32,6 @@ __device__ __forceinline__ float order_val(uint32_t k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
-__global__ void k_init_scene_keys(uint32_t* keys) {
-    int t = threadIdx.x;
-    if (t < 3) keys[t] = 0xffffffffu;       // running minimum
-    else if (t < 6) keys[t] = 0u;           // running maximum
-}
-
 // Triangle::worldBound = bUnion(Bounds3f(p0, p1), p2) with the Bounds3 constructor re-applied.
 __device__ __forceinline__ void tri_box(const float* t, float* mn, float* mx) {
 #pragma unroll
@@ -46,7 +44,7 @@ __device__ __forceinline__ void tri_box(const float* t, float* mn, float* mx) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_prim_bounds(const float* __restrict__ soup, int n, float* __restrict__ primBox,
-                                                        uint32_t* __restrict__ sceneKeys) {
+                                                        uint32_t* __restrict__ partial) {
     __shared__ uint32_t red[6][kBlock / 64];
     int i = blockIdx.x * kBlock + threadIdx.x;
     uint32_t k[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
@@ -64,7 +62,6 @@ __global__ __launch_bounds__(kBlock) void k_prim_bounds(const float* __restrict_
             k[3 + a] = order_key(mx[a]);
         }
     }
-    // wave reduction, then one atomic per component per block
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
@@ -81,7 +78,39 @@ __global__ __launch_bounds__(kBlock) void k_prim_bounds(const float* __restrict_
         int a = threadIdx.x;
         uint32_t v = red[a][0];
         for (int w = 1; w < kBlock / 64; w++) v = a < 3 ? min(v, red[a][w]) : max(v, red[a][w]);
-        if (a < 3) atomicMin(&sceneKeys[a], v); else atomicMax(&sceneKeys[a], v);
+        partial[6 * (size_t)blockIdx.x + a] = v;
+    }
+}
+
+// One block reduces the per-block partial keys into the six scene-bound keys.
+__global__ __launch_bounds__(1024) void k_reduce_scene_keys(const uint32_t* __restrict__ partial, int blocks,
+                                                            uint32_t* __restrict__ sceneKeys) {
+    __shared__ uint32_t red[6][16];
+    uint32_t k[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    for (int b = threadIdx.x; b < blocks; b += 1024) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            k[a] = min(k[a], partial[6 * (size_t)b + a]);
+            k[3 + a] = max(k[3 + a], partial[6 * (size_t)b + 3 + a]);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            k[a] = min(k[a], (uint32_t)__shfl_xor((int)k[a], off));
+            k[3 + a] = max(k[3 + a], (uint32_t)__shfl_xor((int)k[3 + a], off));
+        }
+    }
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int a = 0; a < 6; a++) red[a][wave] = k[a];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int a = threadIdx.x;
+        uint32_t v = red[a][0];
+        for (int w = 1; w < 16; w++) v = a < 3 ? min(v, red[a][w]) : max(v, red[a][w]);
+        sceneKeys[a] = v;
     }
 }
 
@@ -258,7 +287,8 @@ __device__ __forceinline__ int delta_at(const uint32_t* __restrict__ codes, int 
 }
 
 __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ codes, int n, int32_t* __restrict__ left,
-                                                   int32_t* __restrict__ right, int32_t* __restrict__ parent) {
+                                                   int32_t* __restrict__ right, int32_t* __restrict__ parent,
+                                                   int32_t* __restrict__ rangeLo, int32_t* __restrict__ rangeHi) {
     int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n - 1) return;
     int d = delta_at(codes, n, i, i + 1) - delta_at(codes, n, i, i - 1);
@@ -280,6 +310,8 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
     int rc = (hi == gamma + 1) ? (n - 1) + gamma + 1 : gamma + 1;
     left[i] = lc;
     right[i] = rc;
+    rangeLo[i] = lo;
+    rangeHi[i] = hi;
     parent[lc] = i;
     parent[rc] = i;
     if (i == 0) parent[0] = -1;
@@ -300,32 +332,153 @@ __device__ __forceinline__ void union_box(const float* a, const float* b, float*
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_refit(const float* __restrict__ primBox, const int32_t* __restrict__ perm, int n, int mode,
-                                                  const int32_t* __restrict__ left, const int32_t* __restrict__ right,
-                                                  const int32_t* __restrict__ parent, float* box, int32_t* flags) {
-    int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= n) return;
-    int leaf = (n - 1) + k;
-    int prim = mode == 0 ? k : perm[k];
+constexpr int kChunk = 1024;            // sorted leaves per level-1 workgroup (subtrees refit in LDS)
+constexpr int kChunk2 = 64 * kChunk;     // sorted leaves per level-2 workgroup
+// level-1 roots filed per level-2 chunk: each is a child of a node containing one of the
+// kChunk2 / kChunk + 1 chunk boundaries of that range, and at most 58 nodes contain a boundary
+constexpr int kBucketCap = 2 * (kChunk2 / kChunk + 1) * 58;
+
+__device__ __forceinline__ void load_box(const float* box, int node, float* o) {
 #pragma unroll
-    for (int a = 0; a < 6; a++) box[6 * (size_t)leaf + a] = primBox[6 * (size_t)prim + a];
-    if (n == 1) return;
-    int node = parent[leaf];
-    // a Karras tree over n <= 2^26 primitives is at most 57 levels deep; the bound only guards
-    // against a malformed parent array
+    for (int q = 0; q < 6; q++) o[q] = box[6 * (size_t)node + q];
+}
+__device__ __forceinline__ void store_box(float* box, int node, const float* o) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) box[6 * (size_t)node + q] = o[q];
+}
+
+// Level 1: one workgroup per chunk of kChunk sorted leaves. Leaf boxes, the records of the
+// chunk's interior nodes and the boxes of every interior node whose leaf range lies inside the
+// chunk are held in LDS; the leaf-to-root climbs meet through LDS arrival bits. A climb that
+// reaches a node crossing the chunk boundary files the node below it as a level-2 root.
+__global__ __launch_bounds__(kChunk) void k_refit_l1(const float* __restrict__ primBox, const int32_t* __restrict__ perm, int n,
+                                                     int mode, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+                                                     const int32_t* __restrict__ parent, const int32_t* __restrict__ rangeLo,
+                                                     const int32_t* __restrict__ rangeHi, float* __restrict__ box,
+                                                     int32_t* __restrict__ bucket, int32_t* __restrict__ bucketCount) {
+    __shared__ float lbox[kChunk * 6];
+    __shared__ float ibox[kChunk * 6];
+    __shared__ int32_t sl[kChunk], sr[kChunk], sp[kChunk];
+    __shared__ uint32_t arrive[kChunk / 32], local[kChunk / 32];
+    const int t = threadIdx.x;
+    const int cs = blockIdx.x * kChunk, ce = min(cs + kChunk, n);
+    if (t < kChunk / 32) { arrive[t] = 0; local[t] = 0; }
+    __syncthreads();
+    const int k = cs + t;
+    if (k < ce) {
+        const int prim = mode == 0 ? k : perm[k];
+        float b[6];
+        load_box(primBox, prim, b);
+#pragma unroll
+        for (int q = 0; q < 6; q++) lbox[6 * t + q] = b[q];
+        store_box(box, (n - 1) + k, b);
+    }
+    if (k < ce && k < n - 1) {  // interior node k (an interior node is an end of its own range)
+        sl[t] = left[k]; sr[t] = right[k]; sp[t] = parent[k];
+        if (rangeLo[k] >= cs && rangeHi[k] < ce) atomicOr(&local[t >> 5], 1u << (t & 31));
+    }
+    __syncthreads();
+    if (k < ce && n > 1) {
+        int cur = (n - 1) + k;
+        int node = parent[cur];
+        for (int step = 0; step < 128; step++) {
+            const int j = node - cs;
+            const bool isLocal = node >= cs && node < ce && node < n - 1 && ((local[j >> 5] >> (j & 31)) & 1u);
+            if (!isLocal) {
+                if (node >= 0) {
+                    int b = cs / kChunk2;
+                    int slot = atomicAdd(&bucketCount[b], 1);
+                    if (slot < kBucketCap) bucket[(size_t)b * kBucketCap + slot] = cur;
+                }
+                break;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            uint32_t old = atomicOr(&arrive[j >> 5], 1u << (j & 31));
+            if (!((old >> (j & 31)) & 1u)) break;  // first arrival
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            float a[6], c[6], o[6];
+            const int l = sl[j], r = sr[j];
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                a[q] = l >= n - 1 ? lbox[6 * (l - (n - 1) - cs) + q] : ibox[6 * (l - cs) + q];
+                c[q] = r >= n - 1 ? lbox[6 * (r - (n - 1) - cs) + q] : ibox[6 * (r - cs) + q];
+            }
+            union_box(a, c, o);
+#pragma unroll
+            for (int q = 0; q < 6; q++) ibox[6 * j + q] = o[q];
+            if (node == 0) break;
+            cur = node;
+            node = sp[j];
+        }
+    }
+    __syncthreads();
+    if (k < ce && k < n - 1 && ((local[t >> 5] >> (t & 31)) & 1u)) {
+        float o[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) o[q] = ibox[6 * t + q];
+        store_box(box, k, o);
+    }
+}
+
+// Level 2: one workgroup per chunk of kChunk2 leaves climbs from its level-1 roots through the
+// nodes inside the chunk (arrival bits in LDS, boxes in global memory, workgroup-scope fences).
+__global__ __launch_bounds__(1024) void k_refit_l2(int n, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+                                                   const int32_t* __restrict__ parent, const int32_t* __restrict__ rangeLo,
+                                                   const int32_t* __restrict__ rangeHi, float* box, const int32_t* __restrict__ bucket,
+                                                   const int32_t* __restrict__ bucketCount, int32_t* __restrict__ roots,
+                                                   int32_t* __restrict__ rootCount) {
+    __shared__ uint32_t arrive[kChunk2 / 32];
+    const int cs = blockIdx.x * kChunk2, ce = min(cs + kChunk2, n);
+    for (int w = threadIdx.x; w < kChunk2 / 32; w += 1024) arrive[w] = 0;
+    __syncthreads();
+    const int cnt = min(bucketCount[blockIdx.x], kBucketCap);
+    for (int e = threadIdx.x; e < cnt; e += 1024) {
+        int cur = bucket[(size_t)blockIdx.x * kBucketCap + e];
+        int node = parent[cur];
+        for (int step = 0; step < 128 && node >= 0 && node < n - 1; step++) {
+            if (rangeLo[node] < cs || rangeHi[node] >= ce) {
+                int slot = atomicAdd(rootCount, 1);
+                roots[slot] = cur;
+                break;
+            }
+            const int j = node - cs;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            uint32_t old = atomicOr(&arrive[j >> 5], 1u << (j & 31));
+            if (!((old >> (j & 31)) & 1u)) break;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            float a[6], c[6], o[6];
+            load_box(box, left[node], a);
+            load_box(box, right[node], c);
+            union_box(a, c, o);
+            store_box(box, node, o);
+            if (node == 0) break;
+            cur = node;
+            node = parent[node];
+        }
+    }
+}
+
+// Level 3: nodes whose leaf range crosses a level-2 chunk boundary, climbed from the level-2 roots
+// with device-scope ordering.
+__global__ __launch_bounds__(kBlock) void k_refit_global(int n, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+                                                         const int32_t* __restrict__ parent, float* box, int32_t* flags,
+                                                         const int32_t* __restrict__ roots, const int32_t* __restrict__ rootCount) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= *rootCount) return;
+    int node = parent[roots[t]];
     for (int step = 0; step < 128 && node >= 0 && node < n - 1; step++) {
         __threadfence();
         int old = __hip_atomic_fetch_add(&flags[node], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) return;  // first arrival: the sibling subtree is not finished yet
+        if (old == 0) return;
         __threadfence();
-        float a[6], b[6], o[6];
+        float a[6], c[6], o[6];
         int l = left[node], r = right[node];
 #pragma unroll
         for (int q = 0; q < 6; q++) {
             a[q] = __hip_atomic_load(&box[6 * (size_t)l + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            b[q] = __hip_atomic_load(&box[6 * (size_t)r + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            c[q] = __hip_atomic_load(&box[6 * (size_t)r + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        union_box(a, b, o);
+        union_box(a, c, o);
 #pragma unroll
         for (int q = 0; q < 6; q++) __hip_atomic_store(&box[6 * (size_t)node + q], o[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (node == 0) return;
@@ -385,6 +538,11 @@ hipError_t bvh_reserve(BvhBuffers& b, int n) {
     TRY(dalloc(&b.left, n)); TRY(dalloc(&b.right, n));
     TRY(dalloc(&b.parent, 2 * (size_t)n));
     TRY(dalloc(&b.flags, n));
+    TRY(dalloc(&b.rangeLo, n)); TRY(dalloc(&b.rangeHi, n));
+    TRY(dalloc(&b.roots, n));
+    TRY(dalloc(&b.rootCount, (n + kChunk2 - 1) / kChunk2 + 1));
+    TRY(dalloc(&b.bucket, (size_t)((n + kChunk2 - 1) / kChunk2) * kBucketCap));
+    TRY(dalloc(&b.partial, 6 * (size_t)blocks_for(n)));
     TRY(dalloc(&b.sceneKeys, 8));
     TRY(dalloc(&b.nodes, n));
     TRY(dalloc(&b.tris, n));
@@ -395,7 +553,7 @@ hipError_t bvh_reserve(BvhBuffers& b, int n) {
 
 void bvh_release(BvhBuffers& b) {
     void* ps[] = {b.primBox, b.box, b.keysA, b.keysB, b.valsA, b.valsB, b.counts, b.digitBase, b.left, b.right,
-                  b.parent, b.flags, b.sceneKeys, b.nodes, b.tris};
+                  b.parent, b.flags, b.sceneKeys, b.nodes, b.tris, b.rangeLo, b.rangeHi, b.roots, b.rootCount, b.bucket, b.partial};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     b = BvhBuffers();
@@ -406,8 +564,8 @@ hipError_t bvh_enqueue(BvhBuffers& b, const float* soup, int n, int mode, bool p
     b.n = n;
     b.tiles = (n + kTile - 1) / kTile;
     if (ev) (void)hipEventRecord(ev->e[0], s);
-    k_init_scene_keys<<<1, 64, 0, s>>>(b.sceneKeys);
-    k_prim_bounds<<<blocks_for(n), kBlock, 0, s>>>(soup, n, b.primBox, b.sceneKeys);
+    k_prim_bounds<<<blocks_for(n), kBlock, 0, s>>>(soup, n, b.primBox, b.partial);
+    k_reduce_scene_keys<<<1, 1024, 0, s>>>(b.partial, blocks_for(n), b.sceneKeys);
     k_morton<<<blocks_for(n), kBlock, 0, s>>>(b.primBox, n, b.sceneKeys, b.keysA, b.valsA);
     if (ev) (void)hipEventRecord(ev->e[1], s);
     uint32_t* kin = b.keysA; uint32_t* kout = b.keysB;
@@ -426,11 +584,23 @@ hipError_t bvh_enqueue(BvhBuffers& b, const float* soup, int n, int mode, bool p
     if (n > 1) {
         k_fill_i32<<<blocks_for(n - 1), kBlock, 0, s>>>(b.flags, n - 1, 0);
         k_fill_i32<<<blocks_for(2 * n - 1), kBlock, 0, s>>>(b.parent, 2 * n - 1, -1);
-        k_karras<<<blocks_for(n - 1), kBlock, 0, s>>>(b.keysA, n, b.left, b.right, b.parent);
+        k_karras<<<blocks_for(n - 1), kBlock, 0, s>>>(b.keysA, n, b.left, b.right, b.parent, b.rangeLo, b.rangeHi);
     } else {
         k_fill_i32<<<1, kBlock, 0, s>>>(b.parent, 1, -1);
     }
-    k_refit<<<blocks_for(n), kBlock, 0, s>>>(b.primBox, b.valsA, n, mode, b.left, b.right, b.parent, b.box, b.flags);
+    const int chunks = (n + kChunk - 1) / kChunk, chunks2 = (n + kChunk2 - 1) / kChunk2;
+    k_fill_i32<<<blocks_for(chunks2 + 1), kBlock, 0, s>>>(b.rootCount, chunks2 + 1, 0);  // level-3 count + buckets
+    k_refit_l1<<<chunks, kChunk, 0, s>>>(b.primBox, b.valsA, n, mode, b.left, b.right, b.parent, b.rangeLo, b.rangeHi, b.box,
+                                         b.bucket, b.rootCount + 1);
+    if (chunks > 1)
+        k_refit_l2<<<chunks2, 1024, 0, s>>>(n, b.left, b.right, b.parent, b.rangeLo, b.rangeHi, b.box, b.bucket,
+                                            b.rootCount + 1, b.roots, b.rootCount);
+    if (chunks2 > 1) {
+        // level-3 roots: at most two per level-2-crossing node, and each of the chunks2 - 1
+        // boundaries is crossed by at most 58 nodes
+        const long bound = std::min<long>((long)n, 2L * chunks2 * 58);
+        k_refit_global<<<blocks_for(bound), kBlock, 0, s>>>(n, b.left, b.right, b.parent, b.box, b.flags, b.roots, b.rootCount);
+    }
     if (ev) (void)hipEventRecord(ev->e[3], s);
     if (pack) {
         if (n > 1) k_pack_nodes<<<blocks_for(n - 1), kBlock, 0, s>>>(b.box, b.left, b.right, n, b.nodes);

@@ -4,19 +4,19 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <sstream>
 #include <string>
 #include <vector>
 
 #include "image_io.h"
 #include "lts_internal.h"
 #include "scene_host.h"
-
-namespace lts {
-hipError_t upload_perlin_table(const int* table512);
-}
 
 using namespace lts;
 
@@ -73,7 +73,17 @@ struct lts_ctx {
     float* rgb = nullptr;
     size_t filmCap = 0;
     unsigned long long* counters = nullptr;
+    int* perm = nullptr;  // Perlin permutation (512 ints)
     hipEvent_t ev[8] = {};
+    // wavefront integrator
+    WfBuffers wf;
+    hipGraphExec_t wfExec = nullptr;  // captured pass for (wfKeyPaths, wfKeyBounce, wfKeyTune, wfKeyBins)
+    const void* wfKeyPaths = nullptr;
+    int wfKeyBounce = -1, wfKeyBins = -1;
+    WfTuning wfKeyTune;
+    WfParams wfParams{};                  // host copies of the per-pass parameters (stable for async H2D)
+    std::vector<WfSceneSlot> wfSlots;
+    std::map<std::string, WfTuning> tuned;  // autotuner results of this context
 };
 
 namespace {
@@ -158,6 +168,294 @@ int run_build(lts_ctx* c, BvhBuffers& b, const float* dsoup, int n, int mode, bo
     return LTS_OK;
 }
 
+int check_render_params(lts_ctx* c, const lts_render_params* p) {
+    if (p->spp < 1 || p->max_bounce < 0 || p->spp_per_pass < 0) return fail(c, LTS_E_ARG, "bad render parameters");
+    if (p->stratified) {
+        int s = (int)std::floor(std::sqrt((float)p->spp));
+        if (s * s != p->spp) return fail(c, LTS_E_ARG, "stratified film sampling needs a square sample count");
+    }
+    if (p->integrator == LTS_INTEGRATOR_WAVEFRONT) {
+        if (p->max_bounce > kWfMaxBounce) return fail(c, LTS_E_ARG, "wavefront max_bounce above 64");
+        if (p->trace_block != 0 && p->trace_block != 64 && p->trace_block != 128 && p->trace_block != 256)
+            return fail(c, LTS_E_ARG, "trace_block must be 64, 128 or 256");
+        if (p->persistent_blocks < 0 || p->persistent_blocks > (1 << 20)) return fail(c, LTS_E_ARG, "bad persistent_blocks");
+    } else if (p->integrator != LTS_INTEGRATOR_MEGAKERNEL) {
+        return fail(c, LTS_E_ARG, "unsupported integrator");
+    }
+    return LTS_OK;
+}
+
+int ensure_film(lts_ctx* c, size_t npix) {
+    if (npix <= c->filmCap) return LTS_OK;
+    if (c->film) (void)hipFree(c->film);
+    if (c->rgb) (void)hipFree(c->rgb);
+    c->film = c->rgb = nullptr;
+    c->filmCap = 0;
+    HIPCHK(c, hipMalloc((void**)&c->film, npix * 4 * sizeof(float)));
+    HIPCHK(c, hipMalloc((void**)&c->rgb, npix * 3 * sizeof(float)));
+    c->filmCap = npix;
+    return LTS_OK;
+}
+
+int ensure_samples(lts_ctx* c, size_t floats) {
+    if (floats <= c->sampleCap) return LTS_OK;
+    if (c->sampleL) (void)hipFree(c->sampleL);
+    c->sampleL = nullptr;
+    c->sampleCap = 0;
+    HIPCHK(c, hipMalloc((void**)&c->sampleL, floats * sizeof(float)));
+    c->sampleCap = floats;
+    return LTS_OK;
+}
+
+bool same_tuning(const WfTuning& a, const WfTuning& b) {
+    return a.traceBlock == b.traceBlock && a.persistentBlocks == b.persistentBlocks && a.sweepBlocks == b.sweepBlocks &&
+           a.sortMaterials == b.sortMaterials && a.useGraph == b.useGraph;
+}
+
+// Runs the pass whose parameters are resident in c->wf: replays the captured graph, capturing it
+// first when the buffers, bounce count or tuning changed since the last capture.
+int wf_run(lts_ctx* c, const WfTuning& t, int maxBounce) {
+    hipStream_t s = c->stream;
+    if (!t.useGraph) {
+        HIPCHK(c, wf_enqueue_pass(c->wf, t, maxBounce, s));
+        return LTS_OK;
+    }
+    if (!c->wfExec || c->wfKeyPaths != c->wf.paths || c->wfKeyBounce != maxBounce || c->wfKeyBins != c->wf.matBins ||
+        !same_tuning(c->wfKeyTune, t)) {
+        if (c->wfExec) (void)hipGraphExecDestroy(c->wfExec);
+        c->wfExec = nullptr;
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipError_t e = wf_enqueue_pass(c->wf, t, maxBounce, s);
+        hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&c->wfExec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        HIPCHK(c, e);
+        c->wfKeyPaths = c->wf.paths;
+        c->wfKeyBounce = maxBounce;
+        c->wfKeyBins = c->wf.matBins;
+        c->wfKeyTune = t;
+    }
+    HIPCHK(c, hipGraphLaunch(c->wfExec, s));
+    return LTS_OK;
+}
+
+// Writes the pass parameters for samples [s0, s1) of every scene to the device.
+int wf_upload_pass(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_render_params* p, int s0, int s1,
+                   int64_t* paths) {
+    const int n = (int)scenes.size();
+    c->wfSlots.resize(n);
+    int64_t base = 0;
+    for (int i = 0; i < n; i++) {
+        const CameraDesc& cam = scenes[i]->scene.view.cam;
+        WfSceneSlot& sl = c->wfSlots[i];
+        sl.pathBase = base;
+        sl.width = cam.width; sl.height = cam.height;
+        sl.spp = p->spp; sl.stratified = p->stratified; sl.seed = p->seed; sl.s0 = s0;
+        sl.sampleL = c->sampleL + 3 * base;
+        base += (int64_t)cam.width * cam.height * (s1 - s0);
+    }
+    c->wfParams.nPaths = base;
+    c->wfParams.nScenes = n;
+    c->wfParams.maxBounce = p->max_bounce;
+    c->wfParams.rrStart = p->rr_start;
+    HIPCHK(c, hipMemcpyAsync(c->wf.params, &c->wfParams, sizeof(WfParams), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->wf.slots, c->wfSlots.data(), n * sizeof(WfSceneSlot), hipMemcpyHostToDevice, c->stream));
+    *paths = base;
+    return LTS_OK;
+}
+
+std::string tune_key(const std::vector<lts_ctx*>& scenes, const lts_render_params* p, int pass) {
+    std::ostringstream k;
+    k << "gfx950";
+    for (lts_ctx* x : scenes) k << ":" << x->scene.view.nPrim << "/" << x->scene.view.cam.width << "x" << x->scene.view.cam.height;
+    k << ":pass" << pass << ":b" << p->max_bounce << ":rr" << p->rr_start;
+    return k.str();
+}
+
+bool tune_cache_read(const std::string& key, WfTuning* t) {
+    const char* path = std::getenv("LTS_TUNE_CACHE");
+    if (!path || !*path) return false;
+    std::ifstream in(path);
+    std::string k;
+    int tb, pb, so;
+    bool found = false;
+    while (in >> k >> tb >> pb >> so)
+        if (k == key && (tb == 64 || tb == 128 || tb == 256) && pb > 0) {
+            t->traceBlock = tb; t->persistentBlocks = pb; t->sortMaterials = so != 0;
+            found = true;  // the last entry for a key wins
+        }
+    return found;
+}
+
+void tune_cache_append(const std::string& key, const WfTuning& t) {
+    const char* path = std::getenv("LTS_TUNE_CACHE");
+    if (!path || !*path) return;
+    std::ofstream out(path, std::ios::app);
+    out << key << " " << t.traceBlock << " " << t.persistentBlocks << " " << t.sortMaterials << "\n";
+}
+
+// Mean device time of one pass with tuning t (one untimed warm-up, then `reps` passes).
+int time_pass(lts_ctx* c, const WfTuning& t, int maxBounce, int reps, double* ms) {
+    WfTuning u = t;
+    u.useGraph = 0;
+    int rc = wf_run(c, u, maxBounce);
+    if (rc != LTS_OK) return rc;
+    HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+    for (int r = 0; r < reps; r++)
+        if ((rc = wf_run(c, u, maxBounce)) != LTS_OK) return rc;
+    HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[6]));
+    *ms = ms_between(c->ev[5], c->ev[6]) / reps;
+    return LTS_OK;
+}
+
+// Coordinate search over the traversal block size, the persistent grid and material sorting, on
+// a pass of at most two samples per pixel (the parameters of that pass are uploaded here).
+int autotune(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_render_params* p, WfTuning* best) {
+    const int trialSamples = std::min(p->spp, 2);
+    int64_t paths = 0;
+    int rc = wf_upload_pass(c, scenes, p, 0, trialSamples, &paths);
+    if (rc != LTS_OK) return rc;
+    WfTuning cur = *best;
+    double bestMs = 0;
+    if ((rc = time_pass(c, cur, p->max_bounce, 2, &bestMs)) != LTS_OK) return rc;
+    auto consider = [&](WfTuning t) -> int {
+        double ms = 0;
+        int r = time_pass(c, t, p->max_bounce, 2, &ms);
+        if (r != LTS_OK) return r;
+        if (ms < bestMs) { bestMs = ms; cur = t; }
+        return LTS_OK;
+    };
+    for (int tb : {64, 128, 256}) {
+        if (tb == cur.traceBlock) continue;
+        WfTuning t = cur; t.traceBlock = tb;
+        if ((rc = consider(t)) != LTS_OK) return rc;
+    }
+    const WfTuning afterBlock = cur;
+    for (int pb : {512, 1024, 2048, 4096, 8192}) {
+        if (pb == afterBlock.persistentBlocks) continue;
+        WfTuning t = afterBlock; t.persistentBlocks = pb;
+        if ((rc = consider(t)) != LTS_OK) return rc;
+    }
+    {
+        WfTuning t = cur; t.sortMaterials = !cur.sortMaterials;
+        if ((rc = consider(t)) != LTS_OK) return rc;
+    }
+    best->traceBlock = cur.traceBlock;
+    best->persistentBlocks = cur.persistentBlocks;
+    best->sortMaterials = cur.sortMaterials;
+    return LTS_OK;
+}
+
+// The wavefront integrator over one or more scenes, executed on c's stream and buffers.
+int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_render_params* p, float* const* rgb_out,
+                     float* sample_rgb, lts_stats* st) {
+    hipStream_t s = c->stream;
+    size_t totalPix = 0;
+    int maxMats = 1;
+    for (lts_ctx* x : scenes) {
+        totalPix += (size_t)x->scene.view.cam.width * x->scene.view.cam.height;
+        maxMats = std::max(maxMats, (int)x->scene.host.mats.size());
+    }
+    int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, ((size_t)1 << 24) / totalPix);
+    if (sample_rgb) pass = p->spp;
+    pass = std::min(pass, p->spp);
+    const size_t cap = totalPix * (size_t)pass;
+    if (cap > (size_t)INT32_MAX) return fail(c, LTS_E_ARG, "pass holds more than 2^31 paths");
+    const bool realloc = cap > c->wf.cap || (int)scenes.size() > c->wf.maxScenes;
+    HIPCHK(c, wf_reserve(c->wf, cap, (int)scenes.size()));
+    if (realloc && c->wfExec) {
+        (void)hipGraphExecDestroy(c->wfExec);
+        c->wfExec = nullptr;
+    }
+    c->wf.matBins = std::min(maxMats, kWfMaxMatBins - 1) + 1;
+    int rc = ensure_samples(c, cap * 3);
+    if (rc != LTS_OK) return rc;
+    std::vector<SceneView> views;
+    for (lts_ctx* x : scenes) {
+        views.push_back(x->scene.view);
+        if ((rc = ensure_film(x, (size_t)x->scene.view.cam.width * x->scene.view.cam.height)) != LTS_OK)
+            return fail(c, rc, x->err);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->wf.scenes, views.data(), views.size() * sizeof(SceneView), hipMemcpyHostToDevice, s));
+    for (lts_ctx* x : scenes)
+        HIPCHK(c, launch_film_clear(x->film, x->scene.view.cam.width * x->scene.view.cam.height, s));
+
+    WfTuning t;
+    double tuneMs = 0;
+    if (p->autotune) {
+        const std::string key = tune_key(scenes, p, pass);
+        auto it = c->tuned.find(key);
+        if (it != c->tuned.end()) {
+            t = it->second;
+        } else if (tune_cache_read(key, &t)) {
+            c->tuned[key] = t;
+        } else {
+            HIPCHK(c, hipEventRecord(c->ev[3], s));
+            if ((rc = autotune(c, scenes, p, &t)) != LTS_OK) return rc;
+            HIPCHK(c, hipEventRecord(c->ev[4], s));
+            HIPCHK(c, hipEventSynchronize(c->ev[4]));
+            tuneMs = ms_between(c->ev[3], c->ev[4]);
+            c->tuned[key] = t;
+            tune_cache_append(key, t);
+        }
+    }
+    if (p->trace_block) t.traceBlock = p->trace_block;
+    if (p->persistent_blocks) t.persistentBlocks = p->persistent_blocks;
+    if (p->sort_materials >= 0) t.sortMaterials = p->sort_materials != 0;
+    if (p->use_graph >= 0) t.useGraph = p->use_graph != 0;
+
+    double renderMs = 0, resolveMs = 0;
+    int64_t launches = 0;
+    for (int s0 = 0; s0 < p->spp; s0 += pass) {
+        const int s1 = std::min(p->spp, s0 + pass);
+        int64_t paths = 0;
+        if ((rc = wf_upload_pass(c, scenes, p, s0, s1, &paths)) != LTS_OK) return rc;
+        HIPCHK(c, hipEventRecord(c->ev[5], s));
+        if ((rc = wf_run(c, t, p->max_bounce)) != LTS_OK) return rc;
+        HIPCHK(c, hipEventRecord(c->ev[6], s));
+        for (size_t i = 0; i < scenes.size(); i++) {
+            RenderLaunch rl{};
+            rl.spp = p->spp; rl.maxBounce = p->max_bounce; rl.rrStart = p->rr_start; rl.seed = p->seed;
+            rl.stratified = p->stratified; rl.s0 = s0; rl.s1 = s1;
+            rl.sampleL = c->wfSlots[i].sampleL; rl.film = scenes[i]->film;
+            HIPCHK(c, launch_film_gather(views[i], rl, s));
+        }
+        HIPCHK(c, hipEventRecord(c->ev[7], s));
+        HIPCHK(c, hipEventSynchronize(c->ev[7]));
+        renderMs += ms_between(c->ev[5], c->ev[6]);
+        resolveMs += ms_between(c->ev[6], c->ev[7]);
+        launches++;
+        if (sample_rgb)
+            HIPCHK(c, hipMemcpy(sample_rgb, c->sampleL, (size_t)paths * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    for (size_t i = 0; i < scenes.size(); i++) {
+        lts_ctx* x = scenes[i];
+        HIPCHK(c, launch_get_render(views[i], x->film, x->rgb, s));
+        if (rgb_out && rgb_out[i])
+            HIPCHK(c, hipMemcpyAsync(rgb_out[i], x->rgb, (size_t)views[i].cam.width * views[i].cam.height * 3 * sizeof(float),
+                                     hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->render_ms = renderMs;
+        st->resolve_ms = resolveMs;
+        st->launches = launches;
+        st->kernel_avg_ms = launches ? renderMs / launches : 0;
+        st->samples = (int64_t)totalPix * p->spp;
+        st->msamples_per_s = (renderMs + resolveMs) > 0 ? st->samples / ((renderMs + resolveMs) * 1e3) : 0;
+        st->trace_block = t.traceBlock;
+        st->persistent_blocks = t.persistentBlocks;
+        st->sort_materials = t.sortMaterials;
+        st->use_graph = t.useGraph;
+        st->autotune_ms = tuneMs;
+    }
+    return LTS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -179,7 +477,8 @@ int lts_create(lts_ctx** out, int dev) {
         if (hipEventCreate(&e) != hipSuccess) { delete c; return LTS_E_HIP; }
     int table[512];
     for (int i = 0; i < 512; i++) table[i] = kPerm256[i & 255];
-    if (upload_perlin_table(table) != hipSuccess) { delete c; return LTS_E_HIP; }
+    if (hipMalloc((void**)&c->perm, sizeof(table)) != hipSuccess ||
+        hipMemcpy(c->perm, table, sizeof(table), hipMemcpyHostToDevice) != hipSuccess) { lts_destroy(c); return LTS_E_HIP; }
     if (hipMalloc((void**)&c->counters, 2 * sizeof(unsigned long long)) != hipSuccess) { delete c; return LTS_E_HIP; }
     *out = c;
     return LTS_OK;
@@ -191,11 +490,13 @@ void lts_destroy(lts_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     free_scene(c->scene);
     bvh_release(c->build);
-    void* ps[] = {c->buildSoup, c->sampleL, c->film, c->rgb, c->counters};
+    void* ps[] = {c->buildSoup, c->sampleL, c->film, c->rgb, c->counters, c->perm};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->wfExec) (void)hipGraphExecDestroy(c->wfExec);
+    wf_release(c->wf);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -261,6 +562,7 @@ int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* s
     v.tris = S.bvh.tris; v.nodes = S.bvh.nodes; v.shade = S.shade; v.mats = S.mats; v.texs = S.texs;
     v.mips = S.mips; v.mipData = S.mipData; v.lights = S.lights; v.lfunc = S.lfunc; v.lcdf = S.lcdf;
     v.funcInt = H.funcInt; v.nLights = (int)H.lights.size(); v.nPrim = n; v.infiniteLight = H.infiniteLight;
+    v.perm = c->perm;
     v.cam = H.cam;
     std::memcpy(v.filter, H.filter, sizeof(v.filter));
     S.loaded = true;
@@ -296,7 +598,11 @@ int lts_bvh_build(lts_ctx* c, const float* tri, int32_t n, int32_t mode, lts_bvh
 }
 
 int lts_bvh_build_device(lts_ctx* c, const float* dtri, int32_t n, int32_t mode, int32_t repeat, lts_bvh_timing* t) {
-    if (!c || !dtri || n < 1 || n > kMaxPrims || (mode != 0 && mode != 1) || repeat < 1) return fail(c, LTS_E_ARG, "bad build arguments");
+    if (!c || n < 1 || n > kMaxPrims || (mode != 0 && mode != 1) || repeat < 1) return fail(c, LTS_E_ARG, "bad build arguments");
+    if (!dtri) {
+        if (!c->buildSoup || c->buildSoupCap < 9 * (size_t)n * sizeof(float)) return fail(c, LTS_E_STATE, "no uploaded soup of that size");
+        dtri = c->buildSoup;
+    }
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, bvh_reserve(c->build, n));
     lts_bvh_timing acc{};
@@ -323,36 +629,21 @@ int lts_bvh_export_last(lts_ctx* c, lts_bvh_export* out) {
 int lts_render(lts_ctx* c, const lts_render_params* p, float* rgb_out, float* sample_rgb, lts_stats* st) {
     if (!c || !p) return fail(c, LTS_E_ARG, "null argument");
     if (!c->scene.loaded) return fail(c, LTS_E_STATE, "no scene loaded");
-    if (p->spp < 1 || p->max_bounce < 0 || p->integrator != LTS_INTEGRATOR_MEGAKERNEL)
-        return fail(c, LTS_E_ARG, "unsupported render parameters");
-    if (p->stratified) {
-        int s = (int)std::floor(std::sqrt((float)p->spp));
-        if (s * s != p->spp) return fail(c, LTS_E_ARG, "stratified film sampling needs a square sample count");
-    }
+    int rc = check_render_params(c, p);
+    if (rc != LTS_OK) return rc;
     HIPCHK(c, hipSetDevice(c->device));
+    if (p->integrator == LTS_INTEGRATOR_WAVEFRONT) {
+        std::vector<lts_ctx*> one{c};
+        return render_wavefront(c, one, p, &rgb_out, sample_rgb, st);
+    }
     const SceneView& sv = c->scene.view;
     const int W = sv.cam.width, H = sv.cam.height;
     const size_t npix = (size_t)W * H;
     int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, std::min<size_t>((size_t)p->spp, (size_t)(1u << 28) / (npix * 12)));
     if (sample_rgb) pass = p->spp;  // all samples are returned: one pass
     pass = std::min(pass, p->spp);
-    size_t need = npix * (size_t)pass * 3;
-    if (need > c->sampleCap) {
-        if (c->sampleL) (void)hipFree(c->sampleL);
-        c->sampleL = nullptr;
-        c->sampleCap = 0;
-        HIPCHK(c, hipMalloc((void**)&c->sampleL, need * sizeof(float)));
-        c->sampleCap = need;
-    }
-    if (npix > c->filmCap) {
-        if (c->film) (void)hipFree(c->film);
-        if (c->rgb) (void)hipFree(c->rgb);
-        c->film = c->rgb = nullptr;
-        c->filmCap = 0;
-        HIPCHK(c, hipMalloc((void**)&c->film, npix * 4 * sizeof(float)));
-        HIPCHK(c, hipMalloc((void**)&c->rgb, npix * 3 * sizeof(float)));
-        c->filmCap = npix;
-    }
+    if ((rc = ensure_samples(c, npix * (size_t)pass * 3)) != LTS_OK) return rc;
+    if ((rc = ensure_film(c, npix)) != LTS_OK) return rc;
     hipStream_t s = c->stream;
     HIPCHK(c, launch_film_clear(c->film, (int)npix, s));
     RenderLaunch rl{};
@@ -387,6 +678,45 @@ int lts_render(lts_ctx* c, const lts_render_params* p, float* rgb_out, float* sa
         st->samples = (int64_t)npix * p->spp;
         st->msamples_per_s = (renderMs + resolveMs) > 0 ? st->samples / ((renderMs + resolveMs) * 1e3) : 0;
     }
+    return LTS_OK;
+}
+
+int lts_render_batch(lts_ctx* const* ctxs, int32_t n, const lts_render_params* p, float* const* rgb_out, lts_stats* st) {
+    if (!ctxs || n < 1 || !ctxs[0] || !p) return LTS_E_ARG;
+    lts_ctx* c = ctxs[0];
+    if (p->integrator != LTS_INTEGRATOR_WAVEFRONT) return fail(c, LTS_E_ARG, "batched rendering uses the wavefront integrator");
+    std::vector<lts_ctx*> scenes;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i]) return fail(c, LTS_E_ARG, "null context in batch");
+        if (ctxs[i]->device != c->device) return fail(c, LTS_E_ARG, "batch contexts on different devices");
+        if (!ctxs[i]->scene.loaded) return fail(c, LTS_E_STATE, "batch context without a scene");
+        for (lts_ctx* x : scenes)
+            if (x == ctxs[i]) return fail(c, LTS_E_ARG, "context repeated in batch");
+        scenes.push_back(ctxs[i]);
+    }
+    int rc = check_render_params(c, p);
+    if (rc != LTS_OK) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    for (lts_ctx* x : scenes) HIPCHK(c, hipStreamSynchronize(x->stream));
+    return render_wavefront(c, scenes, p, rgb_out, nullptr, st);
+}
+
+int lts_device_copy_benchmark(lts_ctx* c, uint64_t bytes, int32_t repeat, double* ms) {
+    if (!c || bytes < 16 || repeat < 1 || !ms) return fail(c, LTS_E_ARG, "bad copy arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(c, hipMalloc(&a, bytes));
+    hipError_t e = hipMalloc(&b, bytes);
+    if (e != hipSuccess) { (void)hipFree(a); return fail(c, LTS_E_NOMEM, "copy buffers"); }
+    e = launch_copy(b, a, bytes, c->stream);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(c->ev[5], c->stream);
+    for (int r = 0; e == hipSuccess && r < repeat; r++) e = launch_copy(b, a, bytes, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[6], c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev[6]);
+    if (e == hipSuccess) *ms = ms_between(c->ev[5], c->ev[6]) / repeat;
+    (void)hipFree(a);
+    (void)hipFree(b);
+    HIPCHK(c, e);
     return LTS_OK;
 }
 

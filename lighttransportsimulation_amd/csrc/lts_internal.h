@@ -24,6 +24,12 @@ struct BvhBuffers {
     int32_t* right = nullptr;     // n-1
     int32_t* parent = nullptr;    // 2n-1
     int32_t* flags = nullptr;     // n-1
+    int32_t* rangeLo = nullptr;   // n-1 first sorted leaf under interior i
+    int32_t* rangeHi = nullptr;   // n-1 last sorted leaf under interior i
+    int32_t* roots = nullptr;     // phase-1 subtree roots handed to the global refit
+    int32_t* rootCount = nullptr;  // [0] level-3 roots, [1 + c] level-1 roots filed for 64K-leaf chunk c
+    int32_t* bucket = nullptr;     // level-1 roots per 64K-leaf chunk
+    uint32_t* partial = nullptr;   // per-block scene-bound keys
     uint32_t* sceneKeys = nullptr;// 6 ordered keys of the scene bounds
     BvhNode* nodes = nullptr;     // n-1 (or 1 for n == 1)
     TriLeaf* tris = nullptr;      // n
@@ -59,5 +65,61 @@ hipError_t launch_megakernel(const SceneView& sv, const RenderLaunch& rl, hipStr
 hipError_t launch_film_gather(const SceneView& sv, const RenderLaunch& rl, hipStream_t stream);
 hipError_t launch_film_clear(float* film, int npix, hipStream_t stream);
 hipError_t launch_get_render(const SceneView& sv, const float* film, float* rgb, hipStream_t stream);
+// plain copy of `bytes` (a multiple of 16) with 16-byte vector loads and stores
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
+// ---------------------------------------------------------------- wavefront tracer (wavefront.hip)
+constexpr int kWfAlive = 1, kWfSpecular = 2;
+constexpr int kWfCountsPerBounce = 8;  // queue, shadow, mis counts; extend, shadow, mis work counters
+constexpr int kWfMaxMatBins = 64;
+constexpr int kWfMaxBounce = 64;
+
+struct alignas(16) WfPath { float beta[3]; float L[3]; int32_t pixel, sample, bounce, flags, scene, pad; };
+struct alignas(16) WfRay { float o[3]; float d[3]; float pad[2]; };
+struct alignas(16) WfHit { int32_t leaf; float u, v, invDet; };
+struct alignas(16) WfNee { float nb[3]; float lpdf; float p1[3]; float p2[3]; int32_t used; int32_t pad; };
+struct alignas(16) WfShadow { float o[3]; float d[3]; int32_t path; int32_t pad; float c[3]; float pad2; };
+struct alignas(16) WfMis { float o[3]; float d[3]; int32_t path; int32_t light; float f[3]; float weight; float pdf; int32_t pad[3]; };
+
+// Read by the kernels from device memory, so a captured pass graph can be replayed for every pass.
+struct WfParams { int64_t nPaths; int32_t nScenes, maxBounce, rrStart, pad; };
+struct WfSceneSlot {
+    int64_t pathBase;  // first path of this scene in the pass
+    int32_t width, height, spp, stratified;
+    uint32_t seed;
+    int32_t s0;        // first sample index of the pass
+    float* sampleL;    // [pass samples][H][W][3]
+};
+
+struct WfTuning {
+    int traceBlock = 128;         // block size of the persistent traversal kernels (64, 128, 256)
+    int persistentBlocks = 2048;  // grid of the traversal kernels
+    int sweepBlocks = 2048;       // grid of the generate / shade / finalize sweeps
+    int sortMaterials = 0;        // counting-sort the shade queue by material
+    int useGraph = 1;             // capture the pass in a hipGraph and replay it
+};
+
+struct WfBuffers {
+    size_t cap = 0;       // paths per pass
+    int maxScenes = 0;
+    int matBins = 2;
+    SceneView* scenes = nullptr;
+    WfParams* params = nullptr;
+    WfSceneSlot* slots = nullptr;
+    WfPath* paths = nullptr;
+    WfRay* rays = nullptr;
+    WfHit* hits = nullptr;
+    WfNee* nee = nullptr;
+    int32_t* queue[2] = {nullptr, nullptr};
+    int32_t* sorted = nullptr;
+    WfShadow* shadows = nullptr;
+    WfMis* mis = nullptr;
+    int32_t* counts = nullptr;
+};
+
+hipError_t wf_reserve(WfBuffers& b, size_t cap, int maxScenes);
+void wf_release(WfBuffers& b);
+// Enqueues one pass: generate, then per bounce extend / (sort) / shade / shadow / mis / finalize.
+hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce, hipStream_t stream);
 
 }  // namespace lts

@@ -101,6 +101,7 @@ struct SceneView {
     const LightDesc* lights;
     const float* lfunc;
     const float* lcdf;
+    const int* perm;  // Perlin permutation, 512 entries
     float funcInt;
     int32_t nLights;
     int32_t nPrim;

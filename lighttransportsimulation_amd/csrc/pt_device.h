@@ -7,8 +7,6 @@
 
 namespace lts {
 
-__constant__ int c_perm[512];  // Perlin permutation table (materials/textures/Procedural.cuh:79-107), twice
-
 struct Ray {
     V3 o, d;
     float tMax;
@@ -499,60 +497,60 @@ LTS_D float smooth_step(float mn, float mx, float val) {
     float v = clampf((val - mn) / (mx - mn), 0, 1);
     return v * v * (-2 * v + 3);
 }
-LTS_D float grad(int x, int y, int z, float dx, float dy, float dz) {
-    int h = c_perm[c_perm[c_perm[x] + y] + z];
+LTS_D float grad(const int* __restrict__ perm, int x, int y, int z, float dx, float dy, float dz) {
+    int h = perm[perm[perm[x] + y] + z];
     h &= 15;
     float u = h < 8 || h == 12 || h == 13 ? dx : dy;
     float v = h < 4 || h == 12 || h == 13 ? dy : dz;
     return ((h & 1) ? -u : u) + ((h & 2) ? -v : v);
 }
-LTS_D float perlin(V3 p) {
+LTS_D float perlin(const int* __restrict__ P, V3 p) {
     int ix = (int)floorf(p.x), iy = (int)floorf(p.y), iz = (int)floorf(p.z);
     float dx = p.x - ix, dy = p.y - iy, dz = p.z - iz;
     ix &= 255; iy &= 255; iz &= 255;
-    float w000 = grad(ix, iy, iz, dx, dy, dz);
-    float w100 = grad(ix + 1, iy, iz, dx - 1, dy, dz);
-    float w010 = grad(ix, iy + 1, iz, dx, dy - 1, dz);
-    float w001 = grad(ix, iy, iz + 1, dx, dy, dz - 1);
-    float w110 = grad(ix + 1, iy + 1, iz, dx - 1, dy - 1, dz);
-    float w011 = grad(ix, iy + 1, iz + 1, dx, dy - 1, dz - 1);
-    float w101 = grad(ix + 1, iy, iz + 1, dx - 1, dy, dz - 1);
-    float w111 = grad(ix + 1, iy + 1, iz + 1, dx - 1, dy - 1, dz - 1);
+    float w000 = grad(P, ix, iy, iz, dx, dy, dz);
+    float w100 = grad(P, ix + 1, iy, iz, dx - 1, dy, dz);
+    float w010 = grad(P, ix, iy + 1, iz, dx, dy - 1, dz);
+    float w001 = grad(P, ix, iy, iz + 1, dx, dy, dz - 1);
+    float w110 = grad(P, ix + 1, iy + 1, iz, dx - 1, dy - 1, dz);
+    float w011 = grad(P, ix, iy + 1, iz + 1, dx, dy - 1, dz - 1);
+    float w101 = grad(P, ix + 1, iy, iz + 1, dx - 1, dy, dz - 1);
+    float w111 = grad(P, ix + 1, iy + 1, iz + 1, dx - 1, dy - 1, dz - 1);
     float wx = noise_weight(dx), wy = noise_weight(dy), wz = noise_weight(dz);
     float x00 = lerpf(wx, w000, w100), x10 = lerpf(wx, w010, w110);
     float x01 = lerpf(wx, w001, w101), x11 = lerpf(wx, w011, w111);
     float y0 = lerpf(wy, x00, x10), y1 = lerpf(wy, x01, x11);
     return lerpf(wz, y0, y1);
 }
-LTS_D float fbm(V3 p, V3 dpdx, V3 dpdy, float omega, int maxOct) {
+LTS_D float fbm(const int* P, V3 p, V3 dpdx, V3 dpdy, float omega, int maxOct) {
     float len2 = fmaxf(lengthSq(dpdx), lengthSq(dpdy));
     float n = clampf(-1 - 0.5f * f_log2(len2), 0, (float)maxOct);
     int nInt = (int)floorf(n);
     float sum = 0, lambda = 1, o = 1;
-    for (int i = 0; i < nInt; i++) { sum += o * perlin(p * lambda); lambda *= 1.99f; o *= omega; }
+    for (int i = 0; i < nInt; i++) { sum += o * perlin(P, p * lambda); lambda *= 1.99f; o *= omega; }
     float nPartial = n - nInt;
-    sum += o * smooth_step(0.3f, 0.7f, nPartial) * perlin(p * lambda);
+    sum += o * smooth_step(0.3f, 0.7f, nPartial) * perlin(P, p * lambda);
     return sum;
 }
-LTS_D float turbulence(V3 p, V3 dpdx, V3 dpdy, float omega, int maxOct) {
+LTS_D float turbulence(const int* P, V3 p, V3 dpdx, V3 dpdy, float omega, int maxOct) {
     float len2 = fmaxf(lengthSq(dpdx), lengthSq(dpdy));
     float n = clampf(-1 - 0.5f * f_log2(len2), 0, (float)maxOct);
     int nInt = (int)floorf(n);
     float sum = 0, lambda = 1, o = 1;
-    for (int i = 0; i < nInt; i++) { sum += o * fabsf(perlin(p * lambda)); lambda *= 1.99f; o *= omega; }
+    for (int i = 0; i < nInt; i++) { sum += o * fabsf(perlin(P, p * lambda)); lambda *= 1.99f; o *= omega; }
     float nPartial = n - nInt;
-    sum += o * lerpf(smooth_step(0.3f, 0.7f, nPartial), 0.2f, fabsf(perlin(p * lambda)));
+    sum += o * lerpf(smooth_step(0.3f, 0.7f, nPartial), 0.2f, fabsf(perlin(P, p * lambda)));
     for (int i = nInt; i < maxOct; i++) { sum += o * 0.2f; o *= omega; }
     return sum;
 }
-LTS_D Spec marble_tex(const TexDesc& X, V3 p, V3 dpdx, V3 dpdy) {
+LTS_D Spec marble_tex(const int* P, const TexDesc& X, V3 p, V3 dpdx, V3 dpdy) {
     // MarbleTexture colours (Procedural.cuh:278-282); the zero row stands in for the read past the
     // table at t == 1, where its weight is exactly 0
     const float C[10][3] = {{.58f, .58f, .6f}, {.58f, .58f, .6f}, {.58f, .58f, .6f}, {.5f, .5f, .5f},
                             {.6f, .59f, .58f}, {.58f, .58f, .6f}, {.58f, .58f, .6f}, {.2f, .2f, .33f},
                             {.58f, .58f, .6f}, {0, 0, 0}};
     p = p * X.scale;
-    float m = p.y + X.variation * fbm(p, dpdx * X.scale, dpdy * X.scale, X.omega, X.octaves);
+    float m = p.y + X.variation * fbm(P, p, dpdx * X.scale, dpdy * X.scale, X.omega, X.octaves);
     float t = 0.5f + 0.5f * f_sin(m);
     int first = (int)floorf(t * 6.0f);
     first = first < 0 ? 0 : (first > 6 ? 6 : first);  // t is in [0, 1]; the clamp only fences NaN
@@ -603,9 +601,9 @@ LTS_D Spec eval_tex(const SceneView& sv, int ti, const SurfacePoint& s) {
     switch (X.type) {
         case TEX_CONST: return spec(X.c[0], X.c[1], X.c[2]);
         case TEX_IMAGE: return mip_lookup(sv, sv.mips[X.img], v2(1 * s.uv.x + 0, 1 * s.uv.y + 0));
-        case TEX_FBM: return spec(fbm(identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy), X.omega, X.octaves));
-        case TEX_WRINKLED: return spec(turbulence(identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy), X.omega, X.octaves));
-        case TEX_MARBLE: return marble_tex(X, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy));
+        case TEX_FBM: return spec(fbm(sv.perm, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy), X.omega, X.octaves));
+        case TEX_WRINKLED: return spec(turbulence(sv.perm, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy), X.omega, X.octaves));
+        case TEX_MARBLE: return marble_tex(sv.perm, X, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy));
     }
     return spec(0.0f);
 }
@@ -788,6 +786,61 @@ LTS_D V2 film_sample(int px, int py, int sample, int spp, int stratified, const 
         }
     }
     return v2((float)(px + film.x), (float)(py + film.y));
+}
+
+// ---------------------------------------------------------------- camera (rendering/Camera.cu:51-81)
+struct CamRays {
+    Ray r;
+    V3 rxO, rxD, ryO, ryD;
+};
+
+LTS_D void camera_ray(const CameraDesc& c, V2 pf, V2 lens, V3* o, V3* d) {
+    V3 cu = v3(c.u[0], c.u[1], c.u[2]), cv = v3(c.v[0], c.v[1], c.v[2]);
+    V3 co = v3(c.o[0], c.o[1], c.o[2]);
+    V3 offset = (cu * lens.x + cv * lens.y) * c.lensRadius;
+    V2 pr = v2(pf.x / (c.width - 1), pf.y / (c.height - 1));
+    *o = co + offset;
+    *d = normalize(v3(c.llc[0], c.llc[1], c.llc[2]) + v3(c.right[0], c.right[1], c.right[2]) * pr.x +
+                   v3(c.up[0], c.up[1], c.up[2]) * pr.y - co - offset);
+}
+
+// generateDifferentialRay + Ray::scaleDifferentials(rsqrt(spp)) (Camera.cu:67-81, Ray.cuh:35-40)
+LTS_D CamRays camera_rays(const CameraDesc& c, V2 pFilm, V2 lens, int spp) {
+    CamRays cr;
+    camera_ray(c, pFilm, lens, &cr.r.o, &cr.r.d);
+    camera_ray(c, pFilm + v2(1, 0), lens, &cr.rxO, &cr.rxD);
+    camera_ray(c, pFilm + v2(0, 1), lens, &cr.ryO, &cr.ryD);
+    cr.r.tMax = INFINITY;
+    float sc = quake((float)spp);
+    cr.rxO = cr.r.o + (cr.rxO - cr.r.o) * sc;
+    cr.ryO = cr.r.o + (cr.ryO - cr.r.o) * sc;
+    cr.rxD = cr.r.d + (cr.rxD - cr.r.d) * sc;
+    cr.ryD = cr.r.d + (cr.ryD - cr.r.d) * sc;
+    return cr;
+}
+
+// Camera rays of one (pixel, sample): film sample from dims 0-1, lens sample from dims 2-3.
+LTS_D CamRays camera_sample(const SceneView& sv, int px, int py, int sample, int spp, int stratified, uint32_t seed) {
+    Sampler smp;
+    smp.seed = seed;
+    smp.pixel = (uint32_t)(py * sv.cam.width + px);
+    smp.sample = (uint32_t)sample;
+    float d[4];
+    smp.block(0, d);
+    V2 pFilm = film_sample(px, py, sample, spp, stratified, d);
+    V2 lens = uniform_disk(v2(d[2], d[3]));
+    return camera_rays(sv.cam, pFilm, lens, spp);
+}
+
+// SurfaceInteraction::computeDifferentials, dpdx/dpdy part (Interaction.cuh:118-133)
+LTS_D void differentials(SurfacePoint& s, const CamRays& cr) {
+    float d = -dot(s.n, s.p);
+    float tx = (-dot(s.n, cr.rxO) - d) / dot(s.n, cr.rxD);
+    V3 px = cr.rxO + cr.rxD * tx;
+    float ty = (-dot(s.n, cr.ryO) - d) / dot(s.n, cr.ryD);
+    V3 py = cr.ryO + cr.ryD * ty;
+    s.dpdx = px - s.p;
+    s.dpdy = py - s.p;
 }
 
 }  // namespace lts

@@ -13,58 +13,13 @@ namespace {
 
 constexpr int kTileSide = 16;
 
-struct CamRays {
-    Ray r;
-    V3 rxO, rxD, ryO, ryD;
-};
-
-// Camera::generateRay / generateDifferentialRay + Ray::scaleDifferentials (Camera.cu:51-81, Ray.cuh:35-40)
-LTS_D void camera_ray(const CameraDesc& c, V2 pf, V2 lens, V3* o, V3* d) {
-    V3 cu = v3(c.u[0], c.u[1], c.u[2]), cv = v3(c.v[0], c.v[1], c.v[2]);
-    V3 co = v3(c.o[0], c.o[1], c.o[2]);
-    V3 offset = (cu * lens.x + cv * lens.y) * c.lensRadius;
-    V2 pr = v2(pf.x / (c.width - 1), pf.y / (c.height - 1));
-    *o = co + offset;
-    *d = normalize(v3(c.llc[0], c.llc[1], c.llc[2]) + v3(c.right[0], c.right[1], c.right[2]) * pr.x +
-                   v3(c.up[0], c.up[1], c.up[2]) * pr.y - co - offset);
-}
-
-LTS_D CamRays camera_rays(const CameraDesc& c, V2 pFilm, V2 lens, int spp) {
-    CamRays cr;
-    camera_ray(c, pFilm, lens, &cr.r.o, &cr.r.d);
-    camera_ray(c, pFilm + v2(1, 0), lens, &cr.rxO, &cr.rxD);
-    camera_ray(c, pFilm + v2(0, 1), lens, &cr.ryO, &cr.ryD);
-    cr.r.tMax = INFINITY;
-    float sc = quake((float)spp);
-    cr.rxO = cr.r.o + (cr.rxO - cr.r.o) * sc;
-    cr.ryO = cr.r.o + (cr.ryO - cr.r.o) * sc;
-    cr.rxD = cr.r.d + (cr.rxD - cr.r.d) * sc;
-    cr.ryD = cr.r.d + (cr.ryD - cr.r.d) * sc;
-    return cr;
-}
-
-// SurfaceInteraction::computeDifferentials, dpdx/dpdy part (Interaction.cuh:118-133)
-LTS_D void differentials(SurfacePoint& s, const CamRays& cr) {
-    float d = -dot(s.n, s.p);
-    float tx = (-dot(s.n, cr.rxO) - d) / dot(s.n, cr.rxD);
-    V3 px = cr.rxO + cr.rxD * tx;
-    float ty = (-dot(s.n, cr.ryO) - d) / dot(s.n, cr.ryD);
-    V3 py = cr.ryO + cr.ryD * ty;
-    s.dpdx = px - s.p;
-    s.dpdy = py - s.p;
-}
-
 // One camera sample of PathTracingIntegrator::evaluate.
 LTS_D Spec radiance(const SceneView& sv, int px, int py, int sample, const RenderLaunch& rl, unsigned long long* cnt) {
     Sampler smp;
     smp.seed = rl.seed;
     smp.pixel = (uint32_t)(py * sv.cam.width + px);
     smp.sample = (uint32_t)sample;
-    float d[4];
-    smp.block(0, d);
-    V2 pFilm = film_sample(px, py, sample, rl.spp, rl.stratified, d);
-    V2 lens = uniform_disk(v2(d[2], d[3]));
-    CamRays cr = camera_rays(sv.cam, pFilm, lens, rl.spp);
+    CamRays cr = camera_sample(sv, px, py, sample, rl.spp, rl.stratified, rl.seed);
     Ray r = cr.r;
     Spec L = spec(0.0f), beta = spec(1.0f);
     bool specularBounce = false;
@@ -208,8 +163,16 @@ hipError_t launch_get_render(const SceneView& sv, const float* film, float* rgb,
     return hipGetLastError();
 }
 
-hipError_t upload_perlin_table(const int* table512) {
-    return hipMemcpyToSymbol(HIP_SYMBOL(c_perm), table512, 512 * sizeof(int));
+__global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const float4* __restrict__ src, size_t n4) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)gridDim.x * 256) dst[i] = src[i];
 }
+
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+    size_t n4 = bytes / 16;
+    k_copy<<<2048, 256, 0, stream>>>(reinterpret_cast<float4*>(dst), reinterpret_cast<const float4*>(src), n4);
+    return hipGetLastError();
+}
+
+
 
 }  // namespace lts

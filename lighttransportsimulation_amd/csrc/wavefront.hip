@@ -1,0 +1,503 @@
+// Wavefront form of the plain path tracer (gfx950).
+//
+// The reference integrator (intergrators/PathTracingIntegrator.cuh:38-104 with estimateDirect,
+// IntegratorHelper.cuh:51-125) is cut at its three ray casts:
+//   extend   closest hit of every live path ray                     (persistent, wave-batched)
+//   sort     optional counting sort of the hit queue by material
+//   shade    emission, BSDF, light sample + MIS-BSDF sample (NEE set-up), continuation, roulette;
+//            appends one shadow ray and one MIS ray per path at most
+//   shadow   any hit of the shadow rays                             (persistent, wave-batched)
+//   mis      closest hit of the MIS-BSDF rays, light identity test  (persistent, wave-batched)
+//   finalize L += beta * ((Ld_light + Ld_bsdf) / p_pick), the reference's order of additions;
+//            terminated paths write their radiance
+// Each path's arithmetic is the megakernel's, so the image does not depend on queue order.
+// Every count lives in device memory and is reset by kernels, so the bounce loop of a pass is
+// captured once in a hipGraph and replayed.
+#include "lts_internal.h"
+#include "pt_device.h"
+
+namespace lts {
+
+namespace {
+
+constexpr int kShadeBlock = 256;
+
+// Path i of the pass -> (scene, pixel, sample).
+LTS_D void path_origin(const WfParams& P, const WfSceneSlot* slots, long i, int& scene, int& pixel, int& sample) {
+    int s = 0;
+    while (s + 1 < P.nScenes && i >= slots[s + 1].pathBase) s++;
+    const WfSceneSlot& sl = slots[s];
+    const long local = i - sl.pathBase;
+    const long npix = (long)sl.width * sl.height;
+    scene = s;
+    sample = sl.s0 + (int)(local / npix);
+    pixel = (int)(local % npix);
+}
+
+__global__ __launch_bounds__(256) void k_wf_generate(const SceneView* __restrict__ scenes, const WfParams* __restrict__ prm,
+                                                     const WfSceneSlot* __restrict__ slots, WfPath* __restrict__ paths,
+                                                     WfRay* __restrict__ rays, int32_t* __restrict__ queue,
+                                                     int32_t* __restrict__ counts) {
+    const WfParams P = *prm;
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i == 0) counts[0] = (int)P.nPaths;  // bounce-0 queue holds every path of the pass
+    for (long k = i; k < P.nPaths; k += (long)gridDim.x * 256) {
+        int sc, pixel, sample;
+        path_origin(P, slots, k, sc, pixel, sample);
+        const SceneView& sv = scenes[sc];
+        const WfSceneSlot& sl = slots[sc];
+        const int px = pixel % sv.cam.width, py = pixel / sv.cam.width;
+        CamRays cr = camera_sample(sv, px, py, sample, sl.spp, sl.stratified, sl.seed);
+        WfPath p;
+        p.beta[0] = p.beta[1] = p.beta[2] = 1.0f;
+        p.L[0] = p.L[1] = p.L[2] = 0.0f;
+        p.pixel = pixel; p.sample = sample; p.flags = kWfAlive; p.scene = sc;
+        paths[k] = p;
+        WfRay r;
+        r.o[0] = cr.r.o.x; r.o[1] = cr.r.o.y; r.o[2] = cr.r.o.z;
+        r.d[0] = cr.r.d.x; r.d[1] = cr.r.d.y; r.d[2] = cr.r.d.z;
+        r.pad[0] = r.pad[1] = 0;
+        rays[k] = r;
+        queue[k] = (int)k;
+    }
+}
+
+// Wave-batched work fetch: lane 0 takes 64 items from the launch's work counter.
+LTS_D int wave_fetch(int32_t* counter) {
+    int base = 0;
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(counter, 64);
+    return __shfl(base, 0) + (threadIdx.x & 63);
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_wf_extend(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
+                                                  const WfRay* __restrict__ rays, WfHit* __restrict__ hits,
+                                                  const int32_t* __restrict__ queue, const int32_t* __restrict__ count,
+                                                  int32_t* __restrict__ work) {
+    const int n = *count;
+    for (;;) {
+        const int k = wave_fetch(work);
+        if (k - (int)(threadIdx.x & 63) >= n) break;
+        if (k >= n) continue;
+        const int pi = queue[k];
+        const WfRay rr = rays[pi];
+        Ray r;
+        r.o = v3(rr.o[0], rr.o[1], rr.o[2]);
+        r.d = v3(rr.d[0], rr.d[1], rr.d[2]);
+        r.tMax = INFINITY;
+        Hit h = trace_closest(scenes[paths[pi].scene], r, nullptr, nullptr);
+        WfHit o;
+        o.leaf = h.leaf; o.u = h.u; o.v = h.v; o.invDet = h.invDet;
+        hits[pi] = o;
+    }
+}
+
+// Counting sort of a queue by the material of each path's hit (misses in the last bin). Each block
+// counts in LDS and reserves its range of every bin with one atomic per bin.
+LTS_D int material_bin(const SceneView* scenes, const WfPath* paths, const WfHit* hits, int pi, int nBins) {
+    const WfHit h = hits[pi];
+    if (h.leaf < 0) return nBins - 1;
+    const SceneView& sv = scenes[paths[pi].scene];
+    return min(sv.shade[sv.tris[h.leaf].prim].mat, nBins - 2);
+}
+
+__global__ __launch_bounds__(256) void k_wf_mat_hist(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
+                                                     const WfHit* __restrict__ hits, const int32_t* __restrict__ queue,
+                                                     const int32_t* __restrict__ count, int32_t* __restrict__ hist, int nBins) {
+    __shared__ int32_t h[kWfMaxMatBins];
+    if (threadIdx.x < kWfMaxMatBins) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int n = *count;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256)
+        atomicAdd(&h[material_bin(scenes, paths, hits, queue[k], nBins)], 1);
+    __syncthreads();
+    if (threadIdx.x < nBins && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_wf_mat_scan(int32_t* __restrict__ hist, int nBins) {
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int b = 0; b < nBins; b++) { int c = hist[b]; hist[b] = run; run += c; }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_wf_mat_scatter(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
+                                                        const WfHit* __restrict__ hits, const int32_t* __restrict__ queue,
+                                                        const int32_t* __restrict__ count, int32_t* __restrict__ offs, int nBins,
+                                                        int32_t* __restrict__ sorted) {
+    __shared__ int32_t h[kWfMaxMatBins], base[kWfMaxMatBins];
+    const int n = *count;
+    const int stride = gridDim.x * 256;
+    const int rounds = (n + stride - 1) / stride;
+    for (int round = 0; round < rounds; round++) {
+        if (threadIdx.x < kWfMaxMatBins) h[threadIdx.x] = 0;
+        __syncthreads();
+        const int k = round * stride + blockIdx.x * 256 + threadIdx.x;
+        int pi = -1, bin = 0, slot = 0;
+        if (k < n) {
+            pi = queue[k];
+            bin = material_bin(scenes, paths, hits, pi, nBins);
+            slot = atomicAdd(&h[bin], 1);
+        }
+        __syncthreads();
+        if (threadIdx.x < nBins) base[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&offs[threadIdx.x], h[threadIdx.x]) : 0;
+        __syncthreads();
+        if (pi >= 0) sorted[base[bin] + slot] = pi;
+        __syncthreads();
+    }
+}
+
+// Wave-aggregated append: one atomic per wave.
+LTS_D int wave_append(int32_t* counter, bool want) {
+    const uint64_t mask = __ballot(want);
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    const int leader = __ffsll((long long)mask) - 1;
+    if (mask && lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader < 0 ? 0 : leader);
+    return base + (int)__popcll(mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+}
+
+__global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __restrict__ scenes, const WfParams* __restrict__ prm,
+                                                          const WfSceneSlot* __restrict__ slots, int bounce,
+                                                          WfPath* __restrict__ paths, WfRay* __restrict__ rays,
+                                                          const WfHit* __restrict__ hits, WfNee* __restrict__ nee,
+                                                          const int32_t* __restrict__ queue, const int32_t* __restrict__ count,
+                                                          int32_t* __restrict__ nextQueue, int32_t* __restrict__ nextCount,
+                                                          WfShadow* __restrict__ shadows, int32_t* __restrict__ shadowCount,
+                                                          WfMis* __restrict__ misRays, int32_t* __restrict__ misCount) {
+    const int n = *count;
+    const int maxBounce = prm->maxBounce, rrStart = prm->rrStart;
+    const int stride = gridDim.x * kShadeBlock;
+    const int rounds = (n + stride - 1) / stride;  // uniform trip count: the appends below are wave-collective
+    for (int round = 0; round < rounds; round++) {
+        const int k = round * stride + blockIdx.x * kShadeBlock + threadIdx.x;
+        const bool active = k < n;
+        const int pi = active ? queue[k] : 0;
+        bool wantShadow = false, wantMis = false, wantNext = false;
+        WfShadow sh;
+        WfMis ms;
+        if (active) {
+            WfPath p = paths[pi];
+            const WfRay rr = rays[pi];
+            const WfHit hh = hits[pi];
+            const SceneView& sv = scenes[p.scene];
+            const WfSceneSlot& sl = slots[p.scene];
+            Ray r;
+            r.o = v3(rr.o[0], rr.o[1], rr.o[2]);
+            r.d = v3(rr.d[0], rr.d[1], rr.d[2]);
+            r.tMax = INFINITY;
+            Spec L = spec(p.L[0], p.L[1], p.L[2]), beta = spec(p.beta[0], p.beta[1], p.beta[2]);
+            const bool specularBounce = (p.flags & kWfSpecular) != 0;
+            WfNee ne;
+            ne.used = 0;
+            ne.p1[0] = ne.p1[1] = ne.p1[2] = 0.0f;
+            ne.p2[0] = ne.p2[1] = ne.p2[2] = 0.0f;
+            bool alive = true;
+            if (bounce == 0 || specularBounce) {
+                if (hh.leaf >= 0) {
+                    const int l = sv.shade[sv.tris[hh.leaf].prim].light;
+                    if (l >= 0) {
+                        SurfacePoint e = make_surface(sv, hh.leaf, r, hh.u, hh.v, hh.invDet);
+                        Spec le = dot(e.n, -r.d) > 0.f ? light_le(sv.lights[l]) : spec(0.0f);
+                        L = L + beta * le;
+                    } else {
+                        L = L + beta * spec(0.0f);
+                    }
+                } else {
+                    for (int l = 0; l < sv.nLights; l++) L = L + beta * (sv.lights[l].kind == 1 ? light_le(sv.lights[l]) : spec(0.0f));
+                }
+            }
+            if (hh.leaf < 0) {
+                alive = false;
+            } else {
+                SurfacePoint si = make_surface(sv, hh.leaf, r, hh.u, hh.v, hh.invDet);
+                if (bounce == 0) {
+                    const int px = p.pixel % sv.cam.width, py = p.pixel / sv.cam.width;
+                    differentials(si, camera_sample(sv, px, py, p.sample, sl.spp, sl.stratified, sl.seed));
+                }
+                Bsdf bsdf = make_bsdf(sv, si);
+                Sampler smp;
+                smp.seed = sl.seed; smp.pixel = (uint32_t)p.pixel; smp.sample = (uint32_t)p.sample;
+                float u0[4], u1[4];
+                smp.block(1 + 2 * bounce, u0);
+                smp.block(2 + 2 * bounce, u1);
+                if (sv.nLights > 0) {
+                    float lpdf;
+                    const int li = sample_light(sv, u0[0], &lpdf);
+                    const LightDesc& Lt = sv.lights[li];
+                    ne.used = 1;
+                    ne.nb[0] = beta.r; ne.nb[1] = beta.g; ne.nb[2] = beta.b;
+                    ne.lpdf = lpdf;
+                    const int flags = specularBounce ? B_ALL : (B_ALL & ~B_SPEC);
+                    // light sample with MIS (IntegratorHelper.cuh:76-95)
+                    V3 wi;
+                    float lightPdf = 0.0f, scatteringPdf = 0.0f;
+                    LightPoint vis;
+                    Spec Li = sample_li(sv, Lt, si, v2(u0[1], u0[2]), &wi, &lightPdf, &vis);
+                    if (lightPdf > 0 && !isBlack(Li)) {
+                        Spec f = bsdf.f(si.wo, wi, flags) * absDot(wi, si.sn);
+                        scatteringPdf = bsdf.pdf(si.wo, wi, flags);
+                        if (!isBlack(f)) {
+                            float weight = power_heuristic(lightPdf, scatteringPdf);
+                            Spec c = f * Li * weight / lightPdf;
+                            V3 origin = offsetRayOrigin(si.p, si.pError, si.n, vis.p - si.p);
+                            V3 target = offsetRayOrigin(vis.p, vis.pError, vis.n, origin - vis.p);
+                            V3 d = target - origin;
+                            sh.o[0] = origin.x; sh.o[1] = origin.y; sh.o[2] = origin.z;
+                            sh.d[0] = d.x; sh.d[1] = d.y; sh.d[2] = d.z;
+                            sh.path = pi;
+                            sh.c[0] = c.r; sh.c[1] = c.g; sh.c[2] = c.b;
+                            wantShadow = true;
+                        }
+                    }
+                    // BSDF sample with MIS (IntegratorHelper.cuh:96-122)
+                    int sampledType = 0;
+                    Spec f2 = bsdf.sampleF(si.wo, &wi, v2(u0[3], u1[0]), &scatteringPdf, flags, &sampledType);
+                    f2 = f2 * absDot(wi, si.sn);
+                    const bool sampledSpecular = (sampledType & B_SPEC) != 0;
+                    if (!isBlack(f2) && scatteringPdf > 0) {
+                        float weight = 1;
+                        bool cast = true;
+                        if (!sampledSpecular) {
+                            lightPdf = Lt.kind == 0 ? tri_pdf(sv, Lt, si, wi) : inf_pdf(wi);
+                            if (lightPdf == 0) cast = false;
+                            else weight = power_heuristic(scatteringPdf, lightPdf);
+                        }
+                        if (cast) {
+                            Ray mr = spawn_ray(si, wi);
+                            ms.o[0] = mr.o.x; ms.o[1] = mr.o.y; ms.o[2] = mr.o.z;
+                            ms.d[0] = wi.x; ms.d[1] = wi.y; ms.d[2] = wi.z;
+                            ms.path = pi; ms.light = li;
+                            ms.f[0] = f2.r; ms.f[1] = f2.g; ms.f[2] = f2.b;
+                            ms.weight = weight; ms.pdf = scatteringPdf;
+                            wantMis = true;
+                        }
+                    }
+                }
+                // continuation (PathTracingIntegrator.cuh:79-103)
+                V3 wo = -r.d, wi;
+                float pdf;
+                int flags = 0;
+                Spec f = bsdf.sampleF(wo, &wi, v2(u1[1], u1[2]), &pdf, B_ALL, &flags);
+                if (isBlack(f) || pdf == 0) {
+                    alive = false;
+                } else {
+                    p.flags = (flags & B_SPEC) ? (p.flags | kWfSpecular) : (p.flags & ~kWfSpecular);
+                    beta = beta * (f * absDot(wi, si.sn) / pdf);
+                    Ray nr = spawn_ray(si, wi);
+                    WfRay o;
+                    o.o[0] = nr.o.x; o.o[1] = nr.o.y; o.o[2] = nr.o.z;
+                    o.d[0] = wi.x; o.d[1] = wi.y; o.d[2] = wi.z;
+                    o.pad[0] = o.pad[1] = 0;
+                    rays[pi] = o;
+                    if (bounce >= rrStart) {
+                        float q = fmaxf(0.05f, 1 - lum(beta));
+                        if (u1[3] < q) alive = false;
+                        else beta = beta / (1 - q);
+                    }
+                }
+            }
+            p.L[0] = L.r; p.L[1] = L.g; p.L[2] = L.b;
+            p.beta[0] = beta.r; p.beta[1] = beta.g; p.beta[2] = beta.b;
+            if (!alive) p.flags &= ~kWfAlive;
+            paths[pi] = p;
+            nee[pi] = ne;
+            wantNext = alive && bounce + 1 < maxBounce;
+        }
+        const int si = wave_append(shadowCount, wantShadow);
+        if (wantShadow) shadows[si] = sh;
+        const int mi = wave_append(misCount, wantMis);
+        if (wantMis) misRays[mi] = ms;
+        const int ni = wave_append(nextCount, wantNext);
+        if (wantNext) nextQueue[ni] = pi;
+    }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_wf_shadow(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
+                                                  const WfShadow* __restrict__ shadows, const int32_t* __restrict__ count,
+                                                  WfNee* __restrict__ nee, int32_t* __restrict__ work) {
+    const int n = *count;
+    for (;;) {
+        const int k = wave_fetch(work);
+        if (k - (int)(threadIdx.x & 63) >= n) break;
+        if (k >= n) continue;
+        const WfShadow s = shadows[k];
+        Ray r;
+        r.o = v3(s.o[0], s.o[1], s.o[2]);
+        r.d = v3(s.d[0], s.d[1], s.d[2]);
+        r.tMax = 1 - SHADOW_EPS;
+        if (!trace_any(scenes[paths[s.path].scene], r)) {
+            nee[s.path].p1[0] = s.c[0];
+            nee[s.path].p1[1] = s.c[1];
+            nee[s.path].p1[2] = s.c[2];
+        }
+    }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_wf_mis(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
+                                               const WfMis* __restrict__ misRays, const int32_t* __restrict__ count,
+                                               WfNee* __restrict__ nee, int32_t* __restrict__ work) {
+    const int n = *count;
+    for (;;) {
+        const int k = wave_fetch(work);
+        if (k - (int)(threadIdx.x & 63) >= n) break;
+        if (k >= n) continue;
+        const WfMis m = misRays[k];
+        const SceneView& sv = scenes[paths[m.path].scene];
+        const LightDesc& Lt = sv.lights[m.light];
+        Ray r;
+        r.o = v3(m.o[0], m.o[1], m.o[2]);
+        r.d = v3(m.d[0], m.d[1], m.d[2]);
+        r.tMax = INFINITY;
+        const V3 wi = r.d;
+        Hit h = trace_closest(sv, r, nullptr, nullptr);
+        Spec Li2 = spec(0.f);
+        if (h.leaf >= 0) {
+            SurfacePoint lt = make_surface(sv, h.leaf, r, h.u, h.v, h.invDet);
+            if (sv.shade[lt.prim].light == m.light) Li2 = dot(lt.n, -wi) > 0.f ? light_le(Lt) : spec(0.0f);
+        } else if (Lt.kind == 1) {
+            Li2 = light_le(Lt);
+        }
+        if (!isBlack(Li2)) {
+            Spec c = spec(m.f[0], m.f[1], m.f[2]) * Li2 * spec(1.0f) * m.weight / m.pdf;
+            nee[m.path].p2[0] = c.r;
+            nee[m.path].p2[1] = c.g;
+            nee[m.path].p2[2] = c.b;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_wf_finalize(const WfParams* __restrict__ prm, const WfSceneSlot* __restrict__ slots,
+                                                     int bounce, WfPath* __restrict__ paths, const WfNee* __restrict__ nee,
+                                                     const int32_t* __restrict__ queue, const int32_t* __restrict__ count) {
+    const int n = *count;
+    const bool last = bounce + 1 >= prm->maxBounce;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        const int pi = queue[k];
+        WfPath p = paths[pi];
+        const WfNee ne = nee[pi];
+        Spec L = spec(p.L[0], p.L[1], p.L[2]);
+        if (ne.used) {
+            Spec Ld = spec(ne.p1[0], ne.p1[1], ne.p1[2]) + spec(ne.p2[0], ne.p2[1], ne.p2[2]);
+            L = L + spec(ne.nb[0], ne.nb[1], ne.nb[2]) * (Ld / ne.lpdf);
+            p.L[0] = L.r; p.L[1] = L.g; p.L[2] = L.b;
+            paths[pi] = p;
+        }
+        if (!(p.flags & kWfAlive) || last) {
+            const WfSceneSlot& sl = slots[p.scene];
+            float* o = sl.sampleL + 3 * ((size_t)(p.sample - sl.s0) * sl.width * sl.height + p.pixel);
+            o[0] = L.r; o[1] = L.g; o[2] = L.b;
+        }
+    }
+}
+
+// Passes with max_bounce == 0 produce black samples.
+__global__ __launch_bounds__(256) void k_wf_zero(const WfParams* __restrict__ prm, const WfSceneSlot* __restrict__ slots) {
+    const WfParams P = *prm;
+    for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < P.nPaths; k += (long)gridDim.x * 256) {
+        int s = 0;
+        while (s + 1 < P.nScenes && k >= slots[s + 1].pathBase) s++;
+        const WfSceneSlot& sl = slots[s];
+        sl.sampleL[3 * (k - sl.pathBase) + 0] = 0.0f;
+        sl.sampleL[3 * (k - sl.pathBase) + 1] = 0.0f;
+        sl.sampleL[3 * (k - sl.pathBase) + 2] = 0.0f;
+    }
+}
+
+__global__ void k_wf_fill(int32_t* p, int n, int32_t v) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = v;
+}
+
+}  // namespace
+
+hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce, hipStream_t s) {
+    const int gridAll = t.sweepBlocks;
+    const int ncounts = kWfCountsPerBounce * (maxBounce + 1) + kWfMaxMatBins * (maxBounce + 1);
+    k_wf_fill<<<(ncounts + 255) / 256, 256, 0, s>>>(b.counts, ncounts, 0);
+    if (maxBounce <= 0) {
+        k_wf_zero<<<gridAll, 256, 0, s>>>(b.params, b.slots);
+        return hipGetLastError();
+    }
+    k_wf_generate<<<gridAll, 256, 0, s>>>(b.scenes, b.params, b.slots, b.paths, b.rays, b.queue[0], b.counts);
+    for (int bounce = 0; bounce < maxBounce; bounce++) {
+        const int q = bounce & 1;
+        int32_t* cnt = b.counts + kWfCountsPerBounce * bounce;
+        int32_t* nxt = b.counts + kWfCountsPerBounce * (bounce + 1);
+        switch (t.traceBlock) {
+            case 64: k_wf_extend<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3); break;
+            case 128: k_wf_extend<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3); break;
+            default: k_wf_extend<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3); break;
+        }
+        const int32_t* shadeQueue = b.queue[q];
+        if (t.sortMaterials) {
+            int32_t* hist = b.counts + kWfCountsPerBounce * (maxBounce + 1) + kWfMaxMatBins * bounce;
+            k_wf_mat_hist<<<gridAll, 256, 0, s>>>(b.scenes, b.paths, b.hits, b.queue[q], cnt, hist, b.matBins);
+            k_wf_mat_scan<<<1, 64, 0, s>>>(hist, b.matBins);
+            k_wf_mat_scatter<<<gridAll, 256, 0, s>>>(b.scenes, b.paths, b.hits, b.queue[q], cnt, hist, b.matBins, b.sorted);
+            shadeQueue = b.sorted;
+        }
+        k_wf_shade<<<gridAll, kShadeBlock, 0, s>>>(b.scenes, b.params, b.slots, bounce, b.paths, b.rays, b.hits, b.nee, shadeQueue,
+                                                   cnt, b.queue[q ^ 1], nxt, b.shadows, cnt + 1, b.mis, cnt + 2);
+        switch (t.traceBlock) {
+            case 64:
+                k_wf_shadow<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.shadows, cnt + 1, b.nee, cnt + 4);
+                k_wf_mis<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.mis, cnt + 2, b.nee, cnt + 5);
+                break;
+            case 128:
+                k_wf_shadow<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.shadows, cnt + 1, b.nee, cnt + 4);
+                k_wf_mis<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.mis, cnt + 2, b.nee, cnt + 5);
+                break;
+            default:
+                k_wf_shadow<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.shadows, cnt + 1, b.nee, cnt + 4);
+                k_wf_mis<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.mis, cnt + 2, b.nee, cnt + 5);
+                break;
+        }
+        k_wf_finalize<<<gridAll, 256, 0, s>>>(b.params, b.slots, bounce, b.paths, b.nee, shadeQueue, cnt);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace lts
+
+namespace lts {
+
+template <typename T>
+static hipError_t wf_alloc(T** p, size_t count) {
+    return hipMalloc((void**)p, (count ? count : 1) * sizeof(T));
+}
+
+hipError_t wf_reserve(WfBuffers& b, size_t cap, int maxScenes) {
+    if (cap <= b.cap && maxScenes <= b.maxScenes) return hipSuccess;
+    wf_release(b);
+    hipError_t e = hipSuccess;
+#define WTRY(x) do { e = (x); if (e != hipSuccess) { wf_release(b); return e; } } while (0)
+    WTRY(wf_alloc(&b.scenes, maxScenes));
+    WTRY(wf_alloc(&b.params, 1));
+    WTRY(wf_alloc(&b.slots, maxScenes));
+    WTRY(wf_alloc(&b.paths, cap));
+    WTRY(wf_alloc(&b.rays, cap));
+    WTRY(wf_alloc(&b.hits, cap));
+    WTRY(wf_alloc(&b.nee, cap));
+    WTRY(wf_alloc(&b.queue[0], cap));
+    WTRY(wf_alloc(&b.queue[1], cap));
+    WTRY(wf_alloc(&b.sorted, cap));
+    WTRY(wf_alloc(&b.shadows, cap));
+    WTRY(wf_alloc(&b.mis, cap));
+    WTRY(wf_alloc(&b.counts, (size_t)(kWfCountsPerBounce + kWfMaxMatBins) * (kWfMaxBounce + 1)));
+#undef WTRY
+    b.cap = cap;
+    b.maxScenes = maxScenes;
+    return hipSuccess;
+}
+
+void wf_release(WfBuffers& b) {
+    void* ps[] = {b.scenes, b.params, b.slots, b.paths, b.rays, b.hits, b.nee, b.queue[0], b.queue[1], b.sorted, b.shadows, b.mis, b.counts};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    b = WfBuffers();
+}
+
+}  // namespace lts

@@ -182,16 +182,22 @@ def triangle_soup(n: int, seed: int = 3) -> np.ndarray:
     c = rng.random((n, 3))
     s = rng.lognormal(-5.0, 0.7, n)
     u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
-    q = np.stack([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
-                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3)], axis=1)
-    x, y, z, w = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
-    R = np.stack([
-        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], 1),
-        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], 1),
-        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], 1)], 1)
     base = np.array([[0.0, 0.57735027, 0.0], [-0.5, -0.28867513, 0.0], [0.5, -0.28867513, 0.0]])
-    verts = c[:, None, :] + s[:, None, None] * np.einsum("nij,vj->nvi", R, base)
-    return _nonneg_zero(verts.reshape(n, 9))
+    out = np.empty((n, 9), np.float32)
+    step = 1 << 20
+    for a in range(0, n, step):
+        b = min(n, a + step)
+        t1, t2, t3 = u1[a:b], 2 * np.pi * u2[a:b], 2 * np.pi * u3[a:b]
+        x, y = np.sqrt(1 - t1) * np.sin(t2), np.sqrt(1 - t1) * np.cos(t2)
+        z, w = np.sqrt(t1) * np.sin(t3), np.sqrt(t1) * np.cos(t3)
+        R = np.empty((b - a, 3, 3))
+        R[:, 0, 0], R[:, 0, 1], R[:, 0, 2] = 1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)
+        R[:, 1, 0], R[:, 1, 1], R[:, 1, 2] = 2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)
+        R[:, 2, 0], R[:, 2, 1], R[:, 2, 2] = 2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)
+        v = c[a:b, None, :] + s[a:b, None, None] * np.matmul(R[:, None, :, :], base[None, :, :, None])[..., 0]
+        out[a:b] = v.reshape(b - a, 9)
+    out[out == 0] = 0.0  # no -0.0 anywhere in generated data
+    return out
 
 
 # ----------------------------------------------------------------------------- C3 scene

@@ -68,3 +68,19 @@ def test_lbvh_1m_bit_exact(gpu_ctx):
         o = oracle_lib.lbvh_build(tri, mode)
         _assert_same(g, o, f"1M mode={mode}")
         assert t["total_ms"] > 0
+
+
+@pytest.mark.gpu
+def test_lbvh_8m_bit_exact(gpu_ctx):
+    """The 8M-triangle LBVH-build workload in the reference leaf layout."""
+    n = 1 << 23
+    tri = scenegen.triangle_soup(n, seed=3)
+    print("[lbvh-8M]", flush=True)
+    g, t = gpu_ctx.bvh_build(tri, 0)
+    o = oracle_lib.lbvh_build(tri, 0)
+    _assert_same(g, o, "8M mode=0")
+    # size-independent properties: a permutation, sorted codes, every node has one parent
+    assert np.array_equal(np.sort(g["perm"]), np.arange(n))
+    assert np.all(np.diff(g["codes_sorted"].astype(np.int64)) >= 0)
+    par = g["parent"]
+    assert par[0] == -1 and np.all(par[1:] >= 0)

@@ -1,0 +1,130 @@
+"""Wavefront integrator against the oracle and against the megakernel.
+
+The wavefront form splits every path at its ray casts into queue kernels; each path still runs the
+megakernel's fp32 operation sequence, so per-sample radiance must equal the oracle's (same bar as
+the megakernel tests) whatever the queue order, block size, persistent grid, material sort or
+graph replay, and images must equal the megakernel's.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+from lighttransportsimulation_amd import (INTEGRATOR_MEGAKERNEL, INTEGRATOR_WAVEFRONT, Context, LtsError,
+                                          render_batch, scenegen)
+from test_gpu_render import SEED, _compare
+
+
+def _identical(a, b):
+    return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def cornell(tmp_path_factory):
+    d = tmp_path_factory.mktemp("cornell_wf")
+    cfg = scenegen.gen_cornell_sphere(str(d), width=64, height=48, spp=4)
+    o = oracle_lib.OracleScene(str(d), cfg["scene"], cfg["subject"], 64, 48)
+    o_rgb, o_smp, _ = o.render(4, 5, 3, SEED, stratified=1, want_samples=True)
+    return str(d), cfg, o_rgb, o_smp
+
+
+@pytest.fixture(scope="module")
+def displaced(tmp_path_factory):
+    d = tmp_path_factory.mktemp("displaced_wf")
+    W, H, spp = 96, 54, 4
+    cfg = scenegen.gen_displaced_sphere(str(d), width=W, height=H, spp=spp, n_lat=40, n_lon=80, image_size=64)
+    o = oracle_lib.OracleScene(str(d), cfg["scene"], cfg["subject"], W, H)
+    o_rgb, o_smp, _ = o.render(spp, 8, 3, SEED, want_samples=True)
+    return str(d), cfg, o_rgb, o_smp
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("trace_block,persistent_blocks,sort,graph", [
+    (128, 2048, 0, 1), (64, 512, 1, 1), (256, 4096, 1, 0), (128, 16, 0, 0)])
+def test_cornell_wavefront_matches_oracle(gpu_ctx, cornell, trace_block, persistent_blocks, sort, graph):
+    d, cfg, o_rgb, o_smp = cornell
+    print(f"[wavefront cornell tb={trace_block} pb={persistent_blocks} sort={sort} graph={graph}]", flush=True)
+    gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], 64, 48)
+    g_rgb, g_smp, st = gpu_ctx.render(4, 5, 3, SEED, stratified=1, integrator=INTEGRATOR_WAVEFRONT, want_samples=True,
+                                      trace_block=trace_block, persistent_blocks=persistent_blocks,
+                                      sort_materials=sort, use_graph=graph)
+    assert (st["trace_block"], st["persistent_blocks"], st["sort_materials"], st["use_graph"]) == (
+        trace_block, persistent_blocks, sort, graph)
+    assert np.isfinite(g_rgb).all() and g_rgb.mean() > 0
+    _compare(g_smp, o_smp, g_rgb, o_rgb, f"wavefront cornell tb{trace_block} sort{sort}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sort", [0, 1])
+def test_displaced_wavefront_matches_oracle(gpu_ctx, displaced, sort):
+    d, cfg, o_rgb, o_smp = displaced
+    print(f"[wavefront displaced sort={sort}]", flush=True)
+    gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], 96, 54)
+    g_rgb, g_smp, _ = gpu_ctx.render(4, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT, want_samples=True,
+                                     sort_materials=sort)
+    assert np.isfinite(g_rgb).all() and g_rgb.mean() > 0
+    _compare(g_smp, o_smp, g_rgb, o_rgb, f"wavefront displaced sort{sort}")
+
+
+@pytest.mark.gpu
+def test_wavefront_equals_megakernel_over_passes(gpu_ctx, displaced):
+    """Several passes through one captured graph (2 samples per pass, 8 samples) give the
+    megakernel's image bit for bit."""
+    d, cfg, _, _ = displaced
+    print("[wavefront vs megakernel, 4 passes]", flush=True)
+    gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], 96, 54)
+    m_rgb, _, _ = gpu_ctx.render(8, 8, 3, SEED, integrator=INTEGRATOR_MEGAKERNEL, spp_per_pass=3)
+    w_rgb, _, st = gpu_ctx.render(8, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT, spp_per_pass=2, sort_materials=1)
+    assert st["launches"] == 4
+    assert _identical(m_rgb, w_rgb)
+
+
+@pytest.mark.gpu
+def test_wavefront_zero_bounces_is_black(gpu_ctx, cornell):
+    d, cfg, _, _ = cornell
+    gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], 64, 48)
+    m_rgb, _, _ = gpu_ctx.render(4, 0, 3, SEED, stratified=1)
+    w_rgb, _, _ = gpu_ctx.render(4, 0, 3, SEED, stratified=1, integrator=INTEGRATOR_WAVEFRONT)
+    assert _identical(m_rgb, w_rgb) and not w_rgb.any()
+
+
+@pytest.mark.gpu
+def test_wavefront_rejects_bad_tuning(gpu_ctx, cornell):
+    d, cfg, _, _ = cornell
+    gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], 64, 48)
+    with pytest.raises(LtsError):
+        gpu_ctx.render(4, 5, 3, SEED, stratified=1, integrator=INTEGRATOR_WAVEFRONT, trace_block=100)
+    with pytest.raises(LtsError):
+        gpu_ctx.render(4, 65, 3, SEED, stratified=1, integrator=INTEGRATOR_WAVEFRONT)
+
+
+@pytest.mark.gpu
+def test_autotune_keeps_the_image(gpu_ctx, displaced):
+    d, cfg, _, _ = displaced
+    print("[wavefront autotune]", flush=True)
+    gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], 96, 54)
+    ref, _, _ = gpu_ctx.render(4, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT)
+    a_rgb, _, st = gpu_ctx.render(4, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT, autotune=1)
+    assert st["trace_block"] in (64, 128, 256) and st["persistent_blocks"] > 0 and st["autotune_ms"] > 0
+    b_rgb, _, st2 = gpu_ctx.render(4, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT, autotune=1)
+    assert st2["autotune_ms"] == 0  # cached in the context
+    assert (st2["trace_block"], st2["persistent_blocks"], st2["sort_materials"]) == (
+        st["trace_block"], st["persistent_blocks"], st["sort_materials"])
+    assert _identical(ref, a_rgb) and _identical(ref, b_rgb)
+
+
+@pytest.mark.gpu
+def test_batch_equals_single_renders(gpu_ctx, cornell, displaced):
+    """Two scenes of different sizes in one batch: each image equals its own render."""
+    print("[wavefront batch of 2 scenes]", flush=True)
+    dc, cc, _, _ = cornell
+    dd, cd, _, _ = displaced
+    gpu_ctx.load_scene(dc, cc["scene"], cc["subject"], 64, 48)
+    with Context(0) as other:
+        other.load_scene(dd, cd["scene"], cd["subject"], 96, 54)
+        a, _, _ = gpu_ctx.render(4, 6, 3, SEED, integrator=INTEGRATOR_WAVEFRONT)
+        b, _, _ = other.render(4, 6, 3, SEED, integrator=INTEGRATOR_WAVEFRONT)
+        (ba, bb), st = render_batch([gpu_ctx, other], 4, 6, 3, SEED, spp_per_pass=2)
+        assert st["launches"] == 2 and st["samples"] == 4 * (64 * 48 + 96 * 54)
+        assert _identical(a, ba) and _identical(b, bb)
+        with pytest.raises(LtsError):
+            render_batch([gpu_ctx, gpu_ctx], 4, 6, 3, SEED)
