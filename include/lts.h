@@ -134,6 +134,43 @@ int lts_render(lts_ctx* ctx, const lts_render_params* params, float* rgb_out, fl
 int lts_render_batch(lts_ctx* const* ctxs, int32_t n, const lts_render_params* params, float* const* rgb_out,
                      lts_stats* stats);
 
+/* ---- Learned radiance cache. No reference counterpart: the reference traces every path to the
+ * end. A multilayer perceptron (64-dim encoding of a 16-float record -> 5 hidden layers of 64,
+ * ReLU, no biases -> RGB) is trained online on the radiance of path suffixes and queried where a
+ * cached render cuts paths short (LTS_INTEGRATOR_CACHED). Weights are fp32 masters; the MFMA
+ * kernels use bf16 or fp16 copies with fp32 accumulation. Record layout (LTS_NRC_RAW floats):
+ * position normalised to the scene bounds (3), outgoing direction theta/pi and phi/2pi + 1/2 (2),
+ * normal theta/pi and phi/2pi + 1/2 (2), roughness (1), diffuse reflectance (3), specular
+ * reflectance (3), 2 unused. Parameters: W_l[in][out] row-major for l = 0..4 (64 x 64), then the
+ * 64 x 16 output matrix whose columns 3..15 are padding and stay zero. */
+#define LTS_NRC_PARAMS 21504
+#define LTS_NRC_RAW 16
+
+typedef struct lts_nrc_config {
+    int32_t precision;      /* 0 bf16 (default), 1 fp16                                             */
+    int32_t batch;          /* training samples per Adam step (default 65536)                       */
+    int32_t query_bounce;   /* cached renders query the cache at the first non-specular hit of      */
+                            /* this bounce or later (default 2)                                     */
+    int32_t train_paths;    /* paths per pass traced to the end as training paths (default 16384)   */
+    int32_t warmup_passes;  /* passes of a cached render traced without queries (default 4, -1: 0)  */
+    uint32_t seed;          /* weight initialisation (Xavier-uniform from a SplitMix64 stream)      */
+    float lr, beta1, beta2, eps;  /* Adam (defaults 1e-2, 0.9, 0.99, 1e-15)                          */
+    int32_t reserved[6];
+} lts_nrc_config;
+
+/* (Re)initialises the cache of this context: weights from the seed, zero Adam state. NULL or zero
+ * fields take the defaults. */
+int lts_nrc_configure(lts_ctx* ctx, const lts_nrc_config* config);
+/* Replaces the weights (LTS_NRC_PARAMS floats; padding entries are ignored) and resets Adam. */
+int lts_nrc_set_weights(lts_ctx* ctx, const float* params);
+int lts_nrc_get_weights(lts_ctx* ctx, float* params);
+/* Network outputs for n records (host arrays: raw n x LTS_NRC_RAW, out n x 3). */
+int lts_nrc_infer(lts_ctx* ctx, const float* raw, int32_t n, float* out);
+/* One Adam step on n <= batch samples with the relative L2 loss
+ * (1/n) sum_i sum_c (y_ic - t_ic)^2 / (y_ic^2 + 0.01), the denominator held constant.
+ * loss (may be NULL) receives the loss before the step, grad (may be NULL) its gradient. */
+int lts_nrc_train_step(lts_ctx* ctx, const float* raw, const float* target, int32_t n, float* loss, float* grad);
+
 /* No reference counterpart: a plain device-to-device copy kernel over `bytes` bytes, timed with
  * HIP events (mean of `repeat` launches). Used to calibrate HBM traffic counters and to measure
  * the achievable copy bandwidth next to the kernels' roofline figures. */

@@ -22,8 +22,12 @@ INTEGRATOR_CACHED = 2
 EXPORTED_SYMBOLS = (
     "lts_create", "lts_destroy", "lts_last_error", "lts_version", "lts_read_app_params", "lts_load_scene",
     "lts_scene_info", "lts_bvh_build", "lts_bvh_build_device", "lts_bvh_export_last", "lts_render",
-    "lts_write_image", "lts_device_copy_benchmark", "lts_render_batch",
+    "lts_write_image", "lts_device_copy_benchmark", "lts_render_batch", "lts_nrc_configure", "lts_nrc_set_weights",
+    "lts_nrc_get_weights", "lts_nrc_infer", "lts_nrc_train_step",
 )
+
+NRC_PARAMS = 21504
+NRC_RAW = 16
 
 
 class LtsError(RuntimeError):
@@ -45,6 +49,12 @@ class RenderParams(C.Structure):
                 ("stratified", C.c_int32), ("integrator", C.c_int32), ("spp_per_pass", C.c_int32),
                 ("trace_block", C.c_int32), ("persistent_blocks", C.c_int32), ("sort_materials", C.c_int32),
                 ("use_graph", C.c_int32), ("autotune", C.c_int32), ("reserved", C.c_int32 * 4)]
+
+
+class NrcConfig(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("batch", C.c_int32), ("query_bounce", C.c_int32), ("train_paths", C.c_int32),
+                ("warmup_passes", C.c_int32), ("seed", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("reserved", C.c_int32 * 6)]
 
 
 class Stats(C.Structure):
@@ -83,6 +93,11 @@ def load_library():
     L.lts_render_batch.argtypes = [C.POINTER(vp), i32, C.POINTER(RenderParams), C.POINTER(vp), C.POINTER(Stats)]
     L.lts_write_image.argtypes = [C.c_char_p, vp, i32, i32, i32]
     L.lts_device_copy_benchmark.argtypes = [vp, C.c_uint64, i32, C.POINTER(C.c_double)]
+    L.lts_nrc_configure.argtypes = [vp, C.POINTER(NrcConfig)]
+    L.lts_nrc_set_weights.argtypes = [vp, vp]
+    L.lts_nrc_get_weights.argtypes = [vp, vp]
+    L.lts_nrc_infer.argtypes = [vp, vp, i32, vp]
+    L.lts_nrc_train_step.argtypes = [vp, vp, vp, i32, C.POINTER(C.c_float), vp]
     _lib = L
     return L
 
@@ -180,6 +195,42 @@ class Context:
         ms = C.c_double()
         self._check(self.lib.lts_device_copy_benchmark(self.h, nbytes, repeat, C.byref(ms)), "lts_device_copy_benchmark")
         return ms.value
+
+    # ------------------------------------------------------------------ radiance cache
+    def nrc_configure(self, precision: str = "bf16", batch: int = 0, query_bounce: int = 0, train_paths: int = 0,
+                      warmup_passes: int = 0, seed: int = 0, lr: float = 0.0, beta1: float = 0.0, beta2: float = 0.0,
+                      eps: float = 0.0):
+        """(Re)initialises the cache (include/lts.h: lts_nrc_configure); zero arguments take the defaults."""
+        cfg = NrcConfig(0 if precision == "bf16" else 1, batch, query_bounce, train_paths, warmup_passes, seed, lr, beta1,
+                        beta2, eps)
+        self._check(self.lib.lts_nrc_configure(self.h, C.byref(cfg)), "lts_nrc_configure")
+
+    def nrc_set_weights(self, params: np.ndarray):
+        params = np.ascontiguousarray(params, np.float32)
+        if params.size != NRC_PARAMS:
+            raise LtsError(f"expected {NRC_PARAMS} parameters")
+        self._check(self.lib.lts_nrc_set_weights(self.h, _ptr(params)), "lts_nrc_set_weights")
+
+    def nrc_get_weights(self) -> np.ndarray:
+        w = np.zeros(NRC_PARAMS, np.float32)
+        self._check(self.lib.lts_nrc_get_weights(self.h, _ptr(w)), "lts_nrc_get_weights")
+        return w
+
+    def nrc_infer(self, raw: np.ndarray) -> np.ndarray:
+        raw = np.ascontiguousarray(raw, np.float32).reshape(-1, NRC_RAW)
+        out = np.zeros((raw.shape[0], 3), np.float32)
+        self._check(self.lib.lts_nrc_infer(self.h, _ptr(raw), raw.shape[0], _ptr(out)), "lts_nrc_infer")
+        return out
+
+    def nrc_train_step(self, raw: np.ndarray, target: np.ndarray, want_grad: bool = False):
+        """One Adam step; returns (loss before the step, gradient or None)."""
+        raw = np.ascontiguousarray(raw, np.float32).reshape(-1, NRC_RAW)
+        target = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+        loss = C.c_float()
+        grad = np.zeros(NRC_PARAMS, np.float32) if want_grad else None
+        self._check(self.lib.lts_nrc_train_step(self.h, _ptr(raw), _ptr(target), raw.shape[0], C.byref(loss), _ptr(grad)),
+                    "lts_nrc_train_step")
+        return loss.value, grad
 
     # ------------------------------------------------------------------ render
     def render(self, spp: int, max_bounce: int, rr_start: int, seed: int, stratified: int = 0,

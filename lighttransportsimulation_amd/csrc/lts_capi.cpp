@@ -84,6 +84,13 @@ struct lts_ctx {
     WfParams wfParams{};                  // host copies of the per-pass parameters (stable for async H2D)
     std::vector<WfSceneSlot> wfSlots;
     std::map<std::string, WfTuning> tuned;  // autotuner results of this context
+    // learned radiance cache
+    NrcBuffers nrc;
+    lts_nrc_config nrcCfg{};
+    bool nrcReady = false;
+    float* nrcIo = nullptr;     // staging for lts_nrc_infer: raw records then outputs
+    size_t nrcIoCap = 0;        // records
+    int32_t* nrcCount = nullptr;  // device record count for lts_nrc_infer
 };
 
 namespace {
@@ -456,6 +463,72 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
     return LTS_OK;
 }
 
+// ---------------------------------------------------------------- learned radiance cache
+lts_nrc_config nrc_defaults(const lts_nrc_config* in) {
+    lts_nrc_config c{};
+    if (in) c = *in;
+    if (c.batch <= 0) c.batch = 65536;
+    if (c.query_bounce <= 0) c.query_bounce = 2;
+    if (c.train_paths <= 0) c.train_paths = 16384;
+    if (c.warmup_passes == 0) c.warmup_passes = 4;
+    if (c.warmup_passes < 0) c.warmup_passes = 0;
+    if (c.lr <= 0) c.lr = 1e-2f;
+    if (c.beta1 <= 0) c.beta1 = 0.9f;
+    if (c.beta2 <= 0) c.beta2 = 0.99f;
+    if (c.eps <= 0) c.eps = 1e-15f;
+    return c;
+}
+
+uint64_t splitmix64(uint64_t& x) {
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// Uploads fp32 master weights (padding columns forced to zero), clears Adam and repacks.
+int nrc_upload_weights(lts_ctx* c, const float* w) {
+    std::vector<float> m(w, w + kNrcParams);
+    for (int in = 0; in < 64; in++)
+        for (int out = 3; out < 16; out++) m[20480 + in * 16 + out] = 0.0f;
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipMemcpyAsync(c->nrc.master, m.data(), kNrcParams * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemsetAsync(c->nrc.m, 0, kNrcParams * sizeof(float), s));
+    HIPCHK(c, hipMemsetAsync(c->nrc.v, 0, kNrcParams * sizeof(float), s));
+    HIPCHK(c, hipMemsetAsync(c->nrc.grad, 0, kNrcParams * sizeof(float), s));
+    HIPCHK(c, hipMemsetAsync(c->nrc.state, 0, sizeof(NrcState), s));
+    HIPCHK(c, nrc_pack(c->nrc, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return LTS_OK;
+}
+
+int nrc_setup(lts_ctx* c, const lts_nrc_config* in) {
+    lts_nrc_config cfg = nrc_defaults(in);
+    if (cfg.precision != 0 && cfg.precision != 1) return fail(c, LTS_E_ARG, "nrc precision must be 0 (bf16) or 1 (fp16)");
+    if (cfg.batch > (1 << 20)) return fail(c, LTS_E_ARG, "nrc batch above 2^20");
+    if (cfg.train_paths > (1 << 22)) return fail(c, LTS_E_ARG, "nrc train_paths above 2^22");
+    HIPCHK(c, hipSetDevice(c->device));
+    c->nrc.precision = cfg.precision;
+    c->nrc.lr = cfg.lr; c->nrc.beta1 = cfg.beta1; c->nrc.beta2 = cfg.beta2; c->nrc.eps = cfg.eps;
+    HIPCHK(c, nrc_reserve(c->nrc, cfg.batch));
+    if (!c->nrcCount) HIPCHK(c, hipMalloc((void**)&c->nrcCount, sizeof(int32_t)));
+    // Xavier-uniform initialisation
+    std::vector<float> w(kNrcParams, 0.0f);
+    uint64_t st = 0x6e72635f696e6974ull ^ cfg.seed;
+    auto uni = [&](float a) { return (float)(((splitmix64(st) >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0) * a; };
+    const float hid = std::sqrt(6.0f / 128.0f), outA = std::sqrt(6.0f / 67.0f);
+    for (int p = 0; p < 20480; p++) w[p] = uni(hid);
+    for (int in = 0; in < 64; in++)
+        for (int out = 0; out < 3; out++) w[20480 + in * 16 + out] = uni(outA);
+    c->nrcCfg = cfg;
+    int rc = nrc_upload_weights(c, w.data());
+    if (rc != LTS_OK) return rc;
+    c->nrcReady = true;
+    return LTS_OK;
+}
+
+int nrc_ensure(lts_ctx* c) { return c->nrcReady ? LTS_OK : nrc_setup(c, nullptr); }
+
 }  // namespace
 
 extern "C" {
@@ -497,6 +570,9 @@ void lts_destroy(lts_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->wfExec) (void)hipGraphExecDestroy(c->wfExec);
     wf_release(c->wf);
+    nrc_release(c->nrc);
+    if (c->nrcIo) (void)hipFree(c->nrcIo);
+    if (c->nrcCount) (void)hipFree(c->nrcCount);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -699,6 +775,73 @@ int lts_render_batch(lts_ctx* const* ctxs, int32_t n, const lts_render_params* p
     HIPCHK(c, hipSetDevice(c->device));
     for (lts_ctx* x : scenes) HIPCHK(c, hipStreamSynchronize(x->stream));
     return render_wavefront(c, scenes, p, rgb_out, nullptr, st);
+}
+
+int lts_nrc_configure(lts_ctx* c, const lts_nrc_config* cfg) {
+    if (!c) return LTS_E_ARG;
+    return nrc_setup(c, cfg);
+}
+
+int lts_nrc_set_weights(lts_ctx* c, const float* w) {
+    if (!c || !w) return fail(c, LTS_E_ARG, "null argument");
+    int rc = nrc_ensure(c);
+    if (rc != LTS_OK) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    return nrc_upload_weights(c, w);
+}
+
+int lts_nrc_get_weights(lts_ctx* c, float* w) {
+    if (!c || !w) return fail(c, LTS_E_ARG, "null argument");
+    int rc = nrc_ensure(c);
+    if (rc != LTS_OK) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpy(w, c->nrc.master, kNrcParams * sizeof(float), hipMemcpyDeviceToHost));
+    return LTS_OK;
+}
+
+int lts_nrc_infer(lts_ctx* c, const float* raw, int32_t n, float* out) {
+    if (!c || n < 0 || (n > 0 && (!raw || !out))) return fail(c, LTS_E_ARG, "bad nrc_infer arguments");
+    int rc = nrc_ensure(c);
+    if (rc != LTS_OK) return rc;
+    if (n == 0) return LTS_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((size_t)n > c->nrcIoCap) {
+        if (c->nrcIo) (void)hipFree(c->nrcIo);
+        c->nrcIo = nullptr;
+        c->nrcIoCap = 0;
+        HIPCHK(c, hipMalloc((void**)&c->nrcIo, (size_t)n * (kNrcRaw + 3) * sizeof(float)));
+        c->nrcIoCap = n;
+    }
+    hipStream_t s = c->stream;
+    float* dRaw = c->nrcIo;
+    float* dOut = c->nrcIo + (size_t)n * kNrcRaw;
+    HIPCHK(c, hipMemcpyAsync(dRaw, raw, (size_t)n * kNrcRaw * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->nrcCount, &n, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIPCHK(c, nrc_infer(c->nrc, dRaw, c->nrcCount, n, dOut, nullptr, nullptr, s));
+    HIPCHK(c, hipMemcpyAsync(out, dOut, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return LTS_OK;
+}
+
+int lts_nrc_train_step(lts_ctx* c, const float* raw, const float* target, int32_t n, float* loss, float* grad) {
+    if (!c || n < 0 || (n > 0 && (!raw || !target))) return fail(c, LTS_E_ARG, "bad nrc_train_step arguments");
+    int rc = nrc_ensure(c);
+    if (rc != LTS_OK) return rc;
+    if (n > c->nrc.batchCap) return fail(c, LTS_E_ARG, "more samples than the configured batch");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    if (n > 0) {
+        HIPCHK(c, hipMemcpyAsync(c->nrc.trainRaw, raw, (size_t)n * kNrcRaw * sizeof(float), hipMemcpyHostToDevice, s));
+        HIPCHK(c, hipMemcpyAsync(c->nrc.trainTarget, target, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(c, hipMemcpyAsync(&c->nrc.state->nTrain, &n, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIPCHK(c, nrc_train_step(c->nrc, s));
+    NrcState st;
+    HIPCHK(c, hipMemcpyAsync(&st, c->nrc.state, sizeof(st), hipMemcpyDeviceToHost, s));
+    if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->nrc.grad, kNrcParams * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (loss) *loss = n > 0 ? st.loss : 0.0f;
+    return LTS_OK;
 }
 
 int lts_device_copy_benchmark(lts_ctx* c, uint64_t bytes, int32_t repeat, double* ms) {

@@ -122,4 +122,55 @@ void wf_release(WfBuffers& b);
 // Enqueues one pass: generate, then per bounce extend / (sort) / shade / shadow / mis / finalize.
 hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce, hipStream_t stream);
 
+// ---------------------------------------------------------------- learned radiance cache (nrc.hip)
+// Network: 16-float raw record -> 64-dim encoding -> 5 hidden layers of 64 (ReLU) -> 3 outputs.
+// Master weights are fp32, W_l[in][out] row-major: layers 0..4 at l*4096, the output layer
+// (64 x 16, columns 3..15 held at zero) at 20480.
+constexpr int kNrcRaw = 16;
+constexpr int kNrcParams = 5 * 4096 + 64 * 16;  // 21504
+constexpr int kNrcBwdElems = 6 * 4096;          // backward fragments (output layer padded)
+constexpr int kNrcKSplit = 1024;                // samples per split of the weight-gradient GEMM
+constexpr int kNrcTrainBlocks = 256;
+
+struct NrcQuery { float beta[3]; float a[3]; int32_t slot; int32_t pad; };  // slot: sample index into sampleL
+struct NrcState {
+    int32_t t;        // Adam steps taken
+    int32_t nTrain;   // samples in the current training batch
+    int32_t nQuery;   // cache queries of the current pass
+    int32_t pad;
+    float c1, c2;     // Adam bias corrections 1/(1-beta1^t), 1/(1-beta2^t) of the current step
+    float loss;       // mean loss of the last step
+    float pad2;
+};
+
+struct NrcBuffers {
+    int precision = 0;  // 0 bf16, 1 fp16
+    int batchCap = 0;   // training samples per step (multiple of kNrcKSplit)
+    float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-15f;
+    float* master = nullptr;      // kNrcParams
+    float* m = nullptr;           // Adam first moment
+    float* v = nullptr;           // Adam second moment
+    float* grad = nullptr;        // gradient of the last step (kNrcParams)
+    uint16_t* fwd = nullptr;      // forward MFMA fragments (kNrcParams)
+    uint16_t* bwd = nullptr;      // backward MFMA fragments (kNrcBwdElems)
+    uint16_t* H = nullptr;        // layer inputs [6][64][batchCap]
+    uint16_t* G = nullptr;        // layer output gradients [6][64][batchCap]
+    float* partial = nullptr;     // [batchCap / kNrcKSplit][kNrcParams]
+    float* lossPartial = nullptr; // kNrcTrainBlocks
+    NrcState* state = nullptr;
+    float* trainRaw = nullptr;    // [batchCap][16]
+    float* trainTarget = nullptr; // [batchCap][3]
+};
+
+hipError_t nrc_reserve(NrcBuffers& b, int batchCap);
+void nrc_release(NrcBuffers& b);
+// master -> MFMA fragments (after weights are set)
+hipError_t nrc_pack(const NrcBuffers& b, hipStream_t stream);
+// Evaluates min(*nDev, nCap) raw records. With meta == nullptr, out receives 3 floats per record;
+// otherwise sampleL[3 slot + c] += beta_c * max(y_c, 0) * a_c for each query.
+hipError_t nrc_infer(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out,
+                     const NrcQuery* meta, float* sampleL, hipStream_t stream);
+// One Adam step on min(state->nTrain, batchCap) samples of trainRaw / trainTarget.
+hipError_t nrc_train_step(const NrcBuffers& b, hipStream_t stream);
+
 }  // namespace lts

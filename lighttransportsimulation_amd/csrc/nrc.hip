@@ -1,0 +1,426 @@
+// Learned radiance cache network on CDNA4 matrix cores (v_mfma_f32_16x16x32_{bf16,f16}).
+//
+// Orientation. Every layer is evaluated as Y^T = W^T X^T: the weights are the A operand (16 output
+// neurons x 32 inputs per fragment) and 16 samples are the B operand's columns. The accumulator of
+// a 16x16 tile holds, in lane l, neurons 4(l/16)+i of sample l%16, which is already the sample
+// column the next layer's B operand needs; only the order of the K (input-neuron) index differs.
+// Since an MFMA sums over K, any permutation of K applied to both operands gives the same product,
+// so the weight fragments are stored with K permuted to match the accumulator registers:
+//   K slot (ks, g, j) of lane group g = l/16  <->  neuron 16 (2 ks + j/4) + 4 g + (j % 4).
+// Activations therefore pass from layer to layer in registers (fp32 accumulate, ReLU, round to
+// bf16/fp16), with no LDS round trip. Weight fragments sit in LDS (43 KB forward, 49 KB backward).
+//
+// Training step (batch B): one fused kernel runs forward, relative-L2 loss and the backward pass of
+// the activations for 16-sample tiles, writing each layer's input H_l and output gradient G_l
+// (rounded as used); a split-K MFMA GEMM forms dW_l = H_l G_l^T per 1024-sample split; one kernel
+// sums the splits in a fixed order and applies Adam, writing the fp32 master weights and both
+// fragment copies. Every reduction has a fixed order, so a step is deterministic.
+#include "lts_device.h"
+#include "lts_internal.h"
+
+namespace lts {
+
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+template <typename T> struct Vec8;
+template <> struct Vec8<__bf16> { typedef bf16x8 type; };
+template <> struct Vec8<_Float16> { typedef f16x8 type; };
+
+LTS_D f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+LTS_D f32x4 mfma(f16x8 a, f16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+
+constexpr int kFwdVec = kNrcParams / 8;    // 16-byte fragment rows of the forward weights
+constexpr int kBwdVec = kNrcBwdElems / 8;
+constexpr int kTrainBlock = 512;
+
+// Neuron held by K slot (ks, g, j).
+LTS_D int slot_neuron(int ks, int g, int j) { return 16 * (2 * ks + (j >> 2)) + 4 * g + (j & 3); }
+
+LTS_D int w_index(int l, int in, int out) { return l < 5 ? l * 4096 + in * 64 + out : 20480 + in * 16 + out; }
+
+// One-blob bin value: triweight kernel (1 - d^2/9)^3 of the distance d to the bin centre in
+// units of the bin width 1/4 (support 3 bins).
+LTS_D float one_blob(float s, int bin) {
+    const float d = (s - ((float)bin + 0.5f) * 0.25f) * 4.0f;
+    const float u = fmaxf(1.0f - d * d * (1.0f / 9.0f), 0.0f);
+    return u * u * u;
+}
+
+// Encoding feature idx (0..63) of one raw record: 36 triangle waves of frequency 2^(f-1) on the
+// normalised position, one-blob (4 bins) of the direction and normal angles and of roughness,
+// diffuse and specular reflectance, and two constant ones.
+LTS_D float nrc_feature(const float* __restrict__ raw, int idx) {
+    if (idx < 36) {
+        const int d = idx / 12, f = idx % 12;
+        const float x = ldexpf(raw[d], f - 1);
+        const float fr = x - floorf(x);
+        return fabsf(2.0f * fr - 1.0f);
+    }
+    if (idx < 52) return one_blob(raw[3 + (idx - 36) / 4], idx & 3);
+    if (idx < 56) return one_blob(raw[7], idx - 52);
+    if (idx < 62) return raw[8 + idx - 56];
+    return 1.0f;
+}
+
+template <typename T>
+LTS_D typename Vec8<T>::type lds_frag(const uint4* w, int row) {
+    const uint4 q = w[row];
+    return __builtin_bit_cast(typename Vec8<T>::type, q);
+}
+
+// Copies `count` 16-byte rows from global to LDS with the whole block.
+LTS_D void stage(uint4* dst, const uint16_t* src, int count) {
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    for (int i = threadIdx.x; i < count; i += blockDim.x) dst[i] = s[i];
+}
+
+// Forward pass of one 16-sample tile; h[l] receives the B-operand fragments of layer l's input.
+// With Reload, weight fragments are read from LDS layer by layer (a compiler barrier keeps the
+// loads from being hoisted out of the tile loop and held in registers across it).
+template <typename T, bool Reload>
+LTS_D f32x4 forward_tile(const uint4* wf, const float* __restrict__ raw, bool valid, int lane,
+                         typename Vec8<T>::type (&h)[6][2]) {
+    const int g = lane >> 4;
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) h[0][ks][j] = valid ? (T)nrc_feature(raw, slot_neuron(ks, g, j)) : (T)0.0f;
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+        if (Reload) asm volatile("" ::: "memory");
+        f32x4 acc[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) acc[mt] = mfma(lds_frag<T>(wf, l * 512 + (mt * 2 + ks) * 64 + lane), h[l][ks], acc[mt]);
+        }
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) h[l + 1][ks][j] = (T)fmaxf(acc[2 * ks + (j >> 2)][j & 3], 0.0f);
+    }
+    if (Reload) asm volatile("" ::: "memory");
+    f32x4 y = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) y = mfma(lds_frag<T>(wf, 2560 + ks * 64 + lane), h[5][ks], y);
+    return y;  // lane group 0 holds outputs 0..3 of its sample
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_nrc_infer(const uint16_t* __restrict__ fwd, const float* __restrict__ raw,
+                                                   const int32_t* __restrict__ nDev, int nCap, float* __restrict__ out,
+                                                   const NrcQuery* __restrict__ meta, float* __restrict__ sampleL) {
+    __shared__ uint4 wf[kFwdVec];
+    stage(wf, fwd, kFwdVec);
+    __syncthreads();
+    const int n = min(*nDev, nCap);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tiles = (n + 15) >> 4;
+    for (int t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
+        const int s = t * 16 + (lane & 15);
+        const bool valid = s < n;
+        typename Vec8<T>::type h[6][2];
+        const f32x4 y = forward_tile<T, false>(wf, raw + (size_t)(valid ? s : 0) * kNrcRaw, valid, lane, h);
+        if (lane < 16 && valid) {
+            if (meta) {
+                const NrcQuery q = meta[s];
+                float* o = sampleL + 3 * (size_t)q.slot;
+                o[0] += q.beta[0] * fmaxf(y[0], 0.0f) * q.a[0];
+                o[1] += q.beta[1] * fmaxf(y[1], 0.0f) * q.a[1];
+                o[2] += q.beta[2] * fmaxf(y[2], 0.0f) * q.a[2];
+            } else {
+                out[3 * (size_t)s + 0] = y[0];
+                out[3 * (size_t)s + 1] = y[1];
+                out[3 * (size_t)s + 2] = y[2];
+            }
+        }
+    }
+}
+
+// Forward + loss + backward of the activations. H / G hold, per layer (stride 64 ld), blocks of 32
+// samples laid out [neuron][32] (element (l, neuron, s) at l 64 ld + (s/32) 2048 + 32 neuron + s%32),
+// so the GEMM's 8-sample operand rows are 16 contiguous bytes. Samples in [n, roundup(n, 32)) are
+// written as zeros so the weight-gradient GEMM can run over whole 32-sample steps.
+template <typename T>
+__global__ __launch_bounds__(kTrainBlock) void k_nrc_fwdbwd(const uint16_t* __restrict__ fwd, const uint16_t* __restrict__ bwd,
+                                                            const float* __restrict__ raw, const float* __restrict__ target,
+                                                            const NrcState* __restrict__ state, int nCap, T* __restrict__ H,
+                                                            T* __restrict__ G, int ld, float* __restrict__ lossPartial) {
+    __shared__ uint4 wf[kFwdVec];
+    __shared__ uint4 wb[kBwdVec];
+    __shared__ float red[kTrainBlock];
+    stage(wf, fwd, kFwdVec);
+    stage(wb, bwd, kBwdVec);
+    __syncthreads();
+    const int n = min(state->nTrain, nCap);
+    const int npad = (n + 31) & ~31;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
+    const int wavesPerBlock = kTrainBlock / 64;
+    float lossAcc = 0.0f;
+    for (int t = blockIdx.x * wavesPerBlock + wave; t < (npad >> 4); t += gridDim.x * wavesPerBlock) {
+        const int s = t * 16 + (lane & 15);
+        const bool valid = s < n;
+        typename Vec8<T>::type h[6][2];
+        const f32x4 y = forward_tile<T, true>(wf, raw + (size_t)(valid ? s : 0) * kNrcRaw, valid, lane, h);
+        // relative L2 loss (y - t)^2 / (y^2 + 0.01) with the denominator held constant
+        float dy[4] = {0.f, 0.f, 0.f, 0.f};
+        if (g == 0 && valid) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float diff = y[c] - target[3 * (size_t)s + c];
+                const float den = y[c] * y[c] + 0.01f;
+                lossAcc += diff * diff / den;
+                dy[c] = 2.0f * diff / den;
+            }
+        }
+        typename Vec8<T>::type gb[2];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            gb[0][j] = j < 4 ? (T)dy[j & 3] : (T)0.0f;
+            gb[1][j] = (T)0.0f;
+        }
+        // layer inputs H_0..H_5 and the output gradient G_5 (rows 4g + j); neuron 4g + c lives at
+        // 32 (4g + c) within the sample's block, so only c varies between the stores
+        const size_t layer = (size_t)64 * ld;
+        T* Hs = H + (size_t)(s >> 5) * 2048 + (s & 31) + 128 * g;
+        T* Gs = G + (size_t)(s >> 5) * 2048 + (s & 31) + 128 * g;
+#pragma unroll
+        for (int l = 0; l < 6; l++)
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) Hs[l * layer + 32 * (slot_neuron(ks, 0, j))] = h[l][ks][j];
+#pragma unroll
+        for (int j = 0; j < 4; j++) Gs[5 * layer + 32 * j] = gb[0][j];
+        // back-propagate: g_l = (W_l g_{l+1}) * [h_l > 0], stored as G_{l-1}
+#pragma unroll
+        for (int l = 5; l >= 1; l--) {
+            asm volatile("" ::: "memory");
+            f32x4 acc[4];
+#pragma unroll
+            for (int it = 0; it < 4; it++) {
+                acc[it] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ks = 0; ks < (l == 5 ? 1 : 2); ks++)
+                    acc[it] = mfma(lds_frag<T>(wb, l * 512 + (it * 2 + ks) * 64 + lane), gb[ks], acc[it]);
+            }
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const float v = acc[2 * ks + (j >> 2)][j & 3];
+                    gb[ks][j] = (float)h[l][ks][j] > 0.0f ? (T)v : (T)0.0f;
+                    Gs[(l - 1) * layer + 32 * slot_neuron(ks, 0, j)] = gb[ks][j];
+                }
+        }
+    }
+    red[threadIdx.x] = lossAcc;
+    __syncthreads();
+    for (int w = kTrainBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lossPartial[blockIdx.x] = red[0];
+}
+
+// dW_l = H_l G_l^T over the samples of split blockIdx.y; 84 16x16 tiles, one per wave.
+template <typename T>
+__global__ __launch_bounds__(256) void k_nrc_wgrad(const T* __restrict__ H, const T* __restrict__ G, int ld,
+                                                   const NrcState* __restrict__ state, int nCap, float* __restrict__ partial) {
+    using V = typename Vec8<T>::type;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
+    const int tile = blockIdx.x * 4 + wave;  // 0..83
+    int l, it, ot;
+    if (tile < 80) { l = tile >> 4; it = (tile >> 2) & 3; ot = tile & 3; }
+    else { l = 5; it = tile - 80; ot = 0; }
+    const int n = min(state->nTrain, nCap);
+    const int npad = (n + 31) & ~31;
+    const int k0 = blockIdx.y * kNrcKSplit, k1 = min(npad, k0 + kNrcKSplit);
+    const T* a = H + (size_t)l * 64 * ld + (size_t)(k0 >> 5) * 2048 + 32 * (16 * it + col) + 8 * g;
+    const T* b = G + (size_t)l * 64 * ld + (size_t)(k0 >> 5) * 2048 + 32 * (16 * ot + col) + 8 * g;
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k = k0; k < k1; k += 32, a += 2048, b += 2048)
+        acc = mfma(*reinterpret_cast<const V*>(a), *reinterpret_cast<const V*>(b), acc);
+    float* p = partial + (size_t)blockIdx.y * kNrcParams;
+#pragma unroll
+    for (int i = 0; i < 4; i++) p[w_index(l, 16 * it + 4 * g + i, 16 * ot + col)] = acc[i];
+}
+
+// Advances the step counter (when the batch is not empty) and the bias corrections.
+__global__ void k_nrc_step_begin(NrcState* __restrict__ state, int nCap, float beta1, float beta2,
+                                 const float* __restrict__ lossPartial, int nPartial) {
+    if (threadIdx.x != 0) return;
+    const int n = min(state->nTrain, nCap);
+    if (n <= 0) return;
+    const int t = state->t + 1;
+    state->t = t;
+    state->c1 = (float)(1.0 / (1.0 - pow((double)beta1, (double)t)));
+    state->c2 = (float)(1.0 / (1.0 - pow((double)beta2, (double)t)));
+    float s = 0.0f;
+    for (int i = 0; i < nPartial; i++) s += lossPartial[i];
+    state->loss = s / (float)n;
+}
+
+LTS_D int fwd_pos(int l, int in, int out) {
+    // A operand of W^T: row = out, K slot of `in`
+    const int q = in >> 4, r = in & 15;
+    const int ks = q >> 1, j = ((q & 1) << 2) | (r & 3), g = r >> 2;
+    const int lane = 16 * g + (out & 15);
+    return l * 4096 + (((out >> 4) * 2 + ks) * 64 + lane) * 8 + j;
+}
+
+LTS_D int bwd_pos(int l, int in, int out) {
+    // A operand of W: row = in, K slot of `out`
+    const int q = out >> 4, r = out & 15;
+    const int ks = q >> 1, j = ((q & 1) << 2) | (r & 3), g = r >> 2;
+    const int lane = 16 * g + (in & 15);
+    return l * 4096 + (((in >> 4) * 2 + ks) * 64 + lane) * 8 + j;
+}
+
+LTS_D void decode_param(int p, int& l, int& in, int& out) {
+    if (p < 20480) { l = p >> 12; in = (p >> 6) & 63; out = p & 63; }
+    else { l = 5; in = (p - 20480) >> 4; out = (p - 20480) & 15; }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_nrc_adam(const float* __restrict__ partial, int splits, const NrcState* __restrict__ state,
+                                                  int nCap, float lr, float beta1, float beta2, float eps,
+                                                  float* __restrict__ master, float* __restrict__ m, float* __restrict__ v,
+                                                  float* __restrict__ gradOut, T* __restrict__ fwd, T* __restrict__ bwd) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= kNrcParams) return;
+    const int n = min(state->nTrain, nCap);
+    if (n <= 0) return;
+    int l, in, out;
+    decode_param(p, l, in, out);
+    if (l == 5 && out >= 3) return;  // padding columns stay zero
+    float sum = 0.0f;
+    for (int s = 0; s < splits; s++) sum += partial[(size_t)s * kNrcParams + p];
+    const float gr = sum / (float)n;
+    gradOut[p] = gr;
+    const float mm = beta1 * m[p] + (1.0f - beta1) * gr;
+    const float vv = beta2 * v[p] + (1.0f - beta2) * gr * gr;
+    m[p] = mm;
+    v[p] = vv;
+    const float w = master[p] - lr * (mm * state->c1) / (sqrtf(vv * state->c2) + eps);
+    master[p] = w;
+    fwd[fwd_pos(l, in, out)] = (T)w;
+    bwd[bwd_pos(l, in, out)] = (T)w;
+}
+
+// Writes every fragment element from the master weights (slots outside the network get zero).
+template <typename T>
+__global__ __launch_bounds__(256) void k_nrc_pack(const float* __restrict__ master, T* __restrict__ fwd, T* __restrict__ bwd) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e < kNrcParams) {
+        const int l = e >> 12, rem = e & 4095;
+        const int j = rem & 7, lane = (rem >> 3) & 63, ks = (rem >> 9) & 1, mt = rem >> 10;
+        const int out = 16 * mt + (lane & 15), in = slot_neuron(ks, lane >> 4, j);
+        fwd[e] = (T)master[w_index(l, in, out)];
+    }
+    if (e < kNrcBwdElems) {
+        const int l = e >> 12, rem = e & 4095;
+        const int j = rem & 7, lane = (rem >> 3) & 63, ks = (rem >> 9) & 1, it = rem >> 10;
+        const int in = 16 * it + (lane & 15), out = slot_neuron(ks, lane >> 4, j);
+        bwd[e] = (l < 5 || out < 16) ? (T)master[w_index(l, in, out)] : (T)0.0f;
+    }
+}
+
+template <typename T>
+hipError_t infer_t(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out, const NrcQuery* meta,
+                   float* sampleL, hipStream_t s) {
+    const int tiles = (nCap + 15) / 16;
+    const int blocks = std::max(1, std::min((tiles + 3) / 4, 2048));
+    k_nrc_infer<T><<<blocks, 256, 0, s>>>(b.fwd, raw, nDev, nCap, out, meta, sampleL);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t train_t(const NrcBuffers& b, hipStream_t s) {
+    T* H = reinterpret_cast<T*>(b.H);
+    T* G = reinterpret_cast<T*>(b.G);
+    const int tiles = b.batchCap / 16;
+    const int blocks = std::min(kNrcTrainBlocks, (tiles + kTrainBlock / 64 - 1) / (kTrainBlock / 64));
+    k_nrc_fwdbwd<T><<<blocks, kTrainBlock, 0, s>>>(b.fwd, b.bwd, b.trainRaw, b.trainTarget, b.state, b.batchCap, H, G,
+                                                     b.batchCap, b.lossPartial);
+    const int splits = b.batchCap / kNrcKSplit;
+    k_nrc_wgrad<T><<<dim3(21, splits), 256, 0, s>>>(H, G, b.batchCap, b.state, b.batchCap, b.partial);
+    k_nrc_step_begin<<<1, 64, 0, s>>>(b.state, b.batchCap, b.beta1, b.beta2, b.lossPartial, blocks);
+    k_nrc_adam<T><<<(kNrcParams + 255) / 256, 256, 0, s>>>(b.partial, splits, b.state, b.batchCap, b.lr, b.beta1, b.beta2,
+                                                           b.eps, b.master, b.m, b.v, b.grad, reinterpret_cast<T*>(b.fwd),
+                                                           reinterpret_cast<T*>(b.bwd));
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t nrc_pack(const NrcBuffers& b, hipStream_t s) {
+    const int blocks = (kNrcBwdElems + 255) / 256;
+    if (b.precision == 1)
+        k_nrc_pack<_Float16><<<blocks, 256, 0, s>>>(b.master, reinterpret_cast<_Float16*>(b.fwd), reinterpret_cast<_Float16*>(b.bwd));
+    else
+        k_nrc_pack<__bf16><<<blocks, 256, 0, s>>>(b.master, reinterpret_cast<__bf16*>(b.fwd), reinterpret_cast<__bf16*>(b.bwd));
+    return hipGetLastError();
+}
+
+hipError_t nrc_infer(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out, const NrcQuery* meta,
+                     float* sampleL, hipStream_t s) {
+    if (nCap <= 0) return hipSuccess;
+    return b.precision == 1 ? infer_t<_Float16>(b, raw, nDev, nCap, out, meta, sampleL, s)
+                            : infer_t<__bf16>(b, raw, nDev, nCap, out, meta, sampleL, s);
+}
+
+hipError_t nrc_train_step(const NrcBuffers& b, hipStream_t s) {
+    return b.precision == 1 ? train_t<_Float16>(b, s) : train_t<__bf16>(b, s);
+}
+
+template <typename X>
+static hipError_t nrc_alloc(X** p, size_t count) {
+    return hipMalloc((void**)p, (count ? count : 1) * sizeof(X));
+}
+
+hipError_t nrc_reserve(NrcBuffers& b, int batchCap) {
+    const int cap = std::max(kNrcKSplit, (batchCap + kNrcKSplit - 1) / kNrcKSplit * kNrcKSplit);
+    if (b.master && cap == b.batchCap) return hipSuccess;
+    const int precision = b.precision;
+    const float lr = b.lr, b1 = b.beta1, b2 = b.beta2, eps = b.eps;
+    nrc_release(b);
+    b.precision = precision;
+    b.lr = lr; b.beta1 = b1; b.beta2 = b2; b.eps = eps;
+    hipError_t e = hipSuccess;
+#define NTRY(x) do { e = (x); if (e != hipSuccess) { nrc_release(b); return e; } } while (0)
+    NTRY(nrc_alloc(&b.master, kNrcParams));
+    NTRY(nrc_alloc(&b.m, kNrcParams));
+    NTRY(nrc_alloc(&b.v, kNrcParams));
+    NTRY(nrc_alloc(&b.grad, kNrcParams));
+    NTRY(nrc_alloc(&b.fwd, kNrcParams));
+    NTRY(nrc_alloc(&b.bwd, kNrcBwdElems));
+    NTRY(nrc_alloc(&b.H, (size_t)6 * 64 * cap));
+    NTRY(nrc_alloc(&b.G, (size_t)6 * 64 * cap));
+    NTRY(nrc_alloc(&b.partial, (size_t)(cap / kNrcKSplit) * kNrcParams));
+    NTRY(nrc_alloc(&b.lossPartial, kNrcTrainBlocks));
+    NTRY(nrc_alloc(&b.state, 1));
+    NTRY(nrc_alloc(&b.trainRaw, (size_t)cap * kNrcRaw));
+    NTRY(nrc_alloc(&b.trainTarget, (size_t)cap * 3));
+#undef NTRY
+    b.batchCap = cap;
+    return hipSuccess;
+}
+
+void nrc_release(NrcBuffers& b) {
+    void* ps[] = {b.master, b.m, b.v, b.grad, b.fwd, b.bwd, b.H, b.G, b.partial, b.lossPartial, b.state, b.trainRaw, b.trainTarget};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    const int precision = b.precision;
+    const float lr = b.lr, b1 = b.beta1, b2 = b.beta2, eps = b.eps;
+    b = NrcBuffers();
+    b.precision = precision;
+    b.lr = lr; b.beta1 = b1; b.beta2 = b2; b.eps = eps;
+}
+
+}  // namespace lts

@@ -90,6 +90,17 @@ float oracle_fr_dielectric(float cos_theta_i, float eta_i, float eta_t);
 // 2: sample with u -> out = {wi.x, wi.y, wi.z, pdf, f.r, sampledType}
 void oracle_bxdf(int32_t kind, float param, int32_t mode, const float wo[3], const float wi[3], const float u[2], float* out);
 float oracle_roughness_to_alpha(float roughness);
+
+// Learned radiance cache network (nrc.cpp). Parameters: 5 hidden 64x64 matrices W[in][out]
+// followed by the 64x16 output matrix (columns 3..15 padding, held at 0), 21504 floats.
+// Raw record (16 floats): normalised position (3), outgoing direction theta/pi, phi/2pi (2),
+// normal theta/pi, phi/2pi (2), roughness (1), diffuse reflectance (3), specular reflectance (3), 2 unused.
+int32_t oracle_nrc_param_count(void);
+void oracle_nrc_encode(const float raw[16], float out[64]);
+// rounding: 0 none (fp64), 1 bf16, 2 fp16 at the GPU kernels' rounding points.
+void oracle_nrc_forward(const float* params, const float* raw, int32_t n, int32_t rounding, double* y);
+double oracle_nrc_grad(const float* params, const float* raw, const float* target, int32_t n, int32_t rounding,
+                       const double* denom_ref, double* grad, double* y_out);
 #ifdef __cplusplus
 }
 #endif

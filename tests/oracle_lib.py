@@ -57,6 +57,12 @@ def lib():
         L.oracle_occluded.restype = C.c_int32
         L.oracle_occluded.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
         L.oracle_filter_table.argtypes = [C.c_void_p]
+        L.oracle_nrc_param_count.restype = C.c_int32
+        L.oracle_nrc_encode.argtypes = [C.c_void_p, C.c_void_p]
+        L.oracle_nrc_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
+        L.oracle_nrc_grad.restype = C.c_double
+        L.oracle_nrc_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]
         _lib = L
     return _lib
 
@@ -134,3 +140,41 @@ class OracleScene:
         if getattr(self, "h", None):
             lib().oracle_free_scene(self.h)
             self.h = None
+
+
+# ---------------------------------------------------------------- learned radiance cache network
+NRC_ROUNDING = {"none": 0, "bf16": 1, "fp16": 2}
+
+
+def nrc_param_count() -> int:
+    return lib().oracle_nrc_param_count()
+
+
+def nrc_encode(raw: np.ndarray) -> np.ndarray:
+    raw = np.ascontiguousarray(raw, np.float32).reshape(-1, 16)
+    out = np.zeros((raw.shape[0], 64), np.float32)
+    for i in range(raw.shape[0]):
+        lib().oracle_nrc_encode(_p(raw[i]), _p(out[i]))
+    return out
+
+
+def nrc_forward(params: np.ndarray, raw: np.ndarray, rounding: str = "none") -> np.ndarray:
+    params = np.ascontiguousarray(params, np.float32)
+    raw = np.ascontiguousarray(raw, np.float32).reshape(-1, 16)
+    y = np.zeros((raw.shape[0], 3), np.float64)
+    lib().oracle_nrc_forward(_p(params), _p(raw), raw.shape[0], NRC_ROUNDING[rounding], _p(y))
+    return y
+
+
+def nrc_grad(params, raw, target, rounding: str = "none", denom_ref=None, want_grad=True):
+    """Returns (loss, grad or None, y) of the relative L2 loss with stop-gradient denominators."""
+    params = np.ascontiguousarray(params, np.float32)
+    raw = np.ascontiguousarray(raw, np.float32).reshape(-1, 16)
+    target = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+    n = raw.shape[0]
+    grad = np.zeros(params.size, np.float64) if want_grad else None
+    y = np.zeros((n, 3), np.float64)
+    d = None if denom_ref is None else np.ascontiguousarray(denom_ref, np.float64)
+    loss = lib().oracle_nrc_grad(_p(params), _p(raw), _p(target), n, NRC_ROUNDING[rounding],
+                                 None if d is None else _p(d), None if grad is None else _p(grad), _p(y))
+    return loss, grad, y
