@@ -87,7 +87,10 @@ typedef struct lts_stats {
     int32_t sort_materials;
     int32_t use_graph;
     double autotune_ms;     /* time spent in the autotuner by this call             */
-    double reserved[3];
+    double nrc_loss;        /* cached: loss of the last training step               */
+    int64_t nrc_queries;    /* cached: cache queries of the last pass               */
+    int32_t nrc_train_samples; /* cached: training samples of the last step         */
+    int32_t nrc_steps;      /* cached: Adam steps taken by the cache so far          */
 } lts_stats;
 
 /* Context. */
@@ -151,8 +154,10 @@ typedef struct lts_nrc_config {
     int32_t batch;          /* training samples per Adam step (default 65536)                       */
     int32_t query_bounce;   /* cached renders query the cache at the first non-specular hit of      */
                             /* this bounce or later (default 2)                                     */
-    int32_t train_paths;    /* paths per pass traced to the end as training paths (default 16384)   */
-    int32_t warmup_passes;  /* passes of a cached render traced without queries (default 4, -1: 0)  */
+    int32_t train_paths;    /* paths per pass traced to the end as training paths (default 16384,   */
+                            /* at most 1/8 of the pixels)                                           */
+    int32_t warmup_passes;  /* the first passes of cached rendering after this call are traced       */
+                            /* without queries while the cache trains (default 4, -1: none)        */
     uint32_t seed;          /* weight initialisation (Xavier-uniform from a SplitMix64 stream)      */
     float lr, beta1, beta2, eps;  /* Adam (defaults 1e-2, 0.9, 0.99, 1e-15)                          */
     int32_t reserved[6];

@@ -62,7 +62,8 @@ class Stats(C.Structure):
                 ("launches", C.c_int64), ("samples", C.c_int64), ("msamples_per_s", C.c_double),
                 ("node_visits", C.c_int64), ("tri_tests", C.c_int64), ("trace_block", C.c_int32),
                 ("persistent_blocks", C.c_int32), ("sort_materials", C.c_int32), ("use_graph", C.c_int32),
-                ("autotune_ms", C.c_double), ("reserved", C.c_double * 3)]
+                ("autotune_ms", C.c_double), ("nrc_loss", C.c_double), ("nrc_queries", C.c_int64),
+                ("nrc_train_samples", C.c_int32), ("nrc_steps", C.c_int32)]
 
 
 _lib = None
@@ -259,7 +260,8 @@ def _stats_dict(st: Stats) -> dict:
                 launches=st.launches, samples=st.samples, msamples_per_s=st.msamples_per_s,
                 node_visits=st.node_visits, tri_tests=st.tri_tests, trace_block=st.trace_block,
                 persistent_blocks=st.persistent_blocks, sort_materials=st.sort_materials, use_graph=st.use_graph,
-                autotune_ms=st.autotune_ms)
+                autotune_ms=st.autotune_ms, nrc_loss=st.nrc_loss, nrc_queries=st.nrc_queries,
+                nrc_train_samples=st.nrc_train_samples, nrc_steps=st.nrc_steps)
 
 
 def render_batch(contexts, spp: int, max_bounce: int, rr_start: int, seed: int, stratified: int = 0,

@@ -54,6 +54,7 @@ struct DeviceScene {
     float* lcdf = nullptr;
     BvhBuffers bvh;
     SceneView view{};
+    float boxMin[3] = {0, 0, 0}, boxInv[3] = {0, 0, 0};  // vertex bounds, for the cache's positions
 };
 
 }  // namespace
@@ -81,6 +82,9 @@ struct lts_ctx {
     const void* wfKeyPaths = nullptr;
     int wfKeyBounce = -1, wfKeyBins = -1;
     WfTuning wfKeyTune;
+    const void* wfKeyNrc = nullptr;     // cache weights of a cached pass (nullptr: plain pass)
+    int wfKeyPrecision = -1;
+    const void* wfKeySamples = nullptr;
     WfParams wfParams{};                  // host copies of the per-pass parameters (stable for async H2D)
     std::vector<WfSceneSlot> wfSlots;
     std::map<std::string, WfTuning> tuned;  // autotuner results of this context
@@ -91,6 +95,7 @@ struct lts_ctx {
     float* nrcIo = nullptr;     // staging for lts_nrc_infer: raw records then outputs
     size_t nrcIoCap = 0;        // records
     int32_t* nrcCount = nullptr;  // device record count for lts_nrc_infer
+    uint64_t nrcPasses = 0;       // cached passes rendered (moves the training-pixel offset)
 };
 
 namespace {
@@ -175,13 +180,22 @@ int run_build(lts_ctx* c, BvhBuffers& b, const float* dsoup, int n, int mode, bo
     return LTS_OK;
 }
 
+uint64_t splitmix64(uint64_t& x) {
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+int nrc_ensure(lts_ctx* c);
+
 int check_render_params(lts_ctx* c, const lts_render_params* p) {
     if (p->spp < 1 || p->max_bounce < 0 || p->spp_per_pass < 0) return fail(c, LTS_E_ARG, "bad render parameters");
     if (p->stratified) {
         int s = (int)std::floor(std::sqrt((float)p->spp));
         if (s * s != p->spp) return fail(c, LTS_E_ARG, "stratified film sampling needs a square sample count");
     }
-    if (p->integrator == LTS_INTEGRATOR_WAVEFRONT) {
+    if (p->integrator == LTS_INTEGRATOR_WAVEFRONT || p->integrator == LTS_INTEGRATOR_CACHED) {
         if (p->max_bounce > kWfMaxBounce) return fail(c, LTS_E_ARG, "wavefront max_bounce above 64");
         if (p->trace_block != 0 && p->trace_block != 64 && p->trace_block != 128 && p->trace_block != 256)
             return fail(c, LTS_E_ARG, "trace_block must be 64, 128 or 256");
@@ -221,19 +235,22 @@ bool same_tuning(const WfTuning& a, const WfTuning& b) {
 
 // Runs the pass whose parameters are resident in c->wf: replays the captured graph, capturing it
 // first when the buffers, bounce count or tuning changed since the last capture.
-int wf_run(lts_ctx* c, const WfTuning& t, int maxBounce) {
+int wf_run(lts_ctx* c, const WfTuning& t, int maxBounce, const NrcBuffers* nrc) {
     hipStream_t s = c->stream;
     if (!t.useGraph) {
-        HIPCHK(c, wf_enqueue_pass(c->wf, t, maxBounce, s));
+        HIPCHK(c, wf_enqueue_pass(c->wf, t, maxBounce, nrc, s));
         return LTS_OK;
     }
+    const void* nrcKey = nrc ? (const void*)nrc->master : nullptr;
+    const int precision = nrc ? nrc->precision : -1;
     if (!c->wfExec || c->wfKeyPaths != c->wf.paths || c->wfKeyBounce != maxBounce || c->wfKeyBins != c->wf.matBins ||
-        !same_tuning(c->wfKeyTune, t)) {
+        !same_tuning(c->wfKeyTune, t) || c->wfKeyNrc != nrcKey || c->wfKeyPrecision != precision ||
+        c->wfKeySamples != c->wf.sampleBase) {
         if (c->wfExec) (void)hipGraphExecDestroy(c->wfExec);
         c->wfExec = nullptr;
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        hipError_t e = wf_enqueue_pass(c->wf, t, maxBounce, s);
+        hipError_t e = wf_enqueue_pass(c->wf, t, maxBounce, nrc, s);
         hipError_t e2 = hipStreamEndCapture(s, &g);
         if (e == hipSuccess) e = e2;
         if (e == hipSuccess) e = hipGraphInstantiate(&c->wfExec, g, nullptr, nullptr, 0);
@@ -243,14 +260,23 @@ int wf_run(lts_ctx* c, const WfTuning& t, int maxBounce) {
         c->wfKeyBounce = maxBounce;
         c->wfKeyBins = c->wf.matBins;
         c->wfKeyTune = t;
+        c->wfKeyNrc = nrcKey;
+        c->wfKeyPrecision = precision;
+        c->wfKeySamples = c->wf.sampleBase;
     }
     HIPCHK(c, hipGraphLaunch(c->wfExec, s));
     return LTS_OK;
 }
 
+// Radiance-cache settings of one pass (queries off and no training paths for plain passes).
+struct PassCache {
+    int queryBounce = 1 << 30;
+    int trainStride = 1, trainOffset = 0, trainSlots = 0;
+};
+
 // Writes the pass parameters for samples [s0, s1) of every scene to the device.
 int wf_upload_pass(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_render_params* p, int s0, int s1,
-                   int64_t* paths) {
+                   int64_t* paths, const PassCache& pc = PassCache()) {
     const int n = (int)scenes.size();
     c->wfSlots.resize(n);
     int64_t base = 0;
@@ -261,12 +287,21 @@ int wf_upload_pass(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_re
         sl.width = cam.width; sl.height = cam.height;
         sl.spp = p->spp; sl.stratified = p->stratified; sl.seed = p->seed; sl.s0 = s0;
         sl.sampleL = c->sampleL + 3 * base;
+        for (int a = 0; a < 3; a++) {
+            sl.boxMin[a] = scenes[i]->scene.boxMin[a];
+            sl.boxInv[a] = scenes[i]->scene.boxInv[a];
+        }
         base += (int64_t)cam.width * cam.height * (s1 - s0);
     }
     c->wfParams.nPaths = base;
     c->wfParams.nScenes = n;
     c->wfParams.maxBounce = p->max_bounce;
     c->wfParams.rrStart = p->rr_start;
+    c->wfParams.queryBounce = pc.queryBounce;
+    c->wfParams.trainStride = pc.trainStride;
+    c->wfParams.trainOffset = pc.trainOffset;
+    c->wfParams.trainSlots = pc.trainSlots;
+    c->wfParams.pad = 0;
     HIPCHK(c, hipMemcpyAsync(c->wf.params, &c->wfParams, sizeof(WfParams), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->wf.slots, c->wfSlots.data(), n * sizeof(WfSceneSlot), hipMemcpyHostToDevice, c->stream));
     *paths = base;
@@ -307,11 +342,11 @@ void tune_cache_append(const std::string& key, const WfTuning& t) {
 int time_pass(lts_ctx* c, const WfTuning& t, int maxBounce, int reps, double* ms) {
     WfTuning u = t;
     u.useGraph = 0;
-    int rc = wf_run(c, u, maxBounce);
+    int rc = wf_run(c, u, maxBounce, nullptr);
     if (rc != LTS_OK) return rc;
     HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
     for (int r = 0; r < reps; r++)
-        if ((rc = wf_run(c, u, maxBounce)) != LTS_OK) return rc;
+        if ((rc = wf_run(c, u, maxBounce, nullptr)) != LTS_OK) return rc;
     HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev[6]));
     *ms = ms_between(c->ev[5], c->ev[6]) / reps;
@@ -366,7 +401,13 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         totalPix += (size_t)x->scene.view.cam.width * x->scene.view.cam.height;
         maxMats = std::max(maxMats, (int)x->scene.host.mats.size());
     }
-    int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, ((size_t)1 << 24) / totalPix);
+    const bool cached = p->integrator == LTS_INTEGRATOR_CACHED;
+    if (cached && scenes.size() != 1) return fail(c, LTS_E_ARG, "the cached integrator renders one scene");
+    int rc = LTS_OK;
+    if (cached && (rc = nrc_ensure(c)) != LTS_OK) return rc;
+    // cached renders train once per pass: one sample per pixel per pass at 1080p
+    const size_t target = cached ? ((size_t)1 << 21) : ((size_t)1 << 24);
+    int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, target / totalPix);
     if (sample_rgb) pass = p->spp;
     pass = std::min(pass, p->spp);
     const size_t cap = totalPix * (size_t)pass;
@@ -378,8 +419,12 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         c->wfExec = nullptr;
     }
     c->wf.matBins = std::min(maxMats, kWfMaxMatBins - 1) + 1;
-    int rc = ensure_samples(c, cap * 3);
-    if (rc != LTS_OK) return rc;
+    if ((rc = ensure_samples(c, cap * 3)) != LTS_OK) return rc;
+    c->wf.sampleBase = c->sampleL;
+    // training paths stay a minority of the pass (at most 1/8 of the pixels) so most paths query
+    const int trainPaths = cached ? (int)std::max<size_t>(1, std::min<size_t>((size_t)c->nrcCfg.train_paths, totalPix / 8)) : 0;
+    if (cached) HIPCHK(c, wf_reserve_cache(c->wf, cap, trainPaths));
+    const NrcBuffers* nrc = cached ? &c->nrc : nullptr;
     std::vector<SceneView> views;
     for (lts_ctx* x : scenes) {
         views.push_back(x->scene.view);
@@ -392,7 +437,7 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
 
     WfTuning t;
     double tuneMs = 0;
-    if (p->autotune) {
+    if (p->autotune && !cached) {
         const std::string key = tune_key(scenes, p, pass);
         auto it = c->tuned.find(key);
         if (it != c->tuned.end()) {
@@ -416,12 +461,23 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
 
     double renderMs = 0, resolveMs = 0;
     int64_t launches = 0;
+    const size_t npix0 = (size_t)views[0].cam.width * views[0].cam.height;
     for (int s0 = 0; s0 < p->spp; s0 += pass) {
         const int s1 = std::min(p->spp, s0 + pass);
+        PassCache pc;
+        if (cached) {
+            // training paths: every stride-th pixel from an offset that moves from pass to pass
+            pc.queryBounce = c->nrcPasses < (uint64_t)c->nrcCfg.warmup_passes ? (1 << 30) : c->nrcCfg.query_bounce;
+            pc.trainStride = (int)((npix0 + trainPaths - 1) / trainPaths);
+            uint64_t h = 0x70617373ull + (uint64_t)c->nrcPasses * 0x9e3779b97f4a7c15ull;
+            pc.trainOffset = (int)(splitmix64(h) % (uint64_t)pc.trainStride);
+            pc.trainSlots = (int)((npix0 - pc.trainOffset + pc.trainStride - 1) / pc.trainStride);
+            c->nrcPasses++;
+        }
         int64_t paths = 0;
-        if ((rc = wf_upload_pass(c, scenes, p, s0, s1, &paths)) != LTS_OK) return rc;
+        if ((rc = wf_upload_pass(c, scenes, p, s0, s1, &paths, pc)) != LTS_OK) return rc;
         HIPCHK(c, hipEventRecord(c->ev[5], s));
-        if ((rc = wf_run(c, t, p->max_bounce)) != LTS_OK) return rc;
+        if ((rc = wf_run(c, t, p->max_bounce, nrc)) != LTS_OK) return rc;
         HIPCHK(c, hipEventRecord(c->ev[6], s));
         for (size_t i = 0; i < scenes.size(); i++) {
             RenderLaunch rl{};
@@ -459,6 +515,16 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         st->sort_materials = t.sortMaterials;
         st->use_graph = t.useGraph;
         st->autotune_ms = tuneMs;
+        if (cached) {
+            NrcState ns;
+            int32_t q = 0;
+            HIPCHK(c, hipMemcpy(&ns, c->nrc.state, sizeof(ns), hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(&q, c->wf.counts + kWfGlobalBase, sizeof(q), hipMemcpyDeviceToHost));
+            st->nrc_loss = ns.loss;
+            st->nrc_queries = q;
+            st->nrc_train_samples = ns.nTrain;
+            st->nrc_steps = ns.t;
+        }
     }
     return LTS_OK;
 }
@@ -477,13 +543,6 @@ lts_nrc_config nrc_defaults(const lts_nrc_config* in) {
     if (c.beta2 <= 0) c.beta2 = 0.99f;
     if (c.eps <= 0) c.eps = 1e-15f;
     return c;
-}
-
-uint64_t splitmix64(uint64_t& x) {
-    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-    return z ^ (z >> 31);
 }
 
 // Uploads fp32 master weights (padding columns forced to zero), clears Adam and repacks.
@@ -521,6 +580,7 @@ int nrc_setup(lts_ctx* c, const lts_nrc_config* in) {
     for (int in = 0; in < 64; in++)
         for (int out = 0; out < 3; out++) w[20480 + in * 16 + out] = uni(outA);
     c->nrcCfg = cfg;
+    c->nrcPasses = 0;
     int rc = nrc_upload_weights(c, w.data());
     if (rc != LTS_OK) return rc;
     c->nrcReady = true;
@@ -612,6 +672,12 @@ int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* s
     if (!load_scene_files(d, scene, subject ? subject : "", width, height, S.host, err)) return fail(c, LTS_E_IO, err);
     const HostScene& H = S.host;
     const int n = (int)H.shade.size();
+    for (int a = 0; a < 3; a++) {
+        float lo = INFINITY, hi = -INFINITY;
+        for (size_t i = a; i < H.soup.size(); i += 3) { lo = std::min(lo, H.soup[i]); hi = std::max(hi, H.soup[i]); }
+        S.boxMin[a] = lo;
+        S.boxInv[a] = hi > lo ? 1.0f / (hi - lo) : 0.0f;
+    }
     if (n > kMaxPrims) return fail(c, LTS_E_ARG, "scene has more than 2^26 triangles");
     hipStream_t s = c->stream;
     HIPCHK(c, upload(&S.soup, H.soup.data(), H.soup.size(), s));
@@ -708,7 +774,7 @@ int lts_render(lts_ctx* c, const lts_render_params* p, float* rgb_out, float* sa
     int rc = check_render_params(c, p);
     if (rc != LTS_OK) return rc;
     HIPCHK(c, hipSetDevice(c->device));
-    if (p->integrator == LTS_INTEGRATOR_WAVEFRONT) {
+    if (p->integrator == LTS_INTEGRATOR_WAVEFRONT || p->integrator == LTS_INTEGRATOR_CACHED) {
         std::vector<lts_ctx*> one{c};
         return render_wavefront(c, one, p, &rgb_out, sample_rgb, st);
     }

@@ -74,7 +74,7 @@ constexpr int kWfCountsPerBounce = 8;  // queue, shadow, mis counts; extend, sha
 constexpr int kWfMaxMatBins = 64;
 constexpr int kWfMaxBounce = 64;
 
-struct alignas(16) WfPath { float beta[3]; float L[3]; int32_t pixel, sample, bounce, flags, scene, pad; };
+struct alignas(16) WfPath { float beta[3]; float L[3]; int32_t pixel, sample, bounce, flags, scene, trainSlot; };
 struct alignas(16) WfRay { float o[3]; float d[3]; float pad[2]; };
 struct alignas(16) WfHit { int32_t leaf; float u, v, invDet; };
 struct alignas(16) WfNee { float nb[3]; float lpdf; float p1[3]; float p2[3]; int32_t used; int32_t pad; };
@@ -82,14 +82,28 @@ struct alignas(16) WfShadow { float o[3]; float d[3]; int32_t path; int32_t pad;
 struct alignas(16) WfMis { float o[3]; float d[3]; int32_t path; int32_t light; float f[3]; float weight; float pdf; int32_t pad[3]; };
 
 // Read by the kernels from device memory, so a captured pass graph can be replayed for every pass.
-struct WfParams { int64_t nPaths; int32_t nScenes, maxBounce, rrStart, pad; };
+struct WfParams {
+    int64_t nPaths;
+    int32_t nScenes, maxBounce, rrStart;
+    int32_t queryBounce;   // cached integrator: paths query the cache at their first non-specular hit
+                           // of this bounce or later (a value above maxBounce disables queries)
+    int32_t trainStride;   // training paths: pixels offset + k stride of scene 0, first sample of the pass
+    int32_t trainOffset;
+    int32_t trainSlots;    // 0: no training paths
+    int32_t pad;
+};
 struct WfSceneSlot {
     int64_t pathBase;  // first path of this scene in the pass
     int32_t width, height, spp, stratified;
     uint32_t seed;
     int32_t s0;        // first sample index of the pass
     float* sampleL;    // [pass samples][H][W][3]
+    float boxMin[3];   // scene bounds, for the cache's normalised positions
+    float boxInv[3];   // 1 / extent per axis
 };
+// One vertex of a training path: radiance gathered before the vertex's own scattering, throughput
+// and reflectance there; the suffix target is (L_final - Lb) / (beta a).
+struct alignas(16) WfTrainRec { float Lb[3]; float beta[3]; float a[3]; int32_t path; int32_t valid; int32_t pad; };
 
 struct WfTuning {
     int traceBlock = 128;         // block size of the persistent traversal kernels (64, 128, 256)
@@ -98,6 +112,9 @@ struct WfTuning {
     int sortMaterials = 0;        // counting-sort the shade queue by material
     int useGraph = 1;             // capture the pass in a hipGraph and replay it
 };
+
+struct NrcQuery;
+struct NrcBuffers;
 
 struct WfBuffers {
     size_t cap = 0;       // paths per pass
@@ -115,12 +132,29 @@ struct WfBuffers {
     WfShadow* shadows = nullptr;
     WfMis* mis = nullptr;
     int32_t* counts = nullptr;
+    // cached integrator (allocated by wf_reserve_cache)
+    size_t cacheCap = 0;          // paths per pass covered by the query arrays
+    int trainCap = 0;             // training-path slots
+    float* qRaw = nullptr;        // [cacheCap][16] query record of each querying path
+    NrcQuery* qMeta = nullptr;    // [cacheCap]
+    int32_t* qQueue = nullptr;    // [cacheCap] querying paths
+    WfTrainRec* tRec = nullptr;   // [trainCap][kWfMaxBounce]
+    float* tRaw = nullptr;        // [trainCap][kWfMaxBounce][16]
+    float* sampleBase = nullptr;  // sample buffer of the pass (path i's radiance at 3 i)
 };
 
+// counts: per bounce kWfCountsPerBounce counters, then per-bounce material histograms, then the
+// pass-wide counters below.
+constexpr int kWfGlobalBase = (kWfCountsPerBounce + kWfMaxMatBins) * (kWfMaxBounce + 1);
+constexpr int kWfGlobalCounts = 8;  // [0] cache queries
+
 hipError_t wf_reserve(WfBuffers& b, size_t cap, int maxScenes);
+hipError_t wf_reserve_cache(WfBuffers& b, size_t cap, int trainCap);
 void wf_release(WfBuffers& b);
 // Enqueues one pass: generate, then per bounce extend / (sort) / shade / shadow / mis / finalize.
-hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce, hipStream_t stream);
+// With nrc (cached integrator): queries, training records, then cache inference into the sample
+// buffer, the training batch of path-suffix radiance and one Adam step.
+hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce, const NrcBuffers* nrc, hipStream_t stream);
 
 // ---------------------------------------------------------------- learned radiance cache (nrc.hip)
 // Network: 16-float raw record -> 64-dim encoding -> 5 hidden layers of 64 (ReLU) -> 3 outputs.
@@ -166,10 +200,11 @@ hipError_t nrc_reserve(NrcBuffers& b, int batchCap);
 void nrc_release(NrcBuffers& b);
 // master -> MFMA fragments (after weights are set)
 hipError_t nrc_pack(const NrcBuffers& b, hipStream_t stream);
-// Evaluates min(*nDev, nCap) raw records. With meta == nullptr, out receives 3 floats per record;
-// otherwise sampleL[3 slot + c] += beta_c * max(y_c, 0) * a_c for each query.
+// Evaluates min(*nDev, nCap) raw records (record s, or record index[s] when index is given). With
+// meta == nullptr, out receives 3 floats per record; otherwise sampleL[3 slot + c] +=
+// beta_c * max(y_c, 0) * a_c for each query.
 hipError_t nrc_infer(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out,
-                     const NrcQuery* meta, float* sampleL, hipStream_t stream);
+                     const NrcQuery* meta, float* sampleL, hipStream_t stream, const int32_t* index = nullptr);
 // One Adam step on min(state->nTrain, batchCap) samples of trainRaw / trainTarget.
 hipError_t nrc_train_step(const NrcBuffers& b, hipStream_t stream);
 

@@ -114,7 +114,8 @@ LTS_D f32x4 forward_tile(const uint4* wf, const float* __restrict__ raw, bool va
 template <typename T>
 __global__ __launch_bounds__(256) void k_nrc_infer(const uint16_t* __restrict__ fwd, const float* __restrict__ raw,
                                                    const int32_t* __restrict__ nDev, int nCap, float* __restrict__ out,
-                                                   const NrcQuery* __restrict__ meta, float* __restrict__ sampleL) {
+                                                   const NrcQuery* __restrict__ meta, float* __restrict__ sampleL,
+                                                   const int32_t* __restrict__ index) {
     __shared__ uint4 wf[kFwdVec];
     stage(wf, fwd, kFwdVec);
     __syncthreads();
@@ -124,11 +125,12 @@ __global__ __launch_bounds__(256) void k_nrc_infer(const uint16_t* __restrict__ 
     for (int t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
         const int s = t * 16 + (lane & 15);
         const bool valid = s < n;
+        const int rec = valid ? (index ? index[s] : s) : 0;
         typename Vec8<T>::type h[6][2];
-        const f32x4 y = forward_tile<T, false>(wf, raw + (size_t)(valid ? s : 0) * kNrcRaw, valid, lane, h);
+        const f32x4 y = forward_tile<T, false>(wf, raw + (size_t)rec * kNrcRaw, valid, lane, h);
         if (lane < 16 && valid) {
             if (meta) {
-                const NrcQuery q = meta[s];
+                const NrcQuery q = meta[rec];
                 float* o = sampleL + 3 * (size_t)q.slot;
                 o[0] += q.beta[0] * fmaxf(y[0], 0.0f) * q.a[0];
                 o[1] += q.beta[1] * fmaxf(y[1], 0.0f) * q.a[1];
@@ -333,10 +335,10 @@ __global__ __launch_bounds__(256) void k_nrc_pack(const float* __restrict__ mast
 
 template <typename T>
 hipError_t infer_t(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out, const NrcQuery* meta,
-                   float* sampleL, hipStream_t s) {
+                   float* sampleL, const int32_t* index, hipStream_t s) {
     const int tiles = (nCap + 15) / 16;
     const int blocks = std::max(1, std::min((tiles + 3) / 4, 2048));
-    k_nrc_infer<T><<<blocks, 256, 0, s>>>(b.fwd, raw, nDev, nCap, out, meta, sampleL);
+    k_nrc_infer<T><<<blocks, 256, 0, s>>>(b.fwd, raw, nDev, nCap, out, meta, sampleL, index);
     return hipGetLastError();
 }
 
@@ -369,10 +371,10 @@ hipError_t nrc_pack(const NrcBuffers& b, hipStream_t s) {
 }
 
 hipError_t nrc_infer(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out, const NrcQuery* meta,
-                     float* sampleL, hipStream_t s) {
+                     float* sampleL, hipStream_t s, const int32_t* index) {
     if (nCap <= 0) return hipSuccess;
-    return b.precision == 1 ? infer_t<_Float16>(b, raw, nDev, nCap, out, meta, sampleL, s)
-                            : infer_t<__bf16>(b, raw, nDev, nCap, out, meta, sampleL, s);
+    return b.precision == 1 ? infer_t<_Float16>(b, raw, nDev, nCap, out, meta, sampleL, index, s)
+                            : infer_t<__bf16>(b, raw, nDev, nCap, out, meta, sampleL, index, s);
 }
 
 hipError_t nrc_train_step(const NrcBuffers& b, hipStream_t s) {

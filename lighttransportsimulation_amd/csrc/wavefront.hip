@@ -37,7 +37,7 @@ LTS_D void path_origin(const WfParams& P, const WfSceneSlot* slots, long i, int&
 __global__ __launch_bounds__(256) void k_wf_generate(const SceneView* __restrict__ scenes, const WfParams* __restrict__ prm,
                                                      const WfSceneSlot* __restrict__ slots, WfPath* __restrict__ paths,
                                                      WfRay* __restrict__ rays, int32_t* __restrict__ queue,
-                                                     int32_t* __restrict__ counts) {
+                                                     int32_t* __restrict__ counts, WfTrainRec* __restrict__ tRec) {
     const WfParams P = *prm;
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i == 0) counts[0] = (int)P.nPaths;  // bounce-0 queue holds every path of the pass
@@ -52,6 +52,15 @@ __global__ __launch_bounds__(256) void k_wf_generate(const SceneView* __restrict
         p.beta[0] = p.beta[1] = p.beta[2] = 1.0f;
         p.L[0] = p.L[1] = p.L[2] = 0.0f;
         p.pixel = pixel; p.sample = sample; p.flags = kWfAlive; p.scene = sc;
+        // training paths of the radiance cache: every trainStride-th pixel of scene 0, first sample
+        p.trainSlot = -1;
+        if (P.trainSlots > 0 && sc == 0 && sample == sl.s0 && pixel >= P.trainOffset) {
+            const int d = pixel - P.trainOffset;
+            if (d % P.trainStride == 0 && d / P.trainStride < P.trainSlots) {
+                p.trainSlot = d / P.trainStride;
+                for (int b = 0; b < P.maxBounce; b++) tRec[(size_t)p.trainSlot * kWfMaxBounce + b].valid = 0;
+            }
+        }
         paths[k] = p;
         WfRay r;
         r.o[0] = cr.r.o.x; r.o[1] = cr.r.o.y; r.o[2] = cr.r.o.z;
@@ -147,6 +156,32 @@ __global__ __launch_bounds__(256) void k_wf_mat_scatter(const SceneView* __restr
     }
 }
 
+LTS_D float clamp01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
+// The cache's 16-float record of a shading point (layout in include/lts.h, lts_nrc_config).
+LTS_D void nrc_raw_record(const SurfacePoint& si, const Bsdf& bsdf, const WfSceneSlot& sl, float* __restrict__ o) {
+    const V3 wo = si.wo, n = faceTowards(si.sn, si.wo);
+    const bool glossy = bsdf.kind == 4;
+    float4 r0, r1, r2, r3;
+    r0.x = clamp01((si.p.x - sl.boxMin[0]) * sl.boxInv[0]);
+    r0.y = clamp01((si.p.y - sl.boxMin[1]) * sl.boxInv[1]);
+    r0.z = clamp01((si.p.z - sl.boxMin[2]) * sl.boxInv[2]);
+    r0.w = acosf(fminf(fmaxf(wo.z, -1.0f), 1.0f)) * INV_PI_F;
+    r1.x = atan2f(wo.y, wo.x) * (0.5f * INV_PI_F) + 0.5f;
+    r1.y = acosf(fminf(fmaxf(n.z, -1.0f), 1.0f)) * INV_PI_F;
+    r1.z = atan2f(n.y, n.x) * (0.5f * INV_PI_F) + 0.5f;
+    r1.w = glossy ? fminf(bsdf.alpha, 1.0f) : 1.0f;
+    r2.x = glossy ? 0.0f : bsdf.R.r;
+    r2.y = glossy ? 0.0f : bsdf.R.g;
+    r2.z = glossy ? 0.0f : bsdf.R.b;
+    r2.w = glossy ? bsdf.R.r : 0.0f;
+    r3.x = glossy ? bsdf.R.g : 0.0f;
+    r3.y = glossy ? bsdf.R.b : 0.0f;
+    r3.z = r3.w = 0.0f;
+    float4* q = reinterpret_cast<float4*>(o);
+    q[0] = r0; q[1] = r1; q[2] = r2; q[3] = r3;
+}
+
 // Wave-aggregated append: one atomic per wave.
 LTS_D int wave_append(int32_t* counter, bool want) {
     const uint64_t mask = __ballot(want);
@@ -165,16 +200,20 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
                                                           const int32_t* __restrict__ queue, const int32_t* __restrict__ count,
                                                           int32_t* __restrict__ nextQueue, int32_t* __restrict__ nextCount,
                                                           WfShadow* __restrict__ shadows, int32_t* __restrict__ shadowCount,
-                                                          WfMis* __restrict__ misRays, int32_t* __restrict__ misCount) {
+                                                          WfMis* __restrict__ misRays, int32_t* __restrict__ misCount,
+                                                          float* __restrict__ qRaw, NrcQuery* __restrict__ qMeta,
+                                                          int32_t* __restrict__ qQueue, int32_t* __restrict__ qCount,
+                                                          WfTrainRec* __restrict__ tRec, float* __restrict__ tRaw) {
     const int n = *count;
-    const int maxBounce = prm->maxBounce, rrStart = prm->rrStart;
+    const int maxBounce = prm->maxBounce, rrStart = prm->rrStart, queryBounce = prm->queryBounce;
+    const bool cached = qRaw != nullptr;
     const int stride = gridDim.x * kShadeBlock;
     const int rounds = (n + stride - 1) / stride;  // uniform trip count: the appends below are wave-collective
     for (int round = 0; round < rounds; round++) {
         const int k = round * stride + blockIdx.x * kShadeBlock + threadIdx.x;
         const bool active = k < n;
         const int pi = active ? queue[k] : 0;
-        bool wantShadow = false, wantMis = false, wantNext = false;
+        bool wantShadow = false, wantMis = false, wantNext = false, wantQuery = false;
         WfShadow sh;
         WfMis ms;
         if (active) {
@@ -217,12 +256,35 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
                     differentials(si, camera_sample(sv, px, py, p.sample, sl.spp, sl.stratified, sl.seed));
                 }
                 Bsdf bsdf = make_bsdf(sv, si);
+                if (cached && (bsdf.kind == 0 || bsdf.kind == 1 || bsdf.kind == 4)) {
+                    if (p.trainSlot >= 0) {
+                        // training vertex: radiance so far and throughput, for the suffix target
+                        const size_t r = (size_t)p.trainSlot * kWfMaxBounce + bounce;
+                        nrc_raw_record(si, bsdf, sl, tRaw + 16 * r);
+                        WfTrainRec tr;
+                        tr.Lb[0] = L.r; tr.Lb[1] = L.g; tr.Lb[2] = L.b;
+                        tr.beta[0] = beta.r; tr.beta[1] = beta.g; tr.beta[2] = beta.b;
+                        tr.a[0] = bsdf.R.r; tr.a[1] = bsdf.R.g; tr.a[2] = bsdf.R.b;
+                        tr.path = pi; tr.valid = 1; tr.pad = 0;
+                        tRec[r] = tr;
+                    } else if (bounce >= queryBounce) {
+                        // the cache stands for all radiance scattered here
+                        nrc_raw_record(si, bsdf, sl, qRaw + 16 * (size_t)pi);
+                        NrcQuery q;
+                        q.beta[0] = beta.r; q.beta[1] = beta.g; q.beta[2] = beta.b;
+                        q.a[0] = bsdf.R.r; q.a[1] = bsdf.R.g; q.a[2] = bsdf.R.b;
+                        q.slot = pi; q.pad = 0;
+                        qMeta[pi] = q;
+                        wantQuery = true;
+                        alive = false;
+                    }
+                }
                 Sampler smp;
                 smp.seed = sl.seed; smp.pixel = (uint32_t)p.pixel; smp.sample = (uint32_t)p.sample;
                 float u0[4], u1[4];
                 smp.block(1 + 2 * bounce, u0);
                 smp.block(2 + 2 * bounce, u1);
-                if (sv.nLights > 0) {
+                if (!wantQuery && sv.nLights > 0) {
                     float lpdf;
                     const int li = sample_light(sv, u0[0], &lpdf);
                     const LightDesc& Lt = sv.lights[li];
@@ -279,8 +341,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
                 V3 wo = -r.d, wi;
                 float pdf;
                 int flags = 0;
-                Spec f = bsdf.sampleF(wo, &wi, v2(u1[1], u1[2]), &pdf, B_ALL, &flags);
-                if (isBlack(f) || pdf == 0) {
+                Spec f = wantQuery ? spec(0.0f) : bsdf.sampleF(wo, &wi, v2(u1[1], u1[2]), &pdf, B_ALL, &flags);
+                if (wantQuery || isBlack(f) || pdf == 0) {
                     alive = false;
                 } else {
                     p.flags = (flags & B_SPEC) ? (p.flags | kWfSpecular) : (p.flags & ~kWfSpecular);
@@ -311,7 +373,56 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
         if (wantMis) misRays[mi] = ms;
         const int ni = wave_append(nextCount, wantNext);
         if (wantNext) nextQueue[ni] = pi;
+        if (cached) {
+            const int qi = wave_append(qCount, wantQuery);
+            if (wantQuery) qQueue[qi] = pi;
+        }
     }
+}
+
+// Builds the training batch from this pass's training records in record order (a block-wide
+// scan, so the batch and the step that follows are deterministic): raw record and the suffix
+// target (L_final - Lb) / beta / a per channel (0 where beta or a is 0).
+__global__ __launch_bounds__(1024) void k_wf_nrc_collect(const WfParams* __restrict__ prm, const WfTrainRec* __restrict__ tRec,
+                                                         const float* __restrict__ tRaw, const float* __restrict__ sampleBase,
+                                                         int batchCap, float* __restrict__ outRaw, float* __restrict__ outTarget,
+                                                         NrcState* __restrict__ state) {
+    __shared__ int waveCount[16];
+    __shared__ int waveBase[16];
+    __shared__ int total;
+    const int maxBounce = prm->maxBounce;
+    const int nRec = prm->trainSlots * maxBounce;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int base = 0;
+    for (int chunk = 0; chunk < nRec && base < batchCap; chunk += 1024) {
+        const int i = chunk + (int)threadIdx.x;
+        const size_t r = i < nRec ? (size_t)(i / maxBounce) * kWfMaxBounce + (i % maxBounce) : 0;
+        const bool valid = i < nRec && tRec[r].valid;
+        const uint64_t mask = __ballot(valid);
+        if (lane == 0) waveCount[wave] = (int)__popcll(mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int run = 0;
+            for (int w = 0; w < 16; w++) { waveBase[w] = run; run += waveCount[w]; }
+            total = run;
+        }
+        __syncthreads();
+        const int pos = base + waveBase[wave] + (int)__popcll(mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+        if (valid && pos < batchCap) {
+            const WfTrainRec t = tRec[r];
+            const float* Lf = sampleBase + 3 * (size_t)t.path;
+            for (int c = 0; c < 3; c++) {
+                const float suffix = t.beta[c] > 0.0f ? (Lf[c] - t.Lb[c]) / t.beta[c] : 0.0f;
+                outTarget[3 * (size_t)pos + c] = t.a[c] > 0.0f ? suffix / t.a[c] : 0.0f;
+            }
+            const float4* src = reinterpret_cast<const float4*>(tRaw + 16 * r);
+            float4* dst = reinterpret_cast<float4*>(outRaw + 16 * (size_t)pos);
+            dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2]; dst[3] = src[3];
+        }
+        base += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) state->nTrain = min(base, batchCap);
 }
 
 template <int BS>
@@ -413,15 +524,18 @@ __global__ void k_wf_fill(int32_t* p, int n, int32_t v) {
 
 }  // namespace
 
-hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce, hipStream_t s) {
+hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce, const NrcBuffers* nrc, hipStream_t s) {
     const int gridAll = t.sweepBlocks;
     const int ncounts = kWfCountsPerBounce * (maxBounce + 1) + kWfMaxMatBins * (maxBounce + 1);
     k_wf_fill<<<(ncounts + 255) / 256, 256, 0, s>>>(b.counts, ncounts, 0);
+    k_wf_fill<<<1, 256, 0, s>>>(b.counts + kWfGlobalBase, kWfGlobalCounts, 0);
     if (maxBounce <= 0) {
         k_wf_zero<<<gridAll, 256, 0, s>>>(b.params, b.slots);
         return hipGetLastError();
     }
-    k_wf_generate<<<gridAll, 256, 0, s>>>(b.scenes, b.params, b.slots, b.paths, b.rays, b.queue[0], b.counts);
+    const bool cached = nrc != nullptr;
+    int32_t* qCount = b.counts + kWfGlobalBase;
+    k_wf_generate<<<gridAll, 256, 0, s>>>(b.scenes, b.params, b.slots, b.paths, b.rays, b.queue[0], b.counts, b.tRec);
     for (int bounce = 0; bounce < maxBounce; bounce++) {
         const int q = bounce & 1;
         int32_t* cnt = b.counts + kWfCountsPerBounce * bounce;
@@ -440,7 +554,8 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
             shadeQueue = b.sorted;
         }
         k_wf_shade<<<gridAll, kShadeBlock, 0, s>>>(b.scenes, b.params, b.slots, bounce, b.paths, b.rays, b.hits, b.nee, shadeQueue,
-                                                   cnt, b.queue[q ^ 1], nxt, b.shadows, cnt + 1, b.mis, cnt + 2);
+                                                   cnt, b.queue[q ^ 1], nxt, b.shadows, cnt + 1, b.mis, cnt + 2,
+                                                   cached ? b.qRaw : nullptr, b.qMeta, b.qQueue, qCount, b.tRec, b.tRaw);
         switch (t.traceBlock) {
             case 64:
                 k_wf_shadow<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.shadows, cnt + 1, b.nee, cnt + 4);
@@ -456,6 +571,14 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
                 break;
         }
         k_wf_finalize<<<gridAll, 256, 0, s>>>(b.params, b.slots, bounce, b.paths, b.nee, shadeQueue, cnt);
+    }
+    if (cached) {
+        hipError_t e = nrc_infer(*nrc, b.qRaw, qCount, (int)b.cap, nullptr, b.qMeta, b.sampleBase, s, b.qQueue);
+        if (e != hipSuccess) return e;
+        k_wf_nrc_collect<<<1, 1024, 0, s>>>(b.params, b.tRec, b.tRaw, b.sampleBase, nrc->batchCap, nrc->trainRaw,
+                                            nrc->trainTarget, nrc->state);
+        e = nrc_train_step(*nrc, s);
+        if (e != hipSuccess) return e;
     }
     return hipGetLastError();
 }
@@ -486,14 +609,37 @@ hipError_t wf_reserve(WfBuffers& b, size_t cap, int maxScenes) {
     WTRY(wf_alloc(&b.sorted, cap));
     WTRY(wf_alloc(&b.shadows, cap));
     WTRY(wf_alloc(&b.mis, cap));
-    WTRY(wf_alloc(&b.counts, (size_t)(kWfCountsPerBounce + kWfMaxMatBins) * (kWfMaxBounce + 1)));
+    WTRY(wf_alloc(&b.counts, (size_t)kWfGlobalBase + kWfGlobalCounts));
 #undef WTRY
     b.cap = cap;
     b.maxScenes = maxScenes;
     return hipSuccess;
 }
 
+hipError_t wf_reserve_cache(WfBuffers& b, size_t cap, int trainCap) {
+    if (cap <= b.cacheCap && trainCap <= b.trainCap) return hipSuccess;
+    void* ps[] = {b.qRaw, b.qMeta, b.qQueue, b.tRec, b.tRaw};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    b.qRaw = nullptr; b.qMeta = nullptr; b.qQueue = nullptr; b.tRec = nullptr; b.tRaw = nullptr;
+    b.cacheCap = 0; b.trainCap = 0;
+    hipError_t e = hipSuccess;
+#define CTRY(x) do { e = (x); if (e != hipSuccess) return e; } while (0)
+    CTRY(wf_alloc(&b.qRaw, cap * kNrcRaw));
+    CTRY(wf_alloc(&b.qMeta, cap));
+    CTRY(wf_alloc(&b.qQueue, cap));
+    CTRY(wf_alloc(&b.tRec, (size_t)trainCap * kWfMaxBounce));
+    CTRY(wf_alloc(&b.tRaw, (size_t)trainCap * kWfMaxBounce * kNrcRaw));
+#undef CTRY
+    b.cacheCap = cap;
+    b.trainCap = trainCap;
+    return hipSuccess;
+}
+
 void wf_release(WfBuffers& b) {
+    void* cs[] = {b.qRaw, b.qMeta, b.qQueue, b.tRec, b.tRaw};
+    for (void* p : cs)
+        if (p) (void)hipFree(p);
     void* ps[] = {b.scenes, b.params, b.slots, b.paths, b.rays, b.hits, b.nee, b.queue[0], b.queue[1], b.sorted, b.shadows, b.mis, b.counts};
     for (void* p : ps)
         if (p) (void)hipFree(p);
