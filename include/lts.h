@@ -89,7 +89,7 @@ typedef struct lts_stats {
     double autotune_ms;     /* time spent in the autotuner by this call             */
     double nrc_loss;        /* cached: loss of the last training step               */
     int64_t nrc_queries;    /* cached: cache queries of the last pass               */
-    int32_t nrc_train_samples; /* cached: training samples of the last step         */
+    int32_t nrc_train_samples; /* cached: training samples collected in the last pass */
     int32_t nrc_steps;      /* cached: Adam steps taken by the cache so far          */
 } lts_stats;
 
@@ -154,13 +154,15 @@ typedef struct lts_nrc_config {
     int32_t batch;          /* training samples per Adam step (default 65536)                       */
     int32_t query_bounce;   /* cached renders query the cache at the first non-specular hit of      */
                             /* this bounce or later (default 2)                                     */
-    int32_t train_paths;    /* paths per pass traced to the end as training paths (default 16384,   */
-                            /* at most 1/8 of the pixels)                                           */
+    int32_t train_paths;    /* paths per pass traced to the end as training paths (default 262144,  */
+                            /* at most 1/8 of the pass's paths)                                     */
     int32_t warmup_passes;  /* the first passes of cached rendering after this call are traced       */
                             /* without queries while the cache trains (default 4, -1: none)        */
     uint32_t seed;          /* weight initialisation (Xavier-uniform from a SplitMix64 stream)      */
     float lr, beta1, beta2, eps;  /* Adam (defaults 1e-2, 0.9, 0.99, 1e-15)                          */
-    int32_t reserved[6];
+    int32_t train_steps;    /* Adam steps per cached pass over the batches collected in the pass,   */
+                            /* cycling through them (default 16)                                    */
+    int32_t reserved[5];
 } lts_nrc_config;
 
 /* (Re)initialises the cache of this context: weights from the seed, zero Adam state. NULL or zero

@@ -54,7 +54,7 @@ class RenderParams(C.Structure):
 class NrcConfig(C.Structure):
     _fields_ = [("precision", C.c_int32), ("batch", C.c_int32), ("query_bounce", C.c_int32), ("train_paths", C.c_int32),
                 ("warmup_passes", C.c_int32), ("seed", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float),
-                ("beta2", C.c_float), ("eps", C.c_float), ("reserved", C.c_int32 * 6)]
+                ("beta2", C.c_float), ("eps", C.c_float), ("train_steps", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 class Stats(C.Structure):
@@ -200,10 +200,10 @@ class Context:
     # ------------------------------------------------------------------ radiance cache
     def nrc_configure(self, precision: str = "bf16", batch: int = 0, query_bounce: int = 0, train_paths: int = 0,
                       warmup_passes: int = 0, seed: int = 0, lr: float = 0.0, beta1: float = 0.0, beta2: float = 0.0,
-                      eps: float = 0.0):
+                      eps: float = 0.0, train_steps: int = 0):
         """(Re)initialises the cache (include/lts.h: lts_nrc_configure); zero arguments take the defaults."""
         cfg = NrcConfig(0 if precision == "bf16" else 1, batch, query_bounce, train_paths, warmup_passes, seed, lr, beta1,
-                        beta2, eps)
+                        beta2, eps, train_steps)
         self._check(self.lib.lts_nrc_configure(self.h, C.byref(cfg)), "lts_nrc_configure")
 
     def nrc_set_weights(self, params: np.ndarray):

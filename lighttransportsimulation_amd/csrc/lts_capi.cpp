@@ -70,6 +70,7 @@ struct lts_ctx {
     DeviceScene scene;
     float* sampleL = nullptr;
     size_t sampleCap = 0;
+    uint64_t sampleGen = 0;   // allocation generation of sampleL
     float* film = nullptr;
     float* rgb = nullptr;
     size_t filmCap = 0;
@@ -78,13 +79,11 @@ struct lts_ctx {
     hipEvent_t ev[8] = {};
     // wavefront integrator
     WfBuffers wf;
-    hipGraphExec_t wfExec = nullptr;  // captured pass for (wfKeyPaths, wfKeyBounce, wfKeyTune, wfKeyBins)
-    const void* wfKeyPaths = nullptr;
+    hipGraphExec_t wfExec = nullptr;  // captured pass, valid for the wfKey* values below
+    uint64_t wfKeyGen = 0, wfKeyCacheGen = 0, wfKeyNrcGen = 0, wfKeySampleGen = 0;
     int wfKeyBounce = -1, wfKeyBins = -1;
     WfTuning wfKeyTune;
-    const void* wfKeyNrc = nullptr;     // cache weights of a cached pass (nullptr: plain pass)
     int wfKeyPrecision = -1;
-    const void* wfKeySamples = nullptr;
     WfParams wfParams{};                  // host copies of the per-pass parameters (stable for async H2D)
     std::vector<WfSceneSlot> wfSlots;
     std::map<std::string, WfTuning> tuned;  // autotuner results of this context
@@ -225,6 +224,7 @@ int ensure_samples(lts_ctx* c, size_t floats) {
     c->sampleCap = 0;
     HIPCHK(c, hipMalloc((void**)&c->sampleL, floats * sizeof(float)));
     c->sampleCap = floats;
+    c->sampleGen = next_buffer_generation();
     return LTS_OK;
 }
 
@@ -241,11 +241,11 @@ int wf_run(lts_ctx* c, const WfTuning& t, int maxBounce, const NrcBuffers* nrc) 
         HIPCHK(c, wf_enqueue_pass(c->wf, t, maxBounce, nrc, s));
         return LTS_OK;
     }
-    const void* nrcKey = nrc ? (const void*)nrc->master : nullptr;
-    const int precision = nrc ? nrc->precision : -1;
-    if (!c->wfExec || c->wfKeyPaths != c->wf.paths || c->wfKeyBounce != maxBounce || c->wfKeyBins != c->wf.matBins ||
-        !same_tuning(c->wfKeyTune, t) || c->wfKeyNrc != nrcKey || c->wfKeyPrecision != precision ||
-        c->wfKeySamples != c->wf.sampleBase) {
+    const uint64_t nrcGen = nrc ? nrc->gen : 0, cacheGen = nrc ? c->wf.cacheGen : 0;
+    const int precision = nrc ? nrc->precision + 2 * nrc->steps : -1;  // kernels and step count of the pass
+    if (!c->wfExec || c->wfKeyGen != c->wf.gen || c->wfKeyCacheGen != cacheGen || c->wfKeyNrcGen != nrcGen ||
+        c->wfKeySampleGen != c->sampleGen || c->wfKeyBounce != maxBounce || c->wfKeyBins != c->wf.matBins ||
+        !same_tuning(c->wfKeyTune, t) || c->wfKeyPrecision != precision) {
         if (c->wfExec) (void)hipGraphExecDestroy(c->wfExec);
         c->wfExec = nullptr;
         hipGraph_t g = nullptr;
@@ -256,13 +256,14 @@ int wf_run(lts_ctx* c, const WfTuning& t, int maxBounce, const NrcBuffers* nrc) 
         if (e == hipSuccess) e = hipGraphInstantiate(&c->wfExec, g, nullptr, nullptr, 0);
         if (g) (void)hipGraphDestroy(g);
         HIPCHK(c, e);
-        c->wfKeyPaths = c->wf.paths;
+        c->wfKeyGen = c->wf.gen;
+        c->wfKeyCacheGen = cacheGen;
+        c->wfKeyNrcGen = nrcGen;
+        c->wfKeySampleGen = c->sampleGen;
         c->wfKeyBounce = maxBounce;
         c->wfKeyBins = c->wf.matBins;
         c->wfKeyTune = t;
-        c->wfKeyNrc = nrcKey;
         c->wfKeyPrecision = precision;
-        c->wfKeySamples = c->wf.sampleBase;
     }
     HIPCHK(c, hipGraphLaunch(c->wfExec, s));
     return LTS_OK;
@@ -405,9 +406,7 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
     if (cached && scenes.size() != 1) return fail(c, LTS_E_ARG, "the cached integrator renders one scene");
     int rc = LTS_OK;
     if (cached && (rc = nrc_ensure(c)) != LTS_OK) return rc;
-    // cached renders train once per pass: one sample per pixel per pass at 1080p
-    const size_t target = cached ? ((size_t)1 << 21) : ((size_t)1 << 24);
-    int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, target / totalPix);
+    int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, ((size_t)1 << 24) / totalPix);
     if (sample_rgb) pass = p->spp;
     pass = std::min(pass, p->spp);
     const size_t cap = totalPix * (size_t)pass;
@@ -421,8 +420,8 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
     c->wf.matBins = std::min(maxMats, kWfMaxMatBins - 1) + 1;
     if ((rc = ensure_samples(c, cap * 3)) != LTS_OK) return rc;
     c->wf.sampleBase = c->sampleL;
-    // training paths stay a minority of the pass (at most 1/8 of the pixels) so most paths query
-    const int trainPaths = cached ? (int)std::max<size_t>(1, std::min<size_t>((size_t)c->nrcCfg.train_paths, totalPix / 8)) : 0;
+    // training paths stay a minority of the pass (at most 1/8 of its paths) so most paths query
+    const int trainPaths = cached ? (int)std::max<size_t>(1, std::min<size_t>((size_t)c->nrcCfg.train_paths, cap / 8)) : 0;
     if (cached) HIPCHK(c, wf_reserve_cache(c->wf, cap, trainPaths));
     const NrcBuffers* nrc = cached ? &c->nrc : nullptr;
     std::vector<SceneView> views;
@@ -466,12 +465,14 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         const int s1 = std::min(p->spp, s0 + pass);
         PassCache pc;
         if (cached) {
-            // training paths: every stride-th pixel from an offset that moves from pass to pass
+            // training paths: every stride-th path of the pass from an offset that moves from pass to pass
+            const size_t paths0 = npix0 * (size_t)(s1 - s0);
             pc.queryBounce = c->nrcPasses < (uint64_t)c->nrcCfg.warmup_passes ? (1 << 30) : c->nrcCfg.query_bounce;
-            pc.trainStride = (int)((npix0 + trainPaths - 1) / trainPaths);
+            pc.trainStride = (int)((paths0 + trainPaths - 1) / trainPaths);
             uint64_t h = 0x70617373ull + (uint64_t)c->nrcPasses * 0x9e3779b97f4a7c15ull;
             pc.trainOffset = (int)(splitmix64(h) % (uint64_t)pc.trainStride);
-            pc.trainSlots = (int)((npix0 - pc.trainOffset + pc.trainStride - 1) / pc.trainStride);
+            pc.trainSlots = (int)std::min<size_t>((size_t)trainPaths,
+                                                  (paths0 - pc.trainOffset + pc.trainStride - 1) / pc.trainStride);
             c->nrcPasses++;
         }
         int64_t paths = 0;
@@ -522,7 +523,7 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
             HIPCHK(c, hipMemcpy(&q, c->wf.counts + kWfGlobalBase, sizeof(q), hipMemcpyDeviceToHost));
             st->nrc_loss = ns.loss;
             st->nrc_queries = q;
-            st->nrc_train_samples = ns.nTrain;
+            st->nrc_train_samples = ns.nTotal;
             st->nrc_steps = ns.t;
         }
     }
@@ -535,7 +536,8 @@ lts_nrc_config nrc_defaults(const lts_nrc_config* in) {
     if (in) c = *in;
     if (c.batch <= 0) c.batch = 65536;
     if (c.query_bounce <= 0) c.query_bounce = 2;
-    if (c.train_paths <= 0) c.train_paths = 16384;
+    if (c.train_paths <= 0) c.train_paths = 262144;
+    if (c.train_steps <= 0) c.train_steps = 16;
     if (c.warmup_passes == 0) c.warmup_passes = 4;
     if (c.warmup_passes < 0) c.warmup_passes = 0;
     if (c.lr <= 0) c.lr = 1e-2f;
@@ -555,7 +557,9 @@ int nrc_upload_weights(lts_ctx* c, const float* w) {
     HIPCHK(c, hipMemsetAsync(c->nrc.m, 0, kNrcParams * sizeof(float), s));
     HIPCHK(c, hipMemsetAsync(c->nrc.v, 0, kNrcParams * sizeof(float), s));
     HIPCHK(c, hipMemsetAsync(c->nrc.grad, 0, kNrcParams * sizeof(float), s));
-    HIPCHK(c, hipMemsetAsync(c->nrc.state, 0, sizeof(NrcState), s));
+    NrcState st{};
+    st.lr = c->nrc.lr; st.beta1 = c->nrc.beta1; st.beta2 = c->nrc.beta2; st.eps = c->nrc.eps;
+    HIPCHK(c, hipMemcpyAsync(c->nrc.state, &st, sizeof(NrcState), hipMemcpyHostToDevice, s));
     HIPCHK(c, nrc_pack(c->nrc, s));
     HIPCHK(c, hipStreamSynchronize(s));
     return LTS_OK;
@@ -566,10 +570,14 @@ int nrc_setup(lts_ctx* c, const lts_nrc_config* in) {
     if (cfg.precision != 0 && cfg.precision != 1) return fail(c, LTS_E_ARG, "nrc precision must be 0 (bf16) or 1 (fp16)");
     if (cfg.batch > (1 << 20)) return fail(c, LTS_E_ARG, "nrc batch above 2^20");
     if (cfg.train_paths > (1 << 22)) return fail(c, LTS_E_ARG, "nrc train_paths above 2^22");
+    if (cfg.train_steps > 64) return fail(c, LTS_E_ARG, "nrc train_steps above 64");
     HIPCHK(c, hipSetDevice(c->device));
     c->nrc.precision = cfg.precision;
     c->nrc.lr = cfg.lr; c->nrc.beta1 = cfg.beta1; c->nrc.beta2 = cfg.beta2; c->nrc.eps = cfg.eps;
-    HIPCHK(c, nrc_reserve(c->nrc, cfg.batch));
+    // training paths yield fewer samples than paths (escaping paths yield none): hold up to
+    // train_paths samples, in batches
+    const int chunks = std::max(1, (cfg.train_paths + cfg.batch - 1) / cfg.batch);
+    HIPCHK(c, nrc_reserve(c->nrc, cfg.batch, cfg.train_steps, chunks));
     if (!c->nrcCount) HIPCHK(c, hipMalloc((void**)&c->nrcCount, sizeof(int32_t)));
     // Xavier-uniform initialisation
     std::vector<float> w(kNrcParams, 0.0f);
@@ -900,8 +908,8 @@ int lts_nrc_train_step(lts_ctx* c, const float* raw, const float* target, int32_
         HIPCHK(c, hipMemcpyAsync(c->nrc.trainRaw, raw, (size_t)n * kNrcRaw * sizeof(float), hipMemcpyHostToDevice, s));
         HIPCHK(c, hipMemcpyAsync(c->nrc.trainTarget, target, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
     }
-    HIPCHK(c, hipMemcpyAsync(&c->nrc.state->nTrain, &n, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    HIPCHK(c, nrc_train_step(c->nrc, s));
+    HIPCHK(c, hipMemcpyAsync(&c->nrc.state->nTotal, &n, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIPCHK(c, nrc_train_step(c->nrc, s, 0));
     NrcState st;
     HIPCHK(c, hipMemcpyAsync(&st, c->nrc.state, sizeof(st), hipMemcpyDeviceToHost, s));
     if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->nrc.grad, kNrcParams * sizeof(float), hipMemcpyDeviceToHost, s));

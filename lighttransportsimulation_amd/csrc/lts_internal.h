@@ -3,9 +3,19 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+
 #include "lts_scene.h"
 
 namespace lts {
+
+// Process-unique number for every (re)allocation of a buffer set. Captured graphs hold raw device
+// pointers, and a reallocation can return an address that was just freed, so a captured graph is
+// matched to its buffers by these numbers, never by addresses.
+inline uint64_t next_buffer_generation() {
+    static std::atomic<uint64_t> gen{0};
+    return ++gen;
+}
 
 // Device buffers of one LBVH build (sized for `cap` primitives; reused across builds).
 struct BvhBuffers {
@@ -87,7 +97,7 @@ struct WfParams {
     int32_t nScenes, maxBounce, rrStart;
     int32_t queryBounce;   // cached integrator: paths query the cache at their first non-specular hit
                            // of this bounce or later (a value above maxBounce disables queries)
-    int32_t trainStride;   // training paths: pixels offset + k stride of scene 0, first sample of the pass
+    int32_t trainStride;   // training paths: paths offset + k stride of scene 0 (path index in the pass)
     int32_t trainOffset;
     int32_t trainSlots;    // 0: no training paths
     int32_t pad;
@@ -117,6 +127,8 @@ struct NrcQuery;
 struct NrcBuffers;
 
 struct WfBuffers {
+    uint64_t gen = 0;        // allocation generation of the pass buffers
+    uint64_t cacheGen = 0;   // allocation generation of the cached-integrator buffers
     size_t cap = 0;       // paths per pass
     int maxScenes = 0;
     int matBins = 2;
@@ -140,6 +152,7 @@ struct WfBuffers {
     int32_t* qQueue = nullptr;    // [cacheCap] querying paths
     WfTrainRec* tRec = nullptr;   // [trainCap][kWfMaxBounce]
     float* tRaw = nullptr;        // [trainCap][kWfMaxBounce][16]
+    int32_t* tCount = nullptr;    // per-block counts / offsets of the batch compaction
     float* sampleBase = nullptr;  // sample buffer of the pass (path i's radiance at 3 i)
 };
 
@@ -171,15 +184,19 @@ struct NrcState {
     int32_t t;        // Adam steps taken
     int32_t nTrain;   // samples in the current training batch
     int32_t nQuery;   // cache queries of the current pass
-    int32_t pad;
+    int32_t nTotal;   // samples collected for this pass's steps (nTrain of step k: the k-th batch of them)
     float c1, c2;     // Adam bias corrections 1/(1-beta1^t), 1/(1-beta2^t) of the current step
     float loss;       // mean loss of the last step
-    float pad2;
+    int32_t first;    // first sample of the current step's batch within trainRaw / trainTarget
+    float lr, beta1, beta2, eps;  // Adam hyper-parameters (device-resident: captured graphs read them)
 };
 
 struct NrcBuffers {
+    uint64_t gen = 0;   // allocation generation (next_buffer_generation)
     int precision = 0;  // 0 bf16, 1 fp16
     int batchCap = 0;   // training samples per step (multiple of kNrcKSplit)
+    int steps = 1;      // Adam steps per cached pass
+    int chunks = 1;     // batches of collected samples trainRaw / trainTarget hold
     float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-15f;
     float* master = nullptr;      // kNrcParams
     float* m = nullptr;           // Adam first moment
@@ -192,11 +209,11 @@ struct NrcBuffers {
     float* partial = nullptr;     // [batchCap / kNrcKSplit][kNrcParams]
     float* lossPartial = nullptr; // kNrcTrainBlocks
     NrcState* state = nullptr;
-    float* trainRaw = nullptr;    // [batchCap][16]
-    float* trainTarget = nullptr; // [batchCap][3]
+    float* trainRaw = nullptr;    // [chunks * batchCap][16]
+    float* trainTarget = nullptr; // [chunks * batchCap][3]
 };
 
-hipError_t nrc_reserve(NrcBuffers& b, int batchCap);
+hipError_t nrc_reserve(NrcBuffers& b, int batchCap, int steps, int chunks);
 void nrc_release(NrcBuffers& b);
 // master -> MFMA fragments (after weights are set)
 hipError_t nrc_pack(const NrcBuffers& b, hipStream_t stream);
@@ -205,7 +222,8 @@ hipError_t nrc_pack(const NrcBuffers& b, hipStream_t stream);
 // beta_c * max(y_c, 0) * a_c for each query.
 hipError_t nrc_infer(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out,
                      const NrcQuery* meta, float* sampleL, hipStream_t stream, const int32_t* index = nullptr);
-// One Adam step on min(state->nTrain, batchCap) samples of trainRaw / trainTarget.
-hipError_t nrc_train_step(const NrcBuffers& b, hipStream_t stream);
+// One Adam step on batch (chunk mod ceil(nTotal / batchCap)) of the state->nTotal samples in
+// trainRaw / trainTarget, so steps beyond the collected batches make further passes over them.
+hipError_t nrc_train_step(const NrcBuffers& b, hipStream_t stream, int chunk = 0);
 
 }  // namespace lts

@@ -161,6 +161,8 @@ __global__ __launch_bounds__(kTrainBlock) void k_nrc_fwdbwd(const uint16_t* __re
     __syncthreads();
     const int n = min(state->nTrain, nCap);
     const int npad = (n + 31) & ~31;
+    raw += (size_t)state->first * kNrcRaw;
+    target += (size_t)state->first * 3;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
     const int wavesPerBlock = kTrainBlock / 64;
     float lossAcc = 0.0f;
@@ -253,16 +255,25 @@ __global__ __launch_bounds__(256) void k_nrc_wgrad(const T* __restrict__ H, cons
     for (int i = 0; i < 4; i++) p[w_index(l, 16 * it + 4 * g + i, 16 * ot + col)] = acc[i];
 }
 
+// Selects batch chunk mod ceil(nTotal / batchCap) of the collected samples for the next step.
+__global__ void k_nrc_step_prepare(NrcState* __restrict__ state, int chunk, int batchCap) {
+    if (threadIdx.x != 0) return;
+    const int total = state->nTotal;
+    const int chunks = max(1, (total + batchCap - 1) / batchCap);
+    const int c = chunk % chunks;
+    state->first = c * batchCap;
+    state->nTrain = max(0, min(batchCap, total - c * batchCap));
+}
+
 // Advances the step counter (when the batch is not empty) and the bias corrections.
-__global__ void k_nrc_step_begin(NrcState* __restrict__ state, int nCap, float beta1, float beta2,
-                                 const float* __restrict__ lossPartial, int nPartial) {
+__global__ void k_nrc_step_begin(NrcState* __restrict__ state, int nCap, const float* __restrict__ lossPartial, int nPartial) {
     if (threadIdx.x != 0) return;
     const int n = min(state->nTrain, nCap);
     if (n <= 0) return;
     const int t = state->t + 1;
     state->t = t;
-    state->c1 = (float)(1.0 / (1.0 - pow((double)beta1, (double)t)));
-    state->c2 = (float)(1.0 / (1.0 - pow((double)beta2, (double)t)));
+    state->c1 = (float)(1.0 / (1.0 - pow((double)state->beta1, (double)t)));
+    state->c2 = (float)(1.0 / (1.0 - pow((double)state->beta2, (double)t)));
     float s = 0.0f;
     for (int i = 0; i < nPartial; i++) s += lossPartial[i];
     state->loss = s / (float)n;
@@ -291,13 +302,14 @@ LTS_D void decode_param(int p, int& l, int& in, int& out) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_nrc_adam(const float* __restrict__ partial, int splits, const NrcState* __restrict__ state,
-                                                  int nCap, float lr, float beta1, float beta2, float eps,
-                                                  float* __restrict__ master, float* __restrict__ m, float* __restrict__ v,
-                                                  float* __restrict__ gradOut, T* __restrict__ fwd, T* __restrict__ bwd) {
+                                                  int nCap, float* __restrict__ master, float* __restrict__ m,
+                                                  float* __restrict__ v, float* __restrict__ gradOut, T* __restrict__ fwd,
+                                                  T* __restrict__ bwd) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= kNrcParams) return;
     const int n = min(state->nTrain, nCap);
     if (n <= 0) return;
+    const float lr = state->lr, beta1 = state->beta1, beta2 = state->beta2, eps = state->eps;
     int l, in, out;
     decode_param(p, l, in, out);
     if (l == 5 && out >= 3) return;  // padding columns stay zero
@@ -343,19 +355,19 @@ hipError_t infer_t(const NrcBuffers& b, const float* raw, const int32_t* nDev, i
 }
 
 template <typename T>
-hipError_t train_t(const NrcBuffers& b, hipStream_t s) {
+hipError_t train_t(const NrcBuffers& b, hipStream_t s, int chunk) {
     T* H = reinterpret_cast<T*>(b.H);
     T* G = reinterpret_cast<T*>(b.G);
     const int tiles = b.batchCap / 16;
     const int blocks = std::min(kNrcTrainBlocks, (tiles + kTrainBlock / 64 - 1) / (kTrainBlock / 64));
+    k_nrc_step_prepare<<<1, 64, 0, s>>>(b.state, chunk, b.batchCap);
     k_nrc_fwdbwd<T><<<blocks, kTrainBlock, 0, s>>>(b.fwd, b.bwd, b.trainRaw, b.trainTarget, b.state, b.batchCap, H, G,
                                                      b.batchCap, b.lossPartial);
     const int splits = b.batchCap / kNrcKSplit;
     k_nrc_wgrad<T><<<dim3(21, splits), 256, 0, s>>>(H, G, b.batchCap, b.state, b.batchCap, b.partial);
-    k_nrc_step_begin<<<1, 64, 0, s>>>(b.state, b.batchCap, b.beta1, b.beta2, b.lossPartial, blocks);
-    k_nrc_adam<T><<<(kNrcParams + 255) / 256, 256, 0, s>>>(b.partial, splits, b.state, b.batchCap, b.lr, b.beta1, b.beta2,
-                                                           b.eps, b.master, b.m, b.v, b.grad, reinterpret_cast<T*>(b.fwd),
-                                                           reinterpret_cast<T*>(b.bwd));
+    k_nrc_step_begin<<<1, 64, 0, s>>>(b.state, b.batchCap, b.lossPartial, blocks);
+    k_nrc_adam<T><<<(kNrcParams + 255) / 256, 256, 0, s>>>(b.partial, splits, b.state, b.batchCap, b.master, b.m, b.v, b.grad,
+                                                           reinterpret_cast<T*>(b.fwd), reinterpret_cast<T*>(b.bwd));
     return hipGetLastError();
 }
 
@@ -377,8 +389,9 @@ hipError_t nrc_infer(const NrcBuffers& b, const float* raw, const int32_t* nDev,
                             : infer_t<__bf16>(b, raw, nDev, nCap, out, meta, sampleL, index, s);
 }
 
-hipError_t nrc_train_step(const NrcBuffers& b, hipStream_t s) {
-    return b.precision == 1 ? train_t<_Float16>(b, s) : train_t<__bf16>(b, s);
+hipError_t nrc_train_step(const NrcBuffers& b, hipStream_t s, int chunk) {
+    if (chunk < 0) return hipErrorInvalidValue;
+    return b.precision == 1 ? train_t<_Float16>(b, s, chunk) : train_t<__bf16>(b, s, chunk);
 }
 
 template <typename X>
@@ -386,9 +399,14 @@ static hipError_t nrc_alloc(X** p, size_t count) {
     return hipMalloc((void**)p, (count ? count : 1) * sizeof(X));
 }
 
-hipError_t nrc_reserve(NrcBuffers& b, int batchCap) {
+hipError_t nrc_reserve(NrcBuffers& b, int batchCap, int steps, int chunks) {
     const int cap = std::max(kNrcKSplit, (batchCap + kNrcKSplit - 1) / kNrcKSplit * kNrcKSplit);
-    if (b.master && cap == b.batchCap) return hipSuccess;
+    steps = std::max(1, steps);
+    chunks = std::max(1, chunks);
+    if (b.master && cap == b.batchCap && chunks == b.chunks) {
+        b.steps = steps;
+        return hipSuccess;
+    }
     const int precision = b.precision;
     const float lr = b.lr, b1 = b.beta1, b2 = b.beta2, eps = b.eps;
     nrc_release(b);
@@ -407,10 +425,13 @@ hipError_t nrc_reserve(NrcBuffers& b, int batchCap) {
     NTRY(nrc_alloc(&b.partial, (size_t)(cap / kNrcKSplit) * kNrcParams));
     NTRY(nrc_alloc(&b.lossPartial, kNrcTrainBlocks));
     NTRY(nrc_alloc(&b.state, 1));
-    NTRY(nrc_alloc(&b.trainRaw, (size_t)cap * kNrcRaw));
-    NTRY(nrc_alloc(&b.trainTarget, (size_t)cap * 3));
+    NTRY(nrc_alloc(&b.trainRaw, (size_t)chunks * cap * kNrcRaw));
+    NTRY(nrc_alloc(&b.trainTarget, (size_t)chunks * cap * 3));
 #undef NTRY
     b.batchCap = cap;
+    b.steps = steps;
+    b.chunks = chunks;
+    b.gen = next_buffer_generation();
     return hipSuccess;
 }
 

@@ -52,10 +52,10 @@ __global__ __launch_bounds__(256) void k_wf_generate(const SceneView* __restrict
         p.beta[0] = p.beta[1] = p.beta[2] = 1.0f;
         p.L[0] = p.L[1] = p.L[2] = 0.0f;
         p.pixel = pixel; p.sample = sample; p.flags = kWfAlive; p.scene = sc;
-        // training paths of the radiance cache: every trainStride-th pixel of scene 0, first sample
+        // training paths of the radiance cache: every trainStride-th path of scene 0 from trainOffset
         p.trainSlot = -1;
-        if (P.trainSlots > 0 && sc == 0 && sample == sl.s0 && pixel >= P.trainOffset) {
-            const int d = pixel - P.trainOffset;
+        if (P.trainSlots > 0 && sc == 0 && k >= P.trainOffset) {
+            const long d = k - P.trainOffset;
             if (d % P.trainStride == 0 && d / P.trainStride < P.trainSlots) {
                 p.trainSlot = d / P.trainStride;
                 for (int b = 0; b < P.maxBounce; b++) tRec[(size_t)p.trainSlot * kWfMaxBounce + b].valid = 0;
@@ -380,49 +380,80 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
     }
 }
 
-// Builds the training batch from this pass's training records in record order (a block-wide
-// scan, so the batch and the step that follows are deterministic): raw record and the suffix
-// target (L_final - Lb) / beta / a per channel (0 where beta or a is 0).
-__global__ __launch_bounds__(1024) void k_wf_nrc_collect(const WfParams* __restrict__ prm, const WfTrainRec* __restrict__ tRec,
-                                                         const float* __restrict__ tRaw, const float* __restrict__ sampleBase,
-                                                         int batchCap, float* __restrict__ outRaw, float* __restrict__ outTarget,
-                                                         NrcState* __restrict__ state) {
-    __shared__ int waveCount[16];
-    __shared__ int waveBase[16];
-    __shared__ int total;
-    const int maxBounce = prm->maxBounce;
-    const int nRec = prm->trainSlots * maxBounce;
+// Builds the training batch from this pass's training records in record order, so the batch and
+// the step that follows are deterministic: (1) each block counts the valid records of its 2048,
+// (2) one block turns the counts into block offsets, (3) each block writes its records at its
+// offset plus an in-block prefix. A record contributes its raw encoding input and the suffix target
+// (L_final - Lb) / beta / a per channel (0 where beta or a is 0).
+constexpr int kCollectPerThread = 8;
+constexpr int kCollectBlock = 256;
+constexpr int kCollectPerBlock = kCollectPerThread * kCollectBlock;
+
+LTS_D size_t train_record(int i, int maxBounce) { return (size_t)(i / maxBounce) * kWfMaxBounce + (i % maxBounce); }
+
+// Block-wide exclusive prefix of v (256 threads); total receives the block sum.
+LTS_D int block_prefix(int v, int* sh, int& total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int base = 0;
-    for (int chunk = 0; chunk < nRec && base < batchCap; chunk += 1024) {
-        const int i = chunk + (int)threadIdx.x;
-        const size_t r = i < nRec ? (size_t)(i / maxBounce) * kWfMaxBounce + (i % maxBounce) : 0;
-        const bool valid = i < nRec && tRec[r].valid;
-        const uint64_t mask = __ballot(valid);
-        if (lane == 0) waveCount[wave] = (int)__popcll(mask);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int run = 0;
-            for (int w = 0; w < 16; w++) { waveBase[w] = run; run += waveCount[w]; }
-            total = run;
-        }
-        __syncthreads();
-        const int pos = base + waveBase[wave] + (int)__popcll(mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-        if (valid && pos < batchCap) {
-            const WfTrainRec t = tRec[r];
-            const float* Lf = sampleBase + 3 * (size_t)t.path;
-            for (int c = 0; c < 3; c++) {
-                const float suffix = t.beta[c] > 0.0f ? (Lf[c] - t.Lb[c]) / t.beta[c] : 0.0f;
-                outTarget[3 * (size_t)pos + c] = t.a[c] > 0.0f ? suffix / t.a[c] : 0.0f;
-            }
-            const float4* src = reinterpret_cast<const float4*>(tRaw + 16 * r);
-            float4* dst = reinterpret_cast<float4*>(outRaw + 16 * (size_t)pos);
-            dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2]; dst[3] = src[3];
-        }
-        base += total;
-        __syncthreads();
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
     }
-    if (threadIdx.x == 0) state->nTrain = min(base, batchCap);
+    if (lane == 63) sh[wave] = x;
+    __syncthreads();
+    int waveBase = 0;
+    for (int w = 0; w < wave; w++) waveBase += sh[w];
+    total = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return waveBase + x - v;
+}
+
+__global__ __launch_bounds__(kCollectBlock) void k_wf_nrc_count(const WfParams* __restrict__ prm, const WfTrainRec* __restrict__ tRec,
+                                                                int32_t* __restrict__ blockCount) {
+    __shared__ int sh[4];
+    const int maxBounce = prm->maxBounce, nRec = prm->trainSlots * maxBounce;
+    const int i0 = blockIdx.x * kCollectPerBlock + threadIdx.x * kCollectPerThread;
+    int c = 0;
+    for (int i = i0; i < min(i0 + kCollectPerThread, nRec); i++) c += tRec[train_record(i, maxBounce)].valid;
+    int total;
+    block_prefix(c, sh, total);
+    if (threadIdx.x == 0) blockCount[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(64) void k_wf_nrc_scan(int32_t* __restrict__ blockCount, int nBlocks, int cap,
+                                                    NrcState* __restrict__ state) {
+    if (threadIdx.x != 0) return;
+    int run = 0;
+    for (int b = 0; b < nBlocks; b++) { const int c = blockCount[b]; blockCount[b] = run; run += c; }
+    state->nTotal = min(run, cap);
+}
+
+__global__ __launch_bounds__(kCollectBlock) void k_wf_nrc_scatter(const WfParams* __restrict__ prm, const WfTrainRec* __restrict__ tRec,
+                                                                  const float* __restrict__ tRaw, const float* __restrict__ sampleBase,
+                                                                  const int32_t* __restrict__ blockBase, int batchCap,
+                                                                  float* __restrict__ outRaw, float* __restrict__ outTarget) {
+    __shared__ int sh[4];
+    const int maxBounce = prm->maxBounce, nRec = prm->trainSlots * maxBounce;
+    const int i0 = blockIdx.x * kCollectPerBlock + threadIdx.x * kCollectPerThread;
+    const int i1 = min(i0 + kCollectPerThread, nRec);
+    int c = 0;
+    for (int i = i0; i < i1; i++) c += tRec[train_record(i, maxBounce)].valid;
+    int total;
+    int pos = blockBase[blockIdx.x] + block_prefix(c, sh, total);
+    for (int i = i0; i < i1 && pos < batchCap; i++) {
+        const size_t r = train_record(i, maxBounce);
+        const WfTrainRec t = tRec[r];
+        if (!t.valid) continue;
+        const float* Lf = sampleBase + 3 * (size_t)t.path;
+        for (int ch = 0; ch < 3; ch++) {
+            const float suffix = t.beta[ch] > 0.0f ? (Lf[ch] - t.Lb[ch]) / t.beta[ch] : 0.0f;
+            outTarget[3 * (size_t)pos + ch] = t.a[ch] > 0.0f ? suffix / t.a[ch] : 0.0f;
+        }
+        const float4* src = reinterpret_cast<const float4*>(tRaw + 16 * r);
+        float4* dst = reinterpret_cast<float4*>(outRaw + 16 * (size_t)pos);
+        dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2]; dst[3] = src[3];
+        pos++;
+    }
 }
 
 template <int BS>
@@ -575,10 +606,16 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
     if (cached) {
         hipError_t e = nrc_infer(*nrc, b.qRaw, qCount, (int)b.cap, nullptr, b.qMeta, b.sampleBase, s, b.qQueue);
         if (e != hipSuccess) return e;
-        k_wf_nrc_collect<<<1, 1024, 0, s>>>(b.params, b.tRec, b.tRaw, b.sampleBase, nrc->batchCap, nrc->trainRaw,
-                                            nrc->trainTarget, nrc->state);
-        e = nrc_train_step(*nrc, s);
-        if (e != hipSuccess) return e;
+        const int nb = (b.trainCap * maxBounce + kCollectPerBlock - 1) / kCollectPerBlock;
+        if (nb > 0) {
+            k_wf_nrc_count<<<nb, kCollectBlock, 0, s>>>(b.params, b.tRec, b.tCount);
+            const int cap = nrc->batchCap * nrc->chunks;
+            k_wf_nrc_scan<<<1, 64, 0, s>>>(b.tCount, nb, cap, nrc->state);
+            k_wf_nrc_scatter<<<nb, kCollectBlock, 0, s>>>(b.params, b.tRec, b.tRaw, b.sampleBase, b.tCount, cap,
+                                                          nrc->trainRaw, nrc->trainTarget);
+        }
+        for (int k = 0; k < nrc->steps; k++)  // chunks cycle when steps > chunks
+            if ((e = nrc_train_step(*nrc, s, k)) != hipSuccess) return e;
     }
     return hipGetLastError();
 }
@@ -613,15 +650,16 @@ hipError_t wf_reserve(WfBuffers& b, size_t cap, int maxScenes) {
 #undef WTRY
     b.cap = cap;
     b.maxScenes = maxScenes;
+    b.gen = next_buffer_generation();
     return hipSuccess;
 }
 
 hipError_t wf_reserve_cache(WfBuffers& b, size_t cap, int trainCap) {
     if (cap <= b.cacheCap && trainCap <= b.trainCap) return hipSuccess;
-    void* ps[] = {b.qRaw, b.qMeta, b.qQueue, b.tRec, b.tRaw};
+    void* ps[] = {b.qRaw, b.qMeta, b.qQueue, b.tRec, b.tRaw, b.tCount};
     for (void* p : ps)
         if (p) (void)hipFree(p);
-    b.qRaw = nullptr; b.qMeta = nullptr; b.qQueue = nullptr; b.tRec = nullptr; b.tRaw = nullptr;
+    b.qRaw = nullptr; b.qMeta = nullptr; b.qQueue = nullptr; b.tRec = nullptr; b.tRaw = nullptr; b.tCount = nullptr;
     b.cacheCap = 0; b.trainCap = 0;
     hipError_t e = hipSuccess;
 #define CTRY(x) do { e = (x); if (e != hipSuccess) return e; } while (0)
@@ -630,14 +668,16 @@ hipError_t wf_reserve_cache(WfBuffers& b, size_t cap, int trainCap) {
     CTRY(wf_alloc(&b.qQueue, cap));
     CTRY(wf_alloc(&b.tRec, (size_t)trainCap * kWfMaxBounce));
     CTRY(wf_alloc(&b.tRaw, (size_t)trainCap * kWfMaxBounce * kNrcRaw));
+    CTRY(wf_alloc(&b.tCount, ((size_t)trainCap * kWfMaxBounce + kCollectPerBlock - 1) / kCollectPerBlock));
 #undef CTRY
     b.cacheCap = cap;
     b.trainCap = trainCap;
+    b.cacheGen = next_buffer_generation();
     return hipSuccess;
 }
 
 void wf_release(WfBuffers& b) {
-    void* cs[] = {b.qRaw, b.qMeta, b.qQueue, b.tRec, b.tRaw};
+    void* cs[] = {b.qRaw, b.qMeta, b.qQueue, b.tRec, b.tRaw, b.tCount};
     for (void* p : cs)
         if (p) (void)hipFree(p);
     void* ps[] = {b.scenes, b.params, b.slots, b.paths, b.rays, b.hits, b.nee, b.queue[0], b.queue[1], b.sorted, b.shadows, b.mis, b.counts};

@@ -32,9 +32,10 @@ def test_cached_without_queries_equals_wavefront(gpu_ctx, cornell):
     print("[cached: warm-up passes only]", flush=True)
     gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], W, H)
     ref, _, _ = gpu_ctx.render(16, 5, 3, SEED, stratified=1, integrator=INTEGRATOR_WAVEFRONT)
-    gpu_ctx.nrc_configure(warmup_passes=1000, seed=3)
+    gpu_ctx.nrc_configure(warmup_passes=1000, seed=3, train_steps=3)
     img, _, st = gpu_ctx.render(16, 5, 3, SEED, stratified=1, integrator=INTEGRATOR_CACHED, spp_per_pass=4)
-    assert st["nrc_queries"] == 0 and st["nrc_steps"] == 4 and st["nrc_train_samples"] > 0
+    # 4 passes of 3 Adam steps each (cycling over the samples collected in the pass)
+    assert st["nrc_queries"] == 0 and st["nrc_steps"] == 12 and st["nrc_train_samples"] > 0
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
 
 
@@ -47,9 +48,9 @@ def test_trained_cache_beats_zero_cache(gpu_ctx, cornell, precision):
     ref, _, _ = gpu_ctx.render(4096, 5, 3, SEED + 1, integrator=INTEGRATOR_WAVEFRONT)
     plain, _, _ = gpu_ctx.render(32, 5, 3, SEED, integrator=INTEGRATOR_WAVEFRONT)
     # online training while rendering: 32 warm-up passes, then 96 cached passes
-    gpu_ctx.nrc_configure(precision=precision, warmup_passes=32, seed=7)
+    gpu_ctx.nrc_configure(precision=precision, warmup_passes=32, seed=7, train_steps=2)
     _, _, st = gpu_ctx.render(128, 5, 3, SEED + 2, integrator=INTEGRATOR_CACHED, spp_per_pass=1)
-    assert st["nrc_steps"] == 128 and st["nrc_queries"] > 0
+    assert st["nrc_steps"] == 256 and st["nrc_queries"] > 0
     trained, _, st = gpu_ctx.render(32, 5, 3, SEED, integrator=INTEGRATOR_CACHED, spp_per_pass=1)
     # the same integrator with an all-zero cache that does not learn (learning rate 1e-30)
     gpu_ctx.nrc_configure(precision=precision, warmup_passes=-1, seed=7, lr=1e-30)
@@ -60,3 +61,37 @@ def test_trained_cache_beats_zero_cache(gpu_ctx, cornell, precision):
           f"loss {st['nrc_loss']:.4f}, queries/pass {st['nrc_queries']}", flush=True)
     assert np.isfinite(trained).all()
     assert e_t < 0.5 * e_z
+
+
+@pytest.mark.gpu
+def test_reconfigured_learning_rate_reaches_the_replayed_pass_graph(gpu_ctx, cornell):
+    """Adam's hyper-parameters live in device memory: a reconfigured cache with the same buffers and
+    step count replays the captured pass graph, which must train with the new learning rate."""
+    d, cfg = cornell
+    gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], W, H)
+    gpu_ctx.nrc_configure(seed=5, lr=1e-30, warmup_passes=-1, train_steps=1)
+    w0 = gpu_ctx.nrc_get_weights()
+    gpu_ctx.render(4, 5, 3, SEED, integrator=INTEGRATOR_CACHED, spp_per_pass=1)
+    assert np.array_equal(gpu_ctx.nrc_get_weights(), w0)  # 1e-30 steps vanish against weights ~0.1
+    gpu_ctx.nrc_configure(seed=5, lr=1e-2, warmup_passes=-1, train_steps=1)
+    assert np.array_equal(gpu_ctx.nrc_get_weights(), w0)
+    _, _, st = gpu_ctx.render(4, 5, 3, SEED, integrator=INTEGRATOR_CACHED, spp_per_pass=1)
+    moved = np.abs(gpu_ctx.nrc_get_weights() - w0).max()
+    assert st["nrc_steps"] == 4 and 5e-3 < moved < 0.1
+
+
+@pytest.mark.gpu
+def test_reallocated_cache_buffers_recapture_the_pass_graph(gpu_ctx, cornell):
+    """Reconfiguring the cache with a different sample capacity but the same step count reallocates
+    its buffers; the next cached pass must be captured again for the new buffers (a freed buffer's
+    address can come back from the allocator, so the graph is matched by allocation generation)."""
+    d, cfg = cornell
+    gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], W, H)
+    results = []
+    for train_paths, batch in [(524288, 0), (262144, 0), (262144, 32768), (524288, 0)]:
+        gpu_ctx.nrc_configure(seed=9, warmup_passes=-1, train_steps=2, train_paths=train_paths, batch=batch)
+        img, _, st = gpu_ctx.render(4, 5, 3, SEED, integrator=INTEGRATOR_CACHED, spp_per_pass=2)
+        assert np.isfinite(img).all() and st["nrc_steps"] == 4 and st["nrc_queries"] > 0
+        results.append(img)
+    # same configuration, same seed: the same image
+    assert np.array_equal(results[0].view(np.uint32), results[3].view(np.uint32))
