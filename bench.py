@@ -1,17 +1,21 @@
-"""Throughput of the plain path tracer on one MI355X, as one JSON line.
+"""Throughput of the path tracer on one MI355X, as one JSON line.
 
-A step is one full frame of the workload (all samples of every pixel) through the megakernel and
-the film gather, with the scene and BVH already resident in HBM. Workloads are generated from
-fixed seeds in the reference's own file formats (lighttransportsimulation_amd/scenegen.py).
+A step is one full frame of the workload (all samples of every pixel) through the wavefront
+integrator (hipGraph-replayed passes, autotuned in the warm-up) and the film gather, with the scene
+and BVH already resident in HBM. Workloads are generated from fixed seeds in the reference's own
+file formats (lighttransportsimulation_amd/scenegen.py).
 
 Extra fields:
-  roofline      the integrator kernel against HBM bandwidth: algorithmic bytes per launch are the
-                BVH node tests (32 B each) and triangle tests (36 B each) that the reference's
-                traversal performs for the same rays, measured per sample by the CPU oracle on
-                the bounded sample below, times the samples of one launch, over the launch's mean
-                duration from HIP events.
+  roofline      the traversal kernels against HBM bandwidth: algorithmic bytes are the BVH node
+                tests (32 B each) and triangle tests (36 B each) that the reference's traversal
+                performs for the same rays, measured per sample by the CPU oracle on the bounded
+                sample below, times the samples of a frame, over the traversal kernels' device time
+                per frame from HIP events recorded between the kernels of the replayed graphs.
   cpu_baseline  the oracle (the reference integrator restated in C++) on all host cores over a
                 bounded band of rows of the same frame.
+  quality       (1080p) equal-time relMSE of the cached integrator (learned radiance cache) and the
+                plain wavefront integrator against a 4096-spp plain reference, and the cache
+                network's MFMA throughput.
 """
 from __future__ import annotations
 
@@ -124,6 +128,106 @@ def run_lbvh(args) -> dict:
     return out
 
 
+# Radiance-cache network work per sample (64 -> 64 x5 -> 3): 2 (5 * 64 * 64 + 64 * 3) flops forward;
+# a training sample costs the forward, the activation gradients and the weight gradients (3x).
+NRC_FLOPS_FORWARD = 2 * (5 * 64 * 64 + 64 * 3)
+NRC_FLOPS_TRAIN = 3 * NRC_FLOPS_FORWARD
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+# cached-integrator settings of the quality comparison (queries at bounce 2 as specified, and at 1)
+CACHE_SETTINGS = dict(warmup_passes=2, train_steps=32, lr=3e-3, seed=1)
+
+
+def relmse(x, r):
+    import numpy as np
+    return float(np.mean((x - r) ** 2 / (r * r + 0.01)))
+
+
+def time_to_reach(curve, target):
+    """Wall ms at which a (spp, ms, relmse) curve reaches `target`, interpolated in log-log between the
+    bracketing points; None when the curve never gets there."""
+    import math
+    for (s0, t0, e0), (s1, t1, e1) in zip(curve, curve[1:]):
+        if e0 >= target >= e1 and e0 > e1:
+            f = (math.log(e0) - math.log(target)) / (math.log(e0) - math.log(e1))
+            return math.exp(math.log(t0) + f * (math.log(t1) - math.log(t0)))
+    if curve and curve[0][2] <= target:
+        return curve[0][1] * curve[0][2] / target  # already below at the first point: scale back as 1/time
+    return None
+
+
+def relmse_at(curve, t):
+    """relMSE of a curve at wall time t (log-log interpolation, 1/time extrapolation beyond the ends)."""
+    import math
+    pts = sorted(curve, key=lambda c: c[1])
+    if t <= pts[0][1]:
+        return pts[0][2] * pts[0][1] / t
+    for (s0, t0, e0), (s1, t1, e1) in zip(pts, pts[1:]):
+        if t0 <= t <= t1:
+            f = (math.log(t) - math.log(t0)) / (math.log(t1) - math.log(t0))
+            return math.exp(math.log(e0) + f * (math.log(e1) - math.log(e0)))
+    return pts[-1][2] * pts[-1][1] / t
+
+
+def quality(ctx, cfg, tuning, spps=(16, 32, 64, 128), ref_spp=4096) -> dict:
+    """Equal-time relMSE of the cached and the plain wavefront tracer against a plain reference."""
+    from lighttransportsimulation_amd import INTEGRATOR_CACHED, INTEGRATOR_WAVEFRONT
+    base = dict(max_bounce=cfg["max_bounce"], rr_start=cfg["rr_start"], stratified=cfg["stratified"])
+    tune = {k: tuning[k] for k in ("trace_block", "persistent_blocks", "sort_materials")} if tuning else {}
+    t0 = time.perf_counter()
+    ref, _, _ = ctx.render(ref_spp, seed=cfg["seed"] + 1000, integrator=INTEGRATOR_WAVEFRONT, **base, **tune)
+    ref_s = time.perf_counter() - t0
+
+    def timed(spp, integrator, **kw):
+        t = time.perf_counter()
+        img, _, st = ctx.render(spp, seed=cfg["seed"], integrator=integrator, **base, **tune, **kw)
+        return img, (time.perf_counter() - t) * 1e3, st
+
+    plain = []
+    for spp in spps:
+        img, ms, _ = timed(spp, INTEGRATOR_WAVEFRONT)
+        plain.append((spp, round(ms, 2), relmse(img, ref)))
+    out = {"relmse": "mean over pixels and channels of (x - r)^2 / (r^2 + 0.01)",
+           "reference": f"plain wavefront, {ref_spp} spp, independent seed ({ref_s:.1f} s)",
+           "time": "wall ms of the render call, fresh cache per point (training and warm-up included)",
+           "plain": [list(p) for p in plain], "cached": {}}
+    nominal = cfg["spp"]
+    p_nom = next(p for p in plain if p[0] == nominal)
+    nrc_perf = None
+    for qb in (2, 1):
+        curve = []
+        for spp in spps:
+            ctx.nrc_configure(query_bounce=qb, **CACHE_SETTINGS)
+            img, ms, st = timed(spp, INTEGRATOR_CACHED, profile_phases=1)
+            curve.append((spp, round(ms, 2), relmse(img, ref)))
+            if qb == 2 and spp == nominal:
+                nrc_perf = st
+        t_reach = time_to_reach(curve, p_nom[2])
+        out["cached"][f"query_bounce_{qb}"] = {
+            "curve": [list(c) for c in curve],
+            "ms_to_plain_relmse_at_nominal_spp": None if t_reach is None else round(t_reach, 1),
+            "equal_time_speedup": None if t_reach is None else round(p_nom[1] / t_reach, 3),
+            "relmse_ratio_plain_over_cached_at_plain_nominal_time": round(relmse_at(plain, p_nom[1]) /
+                                                                         relmse_at(curve, p_nom[1]), 3)}
+    out["plain_relmse_at_nominal_spp"] = p_nom[2]
+    out["cache_settings"] = dict(CACHE_SETTINGS, batch=65536, train_paths=262144, precision="bf16")
+    if nrc_perf:
+        # MFMA work of the cache in the 64-spp cached render: queries and training samples of the
+        # last pass times the passes, over the phase times summed by HIP events
+        passes = nrc_perf["launches"]
+        q = nrc_perf["nrc_queries"] * passes
+        n_total, batch, steps = nrc_perf["nrc_train_samples"], 65536, CACHE_SETTINGS["train_steps"]
+        chunks = max(1, -(-n_total // batch))
+        per_pass = sum(max(0, min(batch, n_total - (k % chunks) * batch)) for k in range(steps))
+        inf_ms, tr_ms = nrc_perf["phase_ms"]["cache_infer"], nrc_perf["phase_ms"]["cache_train"]
+        out["cache_mfma"] = {
+            "infer_tflops": round(NRC_FLOPS_FORWARD * q / (inf_ms * 1e-3) / 1e12, 2) if inf_ms > 0 else None,
+            "train_tflops": round(NRC_FLOPS_TRAIN * per_pass * passes / (tr_ms * 1e-3) / 1e12, 2) if tr_ms > 0 else None,
+            "peak_tflops": MFMA_BF16_PEAK_TFLOPS, "infer_ms": round(inf_ms, 3), "train_ms": round(tr_ms, 3),
+            "queries": q, "train_samples": per_pass * passes,
+            "flops_per_query": NRC_FLOPS_FORWARD, "flops_per_train_sample": NRC_FLOPS_TRAIN}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,8 +236,9 @@ def main() -> int:
     ap.add_argument("--workload", default="displaced_1080p", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle work for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--integrator", default="megakernel", choices=("megakernel", "wavefront"))
+    ap.add_argument("--integrator", default="wavefront", choices=("megakernel", "wavefront"))
     ap.add_argument("--autotune", type=int, default=1, help="wavefront: tune block/grid/sort during warm-up")
+    ap.add_argument("--no-quality", action="store_true", help="skip the cached-vs-plain equal-time comparison")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the plain 1 GiB device copy (traffic-counter calibration)")
     args = ap.parse_args()
@@ -162,7 +267,7 @@ def main() -> int:
             kw = dict(spp=spp, max_bounce=cfg["max_bounce"], rr_start=cfg["rr_start"], seed=cfg["seed"],
                       stratified=cfg["stratified"], want_image=False,
                       integrator=INTEGRATOR_WAVEFRONT if wavefront else INTEGRATOR_MEGAKERNEL,
-                      autotune=int(wavefront and args.autotune))
+                      autotune=int(wavefront and args.autotune), profile_phases=int(wavefront))
             tuning = None
             for _ in range(max(args.warmup, 1 if wavefront and args.autotune else 0)):
                 _, _, st = ctx.render(**kw)
@@ -171,12 +276,36 @@ def main() -> int:
                     tuning["autotune_ms"] = round(st["autotune_ms"], 1)
             t0 = time.perf_counter()
             dev_ms, kern_ms, launches = 0.0, 0.0, 0
+            phase = {}
             for _ in range(args.steps):
                 _, _, st = ctx.render(**kw)
                 dev_ms += st["render_ms"] + st["resolve_ms"]
                 kern_ms += st["render_ms"]
                 launches += st["launches"]
+                for k, v in st["phase_ms"].items():
+                    phase[k] = phase.get(k, 0.0) + v
             wall = time.perf_counter() - t0
+            graph_rate = None
+            if wavefront:
+                # the same frames replayed as captured graphs, without the phase events
+                gk = dict(kw, autotune=0, profile_phases=0, trace_block=tuning["trace_block"],
+                          persistent_blocks=tuning["persistent_blocks"], sort_materials=tuning["sort_materials"])
+                ctx.render(**gk)
+                t1 = time.perf_counter()
+                for _ in range(args.steps):
+                    ctx.render(**gk)
+                graph_rate = W * H * spp * args.steps / (time.perf_counter() - t1) / 1e6
+            mega = None
+            if wavefront:
+                # the megakernel on the same frame, for comparison (not the headline value)
+                mk = dict(kw, integrator=INTEGRATOR_MEGAKERNEL, autotune=0, profile_phases=0)
+                ctx.render(**mk)
+                t1 = time.perf_counter()
+                for _ in range(2):
+                    ctx.render(**mk)
+                mega = W * H * spp * 2 / (time.perf_counter() - t1) / 1e6
+            qual = quality(ctx, cfg, tuning) if (wavefront and not args.no_quality
+                                                  and args.workload == "displaced_1080p") else None
         samples = W * H * spp
         ms_per_step = wall * 1e3 / args.steps
         value = samples * args.steps / wall / 1e6
@@ -190,19 +319,36 @@ def main() -> int:
         }
         if tuning:
             out["config"]["tuning"] = tuning
+        if mega:
+            out["config"]["megakernel_msamples_per_s"] = round(mega, 3)
+        if graph_rate:
+            out["config"]["graph_replay_msamples_per_s"] = round(graph_rate, 3)
+            out["config"]["timed_region"] = "passes launched directly, HIP events between the phases"
+        if phase:
+            out["config"]["phase_ms_per_step"] = {k: round(v / args.steps, 3) for k, v in phase.items()}
         if not args.no_cpu:
             cb = cpu_baseline(d, cfg, args.cpu_budget)
-            launch_s = kern_ms / launches / 1e3
             bytes_per_sample = 32.0 * cb["node_visits_per_sample"] + 36.0 * cb["tri_tests_per_sample"]
-            samples_per_launch = samples * args.steps / launches
-            achieved = bytes_per_sample * samples_per_launch / launch_s / 1e9
+            if wavefront:
+                # traversal kernels (closest hit of path rays, any hit of shadow rays, closest hit of
+                # MIS rays) against all node and triangle tests of the frame
+                trav_ms = (phase["extend"] + phase["shadow"] + phase["mis"]) / args.steps
+                achieved = bytes_per_sample * samples / (trav_ms * 1e-3) / 1e9
+                kernel = "k_wf_extend + k_wf_shadow + k_wf_mis (per frame)"
+                kernel_ms = trav_ms
+            else:
+                launch_s = kern_ms / launches / 1e3
+                achieved = bytes_per_sample * (samples * args.steps / launches) / launch_s / 1e9
+                kernel = "k_megakernel"
+                kernel_ms = kern_ms / launches
             out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                               "kernel": "wavefront pass (graph)" if wavefront else "k_megakernel",
-                               "kernel_avg_ms": round(kern_ms / launches, 3),
+                               "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kernel,
+                               "kernel_ms": round(kernel_ms, 3),
                                "algorithmic_bytes_per_sample": round(bytes_per_sample, 1)}
             out["cpu_baseline"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cb.items()
                                    if k in ("value", "unit", "cores", "kind", "sample")}
+        if qual:
+            out["quality"] = qual
         print(json.dumps(out), flush=True)
     return 0
 

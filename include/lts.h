@@ -70,8 +70,20 @@ typedef struct lts_render_params {
     int32_t autotune;           /* 1: time candidate tunings on this scene before rendering;    */
                                 /*    the choice is kept per scene, film and pass size in the   */
                                 /*    context, and in the file named by $LTS_TUNE_CACHE if set  */
-    int32_t reserved[4];
+    int32_t profile_phases;     /* 1: time the wavefront phases with HIP events (stats.phase_ms); */
+                                /*    passes are then launched directly, not replayed as graphs  */
+    int32_t reserved[3];
 } lts_render_params;
+
+/* Wavefront phases timed when profile_phases is set (indices into lts_stats.phase_ms). */
+#define LTS_PHASE_EXTEND 0      /* closest-hit traversal of path rays                        */
+#define LTS_PHASE_SORT 1        /* material sort of the shading queue                        */
+#define LTS_PHASE_SHADE 2
+#define LTS_PHASE_SHADOW 3      /* any-hit traversal of shadow rays                          */
+#define LTS_PHASE_MIS 4         /* closest-hit traversal of MIS-BSDF rays                    */
+#define LTS_PHASE_FINALIZE 5
+#define LTS_PHASE_CACHE_INFER 6 /* radiance-cache inference (cached integrator)              */
+#define LTS_PHASE_CACHE_TRAIN 7 /* training-batch compaction and Adam steps                   */
 
 typedef struct lts_stats {
     double render_ms;       /* device time of the integrator kernels (HIP events)  */
@@ -91,6 +103,8 @@ typedef struct lts_stats {
     int64_t nrc_queries;    /* cached: cache queries of the last pass               */
     int32_t nrc_train_samples; /* cached: training samples collected in the last pass */
     int32_t nrc_steps;      /* cached: Adam steps taken by the cache so far          */
+    double phase_ms[8];     /* device time per LTS_PHASE_* summed over the passes     */
+    int64_t phase_launches[8]; /* kernel launches per phase                          */
 } lts_stats;
 
 /* Context. */

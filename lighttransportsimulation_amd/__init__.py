@@ -28,6 +28,7 @@ EXPORTED_SYMBOLS = (
 
 NRC_PARAMS = 21504
 NRC_RAW = 16
+PHASES = ("extend", "sort", "shade", "shadow", "mis", "finalize", "cache_infer", "cache_train")
 
 
 class LtsError(RuntimeError):
@@ -48,7 +49,8 @@ class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_int32), ("max_bounce", C.c_int32), ("rr_start", C.c_int32), ("seed", C.c_uint32),
                 ("stratified", C.c_int32), ("integrator", C.c_int32), ("spp_per_pass", C.c_int32),
                 ("trace_block", C.c_int32), ("persistent_blocks", C.c_int32), ("sort_materials", C.c_int32),
-                ("use_graph", C.c_int32), ("autotune", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("use_graph", C.c_int32), ("autotune", C.c_int32), ("profile_phases", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class NrcConfig(C.Structure):
@@ -63,7 +65,8 @@ class Stats(C.Structure):
                 ("node_visits", C.c_int64), ("tri_tests", C.c_int64), ("trace_block", C.c_int32),
                 ("persistent_blocks", C.c_int32), ("sort_materials", C.c_int32), ("use_graph", C.c_int32),
                 ("autotune_ms", C.c_double), ("nrc_loss", C.c_double), ("nrc_queries", C.c_int64),
-                ("nrc_train_samples", C.c_int32), ("nrc_steps", C.c_int32)]
+                ("nrc_train_samples", C.c_int32), ("nrc_steps", C.c_int32), ("phase_ms", C.c_double * 8),
+                ("phase_launches", C.c_int64 * 8)]
 
 
 _lib = None
@@ -237,11 +240,11 @@ class Context:
     def render(self, spp: int, max_bounce: int, rr_start: int, seed: int, stratified: int = 0,
                integrator: int = INTEGRATOR_MEGAKERNEL, spp_per_pass: int = 0, want_image: bool = True,
                want_samples: bool = False, trace_block: int = 0, persistent_blocks: int = 0, sort_materials: int = -1,
-               use_graph: int = -1, autotune: int = 0):
+               use_graph: int = -1, autotune: int = 0, profile_phases: int = 0):
         """Renders the loaded scene (include/lts.h: lts_render). Returns (rgb, samples, stats): rgb is
         [H][W][3] in getRender row order, samples [spp][H][W][3] per-sample radiance when asked for."""
         p = render_params(spp, max_bounce, rr_start, seed, stratified, integrator, spp_per_pass, trace_block,
-                          persistent_blocks, sort_materials, use_graph, autotune)
+                          persistent_blocks, sort_materials, use_graph, autotune, profile_phases)
         rgb = np.zeros((self.height, self.width, 3), np.float32) if want_image else None
         smp = np.zeros((spp, self.height, self.width, 3), np.float32) if want_samples else None
         st = Stats()
@@ -250,9 +253,10 @@ class Context:
 
 
 def render_params(spp, max_bounce, rr_start, seed, stratified=0, integrator=INTEGRATOR_MEGAKERNEL, spp_per_pass=0,
-                  trace_block=0, persistent_blocks=0, sort_materials=-1, use_graph=-1, autotune=0) -> RenderParams:
+                  trace_block=0, persistent_blocks=0, sort_materials=-1, use_graph=-1, autotune=0,
+                  profile_phases=0) -> RenderParams:
     return RenderParams(spp, max_bounce, rr_start, seed, stratified, integrator, spp_per_pass, trace_block,
-                        persistent_blocks, sort_materials, use_graph, autotune)
+                        persistent_blocks, sort_materials, use_graph, autotune, profile_phases)
 
 
 def _stats_dict(st: Stats) -> dict:
@@ -261,7 +265,9 @@ def _stats_dict(st: Stats) -> dict:
                 node_visits=st.node_visits, tri_tests=st.tri_tests, trace_block=st.trace_block,
                 persistent_blocks=st.persistent_blocks, sort_materials=st.sort_materials, use_graph=st.use_graph,
                 autotune_ms=st.autotune_ms, nrc_loss=st.nrc_loss, nrc_queries=st.nrc_queries,
-                nrc_train_samples=st.nrc_train_samples, nrc_steps=st.nrc_steps)
+                nrc_train_samples=st.nrc_train_samples, nrc_steps=st.nrc_steps,
+                phase_ms={k: st.phase_ms[i] for i, k in enumerate(PHASES)},
+                phase_launches={k: st.phase_launches[i] for i, k in enumerate(PHASES)})
 
 
 def render_batch(contexts, spp: int, max_bounce: int, rr_start: int, seed: int, stratified: int = 0,

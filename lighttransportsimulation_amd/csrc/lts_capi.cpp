@@ -84,6 +84,8 @@ struct lts_ctx {
     int wfKeyBounce = -1, wfKeyBins = -1;
     WfTuning wfKeyTune;
     int wfKeyPrecision = -1;
+    bool wfKeyProfile = false;
+    WfProfile* prof = nullptr;  // phase events, created on first use
     WfParams wfParams{};                  // host copies of the per-pass parameters (stable for async H2D)
     std::vector<WfSceneSlot> wfSlots;
     std::map<std::string, WfTuning> tuned;  // autotuner results of this context
@@ -235,22 +237,22 @@ bool same_tuning(const WfTuning& a, const WfTuning& b) {
 
 // Runs the pass whose parameters are resident in c->wf: replays the captured graph, capturing it
 // first when the buffers, bounce count or tuning changed since the last capture.
-int wf_run(lts_ctx* c, const WfTuning& t, int maxBounce, const NrcBuffers* nrc) {
+int wf_run(lts_ctx* c, const WfTuning& t, int maxBounce, const NrcBuffers* nrc, const WfProfile* prof = nullptr) {
     hipStream_t s = c->stream;
     if (!t.useGraph) {
-        HIPCHK(c, wf_enqueue_pass(c->wf, t, maxBounce, nrc, s));
+        HIPCHK(c, wf_enqueue_pass(c->wf, t, maxBounce, nrc, s, prof));
         return LTS_OK;
     }
     const uint64_t nrcGen = nrc ? nrc->gen : 0, cacheGen = nrc ? c->wf.cacheGen : 0;
     const int precision = nrc ? nrc->precision + 2 * nrc->steps : -1;  // kernels and step count of the pass
     if (!c->wfExec || c->wfKeyGen != c->wf.gen || c->wfKeyCacheGen != cacheGen || c->wfKeyNrcGen != nrcGen ||
         c->wfKeySampleGen != c->sampleGen || c->wfKeyBounce != maxBounce || c->wfKeyBins != c->wf.matBins ||
-        !same_tuning(c->wfKeyTune, t) || c->wfKeyPrecision != precision) {
+        !same_tuning(c->wfKeyTune, t) || c->wfKeyPrecision != precision || c->wfKeyProfile != (prof != nullptr)) {
         if (c->wfExec) (void)hipGraphExecDestroy(c->wfExec);
         c->wfExec = nullptr;
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        hipError_t e = wf_enqueue_pass(c->wf, t, maxBounce, nrc, s);
+        hipError_t e = wf_enqueue_pass(c->wf, t, maxBounce, nrc, s, prof);
         hipError_t e2 = hipStreamEndCapture(s, &g);
         if (e == hipSuccess) e = e2;
         if (e == hipSuccess) e = hipGraphInstantiate(&c->wfExec, g, nullptr, nullptr, 0);
@@ -264,6 +266,7 @@ int wf_run(lts_ctx* c, const WfTuning& t, int maxBounce, const NrcBuffers* nrc) 
         c->wfKeyBins = c->wf.matBins;
         c->wfKeyTune = t;
         c->wfKeyPrecision = precision;
+        c->wfKeyProfile = prof != nullptr;
     }
     HIPCHK(c, hipGraphLaunch(c->wfExec, s));
     return LTS_OK;
@@ -457,10 +460,18 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
     if (p->persistent_blocks) t.persistentBlocks = p->persistent_blocks;
     if (p->sort_materials >= 0) t.sortMaterials = p->sort_materials != 0;
     if (p->use_graph >= 0) t.useGraph = p->use_graph != 0;
+    if (p->profile_phases) t.useGraph = 0;  // phase events are recorded between direct launches
 
     double renderMs = 0, resolveMs = 0;
     int64_t launches = 0;
     const size_t npix0 = (size_t)views[0].cam.width * views[0].cam.height;
+    double phase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (p->profile_phases && !c->prof) {
+        c->prof = new WfProfile();
+        for (hipEvent_t& e : c->prof->ev) e = nullptr;
+        for (hipEvent_t& e : c->prof->ev) HIPCHK(c, hipEventCreate(&e));
+    }
+    const WfProfile* prof = p->profile_phases ? c->prof : nullptr;
     for (int s0 = 0; s0 < p->spp; s0 += pass) {
         const int s1 = std::min(p->spp, s0 + pass);
         PassCache pc;
@@ -478,7 +489,7 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         int64_t paths = 0;
         if ((rc = wf_upload_pass(c, scenes, p, s0, s1, &paths, pc)) != LTS_OK) return rc;
         HIPCHK(c, hipEventRecord(c->ev[5], s));
-        if ((rc = wf_run(c, t, p->max_bounce, nrc)) != LTS_OK) return rc;
+        if ((rc = wf_run(c, t, p->max_bounce, nrc, prof)) != LTS_OK) return rc;
         HIPCHK(c, hipEventRecord(c->ev[6], s));
         for (size_t i = 0; i < scenes.size(); i++) {
             RenderLaunch rl{};
@@ -492,6 +503,17 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         renderMs += ms_between(c->ev[5], c->ev[6]);
         resolveMs += ms_between(c->ev[6], c->ev[7]);
         launches++;
+        if (prof && p->max_bounce > 0) {
+            for (int b = 0; b < p->max_bounce; b++) {
+                const hipEvent_t* e = prof->ev + 8 * b;
+                for (int k = 0; k < 6; k++) phase[k] += ms_between(e[k], e[k + 1]);
+            }
+            if (cached) {
+                const hipEvent_t* e = prof->ev + 8 * kWfMaxBounce;
+                phase[LTS_PHASE_CACHE_INFER] += ms_between(e[0], e[1]);
+                phase[LTS_PHASE_CACHE_TRAIN] += ms_between(e[1], e[2]);
+            }
+        }
         if (sample_rgb)
             HIPCHK(c, hipMemcpy(sample_rgb, c->sampleL, (size_t)paths * 3 * sizeof(float), hipMemcpyDeviceToHost));
     }
@@ -516,6 +538,18 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         st->sort_materials = t.sortMaterials;
         st->use_graph = t.useGraph;
         st->autotune_ms = tuneMs;
+        for (int k = 0; k < 8; k++) st->phase_ms[k] = phase[k];
+        if (prof && p->max_bounce > 0) {
+            const int64_t perBounce = launches * p->max_bounce;
+            st->phase_launches[LTS_PHASE_EXTEND] = perBounce;
+            st->phase_launches[LTS_PHASE_SORT] = t.sortMaterials ? 3 * perBounce : 0;
+            st->phase_launches[LTS_PHASE_SHADE] = perBounce;
+            st->phase_launches[LTS_PHASE_SHADOW] = perBounce;
+            st->phase_launches[LTS_PHASE_MIS] = perBounce;
+            st->phase_launches[LTS_PHASE_FINALIZE] = perBounce;
+            st->phase_launches[LTS_PHASE_CACHE_INFER] = cached ? launches : 0;
+            st->phase_launches[LTS_PHASE_CACHE_TRAIN] = cached ? launches * c->nrc.steps : 0;
+        }
         if (cached) {
             NrcState ns;
             int32_t q = 0;
@@ -637,6 +671,11 @@ void lts_destroy(lts_ctx* c) {
     for (hipEvent_t e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->wfExec) (void)hipGraphExecDestroy(c->wfExec);
+    if (c->prof) {
+        for (hipEvent_t e : c->prof->ev)
+            if (e) (void)hipEventDestroy(e);
+        delete c->prof;
+    }
     wf_release(c->wf);
     nrc_release(c->nrc);
     if (c->nrcIo) (void)hipFree(c->nrcIo);

@@ -161,13 +161,21 @@ struct WfBuffers {
 constexpr int kWfGlobalBase = (kWfCountsPerBounce + kWfMaxMatBins) * (kWfMaxBounce + 1);
 constexpr int kWfGlobalCounts = 8;  // [0] cache queries
 
+// Optional phase timing: events recorded between the kernels of a pass launched directly (not
+// captured). Bounce b uses ev[8b + 0..6]: before extend, after extend, sort, shade, shadow, mis and
+// finalize; the cached integrator's tail uses ev[8 kWfMaxBounce + 0..2]: before inference, after
+// inference, after the training steps.
+constexpr int kWfPhaseEvents = 8 * kWfMaxBounce + 3;
+struct WfProfile { hipEvent_t ev[kWfPhaseEvents]; };
+
 hipError_t wf_reserve(WfBuffers& b, size_t cap, int maxScenes);
 hipError_t wf_reserve_cache(WfBuffers& b, size_t cap, int trainCap);
 void wf_release(WfBuffers& b);
 // Enqueues one pass: generate, then per bounce extend / (sort) / shade / shadow / mis / finalize.
 // With nrc (cached integrator): queries, training records, then cache inference into the sample
 // buffer, the training batch of path-suffix radiance and one Adam step.
-hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce, const NrcBuffers* nrc, hipStream_t stream);
+hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce, const NrcBuffers* nrc, hipStream_t stream,
+                           const WfProfile* prof = nullptr);
 
 // ---------------------------------------------------------------- learned radiance cache (nrc.hip)
 // Network: 16-float raw record -> 64-dim encoding -> 5 hidden layers of 64 (ReLU) -> 3 outputs.

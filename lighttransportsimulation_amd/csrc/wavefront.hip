@@ -555,7 +555,11 @@ __global__ void k_wf_fill(int32_t* p, int n, int32_t v) {
 
 }  // namespace
 
-hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce, const NrcBuffers* nrc, hipStream_t s) {
+hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce, const NrcBuffers* nrc, hipStream_t s,
+                           const WfProfile* prof) {
+    auto mark = [&](int i) {
+        if (prof) (void)hipEventRecord(prof->ev[i], s);
+    };
     const int gridAll = t.sweepBlocks;
     const int ncounts = kWfCountsPerBounce * (maxBounce + 1) + kWfMaxMatBins * (maxBounce + 1);
     k_wf_fill<<<(ncounts + 255) / 256, 256, 0, s>>>(b.counts, ncounts, 0);
@@ -571,11 +575,13 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
         const int q = bounce & 1;
         int32_t* cnt = b.counts + kWfCountsPerBounce * bounce;
         int32_t* nxt = b.counts + kWfCountsPerBounce * (bounce + 1);
+        mark(8 * bounce + 0);
         switch (t.traceBlock) {
             case 64: k_wf_extend<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3); break;
             case 128: k_wf_extend<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3); break;
             default: k_wf_extend<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3); break;
         }
+        mark(8 * bounce + 1);
         const int32_t* shadeQueue = b.queue[q];
         if (t.sortMaterials) {
             int32_t* hist = b.counts + kWfCountsPerBounce * (maxBounce + 1) + kWfMaxMatBins * bounce;
@@ -584,28 +590,31 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
             k_wf_mat_scatter<<<gridAll, 256, 0, s>>>(b.scenes, b.paths, b.hits, b.queue[q], cnt, hist, b.matBins, b.sorted);
             shadeQueue = b.sorted;
         }
+        mark(8 * bounce + 2);
         k_wf_shade<<<gridAll, kShadeBlock, 0, s>>>(b.scenes, b.params, b.slots, bounce, b.paths, b.rays, b.hits, b.nee, shadeQueue,
                                                    cnt, b.queue[q ^ 1], nxt, b.shadows, cnt + 1, b.mis, cnt + 2,
                                                    cached ? b.qRaw : nullptr, b.qMeta, b.qQueue, qCount, b.tRec, b.tRaw);
+        mark(8 * bounce + 3);
         switch (t.traceBlock) {
-            case 64:
-                k_wf_shadow<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.shadows, cnt + 1, b.nee, cnt + 4);
-                k_wf_mis<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.mis, cnt + 2, b.nee, cnt + 5);
-                break;
-            case 128:
-                k_wf_shadow<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.shadows, cnt + 1, b.nee, cnt + 4);
-                k_wf_mis<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.mis, cnt + 2, b.nee, cnt + 5);
-                break;
-            default:
-                k_wf_shadow<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.shadows, cnt + 1, b.nee, cnt + 4);
-                k_wf_mis<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.mis, cnt + 2, b.nee, cnt + 5);
-                break;
+            case 64: k_wf_shadow<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.shadows, cnt + 1, b.nee, cnt + 4); break;
+            case 128: k_wf_shadow<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.shadows, cnt + 1, b.nee, cnt + 4); break;
+            default: k_wf_shadow<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.shadows, cnt + 1, b.nee, cnt + 4); break;
         }
+        mark(8 * bounce + 4);
+        switch (t.traceBlock) {
+            case 64: k_wf_mis<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.mis, cnt + 2, b.nee, cnt + 5); break;
+            case 128: k_wf_mis<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.mis, cnt + 2, b.nee, cnt + 5); break;
+            default: k_wf_mis<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.mis, cnt + 2, b.nee, cnt + 5); break;
+        }
+        mark(8 * bounce + 5);
         k_wf_finalize<<<gridAll, 256, 0, s>>>(b.params, b.slots, bounce, b.paths, b.nee, shadeQueue, cnt);
+        mark(8 * bounce + 6);
     }
     if (cached) {
+        mark(8 * kWfMaxBounce + 0);
         hipError_t e = nrc_infer(*nrc, b.qRaw, qCount, (int)b.cap, nullptr, b.qMeta, b.sampleBase, s, b.qQueue);
         if (e != hipSuccess) return e;
+        mark(8 * kWfMaxBounce + 1);
         const int nb = (b.trainCap * maxBounce + kCollectPerBlock - 1) / kCollectPerBlock;
         if (nb > 0) {
             k_wf_nrc_count<<<nb, kCollectBlock, 0, s>>>(b.params, b.tRec, b.tCount);
@@ -616,6 +625,7 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
         }
         for (int k = 0; k < nrc->steps; k++)  // chunks cycle when steps > chunks
             if ((e = nrc_train_step(*nrc, s, k)) != hipSuccess) return e;
+        mark(8 * kWfMaxBounce + 2);
     }
     return hipGetLastError();
 }

@@ -128,3 +128,18 @@ def test_batch_equals_single_renders(gpu_ctx, cornell, displaced):
         assert _identical(a, ba) and _identical(b, bb)
         with pytest.raises(LtsError):
             render_batch([gpu_ctx, gpu_ctx], 4, 6, 3, SEED)
+
+
+@pytest.mark.gpu
+def test_phase_profiling_keeps_the_image_and_accounts_for_the_pass(gpu_ctx, displaced):
+    d, cfg, _, _ = displaced
+    print("[wavefront phase profiling]", flush=True)
+    gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], 96, 54)
+    ref, _, _ = gpu_ctx.render(8, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT, spp_per_pass=4)
+    img, _, st = gpu_ctx.render(8, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT, spp_per_pass=4, profile_phases=1)
+    assert _identical(ref, img)
+    ph = st["phase_ms"]
+    assert ph["extend"] > 0 and ph["shade"] > 0 and ph["shadow"] > 0 and ph["mis"] > 0
+    assert ph["cache_infer"] == 0 and st["phase_launches"]["extend"] == 2 * 8
+    total = sum(ph.values())
+    assert 0.5 * st["render_ms"] < total <= 1.01 * st["render_ms"]
