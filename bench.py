@@ -239,6 +239,9 @@ def main() -> int:
     ap.add_argument("--integrator", default="wavefront", choices=("megakernel", "wavefront"))
     ap.add_argument("--autotune", type=int, default=1, help="wavefront: tune block/grid/sort during warm-up")
     ap.add_argument("--no-quality", action="store_true", help="skip the cached-vs-plain equal-time comparison")
+    ap.add_argument("--cached-frame", action="store_true",
+                    help="render one frame of the workload with the cached integrator (quality-block settings) "
+                         "and print its stats; for kernel profiles of the cache")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the plain 1 GiB device copy (traffic-counter calibration)")
     args = ap.parse_args()
@@ -256,7 +259,22 @@ def main() -> int:
     if WORKLOADS[args.workload]["gen"] == "soup":
         print(json.dumps(run_lbvh(args)), flush=True)
         return 0
-    from lighttransportsimulation_amd import INTEGRATOR_MEGAKERNEL, INTEGRATOR_WAVEFRONT, Context
+    from lighttransportsimulation_amd import INTEGRATOR_CACHED, INTEGRATOR_MEGAKERNEL, INTEGRATOR_WAVEFRONT, Context
+    if args.cached_frame:
+        with tempfile.TemporaryDirectory() as d:
+            cfg = make_workload(args.workload, d)
+            with Context(0) as ctx:
+                ctx.load_scene(d, cfg["scene"], cfg["subject"], cfg["width"], cfg["height"])
+                ctx.nrc_configure(query_bounce=2, **CACHE_SETTINGS)
+                t0 = time.perf_counter()
+                _, _, st = ctx.render(cfg["spp"], cfg["max_bounce"], cfg["rr_start"], cfg["seed"], cfg["stratified"],
+                                      integrator=INTEGRATOR_CACHED, want_image=False)
+                ms = (time.perf_counter() - t0) * 1e3
+        print(json.dumps({"cached_frame_ms": round(ms, 2), "render_ms": round(st["render_ms"], 2),
+                          "passes": st["launches"], "queries_last_pass": st["nrc_queries"],
+                          "train_samples_last_pass": st["nrc_train_samples"], "adam_steps": st["nrc_steps"],
+                          "settings": dict(CACHE_SETTINGS, query_bounce=2)}), flush=True)
+        return 0
     wavefront = args.integrator == "wavefront"
     with tempfile.TemporaryDirectory() as d:
         cfg = make_workload(args.workload, d)
@@ -285,6 +303,8 @@ def main() -> int:
                 for k, v in st["phase_ms"].items():
                     phase[k] = phase.get(k, 0.0) + v
             wall = time.perf_counter() - t0
+            if wavefront and tuning is None:
+                tuning = {k: st[k] for k in ("trace_block", "persistent_blocks", "sort_materials", "use_graph")}
             graph_rate = None
             if wavefront:
                 # the same frames replayed as captured graphs, without the phase events

@@ -93,11 +93,24 @@ LTS_D bool box_hit(const float* mn, const float* mx, const Ray& r, V3 inv, int n
 }
 
 constexpr int kStack = 64;
+constexpr int kShortStack = 24;  // stack entries kept in LDS; deeper entries go to private memory
+
+// Traversal stack of one lane: entries [0, kShortStack) in LDS (entry e at lds[e * stride], so the
+// lanes of a wave touch consecutive words), deeper entries in a private array that only deep
+// traversals reach. A private 64-entry array alone is scratch memory, and its traffic dominated
+// the traversal kernels.
+struct TraceStack {
+    int* lds;
+    int stride;
+};
+
+// LDS stack of thread `tid` in a block-wide array __shared__ int[kShortStack * blockSize].
+LTS_D TraceStack lds_stack(int* shared, int tid, int blockSize) { return TraceStack{shared + tid, blockSize}; }
 
 // Closest hit (BVH::intersect, acceleration/BVH.cuh:365-398) over the same Karras tree, visiting
 // the nearer child first. The reported hit is order independent (the box test is conservative and
 // prunes with the shrinking tMax), except for exactly equal distances.
-LTS_D Hit trace_closest(const SceneView& sv, Ray& r, unsigned long long* nodeCount, unsigned long long* triCount) {
+LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned long long* nodeCount, unsigned long long* triCount) {
     Hit h; h.leaf = -1; h.t = h.u = h.v = h.invDet = 0;
     const int n = sv.nPrim;
     V3 inv = v3((float)1 / r.d.x, (float)1 / r.d.y, (float)1 / r.d.z);
@@ -110,7 +123,7 @@ LTS_D Hit trace_closest(const SceneView& sv, Ray& r, unsigned long long* nodeCou
         tests++;
         if (tri_hit(p0, p1, p2, r, t, u, v, id)) { r.tMax = t; h.leaf = 0; h.t = t; h.u = u; h.v = v; h.invDet = id; }
     } else {
-        int stack[kStack];
+        int spill[kStack - kShortStack];
         int sp = 0;
         // child boxes are tested directly: a ray that passes a child box passes its parent's box,
         // so the root test of the reference never removes a candidate
@@ -132,7 +145,9 @@ LTS_D Hit trace_closest(const SceneView& sv, Ray& r, unsigned long long* nodeCou
                 bool hr = box_hit(rmn, rmx, r, inv, nx, ny, nz, tr);
                 if (hl && hr && sp < kStack) {
                     int nearC = tl <= tr ? lc : rc, farC = tl <= tr ? rc : lc;
-                    stack[sp++] = farC;
+                    if (sp < kShortStack) ts.lds[sp * ts.stride] = farC;
+                    else spill[sp - kShortStack] = farC;
+                    sp++;
                     cur = nearC;
                     continue;
                 } else if (hl) { cur = lc; continue; }
@@ -146,7 +161,8 @@ LTS_D Hit trace_closest(const SceneView& sv, Ray& r, unsigned long long* nodeCou
                 if (tri_hit(p0, p1, p2, r, t, u, v, id)) { r.tMax = t; h.leaf = k; h.t = t; h.u = u; h.v = v; h.invDet = id; }
             }
             if (sp == 0) break;
-            cur = stack[--sp];
+            --sp;
+            cur = sp < kShortStack ? ts.lds[sp * ts.stride] : spill[sp - kShortStack];
         }
     }
     if (nodeCount) { atomicAdd(nodeCount, nodes); atomicAdd(triCount, tests); }
@@ -154,7 +170,7 @@ LTS_D Hit trace_closest(const SceneView& sv, Ray& r, unsigned long long* nodeCou
 }
 
 // Any hit (BVH::simpleIntersect, acceleration/BVH.cuh:331-363).
-LTS_D bool trace_any(const SceneView& sv, const Ray& r) {
+LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts) {
     const int n = sv.nPrim;
     V3 inv = v3((float)1 / r.d.x, (float)1 / r.d.y, (float)1 / r.d.z);
     int nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
@@ -163,7 +179,7 @@ LTS_D bool trace_any(const SceneView& sv, const Ray& r) {
         load_tri(sv.tris, 0, p0, p1, p2, prim);
         return tri_occludes(p0, p1, p2, r);
     }
-    int stack[kStack];
+    int spill[kStack - kShortStack];
     int sp = 0;
     int cur = 0;
     const unsigned total = 2u * (unsigned)n - 1u;
@@ -178,7 +194,13 @@ LTS_D bool trace_any(const SceneView& sv, const Ray& r) {
             float tl, tr;
             bool hl = box_hit(lmn, lmx, r, inv, nx, ny, nz, tl);
             bool hr = box_hit(rmn, rmx, r, inv, nx, ny, nz, tr);
-            if (hl && hr && sp < kStack) { stack[sp++] = rc; cur = lc; continue; }
+            if (hl && hr && sp < kStack) {
+                if (sp < kShortStack) ts.lds[sp * ts.stride] = rc;
+                else spill[sp - kShortStack] = rc;
+                sp++;
+                cur = lc;
+                continue;
+            }
             else if (hl) { cur = lc; continue; }
             else if (hr) { cur = rc; continue; }
         } else {
@@ -187,7 +209,8 @@ LTS_D bool trace_any(const SceneView& sv, const Ray& r) {
             if (tri_occludes(p0, p1, p2, r)) return true;
         }
         if (sp == 0) return false;
-        cur = stack[--sp];
+        --sp;
+        cur = sp < kShortStack ? ts.lds[sp * ts.stride] : spill[sp - kShortStack];
     }
     return false;
 }
@@ -704,11 +727,11 @@ LTS_D Spec sample_li(const SceneView& sv, const LightDesc& L, const SurfacePoint
     return light_le(L);
 }
 
-LTS_D bool unoccluded(const SceneView& sv, const SurfacePoint& it, const LightPoint& p1) {
+LTS_D bool unoccluded(const SceneView& sv, const SurfacePoint& it, const LightPoint& p1, TraceStack ts) {
     V3 origin = offsetRayOrigin(it.p, it.pError, it.n, p1.p - it.p);
     V3 target = offsetRayOrigin(p1.p, p1.pError, p1.n, origin - p1.p);
     Ray r; r.o = origin; r.d = target - origin; r.tMax = 1 - SHADOW_EPS;
-    return !trace_any(sv, r);
+    return !trace_any(sv, r, ts);
 }
 
 // Distribution1D::sampleDiscrete with the reference's search length (Sampling.cuh:155-188)
@@ -729,7 +752,8 @@ LTS_D int sample_light(const SceneView& sv, float u, float* pdf) {
 }
 
 // estimateDirect (intergrators/IntegratorHelper.cuh:51-125)
-LTS_D Spec estimate_direct(const SceneView& sv, const SurfacePoint& si, const Bsdf& bsdf, V2 uScatter, int li, V2 uLight, bool specular) {
+LTS_D Spec estimate_direct(const SceneView& sv, const SurfacePoint& si, const Bsdf& bsdf, V2 uScatter, int li, V2 uLight, bool specular,
+                           TraceStack ts) {
     const LightDesc& L = sv.lights[li];
     Spec Ld = spec(0.0f);
     int flags = specular ? B_ALL : (B_ALL & ~B_SPEC);
@@ -741,7 +765,7 @@ LTS_D Spec estimate_direct(const SceneView& sv, const SurfacePoint& si, const Bs
         Spec f = bsdf.f(si.wo, wi, flags) * absDot(wi, si.sn);
         scatteringPdf = bsdf.pdf(si.wo, wi, flags);
         if (!isBlack(f)) {
-            if (!unoccluded(sv, si, vis)) Li = spec(0.0f);
+            if (!unoccluded(sv, si, vis, ts)) Li = spec(0.0f);
             if (!isBlack(Li)) {
                 float weight = power_heuristic(lightPdf, scatteringPdf);
                 Ld = Ld + f * Li * weight / lightPdf;
@@ -760,7 +784,7 @@ LTS_D Spec estimate_direct(const SceneView& sv, const SurfacePoint& si, const Bs
             weight = power_heuristic(scatteringPdf, lightPdf);
         }
         Ray r = spawn_ray(si, wi);
-        Hit h = trace_closest(sv, r, nullptr, nullptr);
+        Hit h = trace_closest(sv, r, ts, nullptr, nullptr);
         Spec Li2 = spec(0.f);
         if (h.leaf >= 0) {
             SurfacePoint lt = make_surface(sv, h.leaf, r, h.u, h.v, h.invDet);

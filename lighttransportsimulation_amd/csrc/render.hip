@@ -14,7 +14,8 @@ namespace {
 constexpr int kTileSide = 16;
 
 // One camera sample of PathTracingIntegrator::evaluate.
-LTS_D Spec radiance(const SceneView& sv, int px, int py, int sample, const RenderLaunch& rl, unsigned long long* cnt) {
+LTS_D Spec radiance(const SceneView& sv, int px, int py, int sample, const RenderLaunch& rl, unsigned long long* cnt,
+                     TraceStack ts) {
     Sampler smp;
     smp.seed = rl.seed;
     smp.pixel = (uint32_t)(py * sv.cam.width + px);
@@ -24,7 +25,7 @@ LTS_D Spec radiance(const SceneView& sv, int px, int py, int sample, const Rende
     Spec L = spec(0.0f), beta = spec(1.0f);
     bool specularBounce = false;
     for (int bounce = 0; bounce < rl.maxBounce; bounce++) {
-        Hit h = trace_closest(sv, r, cnt, cnt ? cnt + 1 : nullptr);
+        Hit h = trace_closest(sv, r, ts, cnt, cnt ? cnt + 1 : nullptr);
         if (bounce == 0 || specularBounce) {
             if (h.leaf >= 0) {
                 int prim = sv.tris[h.leaf].prim;
@@ -51,7 +52,7 @@ LTS_D Spec radiance(const SceneView& sv, int px, int py, int sample, const Rende
         if (sv.nLights > 0) {
             float lpdf;
             int ln = sample_light(sv, u0[0], &lpdf);
-            Spec ld = estimate_direct(sv, si, bsdf, v2(u0[3], u1[0]), ln, v2(u0[1], u0[2]), specularBounce);
+            Spec ld = estimate_direct(sv, si, bsdf, v2(u0[3], u1[0]), ln, v2(u0[1], u0[2]), specularBounce, ts);
             L = L + beta * (ld / lpdf);
         }
         V3 wo = -r.d, wi;
@@ -72,13 +73,14 @@ LTS_D Spec radiance(const SceneView& sv, int px, int py, int sample, const Rende
 }
 
 __global__ __launch_bounds__(256) void k_megakernel(SceneView sv, RenderLaunch rl) {
+    __shared__ int stk[kShortStack * 256];
     const int W = sv.cam.width, H = sv.cam.height;
     const int tilesX = (W + kTileSide - 1) / kTileSide;
     const int px = (blockIdx.x % tilesX) * kTileSide + (threadIdx.x & (kTileSide - 1));
     const int py = (blockIdx.x / tilesX) * kTileSide + (threadIdx.x / kTileSide);
     const int sample = rl.s0 + blockIdx.y;
     if (px >= W || py >= H || sample >= rl.s1) return;
-    Spec L = radiance(sv, px, py, sample, rl, rl.counters);
+    Spec L = radiance(sv, px, py, sample, rl, rl.counters, lds_stack(stk, threadIdx.x, 256));
     float* o = rl.sampleL + 3 * ((size_t)blockIdx.y * W * H + (size_t)py * W + px);
     o[0] = L.r; o[1] = L.g; o[2] = L.b;
 }

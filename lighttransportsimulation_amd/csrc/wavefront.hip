@@ -83,6 +83,8 @@ __global__ __launch_bounds__(BS) void k_wf_extend(const SceneView* __restrict__ 
                                                   const WfRay* __restrict__ rays, WfHit* __restrict__ hits,
                                                   const int32_t* __restrict__ queue, const int32_t* __restrict__ count,
                                                   int32_t* __restrict__ work) {
+    __shared__ int stk[kShortStack * BS];
+    const TraceStack ts = lds_stack(stk, threadIdx.x, BS);
     const int n = *count;
     for (;;) {
         const int k = wave_fetch(work);
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(BS) void k_wf_extend(const SceneView* __restrict__ 
         r.o = v3(rr.o[0], rr.o[1], rr.o[2]);
         r.d = v3(rr.d[0], rr.d[1], rr.d[2]);
         r.tMax = INFINITY;
-        Hit h = trace_closest(scenes[paths[pi].scene], r, nullptr, nullptr);
+        Hit h = trace_closest(scenes[paths[pi].scene], r, ts, nullptr, nullptr);
         WfHit o;
         o.leaf = h.leaf; o.u = h.u; o.v = h.v; o.invDet = h.invDet;
         hits[pi] = o;
@@ -460,6 +462,8 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_wf_shadow(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
                                                   const WfShadow* __restrict__ shadows, const int32_t* __restrict__ count,
                                                   WfNee* __restrict__ nee, int32_t* __restrict__ work) {
+    __shared__ int stk[kShortStack * BS];
+    const TraceStack ts = lds_stack(stk, threadIdx.x, BS);
     const int n = *count;
     for (;;) {
         const int k = wave_fetch(work);
@@ -470,7 +474,7 @@ __global__ __launch_bounds__(BS) void k_wf_shadow(const SceneView* __restrict__ 
         r.o = v3(s.o[0], s.o[1], s.o[2]);
         r.d = v3(s.d[0], s.d[1], s.d[2]);
         r.tMax = 1 - SHADOW_EPS;
-        if (!trace_any(scenes[paths[s.path].scene], r)) {
+        if (!trace_any(scenes[paths[s.path].scene], r, ts)) {
             nee[s.path].p1[0] = s.c[0];
             nee[s.path].p1[1] = s.c[1];
             nee[s.path].p1[2] = s.c[2];
@@ -482,6 +486,8 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_wf_mis(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
                                                const WfMis* __restrict__ misRays, const int32_t* __restrict__ count,
                                                WfNee* __restrict__ nee, int32_t* __restrict__ work) {
+    __shared__ int stk[kShortStack * BS];
+    const TraceStack ts = lds_stack(stk, threadIdx.x, BS);
     const int n = *count;
     for (;;) {
         const int k = wave_fetch(work);
@@ -495,7 +501,7 @@ __global__ __launch_bounds__(BS) void k_wf_mis(const SceneView* __restrict__ sce
         r.d = v3(m.d[0], m.d[1], m.d[2]);
         r.tMax = INFINITY;
         const V3 wi = r.d;
-        Hit h = trace_closest(sv, r, nullptr, nullptr);
+        Hit h = trace_closest(sv, r, ts, nullptr, nullptr);
         Spec Li2 = spec(0.f);
         if (h.leaf >= 0) {
             SurfacePoint lt = make_surface(sv, h.leaf, r, h.u, h.v, h.invDet);
