@@ -41,6 +41,10 @@ WORKLOADS = {
                          "random orientations; reference leaf layout", gen="soup", n=1 << 20),
     "lbvh_8m": dict(desc="LBVH build only: 8,388,608 fp32 triangles, centroids U[0,1)^3, log-normal(-5, 0.7) edges, "
                          "random orientations; reference leaf layout", gen="soup", n=1 << 23),
+    "batch64": dict(desc="64 seeded scenes in one wavefront pass sequence: FBM-displaced spheres and tori, triangles "
+                         "log-uniform over 10k-2M, square resolutions from {128, 256, 512, 1024}, 1-4 area lights, "
+                         "matte/glass/microfacet with FBM/wrinkled/marble/image textures; 32 spp, 8 bounces, RR from 3",
+                    gen="batch", n_scenes=64, spp=32, bounces=8, rr=3),
     "cornell_256": dict(desc="seeded icosphere (20,480 tris) in a Cornell box, 256x256, 16 spp stratified, "
                              "5 bounces, RR from 3", gen="cornell", width=256, height=256, spp=16, bounces=5, rr=3,
                         stratified=1),
@@ -228,6 +232,64 @@ def quality(ctx, cfg, tuning, spps=(16, 32, 64, 128), ref_spp=4096) -> dict:
     return out
 
 
+def run_batch(args) -> dict:
+    """Aggregate throughput of many scenes rendered together (lts_render_batch), against the same scenes
+    rendered one after another."""
+    from lighttransportsimulation_amd import INTEGRATOR_WAVEFRONT, Context, render_batch, scenegen
+    w = WORKLOADS[args.workload]
+    n = args.batch_scenes or w["n_scenes"]
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.perf_counter()
+        cfgs = scenegen.gen_batch(d, n_scenes=n, spp=w["spp"], bounces=w["bounces"])
+        gen_s = time.perf_counter() - t0
+        ctxs = [Context(0) for _ in cfgs]
+        try:
+            t0 = time.perf_counter()
+            for c, cfg in zip(ctxs, cfgs):
+                c.load_scene(d, cfg["scene"], cfg["subject"], cfg["width"], cfg["height"])
+            load_s = time.perf_counter() - t0
+            tris = [c.scene_info()["tris"] for c in ctxs]
+            kw = dict(spp=w["spp"], max_bounce=w["bounces"], rr_start=w["rr"], seed=0x5EED)
+            for _ in range(args.warmup):
+                render_batch(ctxs, want_images=False, **kw)
+            t0 = time.perf_counter()
+            dev = 0.0
+            for _ in range(args.steps):
+                _, st = render_batch(ctxs, want_images=False, **kw)
+                dev += st["render_ms"] + st["resolve_ms"]
+            wall = time.perf_counter() - t0
+            groups = st["batch_groups"]
+            # every scene in one group: one pass sequence for all 64
+            render_batch(ctxs, want_images=False, group_pixels=-1, **kw)
+            t1 = time.perf_counter()
+            render_batch(ctxs, want_images=False, group_pixels=-1, **kw)
+            one_group_ms = (time.perf_counter() - t1) * 1e3
+            # the same scenes one at a time (each in its own context, warmed up once)
+            single_ms = []
+            for c in ctxs:
+                c.render(integrator=INTEGRATOR_WAVEFRONT, want_image=False, **kw)
+                t1 = time.perf_counter()
+                c.render(integrator=INTEGRATOR_WAVEFRONT, want_image=False, **kw)
+                single_ms.append((time.perf_counter() - t1) * 1e3)
+        finally:
+            for c in ctxs:
+                c.close()
+    samples = sum(cfg["width"] * cfg["height"] for cfg in cfgs) * w["spp"]
+    ms = wall * 1e3 / args.steps
+    return {"metric": "aggregate Msamples/s of a batched multi-scene render, 1 MI355X",
+            "value": round(samples / (ms * 1e-3) / 1e6, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": w["desc"], "scenes": n, "samples_per_step": samples,
+                       "triangles_total": int(sum(tris)), "device_ms_per_step": round(dev / args.steps, 3),
+                       "groups": groups, "one_group_ms": round(one_group_ms, 3),
+                       "one_at_a_time_ms": round(sum(single_ms), 3),
+                       "batch_speedup_over_one_at_a_time": round(sum(single_ms) / ms, 3),
+                       "per_scene": [{"triangles": t, "res": cfg["width"], "ms_alone": round(m, 3)}
+                                     for t, cfg, m in zip(tris, cfgs, single_ms)],
+                       "generate_s": round(gen_s, 1), "load_and_build_s": round(load_s, 1)}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,6 +301,7 @@ def main() -> int:
     ap.add_argument("--integrator", default="wavefront", choices=("megakernel", "wavefront"))
     ap.add_argument("--autotune", type=int, default=1, help="wavefront: tune block/grid/sort during warm-up")
     ap.add_argument("--no-quality", action="store_true", help="skip the cached-vs-plain equal-time comparison")
+    ap.add_argument("--batch-scenes", type=int, default=0, help="batch64: number of scenes (default 64)")
     ap.add_argument("--cached-frame", action="store_true",
                     help="render one frame of the workload with the cached integrator (quality-block settings) "
                          "and print its stats; for kernel profiles of the cache")
@@ -258,6 +321,9 @@ def main() -> int:
         return 2
     if WORKLOADS[args.workload]["gen"] == "soup":
         print(json.dumps(run_lbvh(args)), flush=True)
+        return 0
+    if WORKLOADS[args.workload]["gen"] == "batch":
+        print(json.dumps(run_batch(args)), flush=True)
         return 0
     from lighttransportsimulation_amd import INTEGRATOR_CACHED, INTEGRATOR_MEGAKERNEL, INTEGRATOR_WAVEFRONT, Context
     if args.cached_frame:

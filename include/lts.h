@@ -72,7 +72,10 @@ typedef struct lts_render_params {
                                 /*    context, and in the file named by $LTS_TUNE_CACHE if set  */
     int32_t profile_phases;     /* 1: time the wavefront phases with HIP events (stats.phase_ms); */
                                 /*    passes are then launched directly, not replayed as graphs  */
-    int32_t reserved[3];
+    int32_t batch_group_pixels; /* lts_render_batch: scenes are grouped, largest first, until a   */
+                                /*    group holds this many pixels; a group shares every launch   */
+                                /*    of its passes. 0: 2^20; -1: all scenes in one group         */
+    int32_t reserved[2];
 } lts_render_params;
 
 /* Wavefront phases timed when profile_phases is set (indices into lts_stats.phase_ms). */
@@ -105,6 +108,8 @@ typedef struct lts_stats {
     int32_t nrc_steps;      /* cached: Adam steps taken by the cache so far          */
     double phase_ms[8];     /* device time per LTS_PHASE_* summed over the passes     */
     int64_t phase_launches[8]; /* kernel launches per phase                          */
+    int32_t batch_groups;   /* lts_render_batch: groups the scenes were rendered in    */
+    int32_t pad;
 } lts_stats;
 
 /* Context. */
@@ -145,9 +150,10 @@ int lts_render(lts_ctx* ctx, const lts_render_params* params, float* rgb_out, fl
 
 /* Batched form of lts_render with the wavefront integrator (the reference renders one scene per
  * process, LightTransportSimulation.cu:23-146): the scenes loaded in ctxs[0..n-1] (contexts of one
- * device) are rendered together, every queue kernel of a pass working on the paths of all of
- * them, on ctxs[0]'s stream and buffers. Each scene's image equals what lts_render gives for it
- * alone; rgb_out[i] (entries may be NULL) receives scene i in getRender row order. */
+ * device) are rendered on ctxs[0]'s stream and buffers in groups (params.batch_group_pixels); the
+ * scenes of a group are rendered together, every queue kernel of a pass working on the paths of all
+ * of them. Each scene's image equals what lts_render gives for it alone; rgb_out[i] (entries may be
+ * NULL) receives scene i in getRender row order; stats cover the whole batch. */
 int lts_render_batch(lts_ctx* const* ctxs, int32_t n, const lts_render_params* params, float* const* rgb_out,
                      lts_stats* stats);
 

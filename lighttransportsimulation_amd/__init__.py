@@ -50,7 +50,7 @@ class RenderParams(C.Structure):
                 ("stratified", C.c_int32), ("integrator", C.c_int32), ("spp_per_pass", C.c_int32),
                 ("trace_block", C.c_int32), ("persistent_blocks", C.c_int32), ("sort_materials", C.c_int32),
                 ("use_graph", C.c_int32), ("autotune", C.c_int32), ("profile_phases", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("batch_group_pixels", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class NrcConfig(C.Structure):
@@ -66,7 +66,7 @@ class Stats(C.Structure):
                 ("persistent_blocks", C.c_int32), ("sort_materials", C.c_int32), ("use_graph", C.c_int32),
                 ("autotune_ms", C.c_double), ("nrc_loss", C.c_double), ("nrc_queries", C.c_int64),
                 ("nrc_train_samples", C.c_int32), ("nrc_steps", C.c_int32), ("phase_ms", C.c_double * 8),
-                ("phase_launches", C.c_int64 * 8)]
+                ("phase_launches", C.c_int64 * 8), ("batch_groups", C.c_int32), ("pad", C.c_int32)]
 
 
 _lib = None
@@ -254,9 +254,9 @@ class Context:
 
 def render_params(spp, max_bounce, rr_start, seed, stratified=0, integrator=INTEGRATOR_MEGAKERNEL, spp_per_pass=0,
                   trace_block=0, persistent_blocks=0, sort_materials=-1, use_graph=-1, autotune=0,
-                  profile_phases=0) -> RenderParams:
+                  profile_phases=0, batch_group_pixels=0) -> RenderParams:
     return RenderParams(spp, max_bounce, rr_start, seed, stratified, integrator, spp_per_pass, trace_block,
-                        persistent_blocks, sort_materials, use_graph, autotune, profile_phases)
+                        persistent_blocks, sort_materials, use_graph, autotune, profile_phases, batch_group_pixels)
 
 
 def _stats_dict(st: Stats) -> dict:
@@ -267,12 +267,12 @@ def _stats_dict(st: Stats) -> dict:
                 autotune_ms=st.autotune_ms, nrc_loss=st.nrc_loss, nrc_queries=st.nrc_queries,
                 nrc_train_samples=st.nrc_train_samples, nrc_steps=st.nrc_steps,
                 phase_ms={k: st.phase_ms[i] for i, k in enumerate(PHASES)},
-                phase_launches={k: st.phase_launches[i] for i, k in enumerate(PHASES)})
+                phase_launches={k: st.phase_launches[i] for i, k in enumerate(PHASES)}, batch_groups=st.batch_groups)
 
 
 def render_batch(contexts, spp: int, max_bounce: int, rr_start: int, seed: int, stratified: int = 0,
                  spp_per_pass: int = 0, trace_block: int = 0, persistent_blocks: int = 0, sort_materials: int = -1,
-                 use_graph: int = -1, autotune: int = 0, want_images: bool = True):
+                 use_graph: int = -1, autotune: int = 0, want_images: bool = True, group_pixels: int = 0):
     """Renders the scenes of several contexts together with the wavefront integrator
     (include/lts.h: lts_render_batch). Returns ([rgb per scene], stats)."""
     ctxs = list(contexts)
@@ -280,7 +280,7 @@ def render_batch(contexts, spp: int, max_bounce: int, rr_start: int, seed: int, 
         raise LtsError("render_batch needs at least one context")
     L = ctxs[0].lib
     p = render_params(spp, max_bounce, rr_start, seed, stratified, INTEGRATOR_WAVEFRONT, spp_per_pass, trace_block,
-                      persistent_blocks, sort_materials, use_graph, autotune)
+                      persistent_blocks, sort_materials, use_graph, autotune, 0, group_pixels)
     imgs = [np.zeros((c.height, c.width, 3), np.float32) if want_images else None for c in ctxs]
     handles = (C.c_void_p * len(ctxs))(*[c.h.value for c in ctxs])
     outs = (C.c_void_p * len(ctxs))(*[None if a is None else a.ctypes.data for a in imgs])

@@ -887,7 +887,47 @@ int lts_render_batch(lts_ctx* const* ctxs, int32_t n, const lts_render_params* p
     if (rc != LTS_OK) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     for (lts_ctx* x : scenes) HIPCHK(c, hipStreamSynchronize(x->stream));
-    return render_wavefront(c, scenes, p, rgb_out, nullptr, st);
+    // groups: scenes sorted by pixel count (largest first, stable), a group closes once it holds
+    // batch_group_pixels pixels; large scenes fill the device alone, small ones share launches
+    const size_t limit = p->batch_group_pixels < 0 ? SIZE_MAX
+                         : p->batch_group_pixels == 0 ? ((size_t)1 << 20) : (size_t)p->batch_group_pixels;
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    auto pixels = [&](int i) { return (size_t)scenes[i]->scene.view.cam.width * scenes[i]->scene.view.cam.height; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pixels(a) > pixels(b); });
+    lts_stats total{};
+    size_t k = 0;
+    while (k < order.size()) {
+        std::vector<lts_ctx*> group;
+        std::vector<float*> outs;
+        size_t px = 0;
+        while (k < order.size() && (group.empty() || px + pixels(order[k]) <= limit)) {
+            group.push_back(scenes[order[k]]);
+            outs.push_back(rgb_out ? rgb_out[order[k]] : nullptr);
+            px += pixels(order[k]);
+            k++;
+        }
+        lts_stats gs{};
+        if ((rc = render_wavefront(c, group, p, outs.data(), nullptr, &gs)) != LTS_OK) return rc;
+        total.render_ms += gs.render_ms;
+        total.resolve_ms += gs.resolve_ms;
+        total.launches += gs.launches;
+        total.samples += gs.samples;
+        total.autotune_ms += gs.autotune_ms;
+        for (int q = 0; q < 8; q++) {
+            total.phase_ms[q] += gs.phase_ms[q];
+            total.phase_launches[q] += gs.phase_launches[q];
+        }
+        total.trace_block = gs.trace_block;
+        total.persistent_blocks = gs.persistent_blocks;
+        total.sort_materials = gs.sort_materials;
+        total.use_graph = gs.use_graph;
+        total.batch_groups++;
+    }
+    total.kernel_avg_ms = total.launches ? total.render_ms / total.launches : 0;
+    total.msamples_per_s = (total.render_ms + total.resolve_ms) > 0 ? total.samples / ((total.render_ms + total.resolve_ms) * 1e3) : 0;
+    if (st) *st = total;
+    return LTS_OK;
 }
 
 int lts_nrc_configure(lts_ctx* c, const lts_nrc_config* cfg) {

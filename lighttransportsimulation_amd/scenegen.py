@@ -339,3 +339,103 @@ def gen_displaced_sphere(dir_: str, width=1920, height=1080, spp=64, bounces=8, 
     write_app_params(dir_, width, height, spp, bounces, rr, 1, "c3_out", "c3/displaced", "c3/displaced")
     return dict(scene="c3/displaced", subject="c3/displaced", width=width, height=height, spp=spp,
                 max_bounce=bounces, rr_start=rr, stratified=0, seed=0x5EED)
+
+
+def uv_torus(n_u: int, n_v: int, R: float = 1.0, r: float = 0.38):
+    """Torus of major radius R and minor radius r on an (n_u + 1) x (n_v + 1) grid: positions, outward
+    unit normals of the undisplaced surface, uvs and 2 n_u n_v triangles shaped (n_u, n_v, 2, 3)."""
+    u = np.linspace(0.0, 2 * np.pi, n_u + 1)
+    v = np.linspace(0.0, 2 * np.pi, n_v + 1)
+    U, V = np.meshgrid(u, v, indexing="ij")
+    nrm = np.stack([np.cos(V) * np.cos(U), np.sin(V), np.cos(V) * np.sin(U)], -1).reshape(-1, 3)
+    ctr = np.stack([R * np.cos(U), np.zeros_like(U), R * np.sin(U)], -1).reshape(-1, 3)
+    uv = np.stack([U / (2 * np.pi), V / (2 * np.pi)], -1).reshape(-1, 2)
+    idx = np.arange((n_u + 1) * (n_v + 1)).reshape(n_u + 1, n_v + 1)
+    a, b = idx[:-1, :-1], idx[:-1, 1:]
+    c, d = idx[1:, :-1], idx[1:, 1:]
+    tris = np.stack([np.stack([a, b, c], -1), np.stack([b, d, c], -1)], 2)
+    return ctr + r * nrm, nrm, uv, tris
+
+
+def gen_batch_scene(dir_: str, name: str, seed: int, n_tris: int, width: int, height: int, n_lights: int,
+                    shape: str = "sphere", spp: int = 32, bounces: int = 8, rr: int = 3, image_size: int = 128,
+                    submeshes: int = 4) -> dict:
+    """One scene of the batched workload: an FBM-displaced sphere or torus of about n_tris triangles
+    split into sectors with materials drawn from matte (FBM / wrinkled / marble / image texture) /
+    glass / microfacet mirror, a ground quad, n_lights quad area lights of random size, position and
+    radiance, and (for half of the seeds) an infinite light. Files go to <dir_>/<name>/."""
+    sub = os.path.join(dir_, name)
+    os.makedirs(sub, exist_ok=True)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = max(4, int(round(np.sqrt(n_tris / 4.0))))
+    if shape == "torus":
+        P0, nrm, uv, tris = uv_torus(2 * n, n)
+        P = P0 + nrm * (0.06 * fbm_displacement(P0 * 2.0, octaves=6, omega=0.5, seed=seed))[:, None]
+    else:
+        nrm, uv, tris = uv_sphere(n, 2 * n)
+        P = nrm * (1.0 + 0.1 * fbm_displacement(nrm * 1.5, octaves=6, omega=0.5, seed=seed))[:, None]
+    F_all = tris.reshape(-1, 3)
+    N = vertex_normals(P, F_all)
+    write_ppm(os.path.join(sub, "noise.ppm"), value_noise_image(image_size, seed + 1))
+    st = SceneText(camera=(0.0, 1.1, -4.2, 0.0, 0.05, 0.0, 0, 1, 0, 40, 4.2))
+    st.images = [(0, f"{name}/noise.ppm")]
+    cols_total = tris.shape[1]
+    per = max(1, cols_total // submeshes)
+    for k in range(submeshes):
+        cols = slice(k * per, cols_total if k == submeshes - 1 else (k + 1) * per)
+        F = tris[:, cols].reshape(-1, 3)
+        used = np.unique(F)
+        remap = np.full(P.shape[0], -1, np.int64)
+        remap[used] = np.arange(used.size)
+        Fi = remap[F]
+        write_bin(os.path.join(sub, f"part{k}"), _nonneg_zero(P[used]), Fi, N=_nonneg_zero(N[used]), idx_n=Fi,
+                  UV=_nonneg_zero(uv[used]), idx_uv=Fi)
+        st.subj_meshes.append(f"{name}/part{k}")
+        kind = ["matte", "glass", "microfacet"][int(rng.integers(0, 3))] if k > 0 else "matte"
+        if kind == "glass":
+            st.subj_mats.append("G c 1 1 1 c 1 1 1 1.5")
+        elif kind == "microfacet":
+            col = rng.uniform(0.5, 0.95, 3)
+            st.subj_mats.append(f"F c {_f(col[0])} {_f(col[1])} {_f(col[2])} {_f(rng.uniform(0.05, 0.5))}")
+        else:
+            tex = ["f", "w", "m", "i"][int(rng.integers(0, 4))]
+            octv, om = int(rng.integers(4, 9)), rng.uniform(0.3, 0.7)
+            if tex == "i":
+                spec_ = "i 0"
+            elif tex == "m":
+                spec_ = f"m {octv} {_f(om)} {_f(rng.uniform(1.5, 4.0))} {_f(rng.uniform(0.5, 2.0))}"
+            else:
+                spec_ = f"{tex} {octv} {_f(om)}"
+            st.subj_mats.append(f"L {spec_} {_f(0.0 if rng.random() < 0.5 else rng.uniform(5.0, 30.0))}")
+    Pq, Fq, Nq = quad((0, -1.15, 0), (6, 0, 0), (0, 0, -6))
+    write_bin(os.path.join(sub, "ground"), _nonneg_zero(Pq), Fq, N=_nonneg_zero(Nq), idx_n=Fq,
+              UV=np.array([[0, 0], [4, 0], [4, 4], [0, 4]], np.float32), idx_uv=Fq)
+    st.meshes.append(f"{name}/ground")
+    st.mats.append("L w 6 0.5 0")
+    for j in range(n_lights):
+        c = (rng.uniform(-2.5, 2.5), rng.uniform(2.0, 3.5), rng.uniform(-2.5, 2.5))
+        s = rng.uniform(0.5, 1.2)
+        Pl, Fl, Nl = quad(c, (s, 0, 0), (0, 0, s))  # emits downwards
+        write_bin(os.path.join(sub, f"light{j}"), _nonneg_zero(Pl), Fl, N=_nonneg_zero(Nl), idx_n=Fl)
+        st.meshes.append(f"{name}/light{j}")
+        st.mats.append("L c 0 0 0 0")
+        st.area_les.append(tuple(float(x) for x in rng.uniform(8.0, 40.0) * rng.uniform(0.6, 1.0, 3)))
+    st.infinite = (0.3, 0.3, 0.3) if seed % 2 == 0 else None
+    st.write(dir_, f"{name}/scene")
+    return dict(scene=f"{name}/scene", subject=f"{name}/scene", width=width, height=height, spp=spp,
+                max_bounce=bounces, rr_start=rr, stratified=0, seed=0x5EED, triangles=2 * n * 2 * n + 2 + 2 * n_lights)
+
+
+def gen_batch(dir_: str, n_scenes: int = 64, seed: int = 2024, min_tris: float = 1e4, max_tris: float = 2e6,
+              resolutions=(128, 256, 512, 1024), spp: int = 32, bounces: int = 8) -> list:
+    """The batched workload: n_scenes seeded scenes, triangle counts log-uniform in [min_tris, max_tris],
+    square resolutions drawn from `resolutions`, 1-4 area lights, spheres and tori alternating."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n_scenes):
+        n_tris = int(np.exp(rng.uniform(np.log(min_tris), np.log(max_tris))))
+        res = int(resolutions[int(rng.integers(0, len(resolutions)))])
+        n_lights = int(rng.integers(1, 5))
+        out.append(gen_batch_scene(dir_, f"b{i:02d}", seed + 17 * i, n_tris, res, res, n_lights,
+                                   "sphere" if i % 2 == 0 else "torus", spp=spp, bounces=bounces))
+    return out

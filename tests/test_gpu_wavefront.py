@@ -143,3 +143,26 @@ def test_phase_profiling_keeps_the_image_and_accounts_for_the_pass(gpu_ctx, disp
     assert ph["cache_infer"] == 0 and st["phase_launches"]["extend"] == 2 * 8
     total = sum(ph.values())
     assert 0.5 * st["render_ms"] < total <= 1.01 * st["render_ms"]
+
+
+@pytest.mark.gpu
+def test_uneven_batch_of_spheres_and_tori(gpu_ctx, tmp_path):
+    """Four scenes of the batched workload's generator (sizes, shapes and light counts differ): the
+    batch renders each one exactly as it renders alone."""
+    print("[wavefront batch of 4 uneven scenes]", flush=True)
+    cfgs = scenegen.gen_batch(str(tmp_path), n_scenes=4, max_tris=4e4, resolutions=(32, 48, 64), spp=4)
+    ctxs = [Context(0) for _ in cfgs]
+    try:
+        for c, cfg in zip(ctxs, cfgs):
+            c.load_scene(str(tmp_path), cfg["scene"], cfg["subject"], cfg["width"], cfg["height"])
+        alone = [c.render(4, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT)[0] for c in ctxs]
+        # one group, the automatic grouping, and every scene in a group of its own
+        for gp, groups in ((-1, 1), (0, 1), (1, len(cfgs))):
+            imgs, st = render_batch(ctxs, 4, 8, 3, SEED, group_pixels=gp)
+            assert st["batch_groups"] == groups
+            assert st["samples"] == 4 * sum(cfg["width"] * cfg["height"] for cfg in cfgs)
+            for a, b in zip(alone, imgs):
+                assert np.isfinite(b).all() and _identical(a, b)
+    finally:
+        for c in ctxs:
+            c.close()
