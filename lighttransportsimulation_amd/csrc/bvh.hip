@@ -516,6 +516,85 @@ __global__ __launch_bounds__(kBlock) void k_pack_tris(const float* __restrict__ 
 
 inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
+// ---------------------------------------------------------------- 4-wide render BVH
+constexpr int kScanBlock = 1024;
+
+// flag[i] = 1 when interior node i has even depth (the root has depth 0).
+__global__ __launch_bounds__(kBlock) void k_even_depth(const int32_t* __restrict__ parent, int n, int32_t* __restrict__ flag) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    int depth = 0;
+    for (int node = i; node > 0 && depth < 2 * n; node = parent[node]) depth++;
+    flag[i] = (depth & 1) == 0;
+}
+
+// In-place exclusive prefix sum of v[0, m) within blocks of kScanBlock; totals[block] = block sum.
+__global__ __launch_bounds__(kScanBlock) void k_scan_local(int32_t* __restrict__ v, int m, int32_t* __restrict__ totals) {
+    __shared__ int32_t sh[kScanBlock / 64];
+    const int i = blockIdx.x * kScanBlock + threadIdx.x;
+    const int x0 = i < m ? v[i] : 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = x0;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wave] = x;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < wave; w++) base += sh[w];
+    if (i < m) v[i] = base + x - x0;
+    if (threadIdx.x == kScanBlock - 1) totals[blockIdx.x] = base + x;
+}
+
+// Exclusive prefix of the block totals (one block, sequential over at most a few thousand entries).
+__global__ void k_scan_totals(int32_t* __restrict__ totals, int nb) {
+    if (threadIdx.x != 0) return;
+    int run = 0;
+    for (int b = 0; b < nb; b++) { const int t = totals[b]; totals[b] = run; run += t; }
+    totals[nb] = run;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_add(int32_t* __restrict__ v, int m, const int32_t* __restrict__ totals) {
+    const int i = blockIdx.x * kScanBlock + threadIdx.x;
+    if (i < m) v[i] += totals[blockIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void k_build_bvh4(const BvhNode* __restrict__ nodes, const int32_t* __restrict__ flag,
+                                                       const int32_t* __restrict__ index4, int n, Bvh4Node* __restrict__ out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1 || !flag[i]) return;
+    const BvhNode nd = nodes[i];
+    Bvh4Node o;
+    int cnt = 0;
+    auto add = [&](int c, const float* mn, const float* mx) {
+        o.child[cnt] = c >= n - 1 ? ~(c - (n - 1)) : index4[c];
+        o.minx[cnt] = mn[0]; o.miny[cnt] = mn[1]; o.minz[cnt] = mn[2];
+        o.maxx[cnt] = mx[0]; o.maxy[cnt] = mx[1]; o.maxz[cnt] = mx[2];
+        cnt++;
+    };
+    for (int side = 0; side < 2; side++) {
+        const int c = side ? nd.right : nd.left;
+        const float* mn = side ? nd.rmin : nd.lmin;
+        const float* mx = side ? nd.rmax : nd.lmax;
+        if (c >= n - 1) {
+            add(c, mn, mx);
+        } else {
+            const BvhNode g = nodes[c];  // odd depth: replaced by its two children
+            add(g.left, g.lmin, g.lmax);
+            add(g.right, g.rmin, g.rmax);
+        }
+    }
+    for (int q = cnt; q < 4; q++) {
+        o.child[q] = 0;
+        o.minx[q] = o.miny[q] = o.minz[q] = 0.0f;
+        o.maxx[q] = o.maxy[q] = o.maxz[q] = 0.0f;
+    }
+    o.count = cnt;
+    o.pad[0] = o.pad[1] = o.pad[2] = 0;
+    out[index4[i]] = o;
+}
+
 }  // namespace
 
 template <typename T>
@@ -551,9 +630,37 @@ hipError_t bvh_reserve(BvhBuffers& b, int n) {
     return hipSuccess;
 }
 
+hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t s) {
+    const int n = b.n;
+    if (n < 2) return hipSuccess;
+    const int m = n - 1;
+    const int nb = (m + kScanBlock - 1) / kScanBlock;
+    if (b.cap4 < n) {
+        void* ps[] = {b.nodes4, b.index4, b.scanTotals};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+        b.nodes4 = nullptr; b.index4 = nullptr; b.scanTotals = nullptr; b.cap4 = 0;
+        hipError_t e = dalloc(&b.nodes4, m);
+        if (e == hipSuccess) e = dalloc(&b.index4, m);
+        if (e == hipSuccess) e = dalloc(&b.scanTotals, (size_t)nb + 1);
+        if (e != hipSuccess) return e;
+        b.cap4 = n;
+    }
+    // flags (reusing b.flags) -> exclusive prefix in index4 -> 4-wide nodes
+    k_even_depth<<<blocks_for(m), kBlock, 0, s>>>(b.parent, n, b.flags);
+    hipError_t e = hipMemcpyAsync(b.index4, b.flags, (size_t)m * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    k_scan_local<<<nb, kScanBlock, 0, s>>>(b.index4, m, b.scanTotals);
+    k_scan_totals<<<1, 64, 0, s>>>(b.scanTotals, nb);
+    k_scan_add<<<nb, kScanBlock, 0, s>>>(b.index4, m, b.scanTotals);
+    k_build_bvh4<<<blocks_for(m), kBlock, 0, s>>>(b.nodes, b.flags, b.index4, n, b.nodes4);
+    return hipGetLastError();
+}
+
 void bvh_release(BvhBuffers& b) {
     void* ps[] = {b.primBox, b.box, b.keysA, b.keysB, b.valsA, b.valsB, b.counts, b.digitBase, b.left, b.right,
-                  b.parent, b.flags, b.sceneKeys, b.nodes, b.tris, b.rangeLo, b.rangeHi, b.roots, b.rootCount, b.bucket, b.partial};
+                  b.parent, b.flags, b.sceneKeys, b.nodes, b.tris, b.rangeLo, b.rangeHi, b.roots, b.rootCount, b.bucket, b.partial,
+                  b.nodes4, b.index4, b.scanTotals};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     b = BvhBuffers();

@@ -738,6 +738,8 @@ int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* s
     HIPCHK(c, bvh_reserve(S.bvh, n));
     int rc = run_build(c, S.bvh, S.soup, n, 1, true, nullptr);
     if (rc != LTS_OK) return rc;
+    HIPCHK(c, bvh4_enqueue(S.bvh, s));
+    HIPCHK(c, hipStreamSynchronize(s));
     // leaf index of every area light's triangle (inverse of the Morton permutation)
     std::vector<int32_t> perm(n);
     HIPCHK(c, hipMemcpy(perm.data(), S.bvh.valsA, n * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -748,7 +750,7 @@ int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* s
         if (L.kind == 0) L.leaf = leafOf[L.prim];
     HIPCHK(c, upload(&S.lights, lights.data(), lights.size(), s));
     SceneView& v = S.view;
-    v.tris = S.bvh.tris; v.nodes = S.bvh.nodes; v.shade = S.shade; v.mats = S.mats; v.texs = S.texs;
+    v.tris = S.bvh.tris; v.nodes = S.bvh.nodes; v.nodes4 = S.bvh.nodes4; v.shade = S.shade; v.mats = S.mats; v.texs = S.texs;
     v.mips = S.mips; v.mipData = S.mipData; v.lights = S.lights; v.lfunc = S.lfunc; v.lcdf = S.lcdf;
     v.funcInt = H.funcInt; v.nLights = (int)H.lights.size(); v.nPrim = n; v.infiniteLight = H.infiniteLight;
     v.perm = c->perm;

@@ -43,11 +43,19 @@ struct BvhBuffers {
     uint32_t* sceneKeys = nullptr;// 6 ordered keys of the scene bounds
     BvhNode* nodes = nullptr;     // n-1 (or 1 for n == 1)
     TriLeaf* tris = nullptr;      // n
+    Bvh4Node* nodes4 = nullptr;   // 4-wide collapse for rendering (allocated by bvh4_enqueue)
+    int32_t* index4 = nullptr;    // n-1: 4-wide node of each even-depth interior node
+    int32_t* scanTotals = nullptr;
+    int cap4 = 0;
     // results after the sort: sorted codes and permutation live in keysA / valsA
 };
 
 hipError_t bvh_reserve(BvhBuffers& b, int n);
 void bvh_release(BvhBuffers& b);
+// Collapses the packed tree of the last build (mode 1, pack) into the 4-wide render BVH: interior
+// nodes of even depth become 4-wide nodes (numbered in interior order, so the root is node 0) and
+// take the boxes of their children's children. Deterministic.
+hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t stream);
 
 struct BvhStageEvents {
     hipEvent_t e[5];  // start, after bounds+morton, after sort, after tree, after pack

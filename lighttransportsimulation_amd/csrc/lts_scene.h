@@ -29,6 +29,18 @@ struct alignas(16) BvhNode {
     int32_t pad0, pad1;
 };
 
+// Node of the 4-wide render BVH: the interior nodes of even depth of the Karras tree, each holding
+// the boxes of its children's children (a leaf child is kept as is). Boxes are stored by
+// coordinate (minx of the 4 children, then miny, ...); child >= 0 is a 4-wide node, child < 0 the
+// leaf ~child; `count` children are valid. 128 bytes, one cache line.
+struct alignas(16) Bvh4Node {
+    float minx[4], miny[4], minz[4];
+    float maxx[4], maxy[4], maxz[4];
+    int32_t child[4];
+    int32_t count;
+    int32_t pad[3];
+};
+
 struct alignas(16) ShadeTri {
     float n[9];    // vertex normals (valid when hasN)
     float uv[6];   // vertex uvs (defaults (0,0),(1,0),(1,1) when the mesh has none)
@@ -93,6 +105,7 @@ struct CameraDesc {
 struct SceneView {
     const TriLeaf* tris;
     const BvhNode* nodes;
+    const Bvh4Node* nodes4;  // 4-wide collapse of `nodes`, root at 0 (used by the traversal)
     const ShadeTri* shade;
     const MatDesc* mats;
     const TexDesc* texs;

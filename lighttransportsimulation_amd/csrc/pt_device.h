@@ -92,7 +92,7 @@ LTS_D bool box_hit(const float* mn, const float* mx, const Ray& r, V3 inv, int n
     return (tMin < r.tMax) && (tMax > 0);
 }
 
-constexpr int kStack = 64;
+constexpr int kStack = 96;  // 4-wide traversal: up to 3 entries per level of a tree at most 64 deep
 constexpr int kShortStack = 24;  // stack entries kept in LDS; deeper entries go to private memory
 
 // Traversal stack of one lane: entries [0, kShortStack) in LDS (entry e at lds[e * stride], so the
@@ -107,9 +107,38 @@ struct TraceStack {
 // LDS stack of thread `tid` in a block-wide array __shared__ int[kShortStack * blockSize].
 LTS_D TraceStack lds_stack(int* shared, int tid, int blockSize) { return TraceStack{shared + tid, blockSize}; }
 
-// Closest hit (BVH::intersect, acceleration/BVH.cuh:365-398) over the same Karras tree, visiting
-// the nearer child first. The reported hit is order independent (the box test is conservative and
-// prunes with the shrinking tMax), except for exactly equal distances.
+// Child boxes of a 4-wide node: hit distance per slot (INFINITY for a miss or an empty slot).
+LTS_D void node4_hits(const Bvh4Node* nodes4, int cur, const Ray& r, V3 inv, int nx, int ny, int nz, float (&t)[4],
+                      int (&c)[4], int& count) {
+    const float4* q = reinterpret_cast<const float4*>(nodes4 + cur);
+    const float4 mnx = q[0], mny = q[1], mnz = q[2], mxx = q[3], mxy = q[4], mxz = q[5];
+    const int4 ch = *reinterpret_cast<const int4*>(q + 6);
+    count = reinterpret_cast<const int*>(q + 7)[0];
+    const float lx[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, ly[4] = {mny.x, mny.y, mny.z, mny.w}, lz[4] = {mnz.x, mnz.y, mnz.z, mnz.w};
+    const float hx[4] = {mxx.x, mxx.y, mxx.z, mxx.w}, hy[4] = {mxy.x, mxy.y, mxy.z, mxy.w}, hz[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
+    const int cc[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float mn[3] = {lx[k], ly[k], lz[k]}, mx[3] = {hx[k], hy[k], hz[k]};
+        float tk;
+        const bool hit = k < count && box_hit(mn, mx, r, inv, nx, ny, nz, tk);
+        t[k] = hit ? tk : INFINITY;
+        c[k] = cc[k];
+    }
+}
+
+LTS_D void cswap(float& ta, int& ca, float& tb, int& cb) {
+    if (tb < ta) {
+        const float t = ta; ta = tb; tb = t;
+        const int c = ca; ca = cb; cb = c;
+    }
+}
+
+// Closest hit (BVH::intersect, acceleration/BVH.cuh:365-398) over the 4-wide collapse of the same
+// Karras tree, children visited nearest first. The 4-wide node tests the boxes of the binary
+// tree's grandchildren and skips the children's boxes, which never prunes a candidate the binary
+// traversal would keep (a ray that passes a grandchild box passes its parent's box); the reported
+// hit is therefore the binary traversal's, except for exactly equal distances.
 LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned long long* nodeCount, unsigned long long* triCount) {
     Hit h; h.leaf = -1; h.t = h.u = h.v = h.invDet = 0;
     const int n = sv.nPrim;
@@ -125,35 +154,33 @@ LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned lon
     } else {
         int spill[kStack - kShortStack];
         int sp = 0;
-        // child boxes are tested directly: a ray that passes a child box passes its parent's box,
-        // so the root test of the reference never removes a candidate
-        int cur = 0;
-        const unsigned total = 2u * (unsigned)n - 1u;
-        // every node is entered at most once in a tree; the step bound and the index checks keep a
+        int cur = 0;  // the 4-wide root; entries < 0 are leaves (~leaf)
+        // every node and leaf is entered at most once; the step bound and the index checks keep a
         // malformed node array from running away
-        for (unsigned steps = 0; steps < total + 64u; steps++) {
-            if ((unsigned)cur >= total) break;
-            if (cur < n - 1) {
-                const float4* q = reinterpret_cast<const float4*>(sv.nodes + cur);
-                float4 a = q[0], b = q[1], c = q[2], d = q[3];
-                float lmn[3] = {a.x, a.y, a.z}, lmx[3] = {a.w, b.x, b.y};
-                float rmn[3] = {b.z, b.w, c.x}, rmx[3] = {c.y, c.z, c.w};
-                int lc = __float_as_int(d.x), rc = __float_as_int(d.y);
-                float tl, tr;
-                nodes += 2;
-                bool hl = box_hit(lmn, lmx, r, inv, nx, ny, nz, tl);
-                bool hr = box_hit(rmn, rmx, r, inv, nx, ny, nz, tr);
-                if (hl && hr && sp < kStack) {
-                    int nearC = tl <= tr ? lc : rc, farC = tl <= tr ? rc : lc;
-                    if (sp < kShortStack) ts.lds[sp * ts.stride] = farC;
-                    else spill[sp - kShortStack] = farC;
-                    sp++;
-                    cur = nearC;
-                    continue;
-                } else if (hl) { cur = lc; continue; }
-                else if (hr) { cur = rc; continue; }
+        for (unsigned steps = 0; steps < 2u * (unsigned)n + 64u; steps++) {
+            if (cur >= 0) {
+                if (cur >= n - 1) break;
+                float t[4];
+                int c[4], count;
+                node4_hits(sv.nodes4, cur, r, inv, nx, ny, nz, t, c, count);
+                nodes += count;
+                cswap(t[0], c[0], t[1], c[1]);
+                cswap(t[2], c[2], t[3], c[3]);
+                cswap(t[0], c[0], t[2], c[2]);
+                cswap(t[1], c[1], t[3], c[3]);
+                cswap(t[1], c[1], t[2], c[2]);
+#pragma unroll
+                for (int k = 3; k >= 1; k--) {
+                    if (t[k] < INFINITY && sp < kStack) {
+                        if (sp < kShortStack) ts.lds[sp * ts.stride] = c[k];
+                        else spill[sp - kShortStack] = c[k];
+                        sp++;
+                    }
+                }
+                if (t[0] < INFINITY) { cur = c[0]; continue; }
             } else {
-                int k = cur - (n - 1);
+                const int k = ~cur;
+                if (k >= n) break;
                 V3 p0, p1, p2; int prim;
                 load_tri(sv.tris, k, p0, p1, p2, prim);
                 float t, u, v, id;
@@ -169,7 +196,8 @@ LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned lon
     return h;
 }
 
-// Any hit (BVH::simpleIntersect, acceleration/BVH.cuh:331-363).
+// Any hit (BVH::simpleIntersect, acceleration/BVH.cuh:331-363) over the 4-wide tree: leaf children
+// are tested as soon as their box is hit, interior children are stacked.
 LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts) {
     const int n = sv.nPrim;
     V3 inv = v3((float)1 / r.d.x, (float)1 / r.d.y, (float)1 / r.d.z);
@@ -182,31 +210,26 @@ LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts) {
     int spill[kStack - kShortStack];
     int sp = 0;
     int cur = 0;
-    const unsigned total = 2u * (unsigned)n - 1u;
-    for (unsigned steps = 0; steps < total + 64u; steps++) {
-        if ((unsigned)cur >= total) return false;
-        if (cur < n - 1) {
-            const float4* q = reinterpret_cast<const float4*>(sv.nodes + cur);
-            float4 a = q[0], b = q[1], c = q[2], d = q[3];
-            float lmn[3] = {a.x, a.y, a.z}, lmx[3] = {a.w, b.x, b.y};
-            float rmn[3] = {b.z, b.w, c.x}, rmx[3] = {c.y, c.z, c.w};
-            int lc = __float_as_int(d.x), rc = __float_as_int(d.y);
-            float tl, tr;
-            bool hl = box_hit(lmn, lmx, r, inv, nx, ny, nz, tl);
-            bool hr = box_hit(rmn, rmx, r, inv, nx, ny, nz, tr);
-            if (hl && hr && sp < kStack) {
-                if (sp < kShortStack) ts.lds[sp * ts.stride] = rc;
-                else spill[sp - kShortStack] = rc;
-                sp++;
-                cur = lc;
-                continue;
+    for (unsigned steps = 0; steps < 2u * (unsigned)n + 64u; steps++) {
+        if (cur < 0 || cur >= n - 1) return false;
+        float t[4];
+        int c[4], count;
+        node4_hits(sv.nodes4, cur, r, inv, nx, ny, nz, t, c, count);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (t[k] < INFINITY) {
+                if (c[k] < 0) {
+                    const int leaf = ~c[k];
+                    if (leaf >= n) return false;
+                    V3 p0, p1, p2; int prim;
+                    load_tri(sv.tris, leaf, p0, p1, p2, prim);
+                    if (tri_occludes(p0, p1, p2, r)) return true;
+                } else if (sp < kStack) {
+                    if (sp < kShortStack) ts.lds[sp * ts.stride] = c[k];
+                    else spill[sp - kShortStack] = c[k];
+                    sp++;
+                }
             }
-            else if (hl) { cur = lc; continue; }
-            else if (hr) { cur = rc; continue; }
-        } else {
-            V3 p0, p1, p2; int prim;
-            load_tri(sv.tris, cur - (n - 1), p0, p1, p2, prim);
-            if (tri_occludes(p0, p1, p2, r)) return true;
         }
         if (sp == 0) return false;
         --sp;
