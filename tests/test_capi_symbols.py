@@ -25,3 +25,34 @@ def test_library_exports_every_symbol():
         assert hasattr(lib, name), name
     lib.lts_version.restype = ctypes.c_char_p
     assert b"gfx950" in lib.lts_version()
+
+
+def _read_pfm(path):
+    import numpy as np
+    with open(path, "rb") as f:
+        assert f.readline().strip() == b"PF"
+        w, h = map(int, f.readline().split())
+        scale = float(f.readline())
+        data = np.frombuffer(f.read(), "<f4" if scale < 0 else ">f4")
+    return data.reshape(h, w, 3)
+
+
+def test_host_only_entry_points(tmp_path):
+    """app.params parsing and image writing run on the host: no GPU needed."""
+    import numpy as np
+
+    from lighttransportsimulation_amd import read_app_params, write_image
+    p = tmp_path / "app.params"
+    p.write_text("64 48\n16\n5\n3\n1\nout\nc1/cornell\nc1/cornell\n")
+    a = read_app_params(str(p))
+    assert (a["width"], a["height"], a["spp"], a["max_bounce"], a["rr_start"], a["format"]) == (64, 48, 16, 5, 3, 1)
+    assert (a["output"], a["scene"], a["subject"]) == ("out.pfm", "c1/cornell", "c1/cornell")
+    rgb = np.random.default_rng(0).random((5, 7, 3)).astype(np.float32)
+    write_image(str(tmp_path / "x.pfm"), rgb, 1)
+    assert np.array_equal(_read_pfm(str(tmp_path / "x.pfm")), rgb)
+    write_image(str(tmp_path / "x.hdr"), rgb, 0)
+    body = (tmp_path / "x.hdr").read_text().split()
+    assert body[:4] == ["P3", "7", "5", "255"]
+    # toRGB: int(pow(clamp(x), 1/2.2) * 255 + 0.5)
+    expect = (np.power(np.clip(rgb.reshape(-1), 0, 1).astype(np.float64), 1 / 2.2).astype(np.float32) * 255 + 0.5).astype(int)
+    assert np.array_equal(np.array(body[4:], int), expect)
