@@ -990,7 +990,7 @@ int lts_nrc_train_step(lts_ctx* c, const float* raw, const float* target, int32_
         HIPCHK(c, hipMemcpyAsync(c->nrc.trainTarget, target, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
     }
     HIPCHK(c, hipMemcpyAsync(&c->nrc.state->nTotal, &n, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    HIPCHK(c, nrc_train_step(c->nrc, s, 0));
+    HIPCHK(c, nrc_train(c->nrc, s, 1));
     NrcState st;
     HIPCHK(c, hipMemcpyAsync(&st, c->nrc.state, sizeof(st), hipMemcpyDeviceToHost, s));
     if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->nrc.grad, kNrcParams * sizeof(float), hipMemcpyDeviceToHost, s));

@@ -192,19 +192,23 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
 constexpr int kNrcRaw = 16;
 constexpr int kNrcParams = 5 * 4096 + 64 * 16;  // 21504
 constexpr int kNrcBwdElems = 6 * 4096;          // backward fragments (output layer padded)
-constexpr int kNrcKSplit = 1024;                // samples per split of the weight-gradient GEMM
+constexpr int kNrcKSplit = 256;                 // samples per split of the weight-gradient GEMM
+constexpr int kNrcBatchAlign = 1024;            // batch capacity multiple (4 splits per GEMM block)
+constexpr int kNrcMaxSteps = 64;                // Adam steps per training call
 constexpr int kNrcTrainBlocks = 256;
 
 struct NrcQuery { float beta[3]; float a[3]; int32_t slot; int32_t pad; };  // slot: sample index into sampleL
 struct NrcState {
     int32_t t;        // Adam steps taken
-    int32_t nTrain;   // samples in the current training batch
-    int32_t nQuery;   // cache queries of the current pass
-    int32_t nTotal;   // samples collected for this pass's steps (nTrain of step k: the k-th batch of them)
-    float c1, c2;     // Adam bias corrections 1/(1-beta1^t), 1/(1-beta2^t) of the current step
-    float loss;       // mean loss of the last step
-    int32_t first;    // first sample of the current step's batch within trainRaw / trainTarget
+    int32_t nTotal;   // samples collected in trainRaw / trainTarget for the next training call
+    float loss;       // mean loss of the last step (before its update)
+    int32_t pad;
     float lr, beta1, beta2, eps;  // Adam hyper-parameters (device-resident: captured graphs read them)
+    // per step k of a training call, planned once per call: batch window and bias corrections
+    int32_t stepFirst[kNrcMaxSteps];
+    int32_t stepN[kNrcMaxSteps];
+    float stepC1[kNrcMaxSteps];   // 1 / (1 - beta1^t_k)
+    float stepC2[kNrcMaxSteps];   // 1 / (1 - beta2^t_k)
 };
 
 struct NrcBuffers {
@@ -222,7 +226,7 @@ struct NrcBuffers {
     uint16_t* bwd = nullptr;      // backward MFMA fragments (kNrcBwdElems)
     uint16_t* H = nullptr;        // layer inputs [6][64][batchCap]
     uint16_t* G = nullptr;        // layer output gradients [6][64][batchCap]
-    float* partial = nullptr;     // [batchCap / kNrcKSplit][kNrcParams]
+    float* partial = nullptr;     // [batchCap / (4 kNrcKSplit)][kNrcParams] weight-gradient partials
     float* lossPartial = nullptr; // kNrcTrainBlocks
     NrcState* state = nullptr;
     float* trainRaw = nullptr;    // [chunks * batchCap][16]
@@ -238,8 +242,8 @@ hipError_t nrc_pack(const NrcBuffers& b, hipStream_t stream);
 // beta_c * max(y_c, 0) * a_c for each query.
 hipError_t nrc_infer(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out,
                      const NrcQuery* meta, float* sampleL, hipStream_t stream, const int32_t* index = nullptr);
-// One Adam step on batch (chunk mod ceil(nTotal / batchCap)) of the state->nTotal samples in
-// trainRaw / trainTarget, so steps beyond the collected batches make further passes over them.
-hipError_t nrc_train_step(const NrcBuffers& b, hipStream_t stream, int chunk = 0);
+// `steps` Adam steps over the state->nTotal samples in trainRaw / trainTarget: step k trains on batch
+// (k mod ceil(nTotal / batchCap)), so steps beyond the collected batches make further passes.
+hipError_t nrc_train(const NrcBuffers& b, hipStream_t stream, int steps);
 
 }  // namespace lts

@@ -7,6 +7,8 @@
 // Since an MFMA sums over K, any permutation of K applied to both operands gives the same product,
 // so the weight fragments are stored with K permuted to match the accumulator registers:
 //   K slot (ks, g, j) of lane group g = l/16  <->  neuron 16 (2 ks + j/4) + 4 g + (j % 4).
+// The input layer's K order is free; its slot (ks, g, j) holds encoding feature 4 (8 ks + j) + g so
+// that every unrolled slot computes one kind of feature in all lanes.
 // Activations therefore pass from layer to layer in registers (fp32 accumulate, ReLU, round to
 // bf16/fp16), with no LDS round trip. Weight fragments sit in LDS (43 KB forward, 49 KB backward).
 //
@@ -37,8 +39,13 @@ constexpr int kFwdVec = kNrcParams / 8;    // 16-byte fragment rows of the forwa
 constexpr int kBwdVec = kNrcBwdElems / 8;
 constexpr int kTrainBlock = 512;
 
-// Neuron held by K slot (ks, g, j).
+// Neuron held by K slot (ks, g, j) of a hidden layer's input.
 LTS_D int slot_neuron(int ks, int g, int j) { return 16 * (2 * ks + (j >> 2)) + 4 * g + (j & 3); }
+
+// Encoding feature held by K slot (ks, g, j) of the input layer. The input layer's K order is free
+// (only its weight fragments must agree), so slot (ks, j) holds features 4 (8 ks + j) + g of the four
+// lane groups: four features of one kind, the kind fixed by (ks, j) once the loops are unrolled.
+LTS_D int slot_feature(int ks, int g, int j) { return 4 * (8 * ks + j) + g; }
 
 LTS_D int w_index(int l, int in, int out) { return l < 5 ? l * 4096 + in * 64 + out : 20480 + in * 16 + out; }
 
@@ -53,17 +60,20 @@ LTS_D float one_blob(float s, int bin) {
 // Encoding feature idx (0..63) of one raw record: 36 triangle waves of frequency 2^(f-1) on the
 // normalised position, one-blob (4 bins) of the direction and normal angles and of roughness,
 // diffuse and specular reflectance, and two constant ones.
-LTS_D float nrc_feature(const float* __restrict__ raw, int idx) {
-    if (idx < 36) {
-        const int d = idx / 12, f = idx % 12;
+// Features 4m + g (g = lane group) of block m; m is a compile-time constant in the unrolled callers,
+// so the kind of feature is resolved without branches.
+LTS_D float nrc_feature_block(const float* __restrict__ raw, int m, int g) {
+    if (m < 9) {
+        // triangle waves: coordinate d = 4m / 12 is the same for the block, frequency (4m % 12) + g
+        const int d = (4 * m) / 12, f = (4 * m) % 12 + g;
         const float x = ldexpf(raw[d], f - 1);
         const float fr = x - floorf(x);
         return fabsf(2.0f * fr - 1.0f);
     }
-    if (idx < 52) return one_blob(raw[3 + (idx - 36) / 4], idx & 3);
-    if (idx < 56) return one_blob(raw[7], idx - 52);
-    if (idx < 62) return raw[8 + idx - 56];
-    return 1.0f;
+    if (m < 13) return one_blob(raw[3 + (m - 9)], g);
+    if (m == 13) return one_blob(raw[7], g);
+    if (m == 14) return raw[8 + g];
+    return g < 2 ? raw[12 + g] : 1.0f;
 }
 
 template <typename T>
@@ -88,7 +98,7 @@ LTS_D f32x4 forward_tile(const uint4* wf, const float* __restrict__ raw, bool va
 #pragma unroll
     for (int ks = 0; ks < 2; ks++)
 #pragma unroll
-        for (int j = 0; j < 8; j++) h[0][ks][j] = valid ? (T)nrc_feature(raw, slot_neuron(ks, g, j)) : (T)0.0f;
+        for (int j = 0; j < 8; j++) h[0][ks][j] = valid ? (T)nrc_feature_block(raw, 8 * ks + j, g) : (T)0.0f;
 #pragma unroll
     for (int l = 0; l < 5; l++) {
         if (Reload) asm volatile("" ::: "memory");
@@ -111,6 +121,52 @@ LTS_D f32x4 forward_tile(const uint4* wf, const float* __restrict__ raw, bool va
     return y;  // lane group 0 holds outputs 0..3 of its sample
 }
 
+// Forward pass of NT 16-sample tiles at once (independent MFMA chains interleaved); only the current
+// layer's activations are kept.
+template <typename T, int NT>
+LTS_D void forward_tiles(const uint4* wf, const float* const (&raw)[NT], const bool (&valid)[NT], int lane, f32x4 (&y)[NT]) {
+    const int g = lane >> 4;
+    typename Vec8<T>::type h[NT][2];
+#pragma unroll
+    for (int u = 0; u < NT; u++)
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) h[u][ks][j] = valid[u] ? (T)nrc_feature_block(raw[u], 8 * ks + j, g) : (T)0.0f;
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+        f32x4 acc[NT][4];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int u = 0; u < NT; u++) acc[u][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) {
+                const typename Vec8<T>::type w = lds_frag<T>(wf, l * 512 + (mt * 2 + ks) * 64 + lane);
+#pragma unroll
+                for (int u = 0; u < NT; u++) acc[u][mt] = mfma(w, h[u][ks], acc[u][mt]);
+            }
+#pragma unroll
+        for (int u = 0; u < NT; u++)
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) h[u][ks][j] = (T)fmaxf(acc[u][2 * ks + (j >> 2)][j & 3], 0.0f);
+    }
+#pragma unroll
+    for (int u = 0; u < NT; u++) y[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) {
+        const typename Vec8<T>::type w = lds_frag<T>(wf, 2560 + ks * 64 + lane);
+#pragma unroll
+        for (int u = 0; u < NT; u++) y[u] = mfma(w, h[u][ks], y[u]);
+    }
+}
+
+constexpr int kInferTiles = 2;  // 16-sample tiles per wave iteration
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_nrc_infer(const uint16_t* __restrict__ fwd, const float* __restrict__ raw,
                                                    const int32_t* __restrict__ nDev, int nCap, float* __restrict__ out,
@@ -121,24 +177,33 @@ __global__ __launch_bounds__(256) void k_nrc_infer(const uint16_t* __restrict__ 
     __syncthreads();
     const int n = min(*nDev, nCap);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tiles = (n + 15) >> 4;
-    for (int t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
-        const int s = t * 16 + (lane & 15);
-        const bool valid = s < n;
-        const int rec = valid ? (index ? index[s] : s) : 0;
-        typename Vec8<T>::type h[6][2];
-        const f32x4 y = forward_tile<T, false>(wf, raw + (size_t)rec * kNrcRaw, valid, lane, h);
-        if (lane < 16 && valid) {
+    const int groups = (n + 16 * kInferTiles - 1) / (16 * kInferTiles);
+    for (int t = blockIdx.x * 4 + wave; t < groups; t += gridDim.x * 4) {
+        int s[kInferTiles], rec[kInferTiles];
+        bool valid[kInferTiles];
+        const float* rp[kInferTiles];
+#pragma unroll
+        for (int u = 0; u < kInferTiles; u++) {
+            s[u] = (t * kInferTiles + u) * 16 + (lane & 15);
+            valid[u] = s[u] < n;
+            rec[u] = valid[u] ? (index ? index[s[u]] : s[u]) : 0;
+            rp[u] = raw + (size_t)rec[u] * kNrcRaw;
+        }
+        f32x4 y[kInferTiles];
+        forward_tiles<T, kInferTiles>(wf, rp, valid, lane, y);
+#pragma unroll
+        for (int u = 0; u < kInferTiles; u++) {
+            if (lane >= 16 || !valid[u]) continue;
             if (meta) {
-                const NrcQuery q = meta[rec];
+                const NrcQuery q = meta[rec[u]];
                 float* o = sampleL + 3 * (size_t)q.slot;
-                o[0] += q.beta[0] * fmaxf(y[0], 0.0f) * q.a[0];
-                o[1] += q.beta[1] * fmaxf(y[1], 0.0f) * q.a[1];
-                o[2] += q.beta[2] * fmaxf(y[2], 0.0f) * q.a[2];
+                o[0] += q.beta[0] * fmaxf(y[u][0], 0.0f) * q.a[0];
+                o[1] += q.beta[1] * fmaxf(y[u][1], 0.0f) * q.a[1];
+                o[2] += q.beta[2] * fmaxf(y[u][2], 0.0f) * q.a[2];
             } else {
-                out[3 * (size_t)s + 0] = y[0];
-                out[3 * (size_t)s + 1] = y[1];
-                out[3 * (size_t)s + 2] = y[2];
+                out[3 * (size_t)s[u] + 0] = y[u][0];
+                out[3 * (size_t)s[u] + 1] = y[u][1];
+                out[3 * (size_t)s[u] + 2] = y[u][2];
             }
         }
     }
@@ -151,18 +216,19 @@ __global__ __launch_bounds__(256) void k_nrc_infer(const uint16_t* __restrict__ 
 template <typename T>
 __global__ __launch_bounds__(kTrainBlock) void k_nrc_fwdbwd(const uint16_t* __restrict__ fwd, const uint16_t* __restrict__ bwd,
                                                             const float* __restrict__ raw, const float* __restrict__ target,
-                                                            const NrcState* __restrict__ state, int nCap, T* __restrict__ H,
-                                                            T* __restrict__ G, int ld, float* __restrict__ lossPartial) {
+                                                            const NrcState* __restrict__ state, int step, int nCap,
+                                                            T* __restrict__ H, T* __restrict__ G, int ld,
+                                                            float* __restrict__ lossPartial) {
     __shared__ uint4 wf[kFwdVec];
     __shared__ uint4 wb[kBwdVec];
     __shared__ float red[kTrainBlock];
     stage(wf, fwd, kFwdVec);
     stage(wb, bwd, kBwdVec);
     __syncthreads();
-    const int n = min(state->nTrain, nCap);
+    const int n = min(state->stepN[step], nCap);
     const int npad = (n + 31) & ~31;
-    raw += (size_t)state->first * kNrcRaw;
-    target += (size_t)state->first * 3;
+    raw += (size_t)state->stepFirst[step] * kNrcRaw;
+    target += (size_t)state->stepFirst[step] * 3;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
     const int wavesPerBlock = kTrainBlock / 64;
     float lossAcc = 0.0f;
@@ -193,8 +259,13 @@ __global__ __launch_bounds__(kTrainBlock) void k_nrc_fwdbwd(const uint16_t* __re
         const size_t layer = (size_t)64 * ld;
         T* Hs = H + (size_t)(s >> 5) * 2048 + (s & 31) + 128 * g;
         T* Gs = G + (size_t)(s >> 5) * 2048 + (s & 31) + 128 * g;
+        T* H0 = H + (size_t)(s >> 5) * 2048 + (s & 31) + 32 * g;  // input layer: feature 4 (8 ks + j) + g
 #pragma unroll
-        for (int l = 0; l < 6; l++)
+        for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) H0[32 * 4 * (8 * ks + j)] = h[0][ks][j];
+#pragma unroll
+        for (int l = 1; l < 6; l++)
 #pragma unroll
             for (int ks = 0; ks < 2; ks++)
 #pragma unroll
@@ -232,57 +303,96 @@ __global__ __launch_bounds__(kTrainBlock) void k_nrc_fwdbwd(const uint16_t* __re
     if (threadIdx.x == 0) lossPartial[blockIdx.x] = red[0];
 }
 
-// dW_l = H_l G_l^T over the samples of split blockIdx.y; 84 16x16 tiles, one per wave.
+// dW_l = H_l G_l^T: one wave per (layer, 256-sample split) computes the whole layer (16 tiles of
+// 16 x 16; the output layer 4), loading 4 A and 4 B fragments per 32-sample step for 16 MFMAs.
+// Block: 4 waves = 4 consecutive splits of layer blockIdx.x, summed in LDS in wave order into one
+// partial per block (1024 samples).
 template <typename T>
 __global__ __launch_bounds__(256) void k_nrc_wgrad(const T* __restrict__ H, const T* __restrict__ G, int ld,
-                                                   const NrcState* __restrict__ state, int nCap, float* __restrict__ partial) {
+                                                   const NrcState* __restrict__ state, int step, int nCap,
+                                                   float* __restrict__ partial) {
     using V = typename Vec8<T>::type;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
-    const int tile = blockIdx.x * 4 + wave;  // 0..83
-    int l, it, ot;
-    if (tile < 80) { l = tile >> 4; it = (tile >> 2) & 3; ot = tile & 3; }
-    else { l = 5; it = tile - 80; ot = 0; }
-    const int n = min(state->nTrain, nCap);
+    const int l = blockIdx.x;
+    const int split = blockIdx.y * 4 + wave;
+    const int nOt = l < 5 ? 4 : 1;
+    const int n = min(state->stepN[step], nCap);
     const int npad = (n + 31) & ~31;
-    const int k0 = blockIdx.y * kNrcKSplit, k1 = min(npad, k0 + kNrcKSplit);
-    const T* a = H + (size_t)l * 64 * ld + (size_t)(k0 >> 5) * 2048 + 32 * (16 * it + col) + 8 * g;
-    const T* b = G + (size_t)l * 64 * ld + (size_t)(k0 >> 5) * 2048 + 32 * (16 * ot + col) + 8 * g;
-    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k = k0; k < k1; k += 32, a += 2048, b += 2048)
-        acc = mfma(*reinterpret_cast<const V*>(a), *reinterpret_cast<const V*>(b), acc);
-    float* p = partial + (size_t)blockIdx.y * kNrcParams;
+    const int k0 = split * kNrcKSplit, k1 = min(npad, k0 + kNrcKSplit);
+    const T* a = H + (size_t)l * 64 * ld + (size_t)(k0 >> 5) * 2048 + 32 * col + 8 * g;
+    const T* b = G + (size_t)l * 64 * ld + (size_t)(k0 >> 5) * 2048 + 32 * col + 8 * g;
+    f32x4 acc[4][4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) p[w_index(l, 16 * it + 4 * g + i, 16 * ot + col)] = acc[i];
+    for (int it = 0; it < 4; it++)
+#pragma unroll
+        for (int ot = 0; ot < 4; ot++) acc[it][ot] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k = k0; k < k1; k += 32, a += 2048, b += 2048) {
+        V A[4], B[4];
+#pragma unroll
+        for (int it = 0; it < 4; it++) A[it] = *reinterpret_cast<const V*>(a + 512 * it);
+#pragma unroll
+        for (int ot = 0; ot < 4; ot++)
+            if (ot < nOt) B[ot] = *reinterpret_cast<const V*>(b + 512 * ot);
+#pragma unroll
+        for (int it = 0; it < 4; it++)
+#pragma unroll
+            for (int ot = 0; ot < 4; ot++)
+                if (ot < nOt) acc[it][ot] = mfma(A[it], B[ot], acc[it][ot]);
+    }
+    // per wave: element (it, ot, i, lane) of the layer's 64 x 64 (or 64 x 16) block
+    __shared__ float red[4][4096];
+#pragma unroll
+    for (int it = 0; it < 4; it++)
+#pragma unroll
+        for (int ot = 0; ot < 4; ot++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) red[wave][((it * 4 + ot) * 4 + i) * 64 + lane] = acc[it][ot][i];
+    __syncthreads();
+    float* p = partial + (size_t)blockIdx.y * kNrcParams;
+    for (int e = threadIdx.x; e < 4096; e += 256) {
+        const int ln = e & 63, i = (e >> 6) & 3, ot = (e >> 8) & 3, it = e >> 10;
+        if (ot >= nOt) continue;
+        const float v = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+        p[w_index(l, 16 * it + 4 * (ln >> 4) + i, 16 * ot + (ln & 15))] = v;
+    }
 }
 
-// Selects batch chunk mod ceil(nTotal / batchCap) of the collected samples for the next step.
-__global__ void k_nrc_step_prepare(NrcState* __restrict__ state, int chunk, int batchCap) {
-    if (threadIdx.x != 0) return;
+// Plans the steps of one training call: batch window of step k (cycling over the collected batches)
+// and Adam's bias corrections for step t + k + 1 (nothing advances when no samples were collected).
+__global__ void k_nrc_plan(NrcState* __restrict__ state, int steps, int batchCap) {
+    const int k = threadIdx.x;
+    if (k >= steps) return;
     const int total = state->nTotal;
     const int chunks = max(1, (total + batchCap - 1) / batchCap);
-    const int c = chunk % chunks;
-    state->first = c * batchCap;
-    state->nTrain = max(0, min(batchCap, total - c * batchCap));
+    const int c = k % chunks;
+    state->stepFirst[k] = c * batchCap;
+    state->stepN[k] = max(0, min(batchCap, total - c * batchCap));
+    const double t = (double)state->t + k + 1;
+    state->stepC1[k] = (float)(1.0 / (1.0 - pow((double)state->beta1, t)));
+    state->stepC2[k] = (float)(1.0 / (1.0 - pow((double)state->beta2, t)));
 }
 
-// Advances the step counter (when the batch is not empty) and the bias corrections.
-__global__ void k_nrc_step_begin(NrcState* __restrict__ state, int nCap, const float* __restrict__ lossPartial, int nPartial) {
+// After the steps of a training call: advance the step counter and record the last step's loss.
+__global__ void k_nrc_finish(NrcState* __restrict__ state, int steps, const float* __restrict__ lossPartial, int nPartial) {
     if (threadIdx.x != 0) return;
-    const int n = min(state->nTrain, nCap);
+    const int n = state->stepN[steps - 1];
     if (n <= 0) return;
-    const int t = state->t + 1;
-    state->t = t;
-    state->c1 = (float)(1.0 / (1.0 - pow((double)state->beta1, (double)t)));
-    state->c2 = (float)(1.0 / (1.0 - pow((double)state->beta2, (double)t)));
+    state->t += steps;
     float s = 0.0f;
     for (int i = 0; i < nPartial; i++) s += lossPartial[i];
     state->loss = s / (float)n;
 }
 
 LTS_D int fwd_pos(int l, int in, int out) {
-    // A operand of W^T: row = out, K slot of `in`
-    const int q = in >> 4, r = in & 15;
-    const int ks = q >> 1, j = ((q & 1) << 2) | (r & 3), g = r >> 2;
+    // A operand of W^T: row = out, K slot of `in` (input layer: slot_feature, else slot_neuron)
+    int ks, j, g;
+    if (l == 0) {
+        const int m = in >> 2;
+        g = in & 3; ks = m >> 3; j = m & 7;
+    } else {
+        const int q = in >> 4, r = in & 15;
+        ks = q >> 1; j = ((q & 1) << 2) | (r & 3); g = r >> 2;
+    }
     const int lane = 16 * g + (out & 15);
     return l * 4096 + (((out >> 4) * 2 + ks) * 64 + lane) * 8 + j;
 }
@@ -302,26 +412,33 @@ LTS_D void decode_param(int p, int& l, int& in, int& out) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_nrc_adam(const float* __restrict__ partial, int splits, const NrcState* __restrict__ state,
-                                                  int nCap, float* __restrict__ master, float* __restrict__ m,
+                                                  int step, int nCap, float* __restrict__ master, float* __restrict__ m,
                                                   float* __restrict__ v, float* __restrict__ gradOut, T* __restrict__ fwd,
                                                   T* __restrict__ bwd) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= kNrcParams) return;
-    const int n = min(state->nTrain, nCap);
+    const int n = min(state->stepN[step], nCap);
     if (n <= 0) return;
     const float lr = state->lr, beta1 = state->beta1, beta2 = state->beta2, eps = state->eps;
     int l, in, out;
     decode_param(p, l, in, out);
     if (l == 5 && out >= 3) return;  // padding columns stay zero
-    float sum = 0.0f;
-    for (int s = 0; s < splits; s++) sum += partial[(size_t)s * kNrcParams + p];
-    const float gr = sum / (float)n;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;  // fixed-order reduction with 4 independent chains
+    int s = 0;
+    for (; s + 4 <= splits; s += 4) {
+        s0 += partial[(size_t)(s + 0) * kNrcParams + p];
+        s1 += partial[(size_t)(s + 1) * kNrcParams + p];
+        s2 += partial[(size_t)(s + 2) * kNrcParams + p];
+        s3 += partial[(size_t)(s + 3) * kNrcParams + p];
+    }
+    for (; s < splits; s++) s0 += partial[(size_t)s * kNrcParams + p];
+    const float gr = ((s0 + s1) + (s2 + s3)) / (float)n;
     gradOut[p] = gr;
     const float mm = beta1 * m[p] + (1.0f - beta1) * gr;
     const float vv = beta2 * v[p] + (1.0f - beta2) * gr * gr;
     m[p] = mm;
     v[p] = vv;
-    const float w = master[p] - lr * (mm * state->c1) / (sqrtf(vv * state->c2) + eps);
+    const float w = master[p] - lr * (mm * state->stepC1[step]) / (sqrtf(vv * state->stepC2[step]) + eps);
     master[p] = w;
     fwd[fwd_pos(l, in, out)] = (T)w;
     bwd[bwd_pos(l, in, out)] = (T)w;
@@ -334,7 +451,8 @@ __global__ __launch_bounds__(256) void k_nrc_pack(const float* __restrict__ mast
     if (e < kNrcParams) {
         const int l = e >> 12, rem = e & 4095;
         const int j = rem & 7, lane = (rem >> 3) & 63, ks = (rem >> 9) & 1, mt = rem >> 10;
-        const int out = 16 * mt + (lane & 15), in = slot_neuron(ks, lane >> 4, j);
+        const int out = 16 * mt + (lane & 15);
+        const int in = l == 0 ? slot_feature(ks, lane >> 4, j) : slot_neuron(ks, lane >> 4, j);
         fwd[e] = (T)master[w_index(l, in, out)];
     }
     if (e < kNrcBwdElems) {
@@ -348,26 +466,30 @@ __global__ __launch_bounds__(256) void k_nrc_pack(const float* __restrict__ mast
 template <typename T>
 hipError_t infer_t(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out, const NrcQuery* meta,
                    float* sampleL, const int32_t* index, hipStream_t s) {
-    const int tiles = (nCap + 15) / 16;
-    const int blocks = std::max(1, std::min((tiles + 3) / 4, 2048));
+    // two blocks per CU are resident (LDS weights, 220 VGPRs): a grid of that size stages the
+    // weights once per block and loops over the tiles
+    const int groups = (nCap + 16 * kInferTiles - 1) / (16 * kInferTiles);
+    const int blocks = std::max(1, std::min((groups + 3) / 4, 512));
     k_nrc_infer<T><<<blocks, 256, 0, s>>>(b.fwd, raw, nDev, nCap, out, meta, sampleL, index);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t train_t(const NrcBuffers& b, hipStream_t s, int chunk) {
+hipError_t train_t(const NrcBuffers& b, hipStream_t s, int steps) {
     T* H = reinterpret_cast<T*>(b.H);
     T* G = reinterpret_cast<T*>(b.G);
     const int tiles = b.batchCap / 16;
     const int blocks = std::min(kNrcTrainBlocks, (tiles + kTrainBlock / 64 - 1) / (kTrainBlock / 64));
-    k_nrc_step_prepare<<<1, 64, 0, s>>>(b.state, chunk, b.batchCap);
-    k_nrc_fwdbwd<T><<<blocks, kTrainBlock, 0, s>>>(b.fwd, b.bwd, b.trainRaw, b.trainTarget, b.state, b.batchCap, H, G,
-                                                     b.batchCap, b.lossPartial);
-    const int splits = b.batchCap / kNrcKSplit;
-    k_nrc_wgrad<T><<<dim3(21, splits), 256, 0, s>>>(H, G, b.batchCap, b.state, b.batchCap, b.partial);
-    k_nrc_step_begin<<<1, 64, 0, s>>>(b.state, b.batchCap, b.lossPartial, blocks);
-    k_nrc_adam<T><<<(kNrcParams + 255) / 256, 256, 0, s>>>(b.partial, splits, b.state, b.batchCap, b.master, b.m, b.v, b.grad,
-                                                           reinterpret_cast<T*>(b.fwd), reinterpret_cast<T*>(b.bwd));
+    const int splits = b.batchCap / kNrcKSplit;  // 256-sample GEMM splits; 4 per block and partial
+    k_nrc_plan<<<1, 64, 0, s>>>(b.state, steps, b.batchCap);
+    for (int k = 0; k < steps; k++) {
+        k_nrc_fwdbwd<T><<<blocks, kTrainBlock, 0, s>>>(b.fwd, b.bwd, b.trainRaw, b.trainTarget, b.state, k, b.batchCap, H, G,
+                                                         b.batchCap, b.lossPartial);
+        k_nrc_wgrad<T><<<dim3(6, splits / 4), 256, 0, s>>>(H, G, b.batchCap, b.state, k, b.batchCap, b.partial);
+        k_nrc_adam<T><<<(kNrcParams + 255) / 256, 256, 0, s>>>(b.partial, splits / 4, b.state, k, b.batchCap, b.master, b.m, b.v,
+                                                               b.grad, reinterpret_cast<T*>(b.fwd), reinterpret_cast<T*>(b.bwd));
+    }
+    k_nrc_finish<<<1, 64, 0, s>>>(b.state, steps, b.lossPartial, blocks);
     return hipGetLastError();
 }
 
@@ -389,9 +511,9 @@ hipError_t nrc_infer(const NrcBuffers& b, const float* raw, const int32_t* nDev,
                             : infer_t<__bf16>(b, raw, nDev, nCap, out, meta, sampleL, index, s);
 }
 
-hipError_t nrc_train_step(const NrcBuffers& b, hipStream_t s, int chunk) {
-    if (chunk < 0) return hipErrorInvalidValue;
-    return b.precision == 1 ? train_t<_Float16>(b, s, chunk) : train_t<__bf16>(b, s, chunk);
+hipError_t nrc_train(const NrcBuffers& b, hipStream_t s, int steps) {
+    if (steps < 1 || steps > kNrcMaxSteps) return hipErrorInvalidValue;
+    return b.precision == 1 ? train_t<_Float16>(b, s, steps) : train_t<__bf16>(b, s, steps);
 }
 
 template <typename X>
@@ -400,7 +522,7 @@ static hipError_t nrc_alloc(X** p, size_t count) {
 }
 
 hipError_t nrc_reserve(NrcBuffers& b, int batchCap, int steps, int chunks) {
-    const int cap = std::max(kNrcKSplit, (batchCap + kNrcKSplit - 1) / kNrcKSplit * kNrcKSplit);
+    const int cap = std::max(kNrcBatchAlign, (batchCap + kNrcBatchAlign - 1) / kNrcBatchAlign * kNrcBatchAlign);
     steps = std::max(1, steps);
     chunks = std::max(1, chunks);
     if (b.master && cap == b.batchCap && chunks == b.chunks) {
@@ -422,7 +544,7 @@ hipError_t nrc_reserve(NrcBuffers& b, int batchCap, int steps, int chunks) {
     NTRY(nrc_alloc(&b.bwd, kNrcBwdElems));
     NTRY(nrc_alloc(&b.H, (size_t)6 * 64 * cap));
     NTRY(nrc_alloc(&b.G, (size_t)6 * 64 * cap));
-    NTRY(nrc_alloc(&b.partial, (size_t)(cap / kNrcKSplit) * kNrcParams));
+    NTRY(nrc_alloc(&b.partial, (size_t)(cap / (4 * kNrcKSplit)) * kNrcParams));
     NTRY(nrc_alloc(&b.lossPartial, kNrcTrainBlocks));
     NTRY(nrc_alloc(&b.state, 1));
     NTRY(nrc_alloc(&b.trainRaw, (size_t)chunks * cap * kNrcRaw));

@@ -629,8 +629,7 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
             k_wf_nrc_scatter<<<nb, kCollectBlock, 0, s>>>(b.params, b.tRec, b.tRaw, b.sampleBase, b.tCount, cap,
                                                           nrc->trainRaw, nrc->trainTarget);
         }
-        for (int k = 0; k < nrc->steps; k++)  // chunks cycle when steps > chunks
-            if ((e = nrc_train_step(*nrc, s, k)) != hipSuccess) return e;
+        if ((e = nrc_train(*nrc, s, nrc->steps)) != hipSuccess) return e;  // batches cycle when steps > batches
         mark(8 * kWfMaxBounce + 2);
     }
     return hipGetLastError();
