@@ -129,6 +129,19 @@ int lts_load_scene(lts_ctx* ctx, const char* project_dir, const char* scene, con
                    int32_t width, int32_t height);
 int lts_scene_info(lts_ctx* ctx, int32_t* n_tris, int32_t* n_lights, int32_t* n_meshes);
 
+/* Render tree of the scenes this context loads from now on. Rendering does not require the
+ * reference's tree: any binary tree over the Morton-sorted leaves finds the same closest hits.
+ * LTS_TREE_PLOC (default) clusters the sorted leaves by PLOC (surface-area nearest neighbours,
+ * radius 8) and falls back to the Karras tree when the PLOC tree is too deep for the traversal
+ * stack; LTS_TREE_KARRAS renders with the LBVH itself (BVH::intersect's tree, BVH.cuh:365-398).
+ * The exported LBVH (lts_bvh_build*) is always the bit-exact Karras tree. No reference
+ * counterpart; the reference renders with its LBVH. */
+#define LTS_TREE_KARRAS 0
+#define LTS_TREE_PLOC 1
+int lts_set_render_tree(lts_ctx* ctx, int32_t kind);
+/* Tree the loaded scene renders with, its depth in edges and its 4-wide node count. */
+int lts_render_tree_info(lts_ctx* ctx, int32_t* kind, int32_t* depth, int32_t* nodes4);
+
 /* Replaces CreateBVHTree (acceleration/BVH.cuh:411-452) for a triangle soup of n triangles
  * (9 floats each, host memory). mode 0 reproduces the reference's leaf assignment (leaf k holds
  * primitive k, BVH.cuh:242); mode 1 assigns leaf k to the k-th primitive in Morton order.

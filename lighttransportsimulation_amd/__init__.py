@@ -23,8 +23,11 @@ EXPORTED_SYMBOLS = (
     "lts_create", "lts_destroy", "lts_last_error", "lts_version", "lts_read_app_params", "lts_load_scene",
     "lts_scene_info", "lts_bvh_build", "lts_bvh_build_device", "lts_bvh_export_last", "lts_render",
     "lts_write_image", "lts_device_copy_benchmark", "lts_render_batch", "lts_nrc_configure", "lts_nrc_set_weights",
-    "lts_nrc_get_weights", "lts_nrc_infer", "lts_nrc_train_step",
+    "lts_nrc_get_weights", "lts_nrc_infer", "lts_nrc_train_step", "lts_set_render_tree", "lts_render_tree_info",
 )
+
+TREE_KARRAS = 0
+TREE_PLOC = 1
 
 NRC_PARAMS = 21504
 NRC_RAW = 16
@@ -90,6 +93,8 @@ def load_library():
     L.lts_read_app_params.argtypes = [C.c_char_p] + [C.POINTER(i32)] * 6 + [C.c_char_p] * 3 + [i32]
     L.lts_load_scene.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, i32, i32]
     L.lts_scene_info.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
+    L.lts_set_render_tree.argtypes = [vp, i32]
+    L.lts_render_tree_info.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
     L.lts_bvh_build.argtypes = [vp, vp, i32, i32, C.POINTER(BvhExport), C.POINTER(BvhTiming)]
     L.lts_bvh_build_device.argtypes = [vp, vp, i32, i32, i32, C.POINTER(BvhTiming)]
     L.lts_bvh_export_last.argtypes = [vp, C.POINTER(BvhExport)]
@@ -165,6 +170,15 @@ class Context:
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
         self._check(self.lib.lts_scene_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "lts_scene_info")
         return dict(tris=a.value, lights=b.value, meshes=c.value)
+
+    def set_render_tree(self, kind: int):
+        """Render tree of the scenes loaded from now on: TREE_PLOC (default) or TREE_KARRAS."""
+        self._check(self.lib.lts_set_render_tree(self.h, kind), "lts_set_render_tree")
+
+    def render_tree_info(self) -> dict:
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.lts_render_tree_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "lts_render_tree_info")
+        return dict(kind="ploc" if a.value == TREE_PLOC else "karras", depth=b.value, nodes4=c.value)
 
     # ------------------------------------------------------------------ LBVH
     def bvh_build(self, tri: np.ndarray, mode: int = 0, export: bool = True):

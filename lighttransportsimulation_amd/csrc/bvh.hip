@@ -519,13 +519,22 @@ inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
 // ---------------------------------------------------------------- 4-wide render BVH
 constexpr int kScanBlock = 1024;
 
-// flag[i] = 1 when interior node i has even depth (the root has depth 0).
-__global__ __launch_bounds__(kBlock) void k_even_depth(const int32_t* __restrict__ parent, int n, int32_t* __restrict__ flag) {
+// flag[i] = 1 when interior node i has even depth (the root has depth 0); maxDepth (when given)
+// receives the deepest level of a child, i.e. the tree depth counted in edges down to the leaves.
+__global__ __launch_bounds__(kBlock) void k_even_depth(const int32_t* __restrict__ parent, int n, int32_t* __restrict__ flag,
+                                                       int32_t* __restrict__ maxDepth) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n - 1) return;
-    int depth = 0;
-    for (int node = i; node > 0 && depth < 2 * n; node = parent[node]) depth++;
-    flag[i] = (depth & 1) == 0;
+    int depth = -1;
+    if (i < n - 1) {
+        depth = 0;
+        for (int node = i; node > 0 && depth < 2 * n; node = parent[node]) depth++;
+        flag[i] = (depth & 1) == 0;
+    }
+    if (maxDepth) {
+        int d = depth + 1;
+        for (int o = 32; o > 0; o >>= 1) d = max(d, __shfl_xor(d, o));
+        if ((threadIdx.x & 63) == 0 && d > 0) atomicMax(maxDepth, d);
+    }
 }
 
 // In-place exclusive prefix sum of v[0, m) within blocks of kScanBlock; totals[block] = block sum.
@@ -595,6 +604,148 @@ __global__ __launch_bounds__(kBlock) void k_build_bvh4(const BvhNode* __restrict
     out[index4[i]] = o;
 }
 
+
+// ---------------------------------------------------------------- PLOC render tree
+// Parallel locally-ordered clustering (Meister & Bittner, IEEE TVCG 2018) over the Morton-ordered
+// leaves of the LBVH build: every cluster finds the cluster within kPlocRadius positions whose union
+// box has the smallest surface area; mutual nearest neighbours merge into a new interior node; the
+// survivors are compacted in order; repeat until one cluster is left. Pairs are ordered by (area,
+// lower position, higher position), a total order, so the globally smallest pair is always mutual
+// and every round merges at least one pair. New nodes are numbered by a prefix sum over the merging
+// pairs, counting down from n - 2, so the root is node 0 and the layout matches the Karras tree's
+// (interior 0..n-2, leaf k at n-1+k over the same sorted leaves): the 4-wide collapse and the
+// traversal take either tree. The result depends only on the input, not on scheduling.
+constexpr int kPlocRadius = 8;
+constexpr int kPlocBlock = 256;
+constexpr int kFlagBlock = 1024;
+
+__device__ __forceinline__ float union_area(const float* a, const float* b) {
+    const float dx = fmaxf(a[3], b[3]) - fminf(a[0], b[0]);
+    const float dy = fmaxf(a[4], b[4]) - fminf(a[1], b[1]);
+    const float dz = fmaxf(a[5], b[5]) - fminf(a[2], b[2]);
+    const float s = dx * dy + dy * dz + dz * dx;
+    return s <= 3.0e38f ? s : INFINITY;  // NaN and overflow order as +inf, keeping the order total
+}
+
+__global__ __launch_bounds__(kBlock) void k_ploc_init(const float* __restrict__ primBox, const int32_t* __restrict__ perm, int n,
+                                                      float* __restrict__ box, int32_t* __restrict__ cl, int32_t* __restrict__ parent) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < n) {
+        const int p = perm[k];
+#pragma unroll
+        for (int a = 0; a < 6; a++) box[6 * (size_t)(n - 1 + k) + a] = primBox[6 * (size_t)p + a];
+        cl[k] = n - 1 + k;
+        parent[n - 1 + k] = -1;
+    }
+    if (k < n - 1) parent[k] = -1;
+}
+
+__global__ __launch_bounds__(kPlocBlock) void k_ploc_nearest(const int32_t* __restrict__ cl, int m, const float* __restrict__ box,
+                                                             int32_t* __restrict__ nn) {
+    __shared__ float sb[(kPlocBlock + 2 * kPlocRadius) * 6];
+    const int base = blockIdx.x * kPlocBlock - kPlocRadius;
+    for (int e = threadIdx.x; e < kPlocBlock + 2 * kPlocRadius; e += kPlocBlock) {
+        const int pos = base + e;
+        if (pos >= 0 && pos < m) {
+            const int c = cl[pos];
+#pragma unroll
+            for (int a = 0; a < 6; a++) sb[6 * e + a] = box[6 * (size_t)c + a];
+        }
+    }
+    __syncthreads();
+    const int i = blockIdx.x * kPlocBlock + threadIdx.x;
+    if (i >= m) return;
+    const float* bi = &sb[6 * (threadIdx.x + kPlocRadius)];
+    float best = INFINITY;
+    int bj = -1;
+    const int lo = max(0, i - kPlocRadius), hi = min(m - 1, i + kPlocRadius);
+    for (int j = lo; j <= hi; j++) {
+        if (j == i) continue;
+        const float a = union_area(bi, &sb[6 * (j - base)]);
+        // candidates are visited in increasing j: below i the pair key (j, i) grows with j, above i
+        // the key (i, j) grows with j and every key above i is larger than every key below it, so a
+        // tie keeps the earlier candidate
+        if (bj < 0 || a < best) { best = a; bj = j; }
+    }
+    nn[i] = bj;
+}
+
+// keep[i] / merge[i]: exclusive prefix within blocks of kFlagBlock; sums[2b], sums[2b+1]: block totals.
+__global__ __launch_bounds__(kFlagBlock) void k_ploc_flags(const int32_t* __restrict__ nn, int m, int32_t* __restrict__ keep,
+                                                           int32_t* __restrict__ merge, int32_t* __restrict__ sums) {
+    __shared__ int32_t shK[kFlagBlock / 64], shM[kFlagBlock / 64];
+    const int i = blockIdx.x * kFlagBlock + threadIdx.x;
+    int k0 = 0, m0 = 0;
+    if (i < m) {
+        const int j = nn[i];
+        const bool mutual = nn[j] == i;
+        k0 = !mutual || i < j;
+        m0 = mutual && i < j;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int xk = k0, xm = m0;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int yk = __shfl_up(xk, o), ym = __shfl_up(xm, o);
+        if (lane >= o) { xk += yk; xm += ym; }
+    }
+    if (lane == 63) { shK[wave] = xk; shM[wave] = xm; }
+    __syncthreads();
+    int bk = 0, bm = 0;
+    for (int w = 0; w < wave; w++) { bk += shK[w]; bm += shM[w]; }
+    if (i < m) { keep[i] = bk + xk - k0; merge[i] = bm + xm - m0; }
+    if (threadIdx.x == kFlagBlock - 1) { sums[2 * blockIdx.x] = bk + xk; sums[2 * blockIdx.x + 1] = bm + xm; }
+}
+
+// Exclusive prefix of the block totals in one block; totals[0] = survivors, totals[1] = merges.
+__global__ __launch_bounds__(kFlagBlock) void k_ploc_scan_sums(int32_t* __restrict__ sums, int nb, int32_t* __restrict__ totals) {
+    __shared__ int32_t shK[kFlagBlock / 64], shM[kFlagBlock / 64];
+    __shared__ int32_t carry[2];
+    if (threadIdx.x == 0) { carry[0] = carry[1] = 0; }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < nb; c0 += kFlagBlock) {
+        const int b = c0 + threadIdx.x;
+        const int k0 = b < nb ? sums[2 * b] : 0, m0 = b < nb ? sums[2 * b + 1] : 0;
+        int xk = k0, xm = m0;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int yk = __shfl_up(xk, o), ym = __shfl_up(xm, o);
+            if (lane >= o) { xk += yk; xm += ym; }
+        }
+        if (lane == 63) { shK[wave] = xk; shM[wave] = xm; }
+        __syncthreads();
+        int bk = carry[0], bm = carry[1];
+        for (int w = 0; w < wave; w++) { bk += shK[w]; bm += shM[w]; }
+        if (b < nb) { sums[2 * b] = bk + xk - k0; sums[2 * b + 1] = bm + xm - m0; }
+        __syncthreads();
+        if (threadIdx.x == kFlagBlock - 1) { carry[0] = bk + xk; carry[1] = bm + xm; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { totals[0] = carry[0]; totals[1] = carry[1]; }
+}
+
+__global__ __launch_bounds__(kFlagBlock) void k_ploc_emit(const int32_t* __restrict__ cl, const int32_t* __restrict__ nn, int m,
+                                                          const int32_t* __restrict__ keep, const int32_t* __restrict__ merge,
+                                                          const int32_t* __restrict__ sums, int created, int n,
+                                                          float* __restrict__ box, int32_t* __restrict__ left, int32_t* __restrict__ right,
+                                                          int32_t* __restrict__ parent, int32_t* __restrict__ out) {
+    const int i = blockIdx.x * kFlagBlock + threadIdx.x;
+    if (i >= m) return;
+    const int j = nn[i];
+    const bool mutual = nn[j] == i;
+    const int pos = keep[i] + sums[2 * blockIdx.x];
+    if (!mutual) { out[pos] = cl[i]; return; }
+    if (i > j) return;
+    const int node = n - 2 - (created + merge[i] + sums[2 * blockIdx.x + 1]);
+    const int a = cl[i], c = cl[j];
+    left[node] = a; right[node] = c;
+    parent[a] = node; parent[c] = node;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        box[6 * (size_t)node + q] = fminf(box[6 * (size_t)a + q], box[6 * (size_t)c + q]);
+        box[6 * (size_t)node + 3 + q] = fmaxf(box[6 * (size_t)a + 3 + q], box[6 * (size_t)c + 3 + q]);
+    }
+    out[pos] = node;
+}
 }  // namespace
 
 template <typename T>
@@ -630,8 +781,10 @@ hipError_t bvh_reserve(BvhBuffers& b, int n) {
     return hipSuccess;
 }
 
-hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t s) {
+hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t s, int* depth, int* nodes4) {
     const int n = b.n;
+    *depth = 0;
+    *nodes4 = 0;
     if (n < 2) return hipSuccess;
     const int m = n - 1;
     const int nb = (m + kScanBlock - 1) / kScanBlock;
@@ -642,19 +795,93 @@ hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t s) {
         b.nodes4 = nullptr; b.index4 = nullptr; b.scanTotals = nullptr; b.cap4 = 0;
         hipError_t e = dalloc(&b.nodes4, m);
         if (e == hipSuccess) e = dalloc(&b.index4, m);
-        if (e == hipSuccess) e = dalloc(&b.scanTotals, (size_t)nb + 1);
+        if (e == hipSuccess) e = dalloc(&b.scanTotals, (size_t)nb + 2);
         if (e != hipSuccess) return e;
         b.cap4 = n;
     }
-    // flags (reusing b.flags) -> exclusive prefix in index4 -> 4-wide nodes
-    k_even_depth<<<blocks_for(m), kBlock, 0, s>>>(b.parent, n, b.flags);
-    hipError_t e = hipMemcpyAsync(b.index4, b.flags, (size_t)m * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
+    // flags (reusing b.flags) and the depth -> exclusive prefix in index4 -> 4-wide nodes
+    int32_t* depthDev = b.scanTotals + nb + 1;
+    hipError_t e = hipMemsetAsync(depthDev, 0, sizeof(int32_t), s);
+    if (e != hipSuccess) return e;
+    k_even_depth<<<blocks_for(m), kBlock, 0, s>>>(b.parent, n, b.flags, depthDev);
+    e = hipMemcpyAsync(b.index4, b.flags, (size_t)m * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return e;
     k_scan_local<<<nb, kScanBlock, 0, s>>>(b.index4, m, b.scanTotals);
     k_scan_totals<<<1, 64, 0, s>>>(b.scanTotals, nb);
     k_scan_add<<<nb, kScanBlock, 0, s>>>(b.index4, m, b.scanTotals);
     k_build_bvh4<<<blocks_for(m), kBlock, 0, s>>>(b.nodes, b.flags, b.index4, n, b.nodes4);
-    return hipGetLastError();
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int32_t tail[2];  // scanTotals[nb] = 4-wide node count, then the depth
+    e = hipMemcpyAsync(tail, b.scanTotals + nb, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    *nodes4 = tail[0];
+    *depth = tail[1];
+    return hipSuccess;
+}
+
+
+hipError_t ploc_enqueue(BvhBuffers& b, hipStream_t s, int maxDepth, int* depthOut) {
+    const int n = b.n;
+    *depthOut = 0;
+    if (n < 2) return hipSuccess;
+    // temporaries: unified boxes (interior + leaves), children, parents, two cluster lists,
+    // neighbours, the two prefix arrays and the block totals
+    const int nbMax = (n + kFlagBlock - 1) / kFlagBlock;
+    float* box = nullptr;
+    int32_t *left = nullptr, *right = nullptr, *parent = nullptr, *clA = nullptr, *clB = nullptr, *nn = nullptr, *keep = nullptr,
+            *merge = nullptr, *sums = nullptr, *totals = nullptr;
+    int32_t* host = nullptr;
+    hipError_t e = hipSuccess;
+    auto cleanup = [&]() {
+        void* ps[] = {box, left, right, parent, clA, clB, nn, keep, merge, sums, totals};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+        if (host) (void)hipHostFree(host);
+    };
+#define TRY(x) do { e = (x); if (e != hipSuccess) { cleanup(); return e; } } while (0)
+    TRY(dalloc(&box, 6 * (size_t)(2 * n - 1)));
+    TRY(dalloc(&left, n)); TRY(dalloc(&right, n)); TRY(dalloc(&parent, 2 * (size_t)n));
+    TRY(dalloc(&clA, n)); TRY(dalloc(&clB, n)); TRY(dalloc(&nn, n));
+    TRY(dalloc(&keep, n)); TRY(dalloc(&merge, n));
+    TRY(dalloc(&sums, 2 * (size_t)nbMax + 2)); TRY(dalloc(&totals, 4));
+    TRY(hipHostMalloc((void**)&host, 4 * sizeof(int32_t), hipHostMallocDefault));
+    k_ploc_init<<<blocks_for(n), kBlock, 0, s>>>(b.primBox, b.valsA, n, box, clA, parent);
+    int m = n, created = 0;
+    for (int round = 0; m > 1; round++) {
+        if (round > n) { cleanup(); return hipErrorUnknown; }  // unreachable: every round merges
+        const int nb = (m + kFlagBlock - 1) / kFlagBlock;
+        k_ploc_nearest<<<(m + kPlocBlock - 1) / kPlocBlock, kPlocBlock, 0, s>>>(clA, m, box, nn);
+        k_ploc_flags<<<nb, kFlagBlock, 0, s>>>(nn, m, keep, merge, sums);
+        k_ploc_scan_sums<<<1, kFlagBlock, 0, s>>>(sums, nb, totals);
+        k_ploc_emit<<<nb, kFlagBlock, 0, s>>>(clA, nn, m, keep, merge, sums, created, n, box, left, right, parent, clB);
+        TRY(hipGetLastError());
+        TRY(hipMemcpyAsync(host, totals, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        TRY(hipStreamSynchronize(s));
+        if (host[1] < 1 || host[0] != m - host[1]) { cleanup(); return hipErrorUnknown; }
+        m = host[0];
+        created += host[1];
+        std::swap(clA, clB);
+    }
+    if (created != n - 1) { cleanup(); return hipErrorUnknown; }
+    // depth in edges down to the deepest leaf
+    TRY(hipMemsetAsync(totals + 2, 0, sizeof(int32_t), s));
+    k_even_depth<<<blocks_for(n - 1), kBlock, 0, s>>>(parent, n, b.flags, totals + 2);
+    TRY(hipGetLastError());
+    TRY(hipMemcpyAsync(host + 2, totals + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    *depthOut = host[2];
+    if (host[2] <= maxDepth) {
+        // adopt the tree: the traversal layout and the parents the 4-wide collapse climbs
+        k_pack_nodes<<<blocks_for(n - 1), kBlock, 0, s>>>(box, left, right, n, b.nodes);
+        TRY(hipMemcpyAsync(b.parent, parent, (size_t)(2 * n - 1) * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        TRY(hipGetLastError());
+        TRY(hipStreamSynchronize(s));
+    }
+#undef TRY
+    cleanup();
+    return hipSuccess;
 }
 
 void bvh_release(BvhBuffers& b) {

@@ -38,7 +38,7 @@ const int kPerm256[256] = {
     181, 199, 106, 157, 184, 84,  204, 176, 115, 121, 50,  45,  127, 4,   150, 254, 138, 236, 205, 93,  222, 114,
     67,  29,  24,  72,  243, 141, 128, 195, 78,  66,  215, 61,  156, 180};
 
-constexpr int kMaxPrims = 1 << 26;  // keeps the Karras depth (and the traversal stack) below 64
+constexpr int kMaxPrims = 1 << 26;  // keeps the Karras depth below 58, inside the traversal stack's bound
 
 struct DeviceScene {
     bool loaded = false;
@@ -53,6 +53,7 @@ struct DeviceScene {
     float* lfunc = nullptr;
     float* lcdf = nullptr;
     BvhBuffers bvh;
+    int treeKind = 0, treeDepth = 0, nodes4 = 0;  // render tree actually used
     SceneView view{};
     float boxMin[3] = {0, 0, 0}, boxInv[3] = {0, 0, 0};  // vertex bounds, for the cache's positions
 };
@@ -67,6 +68,7 @@ struct lts_ctx {
     float* buildSoup = nullptr;
     size_t buildSoupCap = 0;
     int lastMode = 0;
+    int renderTree = LTS_TREE_PLOC;  // tree of the next lts_load_scene
     DeviceScene scene;
     float* sampleL = nullptr;
     size_t sampleCap = 0;
@@ -738,8 +740,19 @@ int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* s
     HIPCHK(c, bvh_reserve(S.bvh, n));
     int rc = run_build(c, S.bvh, S.soup, n, 1, true, nullptr);
     if (rc != LTS_OK) return rc;
-    HIPCHK(c, bvh4_enqueue(S.bvh, s));
+    S.treeKind = LTS_TREE_KARRAS;
+    if (c->renderTree == LTS_TREE_PLOC && n > 1) {
+        int depth = 0;
+        HIPCHK(c, ploc_enqueue(S.bvh, s, kMaxRenderDepth, &depth));
+        if (depth <= kMaxRenderDepth) S.treeKind = LTS_TREE_PLOC;
+    }
+    int depth = 0, nodes4 = 0;
+    HIPCHK(c, bvh4_enqueue(S.bvh, s, &depth, &nodes4));
     HIPCHK(c, hipStreamSynchronize(s));
+    // the Karras tree of at most 2^26 leaves is at most 57 deep; the check guards the stack bound
+    if (depth > kMaxRenderDepth) return fail(c, LTS_E_STATE, "render tree deeper than the traversal stack allows");
+    S.treeDepth = depth;
+    S.nodes4 = nodes4;
     // leaf index of every area light's triangle (inverse of the Morton permutation)
     std::vector<int32_t> perm(n);
     HIPCHK(c, hipMemcpy(perm.data(), S.bvh.valsA, n * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -766,6 +779,22 @@ int lts_scene_info(lts_ctx* c, int32_t* nt, int32_t* nl, int32_t* nm) {
     if (nt) *nt = c->scene.view.nPrim;
     if (nl) *nl = c->scene.view.nLights;
     if (nm) *nm = c->scene.host.nMeshes;
+    return LTS_OK;
+}
+
+int lts_set_render_tree(lts_ctx* c, int32_t kind) {
+    if (!c) return LTS_E_ARG;
+    if (kind != LTS_TREE_KARRAS && kind != LTS_TREE_PLOC) return fail(c, LTS_E_ARG, "unknown render tree kind");
+    c->renderTree = kind;
+    return LTS_OK;
+}
+
+int lts_render_tree_info(lts_ctx* c, int32_t* kind, int32_t* depth, int32_t* nodes4) {
+    if (!c) return LTS_E_ARG;
+    if (!c->scene.loaded) return fail(c, LTS_E_STATE, "no scene loaded");
+    if (kind) *kind = c->scene.treeKind;
+    if (depth) *depth = c->scene.treeDepth;
+    if (nodes4) *nodes4 = c->scene.nodes4;
     return LTS_OK;
 }
 

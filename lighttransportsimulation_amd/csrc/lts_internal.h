@@ -54,8 +54,13 @@ hipError_t bvh_reserve(BvhBuffers& b, int n);
 void bvh_release(BvhBuffers& b);
 // Collapses the packed tree of the last build (mode 1, pack) into the 4-wide render BVH: interior
 // nodes of even depth become 4-wide nodes (numbered in interior order, so the root is node 0) and
-// take the boxes of their children's children. Deterministic.
-hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t stream);
+// take the boxes of their children's children. Deterministic. Waits for the stream and returns
+// the tree's depth (edges down to the deepest leaf) and the 4-wide node count.
+hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t stream, int* depth, int* nodes4);
+// Render tree of a scene: PLOC over the sorted leaves of the last bvh_enqueue (which must have run
+// with pack). When the PLOC tree is at most maxDepth deep it replaces b.nodes and b.parent (the
+// 4-wide collapse then takes it); otherwise the Karras tree stays. *depth = the PLOC tree's depth.
+hipError_t ploc_enqueue(BvhBuffers& b, hipStream_t stream, int maxDepth, int* depth);
 
 struct BvhStageEvents {
     hipEvent_t e[5];  // start, after bounds+morton, after sort, after tree, after pack

@@ -123,4 +123,10 @@ struct SceneView {
     float filter[256];
 };
 
+// Traversal stack entries per lane of the 4-wide traversal. A 4-wide node leaves at most 3 entries
+// on the stack, and a leaf at depth D of the binary tree lies under ceil(D / 2) 4-wide nodes, so a
+// render tree is accepted only when it is at most kMaxRenderDepth deep.
+constexpr int kTraceStack = 128;
+constexpr int kMaxRenderDepth = 2 * (kTraceStack / 3);
+
 }  // namespace lts

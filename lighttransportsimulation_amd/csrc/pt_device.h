@@ -92,7 +92,7 @@ LTS_D bool box_hit(const float* mn, const float* mx, const Ray& r, V3 inv, int n
     return (tMin < r.tMax) && (tMax > 0);
 }
 
-constexpr int kStack = 96;  // 4-wide traversal: up to 3 entries per level of a tree at most 64 deep
+constexpr int kStack = kTraceStack;
 constexpr int kShortStack = 24;  // stack entries kept in LDS; deeper entries go to private memory
 
 // Traversal stack of one lane: entries [0, kShortStack) in LDS (entry e at lds[e * stride], so the
