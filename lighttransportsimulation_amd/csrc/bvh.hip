@@ -569,38 +569,84 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_add(int32_t* __restrict__ v
     if (i < m) v[i] += totals[blockIdx.x];
 }
 
+// Smallest exponent e in [-126, 127] with qdecode(o, e, 255) >= hi.
+__device__ int q_exponent(float o, float hi) {
+    int e = -126;
+    const float ext = hi - o;
+    if (ext > 0) {
+        int ee;
+        (void)frexpf(ext / 255.0f, &ee);  // ext / 255 < 2^ee
+        e = max(-126, min(127, ee));
+    }
+    while (e < 127 && !(qdecode(o, e, 255) >= hi)) e++;
+    while (e > -126 && qdecode(o, e - 1, 255) >= hi) e--;
+    return e;
+}
+
+// Largest q with qdecode(o, e, q) <= lo (q = 0 always qualifies: o is the node's minimum).
+__device__ unsigned q_floor(float o, int e, float lo) {
+    const float d = (lo - o) / __int_as_float((e + 127) << 23);
+    int q = d > 0 ? (int)fminf(floorf(d), 255.0f) : 0;
+    while (q > 0 && qdecode(o, e, q) > lo) q--;
+    while (q < 255 && qdecode(o, e, q + 1) <= lo) q++;
+    return (unsigned)q;
+}
+
+// Smallest q with qdecode(o, e, q) >= hi (q = 255 always qualifies by the choice of e).
+__device__ unsigned q_ceil(float o, int e, float hi) {
+    const float d = (hi - o) / __int_as_float((e + 127) << 23);
+    int q = d > 0 ? (int)fminf(ceilf(d), 255.0f) : 0;
+    while (q < 255 && qdecode(o, e, q) < hi) q++;
+    while (q > 0 && qdecode(o, e, q - 1) >= hi) q--;
+    return (unsigned)q;
+}
+
 __global__ __launch_bounds__(kBlock) void k_build_bvh4(const BvhNode* __restrict__ nodes, const int32_t* __restrict__ flag,
-                                                       const int32_t* __restrict__ index4, int n, Bvh4Node* __restrict__ out) {
+                                                       const int32_t* __restrict__ index4, int n, BvhQNode* __restrict__ out) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n - 1 || !flag[i]) return;
     const BvhNode nd = nodes[i];
-    Bvh4Node o;
+    int ch[4];
+    float mn[4][3], mx[4][3];
     int cnt = 0;
-    auto add = [&](int c, const float* mn, const float* mx) {
-        o.child[cnt] = c >= n - 1 ? ~(c - (n - 1)) : index4[c];
-        o.minx[cnt] = mn[0]; o.miny[cnt] = mn[1]; o.minz[cnt] = mn[2];
-        o.maxx[cnt] = mx[0]; o.maxy[cnt] = mx[1]; o.maxz[cnt] = mx[2];
+    auto add = [&](int c, const float* lo, const float* hi) {
+        ch[cnt] = c >= n - 1 ? ~(c - (n - 1)) : index4[c];
+        for (int a = 0; a < 3; a++) { mn[cnt][a] = lo[a]; mx[cnt][a] = hi[a]; }
         cnt++;
     };
     for (int side = 0; side < 2; side++) {
         const int c = side ? nd.right : nd.left;
-        const float* mn = side ? nd.rmin : nd.lmin;
-        const float* mx = side ? nd.rmax : nd.lmax;
+        const float* lo = side ? nd.rmin : nd.lmin;
+        const float* hi = side ? nd.rmax : nd.lmax;
         if (c >= n - 1) {
-            add(c, mn, mx);
+            add(c, lo, hi);
         } else {
             const BvhNode g = nodes[c];  // odd depth: replaced by its two children
             add(g.left, g.lmin, g.lmax);
             add(g.right, g.rmin, g.rmax);
         }
     }
-    for (int q = cnt; q < 4; q++) {
-        o.child[q] = 0;
-        o.minx[q] = o.miny[q] = o.minz[q] = 0.0f;
-        o.maxx[q] = o.maxy[q] = o.maxz[q] = 0.0f;
+    BvhQNode o;
+    float org[3];
+    int ex[3];
+    for (int a = 0; a < 3; a++) {
+        float lo = mn[0][a], hi = mx[0][a];
+        for (int k = 1; k < cnt; k++) { lo = fminf(lo, mn[k][a]); hi = fmaxf(hi, mx[k][a]); }
+        org[a] = lo;
+        ex[a] = q_exponent(lo, hi);
     }
-    o.count = cnt;
-    o.pad[0] = o.pad[1] = o.pad[2] = 0;
+    o.ox = org[0]; o.oy = org[1]; o.oz = org[2];
+    o.ex = (int8_t)ex[0]; o.ey = (int8_t)ex[1]; o.ez = (int8_t)ex[2];
+    o.count = (uint8_t)cnt;
+    for (int k = 0; k < 4; k++) {
+        const bool valid = k < cnt;
+        o.child[k] = valid ? ch[k] : 0;
+        for (int a = 0; a < 3; a++) {
+            o.qlo[a][k] = valid ? (uint8_t)q_floor(org[a], ex[a], mn[k][a]) : 0;
+            o.qhi[a][k] = valid ? (uint8_t)q_ceil(org[a], ex[a], mx[k][a]) : 0;
+        }
+    }
+    o.pad[0] = o.pad[1] = 0;
     out[index4[i]] = o;
 }
 

@@ -43,7 +43,7 @@ struct BvhBuffers {
     uint32_t* sceneKeys = nullptr;// 6 ordered keys of the scene bounds
     BvhNode* nodes = nullptr;     // n-1 (or 1 for n == 1)
     TriLeaf* tris = nullptr;      // n
-    Bvh4Node* nodes4 = nullptr;   // 4-wide collapse for rendering (allocated by bvh4_enqueue)
+    BvhQNode* nodes4 = nullptr;   // 4-wide collapse for rendering (allocated by bvh4_enqueue)
     int32_t* index4 = nullptr;    // n-1: 4-wide node of each even-depth interior node
     int32_t* scanTotals = nullptr;
     int cap4 = 0;

@@ -29,17 +29,22 @@ struct alignas(16) BvhNode {
     int32_t pad0, pad1;
 };
 
-// Node of the 4-wide render BVH: the interior nodes of even depth of the Karras tree, each holding
-// the boxes of its children's children (a leaf child is kept as is). Boxes are stored by
-// coordinate (minx of the 4 children, then miny, ...); child >= 0 is a 4-wide node, child < 0 the
-// leaf ~child; `count` children are valid. 128 bytes, one cache line.
-struct alignas(16) Bvh4Node {
-    float minx[4], miny[4], minz[4];
-    float maxx[4], maxy[4], maxz[4];
+// Node of the 4-wide render BVH: an interior node of even depth of the binary render tree, holding
+// the boxes of its children's children (a leaf child is kept as is). Child boxes are quantized to
+// 8 bits per plane against the node's own box: coordinate = origin + q * 2^e per axis, with qlo
+// rounded down and qhi rounded up so that the decoded box (computed exactly as the traversal does)
+// contains the child's box. child >= 0 is a 4-wide node, child < 0 the leaf ~child; `count`
+// children are valid. 64 bytes, four 16-byte loads.
+struct alignas(64) BvhQNode {
+    float ox, oy, oz;     // origin: minimum corner of the node's box
+    int8_t ex, ey, ez;    // per-axis scale exponents
+    uint8_t count;
     int32_t child[4];
-    int32_t count;
-    int32_t pad[3];
+    uint8_t qlo[3][4];    // [axis][child]
+    uint8_t qhi[3][4];
+    int32_t pad[2];
 };
+static_assert(sizeof(BvhQNode) == 64, "BvhQNode layout");
 
 struct alignas(16) ShadeTri {
     float n[9];    // vertex normals (valid when hasN)
@@ -105,7 +110,7 @@ struct CameraDesc {
 struct SceneView {
     const TriLeaf* tris;
     const BvhNode* nodes;
-    const Bvh4Node* nodes4;  // 4-wide collapse of `nodes`, root at 0 (used by the traversal)
+    const BvhQNode* nodes4;  // 4-wide collapse of `nodes`, root at 0 (used by the traversal)
     const ShadeTri* shade;
     const MatDesc* mats;
     const TexDesc* texs;

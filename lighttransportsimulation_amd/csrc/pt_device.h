@@ -107,19 +107,25 @@ struct TraceStack {
 // LDS stack of thread `tid` in a block-wide array __shared__ int[kShortStack * blockSize].
 LTS_D TraceStack lds_stack(int* shared, int tid, int blockSize) { return TraceStack{shared + tid, blockSize}; }
 
-// Child boxes of a 4-wide node: hit distance per slot (INFINITY for a miss or an empty slot).
-LTS_D void node4_hits(const Bvh4Node* nodes4, int cur, const Ray& r, V3 inv, int nx, int ny, int nz, float (&t)[4],
+// Child boxes of a 4-wide node: hit distance per slot (INFINITY for a miss or an empty slot). The
+// quantized boxes are decoded with qdecode, as the builder did, and contain the exact child boxes;
+// the slab test is monotone in the planes, so a ray that passes an exact box passes its decoded box.
+LTS_D void node4_hits(const BvhQNode* nodes4, int cur, const Ray& r, V3 inv, int nx, int ny, int nz, float (&t)[4],
                       int (&c)[4], int& count) {
-    const float4* q = reinterpret_cast<const float4*>(nodes4 + cur);
-    const float4 mnx = q[0], mny = q[1], mnz = q[2], mxx = q[3], mxy = q[4], mxz = q[5];
-    const int4 ch = *reinterpret_cast<const int4*>(q + 6);
-    count = reinterpret_cast<const int*>(q + 7)[0];
-    const float lx[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, ly[4] = {mny.x, mny.y, mny.z, mny.w}, lz[4] = {mnz.x, mnz.y, mnz.z, mnz.w};
-    const float hx[4] = {mxx.x, mxx.y, mxx.z, mxx.w}, hy[4] = {mxy.x, mxy.y, mxy.z, mxy.w}, hz[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
-    const int cc[4] = {ch.x, ch.y, ch.z, ch.w};
+    const uint4* q = reinterpret_cast<const uint4*>(nodes4 + cur);
+    const uint4 h = q[0], ch = q[1], lo = q[2], hi = q[3];
+    const float ox = __uint_as_float(h.x), oy = __uint_as_float(h.y), oz = __uint_as_float(h.z);
+    const int ex = (int)(int8_t)(h.w & 0xffu), ey = (int)(int8_t)((h.w >> 8) & 0xffu), ez = (int)(int8_t)((h.w >> 16) & 0xffu);
+    count = (int)(h.w >> 24);
+    // lo = (qlo x, qlo y, qlo z, qhi x), hi = (qhi y, qhi z, pad, pad); byte k belongs to child k
+    const int cc[4] = {(int)ch.x, (int)ch.y, (int)ch.z, (int)ch.w};
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const float mn[3] = {lx[k], ly[k], lz[k]}, mx[3] = {hx[k], hy[k], hz[k]};
+        const unsigned s = 8 * k;
+        const float mn[3] = {qdecode(ox, ex, (lo.x >> s) & 0xffu), qdecode(oy, ey, (lo.y >> s) & 0xffu),
+                             qdecode(oz, ez, (lo.z >> s) & 0xffu)};
+        const float mx[3] = {qdecode(ox, ex, (lo.w >> s) & 0xffu), qdecode(oy, ey, (hi.x >> s) & 0xffu),
+                             qdecode(oz, ez, (hi.y >> s) & 0xffu)};
         float tk;
         const bool hit = k < count && box_hit(mn, mx, r, inv, nx, ny, nz, tk);
         t[k] = hit ? tk : INFINITY;
