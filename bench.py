@@ -390,6 +390,15 @@ def main() -> int:
                 for _ in range(2):
                     ctx.render(**mk)
                 mega = W * H * spp * 2 / (time.perf_counter() - t1) / 1e6
+            tree = None
+            if wavefront:
+                # render tree and its traversal statistics: node fetches and triangle tests of the
+                # path rays of one sample per pixel, counted by the megakernel
+                tree = ctx.render_tree_info()
+                _, _, tst = ctx.render(**dict(kw, spp=1, stratified=0, integrator=INTEGRATOR_MEGAKERNEL, autotune=0,
+                                              profile_phases=0, count_traversal=1))
+                tree["path_ray_node_fetches_per_sample"] = round(tst["node_visits"] / tst["samples"], 3)
+                tree["path_ray_tri_tests_per_sample"] = round(tst["tri_tests"] / tst["samples"], 3)
             qual = quality(ctx, cfg, tuning) if (wavefront and not args.no_quality
                                                   and args.workload == "displaced_1080p") else None
         samples = W * H * spp
@@ -407,6 +416,8 @@ def main() -> int:
             out["config"]["tuning"] = tuning
         if mega:
             out["config"]["megakernel_msamples_per_s"] = round(mega, 3)
+        if tree:
+            out["config"]["render_tree"] = tree
         if graph_rate:
             out["config"]["graph_replay_msamples_per_s"] = round(graph_rate, 3)
             out["config"]["timed_region"] = "passes launched directly, HIP events between the phases"

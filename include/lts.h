@@ -75,7 +75,9 @@ typedef struct lts_render_params {
     int32_t batch_group_pixels; /* lts_render_batch: scenes are grouped, largest first, until a   */
                                 /*    group holds this many pixels; a group shares every launch   */
                                 /*    of its passes. 0: 2^20; -1: all scenes in one group         */
-    int32_t reserved[2];
+    int32_t count_traversal;    /* megakernel only, 1: count the render-node fetches and triangle */
+                                /*    tests of the path rays into stats.node_visits / tri_tests    */
+    int32_t reserved[1];
 } lts_render_params;
 
 /* Wavefront phases timed when profile_phases is set (indices into lts_stats.phase_ms). */
@@ -95,7 +97,7 @@ typedef struct lts_stats {
     int64_t launches;       /* integrator launches                                  */
     int64_t samples;        /* camera samples traced                                */
     double msamples_per_s;  /* samples / (render_ms + resolve_ms)                   */
-    int64_t node_visits;    /* filled only when counting is enabled                 */
+    int64_t node_visits;    /* count_traversal: render-node fetches of the path rays */
     int64_t tri_tests;
     int32_t trace_block;    /* wavefront tuning that was used                      */
     int32_t persistent_blocks;

@@ -53,7 +53,7 @@ class RenderParams(C.Structure):
                 ("stratified", C.c_int32), ("integrator", C.c_int32), ("spp_per_pass", C.c_int32),
                 ("trace_block", C.c_int32), ("persistent_blocks", C.c_int32), ("sort_materials", C.c_int32),
                 ("use_graph", C.c_int32), ("autotune", C.c_int32), ("profile_phases", C.c_int32),
-                ("batch_group_pixels", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("batch_group_pixels", C.c_int32), ("count_traversal", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class NrcConfig(C.Structure):
@@ -254,11 +254,12 @@ class Context:
     def render(self, spp: int, max_bounce: int, rr_start: int, seed: int, stratified: int = 0,
                integrator: int = INTEGRATOR_MEGAKERNEL, spp_per_pass: int = 0, want_image: bool = True,
                want_samples: bool = False, trace_block: int = 0, persistent_blocks: int = 0, sort_materials: int = -1,
-               use_graph: int = -1, autotune: int = 0, profile_phases: int = 0):
+               use_graph: int = -1, autotune: int = 0, profile_phases: int = 0, count_traversal: int = 0):
         """Renders the loaded scene (include/lts.h: lts_render). Returns (rgb, samples, stats): rgb is
         [H][W][3] in getRender row order, samples [spp][H][W][3] per-sample radiance when asked for."""
         p = render_params(spp, max_bounce, rr_start, seed, stratified, integrator, spp_per_pass, trace_block,
-                          persistent_blocks, sort_materials, use_graph, autotune, profile_phases)
+                          persistent_blocks, sort_materials, use_graph, autotune, profile_phases,
+                          count_traversal=count_traversal)
         rgb = np.zeros((self.height, self.width, 3), np.float32) if want_image else None
         smp = np.zeros((spp, self.height, self.width, 3), np.float32) if want_samples else None
         st = Stats()
@@ -268,9 +269,10 @@ class Context:
 
 def render_params(spp, max_bounce, rr_start, seed, stratified=0, integrator=INTEGRATOR_MEGAKERNEL, spp_per_pass=0,
                   trace_block=0, persistent_blocks=0, sort_materials=-1, use_graph=-1, autotune=0,
-                  profile_phases=0, batch_group_pixels=0) -> RenderParams:
+                  profile_phases=0, batch_group_pixels=0, count_traversal=0) -> RenderParams:
     return RenderParams(spp, max_bounce, rr_start, seed, stratified, integrator, spp_per_pass, trace_block,
-                        persistent_blocks, sort_materials, use_graph, autotune, profile_phases, batch_group_pixels)
+                        persistent_blocks, sort_materials, use_graph, autotune, profile_phases, batch_group_pixels,
+                        count_traversal)
 
 
 def _stats_dict(st: Stats) -> dict:

@@ -206,6 +206,8 @@ int check_render_params(lts_ctx* c, const lts_render_params* p) {
     } else if (p->integrator != LTS_INTEGRATOR_MEGAKERNEL) {
         return fail(c, LTS_E_ARG, "unsupported integrator");
     }
+    if (p->count_traversal != 0 && (p->count_traversal != 1 || p->integrator != LTS_INTEGRATOR_MEGAKERNEL))
+        return fail(c, LTS_E_ARG, "count_traversal is a megakernel option (0 or 1)");
     return LTS_OK;
 }
 
@@ -868,7 +870,9 @@ int lts_render(lts_ctx* c, const lts_render_params* p, float* rgb_out, float* sa
     HIPCHK(c, launch_film_clear(c->film, (int)npix, s));
     RenderLaunch rl{};
     rl.spp = p->spp; rl.maxBounce = p->max_bounce; rl.rrStart = p->rr_start; rl.seed = p->seed;
-    rl.stratified = p->stratified; rl.sampleL = c->sampleL; rl.film = c->film; rl.counters = nullptr;
+    rl.stratified = p->stratified; rl.sampleL = c->sampleL; rl.film = c->film;
+    rl.counters = p->count_traversal ? c->counters : nullptr;
+    if (rl.counters) HIPCHK(c, hipMemsetAsync(c->counters, 0, 2 * sizeof(unsigned long long), s));
     double renderMs = 0, resolveMs = 0;
     int64_t launches = 0;
     for (int s0 = 0; s0 < p->spp; s0 += pass) {
@@ -897,6 +901,12 @@ int lts_render(lts_ctx* c, const lts_render_params* p, float* rgb_out, float* sa
         st->kernel_avg_ms = launches ? renderMs / launches : 0;
         st->samples = (int64_t)npix * p->spp;
         st->msamples_per_s = (renderMs + resolveMs) > 0 ? st->samples / ((renderMs + resolveMs) * 1e3) : 0;
+        if (rl.counters) {
+            unsigned long long cnt[2];
+            HIPCHK(c, hipMemcpy(cnt, c->counters, sizeof cnt, hipMemcpyDeviceToHost));
+            st->node_visits = (int64_t)cnt[0];
+            st->tri_tests = (int64_t)cnt[1];
+        }
     }
     return LTS_OK;
 }

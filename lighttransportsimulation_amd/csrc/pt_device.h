@@ -169,7 +169,7 @@ LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned lon
                 float t[4];
                 int c[4], count;
                 node4_hits(sv.nodes4, cur, r, inv, nx, ny, nz, t, c, count);
-                nodes += count;
+                nodes++;
                 cswap(t[0], c[0], t[1], c[1]);
                 cswap(t[2], c[2], t[3], c[3]);
                 cswap(t[0], c[0], t[2], c[2]);

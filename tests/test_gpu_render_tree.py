@@ -52,6 +52,27 @@ def test_ploc_tree_renders_like_the_lbvh(gpu_ctx, scenes):
 
 
 @pytest.mark.gpu
+def test_ploc_tree_fetches_fewer_nodes(gpu_ctx, scenes):
+    """On the displaced sphere the PLOC tree needs clearly fewer node fetches per path ray than the
+    LBVH (counted by the megakernel), and the counted render is still the uncounted one."""
+    d, cfg, w, h = scenes[1]
+    fetches = {}
+    for kind in (TREE_KARRAS, TREE_PLOC):
+        gpu_ctx.set_render_tree(kind)
+        gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], w, h)
+        img, _, st = gpu_ctx.render(4, 8, 3, SEED, count_traversal=1)
+        ref, _, st0 = gpu_ctx.render(4, 8, 3, SEED)
+        assert _identical(img, ref) and st0["node_visits"] == 0
+        assert st["node_visits"] > st["samples"] and st["tri_tests"] > 0
+        fetches[kind] = st["node_visits"] / st["samples"]
+    print(f"  node fetches per sample: karras {fetches[TREE_KARRAS]:.2f}, ploc {fetches[TREE_PLOC]:.2f}", flush=True)
+    assert fetches[TREE_PLOC] < 0.9 * fetches[TREE_KARRAS]
+    gpu_ctx.set_render_tree(TREE_PLOC)
+    with pytest.raises(LtsError):
+        gpu_ctx.render(4, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT, count_traversal=1)
+
+
+@pytest.mark.gpu
 def test_render_tree_rejects_unknown_kind(gpu_ctx):
     with pytest.raises(LtsError):
         gpu_ctx.set_render_tree(7)
