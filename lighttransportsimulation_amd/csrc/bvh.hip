@@ -510,7 +510,7 @@ __global__ __launch_bounds__(kBlock) void k_pack_tris(const float* __restrict__ 
     TriLeaf o;
     o.p0x = t[0]; o.p0y = t[1]; o.p0z = t[2]; o.p1x = t[3];
     o.p1y = t[4]; o.p1z = t[5]; o.p2x = t[6]; o.p2y = t[7];
-    o.p2z = t[8]; o.prim = p; o.pad0 = o.pad1 = 0;
+    o.p2z = t[8]; o.prim = p; o.mat = 0; o.pad1 = 0;
     tris[k] = o;
 }
 

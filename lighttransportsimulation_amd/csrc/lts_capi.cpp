@@ -750,6 +750,7 @@ int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* s
     }
     int depth = 0, nodes4 = 0;
     HIPCHK(c, bvh4_enqueue(S.bvh, s, &depth, &nodes4));
+    HIPCHK(c, launch_tri_materials(S.bvh.tris, S.shade, n, s));
     HIPCHK(c, hipStreamSynchronize(s));
     // the Karras tree of at most 2^26 leaves is at most 57 deep; the check guards the stack bound
     if (depth > kMaxRenderDepth) return fail(c, LTS_E_STATE, "render tree deeper than the traversal stack allows");

@@ -87,6 +87,8 @@ struct RenderLaunch {
 hipError_t launch_megakernel(const SceneView& sv, const RenderLaunch& rl, hipStream_t stream);
 hipError_t launch_film_gather(const SceneView& sv, const RenderLaunch& rl, hipStream_t stream);
 hipError_t launch_film_clear(float* film, int npix, hipStream_t stream);
+// tris[k].mat = shade[tris[k].prim].mat for the n leaves of a loaded scene.
+hipError_t launch_tri_materials(TriLeaf* tris, const ShadeTri* shade, int n, hipStream_t stream);
 hipError_t launch_get_render(const SceneView& sv, const float* film, float* rgb, hipStream_t stream);
 // plain copy of `bytes` (a multiple of 16) with 16-byte vector loads and stores
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
@@ -126,7 +128,8 @@ struct WfSceneSlot {
 };
 // One vertex of a training path: radiance gathered before the vertex's own scattering, throughput
 // and reflectance there; the suffix target is (L_final - Lb) / (beta a).
-struct alignas(16) WfTrainRec { float Lb[3]; float beta[3]; float a[3]; int32_t path; int32_t valid; int32_t pad; };
+// slot: the path's sample index into its scene's sample buffer ((sample - s0) * W * H + pixel)
+struct alignas(16) WfTrainRec { float Lb[3]; float beta[3]; float a[3]; int32_t slot; int32_t valid; int32_t pad; };
 
 struct WfTuning {
     int traceBlock = 128;         // block size of the persistent traversal kernels (64, 128, 256)
@@ -154,6 +157,7 @@ struct WfBuffers {
     WfNee* nee = nullptr;
     int32_t* queue[2] = {nullptr, nullptr};
     int32_t* sorted = nullptr;
+    int32_t* bins = nullptr;      // material bin of each path's hit, written by the extend kernel
     WfShadow* shadows = nullptr;
     WfMis* mis = nullptr;
     int32_t* counts = nullptr;

@@ -19,7 +19,8 @@ struct alignas(16) TriLeaf {
     float p1y, p1z, p2x, p2y;
     float p2z;
     int32_t prim;  // primitive number (global triangle order of the scene)
-    int32_t pad0, pad1;
+    int32_t mat;   // material of the primitive (scenes; 0 after a standalone build)
+    int32_t pad1;
 };
 
 struct alignas(16) BvhNode {

@@ -72,6 +72,11 @@ LTS_D Spec radiance(const SceneView& sv, int px, int py, int sample, const Rende
     return L;
 }
 
+__global__ __launch_bounds__(256) void k_tri_materials(TriLeaf* __restrict__ tris, const ShadeTri* __restrict__ shade, int n) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) tris[k].mat = shade[tris[k].prim].mat;
+}
+
 __global__ __launch_bounds__(256) void k_megakernel(SceneView sv, RenderLaunch rl) {
     __shared__ int stk[kShortStack * 256];
     const int W = sv.cam.width, H = sv.cam.height;
@@ -151,6 +156,12 @@ hipError_t launch_film_gather(const SceneView& sv, const RenderLaunch& rl, hipSt
     const int W = sv.cam.width, H = sv.cam.height;
     dim3 grid((W + 15) / 16, (H + 15) / 16);
     k_film_gather<<<grid, 256, 0, stream>>>(sv, rl);
+    return hipGetLastError();
+}
+
+hipError_t launch_tri_materials(TriLeaf* tris, const ShadeTri* shade, int n, hipStream_t s) {
+    if (n < 1) return hipSuccess;
+    k_tri_materials<<<(n + 255) / 256, 256, 0, s>>>(tris, shade, n);
     return hipGetLastError();
 }
 

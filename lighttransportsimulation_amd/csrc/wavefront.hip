@@ -22,6 +22,15 @@ namespace {
 
 constexpr int kShadeBlock = 256;
 
+// Pixel of the j-th path of a sample: the image is cut into bands of 8 rows and each band is walked
+// column by column, so 64 consecutive paths (one wave) cover an 8x8 tile and their rays stay close.
+LTS_D int band_pixel(int j, int W, int H) {
+    const int band = j / (8 * W);
+    const int rows = min(8, H - 8 * band);
+    const int jj = j - band * 8 * W;
+    return (8 * band + jj % rows) * W + jj / rows;
+}
+
 // Path i of the pass -> (scene, pixel, sample).
 LTS_D void path_origin(const WfParams& P, const WfSceneSlot* slots, long i, int& scene, int& pixel, int& sample) {
     int s = 0;
@@ -31,7 +40,7 @@ LTS_D void path_origin(const WfParams& P, const WfSceneSlot* slots, long i, int&
     const long npix = (long)sl.width * sl.height;
     scene = s;
     sample = sl.s0 + (int)(local / npix);
-    pixel = (int)(local % npix);
+    pixel = band_pixel((int)(local % npix), sl.width, sl.height);
 }
 
 __global__ __launch_bounds__(256) void k_wf_generate(const SceneView* __restrict__ scenes, const WfParams* __restrict__ prm,
@@ -78,13 +87,21 @@ LTS_D int wave_fetch(int32_t* counter) {
     return __shfl(base, 0) + (threadIdx.x & 63);
 }
 
+// With `hist` set, the kernel also files each hit under its material bin (misses in the last bin)
+// for the counting sort: bins[path] and a block histogram added to hist at the end.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_wf_extend(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
                                                   const WfRay* __restrict__ rays, WfHit* __restrict__ hits,
                                                   const int32_t* __restrict__ queue, const int32_t* __restrict__ count,
-                                                  int32_t* __restrict__ work) {
+                                                  int32_t* __restrict__ work, int32_t* __restrict__ bins,
+                                                  int32_t* __restrict__ hist, int nBins) {
     __shared__ int stk[kShortStack * BS];
+    __shared__ int32_t lh[kWfMaxMatBins];
     const TraceStack ts = lds_stack(stk, threadIdx.x, BS);
+    if (hist) {
+        for (int b = threadIdx.x; b < kWfMaxMatBins; b += BS) lh[b] = 0;
+        __syncthreads();
+    }
     const int n = *count;
     for (;;) {
         const int k = wave_fetch(work);
@@ -96,35 +113,27 @@ __global__ __launch_bounds__(BS) void k_wf_extend(const SceneView* __restrict__ 
         r.o = v3(rr.o[0], rr.o[1], rr.o[2]);
         r.d = v3(rr.d[0], rr.d[1], rr.d[2]);
         r.tMax = INFINITY;
-        Hit h = trace_closest(scenes[paths[pi].scene], r, ts, nullptr, nullptr);
+        const SceneView& sv = scenes[paths[pi].scene];
+        Hit h = trace_closest(sv, r, ts, nullptr, nullptr);
         WfHit o;
         o.leaf = h.leaf; o.u = h.u; o.v = h.v; o.invDet = h.invDet;
         hits[pi] = o;
+        if (hist) {
+            const int bin = h.leaf < 0 ? nBins - 1 : min(sv.tris[h.leaf].mat, nBins - 2);
+            bins[pi] = bin;
+            atomicAdd(&lh[bin], 1);
+        }
+    }
+    if (hist) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < nBins; b += BS)
+            if (lh[b]) atomicAdd(&hist[b], lh[b]);
     }
 }
 
-// Counting sort of a queue by the material of each path's hit (misses in the last bin). Each block
-// counts in LDS and reserves its range of every bin with one atomic per bin.
-LTS_D int material_bin(const SceneView* scenes, const WfPath* paths, const WfHit* hits, int pi, int nBins) {
-    const WfHit h = hits[pi];
-    if (h.leaf < 0) return nBins - 1;
-    const SceneView& sv = scenes[paths[pi].scene];
-    return min(sv.shade[sv.tris[h.leaf].prim].mat, nBins - 2);
-}
-
-__global__ __launch_bounds__(256) void k_wf_mat_hist(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
-                                                     const WfHit* __restrict__ hits, const int32_t* __restrict__ queue,
-                                                     const int32_t* __restrict__ count, int32_t* __restrict__ hist, int nBins) {
-    __shared__ int32_t h[kWfMaxMatBins];
-    if (threadIdx.x < kWfMaxMatBins) h[threadIdx.x] = 0;
-    __syncthreads();
-    const int n = *count;
-    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256)
-        atomicAdd(&h[material_bin(scenes, paths, hits, queue[k], nBins)], 1);
-    __syncthreads();
-    if (threadIdx.x < nBins && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
-}
-
+// Counting sort of a queue by the material bin the extend kernel filed each hit under: one block
+// turns the histogram into bin offsets, then each block ranks its entries per bin in LDS and
+// reserves its range of every bin with one atomic per bin.
 __global__ void k_wf_mat_scan(int32_t* __restrict__ hist, int nBins) {
     if (threadIdx.x == 0) {
         int run = 0;
@@ -132,8 +141,7 @@ __global__ void k_wf_mat_scan(int32_t* __restrict__ hist, int nBins) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_wf_mat_scatter(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
-                                                        const WfHit* __restrict__ hits, const int32_t* __restrict__ queue,
+__global__ __launch_bounds__(256) void k_wf_mat_scatter(const int32_t* __restrict__ bins, const int32_t* __restrict__ queue,
                                                         const int32_t* __restrict__ count, int32_t* __restrict__ offs, int nBins,
                                                         int32_t* __restrict__ sorted) {
     __shared__ int32_t h[kWfMaxMatBins], base[kWfMaxMatBins];
@@ -147,7 +155,7 @@ __global__ __launch_bounds__(256) void k_wf_mat_scatter(const SceneView* __restr
         int pi = -1, bin = 0, slot = 0;
         if (k < n) {
             pi = queue[k];
-            bin = material_bin(scenes, paths, hits, pi, nBins);
+            bin = bins[pi];
             slot = atomicAdd(&h[bin], 1);
         }
         __syncthreads();
@@ -258,6 +266,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
                     differentials(si, camera_sample(sv, px, py, p.sample, sl.spp, sl.stratified, sl.seed));
                 }
                 Bsdf bsdf = make_bsdf(sv, si);
+                const int slot = (p.sample - sl.s0) * sl.width * sl.height + p.pixel;  // sample index of the path
                 if (cached && (bsdf.kind == 0 || bsdf.kind == 1 || bsdf.kind == 4)) {
                     if (p.trainSlot >= 0) {
                         // training vertex: radiance so far and throughput, for the suffix target
@@ -267,7 +276,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
                         tr.Lb[0] = L.r; tr.Lb[1] = L.g; tr.Lb[2] = L.b;
                         tr.beta[0] = beta.r; tr.beta[1] = beta.g; tr.beta[2] = beta.b;
                         tr.a[0] = bsdf.R.r; tr.a[1] = bsdf.R.g; tr.a[2] = bsdf.R.b;
-                        tr.path = pi; tr.valid = 1; tr.pad = 0;
+                        tr.slot = slot; tr.valid = 1; tr.pad = 0;
                         tRec[r] = tr;
                     } else if (bounce >= queryBounce) {
                         // the cache stands for all radiance scattered here
@@ -275,7 +284,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
                         NrcQuery q;
                         q.beta[0] = beta.r; q.beta[1] = beta.g; q.beta[2] = beta.b;
                         q.a[0] = bsdf.R.r; q.a[1] = bsdf.R.g; q.a[2] = bsdf.R.b;
-                        q.slot = pi; q.pad = 0;
+                        q.slot = slot; q.pad = 0;
                         qMeta[pi] = q;
                         wantQuery = true;
                         alive = false;
@@ -446,7 +455,7 @@ __global__ __launch_bounds__(kCollectBlock) void k_wf_nrc_scatter(const WfParams
         const size_t r = train_record(i, maxBounce);
         const WfTrainRec t = tRec[r];
         if (!t.valid) continue;
-        const float* Lf = sampleBase + 3 * (size_t)t.path;
+        const float* Lf = sampleBase + 3 * (size_t)t.slot;
         for (int ch = 0; ch < 3; ch++) {
             const float suffix = t.beta[ch] > 0.0f ? (Lf[ch] - t.Lb[ch]) / t.beta[ch] : 0.0f;
             outTarget[3 * (size_t)pos + ch] = t.a[ch] > 0.0f ? suffix / t.a[ch] : 0.0f;
@@ -582,18 +591,20 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
         int32_t* cnt = b.counts + kWfCountsPerBounce * bounce;
         int32_t* nxt = b.counts + kWfCountsPerBounce * (bounce + 1);
         mark(8 * bounce + 0);
+        int32_t* hist = t.sortMaterials ? b.counts + kWfCountsPerBounce * (maxBounce + 1) + kWfMaxMatBins * bounce : nullptr;
         switch (t.traceBlock) {
-            case 64: k_wf_extend<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3); break;
-            case 128: k_wf_extend<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3); break;
-            default: k_wf_extend<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3); break;
+            case 64: k_wf_extend<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3,
+                                                                      b.bins, hist, b.matBins); break;
+            case 128: k_wf_extend<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3,
+                                                                         b.bins, hist, b.matBins); break;
+            default: k_wf_extend<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3,
+                                                                        b.bins, hist, b.matBins); break;
         }
         mark(8 * bounce + 1);
         const int32_t* shadeQueue = b.queue[q];
         if (t.sortMaterials) {
-            int32_t* hist = b.counts + kWfCountsPerBounce * (maxBounce + 1) + kWfMaxMatBins * bounce;
-            k_wf_mat_hist<<<gridAll, 256, 0, s>>>(b.scenes, b.paths, b.hits, b.queue[q], cnt, hist, b.matBins);
             k_wf_mat_scan<<<1, 64, 0, s>>>(hist, b.matBins);
-            k_wf_mat_scatter<<<gridAll, 256, 0, s>>>(b.scenes, b.paths, b.hits, b.queue[q], cnt, hist, b.matBins, b.sorted);
+            k_wf_mat_scatter<<<gridAll, 256, 0, s>>>(b.bins, b.queue[q], cnt, hist, b.matBins, b.sorted);
             shadeQueue = b.sorted;
         }
         mark(8 * bounce + 2);
@@ -659,6 +670,7 @@ hipError_t wf_reserve(WfBuffers& b, size_t cap, int maxScenes) {
     WTRY(wf_alloc(&b.queue[0], cap));
     WTRY(wf_alloc(&b.queue[1], cap));
     WTRY(wf_alloc(&b.sorted, cap));
+    WTRY(wf_alloc(&b.bins, cap));
     WTRY(wf_alloc(&b.shadows, cap));
     WTRY(wf_alloc(&b.mis, cap));
     WTRY(wf_alloc(&b.counts, (size_t)kWfGlobalBase + kWfGlobalCounts));
@@ -695,7 +707,7 @@ void wf_release(WfBuffers& b) {
     void* cs[] = {b.qRaw, b.qMeta, b.qQueue, b.tRec, b.tRaw, b.tCount};
     for (void* p : cs)
         if (p) (void)hipFree(p);
-    void* ps[] = {b.scenes, b.params, b.slots, b.paths, b.rays, b.hits, b.nee, b.queue[0], b.queue[1], b.sorted, b.shadows, b.mis, b.counts};
+    void* ps[] = {b.scenes, b.params, b.slots, b.paths, b.rays, b.hits, b.nee, b.queue[0], b.queue[1], b.sorted, b.bins, b.shadows, b.mis, b.counts};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     b = WfBuffers();
