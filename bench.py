@@ -172,6 +172,26 @@ def relmse_at(curve, t):
     return pts[-1][2] * pts[-1][1] / t
 
 
+TRAFFIC_PROFILE = os.path.join("profiles", "traffic_wavefront_displaced_1080p.md")
+
+
+def measured_traffic():
+    """HBM bytes per frame of the traversal kernels, from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE, calibrated with the copy kernel); None when it is absent."""
+    import re
+    try:
+        text = open(os.path.join(ROOT, TRAFFIC_PROFILE)).read()
+    except OSError:
+        return None
+    m = re.search(r"^traversal_hbm_bytes_per_frame: ([0-9.eE+]+)$", text, re.M)
+    return float(m.group(1)) if m else None
+
+
+def progress(msg: str):
+    """A step of the run finished (stderr; stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def quality(ctx, cfg, tuning, spps=(16, 32, 64, 128), ref_spp=4096) -> dict:
     """Equal-time relMSE of the cached and the plain wavefront tracer against a plain reference."""
     from lighttransportsimulation_amd import INTEGRATOR_CACHED, INTEGRATOR_WAVEFRONT
@@ -180,6 +200,7 @@ def quality(ctx, cfg, tuning, spps=(16, 32, 64, 128), ref_spp=4096) -> dict:
     t0 = time.perf_counter()
     ref, _, _ = ctx.render(ref_spp, seed=cfg["seed"] + 1000, integrator=INTEGRATOR_WAVEFRONT, **base, **tune)
     ref_s = time.perf_counter() - t0
+    progress(f"quality: {ref_spp}-spp reference rendered in {ref_s:.1f} s")
 
     def timed(spp, integrator, **kw):
         t = time.perf_counter()
@@ -190,6 +211,7 @@ def quality(ctx, cfg, tuning, spps=(16, 32, 64, 128), ref_spp=4096) -> dict:
     for spp in spps:
         img, ms, _ = timed(spp, INTEGRATOR_WAVEFRONT)
         plain.append((spp, round(ms, 2), relmse(img, ref)))
+    progress("quality: plain curve done")
     out = {"relmse": "mean over pixels and channels of (x - r)^2 / (r^2 + 0.01)",
            "reference": f"plain wavefront, {ref_spp} spp, independent seed ({ref_s:.1f} s)",
            "time": "wall ms of the render call, fresh cache per point (training and warm-up included)",
@@ -205,6 +227,7 @@ def quality(ctx, cfg, tuning, spps=(16, 32, 64, 128), ref_spp=4096) -> dict:
             curve.append((spp, round(ms, 2), relmse(img, ref)))
             if qb == 2 and spp == nominal:
                 nrc_perf = st
+        progress(f"quality: cached curve, query at bounce {qb}, done")
         t_reach = time_to_reach(curve, p_nom[2])
         out["cached"][f"query_bounce_{qb}"] = {
             "curve": [list(c) for c in curve],
@@ -242,12 +265,14 @@ def run_batch(args) -> dict:
         t0 = time.perf_counter()
         cfgs = scenegen.gen_batch(d, n_scenes=n, spp=w["spp"], bounces=w["bounces"])
         gen_s = time.perf_counter() - t0
+        progress(f"batch: {n} scenes generated in {gen_s:.1f} s")
         ctxs = [Context(0) for _ in cfgs]
         try:
             t0 = time.perf_counter()
             for c, cfg in zip(ctxs, cfgs):
                 c.load_scene(d, cfg["scene"], cfg["subject"], cfg["width"], cfg["height"])
             load_s = time.perf_counter() - t0
+            progress(f"batch: scenes loaded and built in {load_s:.1f} s")
             tris = [c.scene_info()["tris"] for c in ctxs]
             kw = dict(spp=w["spp"], max_bounce=w["bounces"], rr_start=w["rr"], seed=0x5EED)
             for _ in range(args.warmup):
@@ -259,11 +284,13 @@ def run_batch(args) -> dict:
                 dev += st["render_ms"] + st["resolve_ms"]
             wall = time.perf_counter() - t0
             groups = st["batch_groups"]
+            progress(f"batch: {args.steps} timed batches, {wall / args.steps * 1e3:.1f} ms each")
             # every scene in one group: one pass sequence for all 64
             render_batch(ctxs, want_images=False, group_pixels=-1, **kw)
             t1 = time.perf_counter()
             render_batch(ctxs, want_images=False, group_pixels=-1, **kw)
             one_group_ms = (time.perf_counter() - t1) * 1e3
+            progress("batch: one-group render done")
             # the same scenes one at a time (each in its own context, warmed up once)
             single_ms = []
             for c in ctxs:
@@ -353,6 +380,7 @@ def main() -> int:
                       integrator=INTEGRATOR_WAVEFRONT if wavefront else INTEGRATOR_MEGAKERNEL,
                       autotune=int(wavefront and args.autotune), profile_phases=int(wavefront))
             tuning = None
+            progress(f"scene loaded: {info['tris']} triangles")
             for _ in range(max(args.warmup, 1 if wavefront and args.autotune else 0)):
                 _, _, st = ctx.render(**kw)
                 if wavefront:
@@ -369,6 +397,7 @@ def main() -> int:
                 for k, v in st["phase_ms"].items():
                     phase[k] = phase.get(k, 0.0) + v
             wall = time.perf_counter() - t0
+            progress(f"{args.steps} timed frames: {wall / args.steps * 1e3:.1f} ms per frame")
             if wavefront and tuning is None:
                 tuning = {k: st[k] for k in ("trace_block", "persistent_blocks", "sort_materials", "use_graph")}
             graph_rate = None
@@ -424,7 +453,9 @@ def main() -> int:
         if phase:
             out["config"]["phase_ms_per_step"] = {k: round(v / args.steps, 3) for k, v in phase.items()}
         if not args.no_cpu:
+            progress("cpu baseline starts")
             cb = cpu_baseline(d, cfg, args.cpu_budget)
+            progress("cpu baseline done")
             bytes_per_sample = 32.0 * cb["node_visits_per_sample"] + 36.0 * cb["tri_tests_per_sample"]
             if wavefront:
                 # traversal kernels (closest hit of path rays, any hit of shadow rays, closest hit of
@@ -438,10 +469,13 @@ def main() -> int:
                 achieved = bytes_per_sample * (samples * args.steps / launches) / launch_s / 1e9
                 kernel = "k_megakernel"
                 kernel_ms = kern_ms / launches
+            traffic = measured_traffic() if (wavefront and args.workload == "displaced_1080p") else None
             out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kernel,
+                               "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kernel,
                                "kernel_ms": round(kernel_ms, 3),
                                "algorithmic_bytes_per_sample": round(bytes_per_sample, 1)}
+            if traffic is not None:
+                out["roofline"]["traffic_source"] = TRAFFIC_PROFILE + " (HBM bytes per frame of the same kernels)"
             out["cpu_baseline"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cb.items()
                                    if k in ("value", "unit", "cores", "kind", "sample")}
         if qual:

@@ -65,7 +65,7 @@ typedef struct lts_render_params {
     /* wavefront only (0 / -1 = default or the autotuned value) */
     int32_t trace_block;        /* block size of the persistent traversal kernels: 64, 128, 256 */
     int32_t persistent_blocks;  /* grid of the persistent traversal kernels                     */
-    int32_t sort_materials;     /* -1 default, 0 off, 1 counting-sort shading work by material  */
+    int32_t sort_materials;     /* -1 default (on), 0 off, 1 counting-sort shading by material  */
     int32_t use_graph;          /* -1 default (on), 0 off: capture each pass in a hipGraph       */
     int32_t autotune;           /* 1: time candidate tunings on this scene before rendering;    */
                                 /*    the choice is kept per scene, film and pass size in the   */

@@ -135,7 +135,7 @@ struct WfTuning {
     int traceBlock = 128;         // block size of the persistent traversal kernels (64, 128, 256)
     int persistentBlocks = 2048;  // grid of the traversal kernels
     int sweepBlocks = 2048;       // grid of the generate / shade / finalize sweeps
-    int sortMaterials = 0;        // counting-sort the shade queue by material
+    int sortMaterials = 1;        // counting-sort the shade queue by material
     int useGraph = 1;             // capture the pass in a hipGraph and replay it
 };
 
