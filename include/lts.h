@@ -221,6 +221,13 @@ int lts_device_copy_benchmark(lts_ctx* ctx, uint64_t bytes, int32_t repeat, doub
 /* Replaces writeToPPM / writeToPFM (core/ImageO.cuh:16-78). format 0: P3 PPM, 1: PFM. */
 int lts_write_image(const char* path, const float* rgb, int32_t width, int32_t height, int32_t format);
 
+/* Replaces parseMeshFile + binMeshWrite (core/FileIO.cuh:68-158, 20-45): converts
+ * <mesh_stem>.obj (v, vt, vn and triangular "f p/uv/n" statements) to <mesh_stem>.bin, the mesh
+ * format lts_load_scene reads. lts_load_scene does the same by itself for a mesh that has only an
+ * .obj, as the reference's parseMesh (core/FileIO.cuh:212-231) does. *n_tris (may be NULL)
+ * receives the triangle count. LTS_E_IO for a missing, malformed or unwritable file. No GPU use. */
+int lts_obj_to_bin(const char* mesh_stem, int32_t* n_tris);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,7 @@ EXPORTED_SYMBOLS = (
     "lts_scene_info", "lts_bvh_build", "lts_bvh_build_device", "lts_bvh_export_last", "lts_render",
     "lts_write_image", "lts_device_copy_benchmark", "lts_render_batch", "lts_nrc_configure", "lts_nrc_set_weights",
     "lts_nrc_get_weights", "lts_nrc_infer", "lts_nrc_train_step", "lts_set_render_tree", "lts_render_tree_info",
+    "lts_obj_to_bin",
 )
 
 TREE_KARRAS = 0
@@ -101,6 +102,7 @@ def load_library():
     L.lts_render.argtypes = [vp, C.POINTER(RenderParams), vp, vp, C.POINTER(Stats)]
     L.lts_render_batch.argtypes = [C.POINTER(vp), i32, C.POINTER(RenderParams), C.POINTER(vp), C.POINTER(Stats)]
     L.lts_write_image.argtypes = [C.c_char_p, vp, i32, i32, i32]
+    L.lts_obj_to_bin.argtypes = [C.c_char_p, C.POINTER(i32)]
     L.lts_device_copy_benchmark.argtypes = [vp, C.c_uint64, i32, C.POINTER(C.c_double)]
     L.lts_nrc_configure.argtypes = [vp, C.POINTER(NrcConfig)]
     L.lts_nrc_set_weights.argtypes = [vp, vp]
@@ -315,6 +317,16 @@ def read_app_params(path: str) -> dict:
     w, h, spp, b, rr, fmt = (x.value for x in v)
     return dict(width=w, height=h, spp=spp, max_bounce=b, rr_start=rr, format=fmt, output=bufs[0].value.decode(),
                 scene=bufs[1].value.decode(), subject=bufs[2].value.decode())
+
+
+def obj_to_bin(mesh_stem: str) -> int:
+    """Converts <mesh_stem>.obj to <mesh_stem>.bin (include/lts.h: lts_obj_to_bin); returns the
+    triangle count."""
+    n = C.c_int32()
+    rc = load_library().lts_obj_to_bin(mesh_stem.encode(), C.byref(n))
+    if rc != 0:
+        raise LtsError(f"lts_obj_to_bin({mesh_stem}) failed ({rc})")
+    return n.value
 
 
 def write_image(path: str, rgb: np.ndarray, fmt: int) -> None:

@@ -1064,4 +1064,16 @@ int lts_write_image(const char* path, const float* rgb, int32_t w, int32_t h, in
     return ok ? LTS_OK : LTS_E_IO;
 }
 
+int lts_obj_to_bin(const char* stem, int32_t* nTris) {
+    if (!stem) return LTS_E_ARG;
+    std::vector<char> bin;
+    std::string err;
+    int n = 0;
+    if (!obj_mesh_bytes(stem, bin, &n, err)) return LTS_E_IO;
+    std::ofstream out(std::string(stem) + ".bin", std::ios::binary);
+    if (!out || !out.write(bin.data(), (std::streamsize)bin.size())) return LTS_E_IO;
+    if (nTris) *nTris = n;
+    return LTS_OK;
+}
+
 }  // extern "C"

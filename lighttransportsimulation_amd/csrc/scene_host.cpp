@@ -12,6 +12,7 @@
 // (the reference indexes lights by per-mesh triangle number), and missing materials are errors.
 #include "scene_host.h"
 
+#include <charconv>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -92,9 +93,16 @@ struct MeshData {
     std::vector<int32_t> idx;
 };
 
+// parseMesh (core/FileIO.cuh:212-231): <stem>.bin when it exists, else <stem>.obj, which is
+// converted and written to <stem>.bin for the next load (when the directory is writable).
 bool read_mesh(const std::string& stem, const Xf& xf, MeshData& m, std::string& err) {
     std::vector<char> b;
-    if (!read_all(stem + ".bin", b)) { err = "missing mesh " + stem + ".bin"; return false; }
+    if (!read_all(stem + ".bin", b)) {
+        int nTri = 0;
+        if (!obj_mesh_bytes(stem, b, &nTri, err)) return false;
+        std::ofstream out(stem + ".bin", std::ios::binary);
+        if (out) out.write(b.data(), (std::streamsize)b.size());
+    }
     if (b.size() < 16) { err = "truncated mesh " + stem; return false; }
     int32_t h[4];
     std::memcpy(h, b.data(), 16);
@@ -122,6 +130,100 @@ bool read_mesh(const std::string& stem, const Xf& xf, MeshData& m, std::string& 
     }
     return true;
 }
+
+}  // namespace
+
+// The token grammar of parseMeshFile: "v" takes three floats, "vt" two, "vn" three, "f" exactly
+// three vertex tokens "p/uv/n" (1-based, split at '/', integers read with std::from_chars); every
+// other token is skipped, so comments and statements such as "o", "s" or "usemtl" pass unless they
+// contain one of the four keywords, exactly as in the reference. The uv index is 0 while no "vt"
+// has been read. When the uv or normal field does not parse (e.g. "3//3" after a "vt"),
+// std::from_chars leaves the integer unchanged and the reference stores the field parsed before it
+// again; that is kept. Fixed instead of undefined: a missing file, a vertex token with fewer than
+// three fields, a point field or a float that does not parse, and an index outside its array are
+// errors (the reference writes an empty mesh, reads past the token, reads an uninitialised
+// integer, throws from stof or reads out of bounds).
+bool obj_mesh_bytes(const std::string& stem, std::vector<char>& bin, int* nTriOut, std::string& err) {
+    std::ifstream obj(stem + ".obj");
+    if (!obj) { err = "missing mesh " + stem + ".bin or " + stem + ".obj"; return false; }
+    std::vector<float> P, UV, N;
+    std::vector<int32_t> idx;
+    int nTri = 0;
+    std::string tok;
+    auto num = [&](float& v) {
+        std::string t;
+        if (!(obj >> t)) return false;
+        try {
+            v = std::stof(t);
+        } catch (const std::exception&) {
+            return false;
+        }
+        return true;
+    };
+    while (obj >> tok) {
+        if (tok == "v" || tok == "vn") {
+            float a, b, c;
+            if (!num(a) || !num(b) || !num(c)) { err = "bad " + tok + " in " + stem + ".obj"; return false; }
+            std::vector<float>& dst = tok == "v" ? P : N;
+            dst.push_back(a); dst.push_back(b); dst.push_back(c);
+        } else if (tok == "vt") {
+            float a, b;
+            if (!num(a) || !num(b)) { err = "bad vt in " + stem + ".obj"; return false; }
+            UV.push_back(a); UV.push_back(b);
+        } else if (tok == "f") {
+            for (int v = 0; v < 3; v++) {
+                std::string t;
+                if (!(obj >> t)) { err = "truncated face in " + stem + ".obj"; return false; }
+                std::string_view f[3];
+                size_t start = 0;
+                int nf = 0;
+                for (; nf < 3; nf++) {
+                    const size_t end = t.find('/', start);
+                    f[nf] = std::string_view(t).substr(start, end == std::string::npos ? std::string::npos : end - start);
+                    if (end == std::string::npos) { nf++; break; }
+                    start = end + 1;
+                }
+                if (nf < 3) { err = "face vertex " + t + " without p/uv/n fields in " + stem + ".obj"; return false; }
+                int i = 0;
+                if (std::from_chars(f[0].data(), f[0].data() + f[0].size(), i).ec != std::errc()) {
+                    err = "bad face vertex " + t + " in " + stem + ".obj";
+                    return false;
+                }
+                idx.push_back(i - 1);
+                if (!UV.empty()) {
+                    (void)std::from_chars(f[1].data(), f[1].data() + f[1].size(), i);  // unchanged on failure
+                    idx.push_back(i - 1);
+                } else {
+                    idx.push_back(0);
+                }
+                (void)std::from_chars(f[2].data(), f[2].data() + f[2].size(), i);  // unchanged on failure
+                idx.push_back(i - 1);
+            }
+            nTri++;
+        }
+    }
+    const int32_t h[4] = {nTri, (int32_t)(P.size() / 3), (int32_t)(UV.size() / 2), (int32_t)(N.size() / 3)};
+    for (size_t k = 0; k < idx.size(); k++) {
+        const int which = (int)(k % 3), lim = which == 0 ? h[1] : which == 1 ? h[2] : h[3];
+        if (which == 1 && lim == 0) continue;
+        if (idx[k] < 0 || idx[k] >= lim) { err = "index out of range in " + stem + ".obj"; return false; }
+    }
+    bin.resize(16 + 4 * (P.size() + UV.size() + N.size() + idx.size()));
+    char* at = bin.data();
+    auto put = [&](const void* src, size_t bytes) {
+        if (bytes) std::memcpy(at, src, bytes);
+        at += bytes;
+    };
+    put(h, 16);
+    put(P.data(), 4 * P.size());
+    put(UV.data(), 4 * UV.size());
+    put(N.data(), 4 * N.size());
+    put(idx.data(), 4 * idx.size());
+    *nTriOut = nTri;
+    return true;
+}
+
+namespace {
 
 // BlockedArray slot of (u, v) with the reference's row-block index (MemoryHelper.cuh:88-94).
 inline int64_t blocked_slot(int ublocks, int u, int v) {

@@ -29,6 +29,11 @@ struct HostScene {
 bool load_scene_files(const std::string& dir, const std::string& scene, const std::string& subject, int width,
                       int height, HostScene& out, std::string& err);
 
+// OBJ import (core/FileIO.cuh:68-158, parseMeshFile with binMeshWrite): parses <stem>.obj and
+// returns the bytes of the equivalent <stem>.bin (header nTri, nP, nUV, nN; points, uvs, normals;
+// 9 indices per triangle). Returns false with `err` set on failure; *nTri = triangles read.
+bool obj_mesh_bytes(const std::string& stem, std::vector<char>& bin, int* nTri, std::string& err);
+
 // Gaussian filter table, radius (1, 1), alpha 1 (rendering/Filter.cuh:32-50, :82-99;
 // LightTransportSimulation.cu:64).
 void gaussian_filter_table(float* out256);
