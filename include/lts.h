@@ -77,8 +77,25 @@ typedef struct lts_render_params {
                                 /*    of its passes. 0: 2^20; -1: all scenes in one group         */
     int32_t count_traversal;    /* megakernel only, 1: count the render-node fetches and triangle */
                                 /*    tests of the path rays into stats.node_visits / tri_tests    */
+    int32_t sample_begin;       /* progressive rendering: samples [sample_begin, sample_end) of  */
+    int32_t sample_end;         /*    the spp-sample image (sample_end 0: spp). sample_begin > 0  */
+                                /*    keeps the film: the previous render of this context (or    */
+                                /*    lts_film_load) must have ended at sample_begin with the same */
+                                /*    scene, spp, seed, stratification, bounces, roulette and      */
+                                /*    integrator. The image then equals one render of             */
+                                /*    [0, sample_end) bit for bit. lts_render_batch: full range.  */
     int32_t reserved[1];
 } lts_render_params;
+
+/* Film checkpoint (no reference counterpart: the reference renders all samples in one launch).
+ * lts_film_save writes the context's film (per-pixel rgb sums and filter weights), the next sample
+ * index and the sampler parameters of the renders that filled it; lts_film_load restores them into
+ * a context whose loaded scene has the same film size, so a render with sample_begin = next sample
+ * continues it. lts_film_resolve applies getRender (rendering/Camera.cu:256-289) to the current
+ * film: rgb_out receives width * height * 3 floats, top row first. */
+int lts_film_save(lts_ctx* ctx, const char* path);
+int lts_film_load(lts_ctx* ctx, const char* path);
+int lts_film_resolve(lts_ctx* ctx, float* rgb_out);
 
 /* Wavefront phases timed when profile_phases is set (indices into lts_stats.phase_ms). */
 #define LTS_PHASE_EXTEND 0      /* closest-hit traversal of path rays                        */
@@ -218,7 +235,10 @@ int lts_nrc_train_step(lts_ctx* ctx, const float* raw, const float* target, int3
  * the achievable copy bandwidth next to the kernels' roofline figures. */
 int lts_device_copy_benchmark(lts_ctx* ctx, uint64_t bytes, int32_t repeat, double* ms_per_copy);
 
-/* Replaces writeToPPM / writeToPFM (core/ImageO.cuh:16-78). format 0: P3 PPM, 1: PFM. */
+/* Replaces writeToPPM / writeToPFM (core/ImageO.cuh:16-78). format 0: P3 PPM, 1: PFM (the
+ * reference's two formats); 2: Radiance RGBE .hdr, the format of the reference's unused outputFile
+ * (core/ImageO.cuh:80-92); 3: 8-bit PNG with the PPM tone map int(pow(clamp(x), 1/2.2) * 255 + 0.5).
+ * rgb rows are top first, as lts_render returns them. */
 int lts_write_image(const char* path, const float* rgb, int32_t width, int32_t height, int32_t format);
 
 /* Replaces parseMeshFile + binMeshWrite (core/FileIO.cuh:68-158, 20-45): converts

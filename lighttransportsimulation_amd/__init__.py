@@ -24,7 +24,7 @@ EXPORTED_SYMBOLS = (
     "lts_scene_info", "lts_bvh_build", "lts_bvh_build_device", "lts_bvh_export_last", "lts_render",
     "lts_write_image", "lts_device_copy_benchmark", "lts_render_batch", "lts_nrc_configure", "lts_nrc_set_weights",
     "lts_nrc_get_weights", "lts_nrc_infer", "lts_nrc_train_step", "lts_set_render_tree", "lts_render_tree_info",
-    "lts_obj_to_bin",
+    "lts_obj_to_bin", "lts_film_save", "lts_film_load", "lts_film_resolve",
 )
 
 TREE_KARRAS = 0
@@ -54,7 +54,8 @@ class RenderParams(C.Structure):
                 ("stratified", C.c_int32), ("integrator", C.c_int32), ("spp_per_pass", C.c_int32),
                 ("trace_block", C.c_int32), ("persistent_blocks", C.c_int32), ("sort_materials", C.c_int32),
                 ("use_graph", C.c_int32), ("autotune", C.c_int32), ("profile_phases", C.c_int32),
-                ("batch_group_pixels", C.c_int32), ("count_traversal", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("batch_group_pixels", C.c_int32), ("count_traversal", C.c_int32), ("sample_begin", C.c_int32),
+                ("sample_end", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class NrcConfig(C.Structure):
@@ -103,6 +104,9 @@ def load_library():
     L.lts_render_batch.argtypes = [C.POINTER(vp), i32, C.POINTER(RenderParams), C.POINTER(vp), C.POINTER(Stats)]
     L.lts_write_image.argtypes = [C.c_char_p, vp, i32, i32, i32]
     L.lts_obj_to_bin.argtypes = [C.c_char_p, C.POINTER(i32)]
+    L.lts_film_save.argtypes = [vp, C.c_char_p]
+    L.lts_film_load.argtypes = [vp, C.c_char_p]
+    L.lts_film_resolve.argtypes = [vp, vp]
     L.lts_device_copy_benchmark.argtypes = [vp, C.c_uint64, i32, C.POINTER(C.c_double)]
     L.lts_nrc_configure.argtypes = [vp, C.POINTER(NrcConfig)]
     L.lts_nrc_set_weights.argtypes = [vp, vp]
@@ -172,6 +176,19 @@ class Context:
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
         self._check(self.lib.lts_scene_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "lts_scene_info")
         return dict(tris=a.value, lights=b.value, meshes=c.value)
+
+    # ------------------------------------------------------------------ film checkpoint
+    def film_save(self, path: str):
+        """Writes the accumulated film and its sample count (include/lts.h: lts_film_save)."""
+        self._check(self.lib.lts_film_save(self.h, path.encode()), "lts_film_save")
+
+    def film_load(self, path: str):
+        self._check(self.lib.lts_film_load(self.h, path.encode()), "lts_film_load")
+
+    def film_resolve(self) -> np.ndarray:
+        rgb = np.zeros((self.height, self.width, 3), np.float32)
+        self._check(self.lib.lts_film_resolve(self.h, _ptr(rgb)), "lts_film_resolve")
+        return rgb
 
     def set_render_tree(self, kind: int):
         """Render tree of the scenes loaded from now on: TREE_PLOC (default) or TREE_KARRAS."""
@@ -256,14 +273,16 @@ class Context:
     def render(self, spp: int, max_bounce: int, rr_start: int, seed: int, stratified: int = 0,
                integrator: int = INTEGRATOR_MEGAKERNEL, spp_per_pass: int = 0, want_image: bool = True,
                want_samples: bool = False, trace_block: int = 0, persistent_blocks: int = 0, sort_materials: int = -1,
-               use_graph: int = -1, autotune: int = 0, profile_phases: int = 0, count_traversal: int = 0):
+               use_graph: int = -1, autotune: int = 0, profile_phases: int = 0, count_traversal: int = 0,
+               sample_begin: int = 0, sample_end: int = 0):
         """Renders the loaded scene (include/lts.h: lts_render). Returns (rgb, samples, stats): rgb is
         [H][W][3] in getRender row order, samples [spp][H][W][3] per-sample radiance when asked for."""
         p = render_params(spp, max_bounce, rr_start, seed, stratified, integrator, spp_per_pass, trace_block,
                           persistent_blocks, sort_materials, use_graph, autotune, profile_phases,
-                          count_traversal=count_traversal)
+                          count_traversal=count_traversal, sample_begin=sample_begin, sample_end=sample_end)
         rgb = np.zeros((self.height, self.width, 3), np.float32) if want_image else None
-        smp = np.zeros((spp, self.height, self.width, 3), np.float32) if want_samples else None
+        n_smp = (sample_end or spp) - sample_begin
+        smp = np.zeros((n_smp, self.height, self.width, 3), np.float32) if want_samples else None
         st = Stats()
         self._check(self.lib.lts_render(self.h, C.byref(p), _ptr(rgb), _ptr(smp), C.byref(st)), "lts_render")
         return rgb, smp, _stats_dict(st)
@@ -271,10 +290,11 @@ class Context:
 
 def render_params(spp, max_bounce, rr_start, seed, stratified=0, integrator=INTEGRATOR_MEGAKERNEL, spp_per_pass=0,
                   trace_block=0, persistent_blocks=0, sort_materials=-1, use_graph=-1, autotune=0,
-                  profile_phases=0, batch_group_pixels=0, count_traversal=0) -> RenderParams:
+                  profile_phases=0, batch_group_pixels=0, count_traversal=0, sample_begin=0,
+                  sample_end=0) -> RenderParams:
     return RenderParams(spp, max_bounce, rr_start, seed, stratified, integrator, spp_per_pass, trace_block,
                         persistent_blocks, sort_materials, use_graph, autotune, profile_phases, batch_group_pixels,
-                        count_traversal)
+                        count_traversal, sample_begin, sample_end)
 
 
 def _stats_dict(st: Stats) -> dict:

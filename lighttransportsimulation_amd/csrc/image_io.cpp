@@ -155,4 +155,76 @@ bool write_ppm(const std::string& path, const float* rgb, int w, int h) {
     return ok;
 }
 
+// Shared-exponent encoding of the Radiance picture format (G. Ward, "Real Pixels", Graphics Gems
+// II, 1991): e = the exponent of the largest channel as frexp gives it, each channel stored as
+// floor(c * 256 / 2^e), and e + 128; values below 1e-32 (and negative ones) encode as zero.
+bool write_hdr(const std::string& path, const float* rgb, int w, int h) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    bool ok = std::fprintf(f, "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n", h, w) > 0;
+    std::vector<unsigned char> row(4 * (size_t)w);
+    for (int y = 0; ok && y < h; y++) {
+        for (int x = 0; x < w; x++) {
+            const float* c = rgb + 3 * ((size_t)y * w + x);
+            const float r = std::fmax(c[0], 0.0f), g = std::fmax(c[1], 0.0f), b = std::fmax(c[2], 0.0f);
+            const float m = std::fmax(r, std::fmax(g, b));
+            unsigned char* o = &row[4 * (size_t)x];
+            if (!(m >= 1e-32f)) {
+                o[0] = o[1] = o[2] = o[3] = 0;
+                continue;
+            }
+            int e;
+            const float scale = (float)std::frexp(m, &e) * 256.0f / m;
+            o[0] = (unsigned char)(r * scale);
+            o[1] = (unsigned char)(g * scale);
+            o[2] = (unsigned char)(b * scale);
+            o[3] = (unsigned char)(e + 128);
+        }
+        ok = std::fwrite(row.data(), 1, row.size(), f) == row.size();
+    }
+    std::fclose(f);
+    return ok;
+}
+
+// PNG (ISO/IEC 15948): signature, IHDR (8-bit RGB, no interlace), one zlib IDAT stream of rows
+// with filter byte 0, IEND; chunk CRCs from zlib's crc32.
+bool write_png(const std::string& path, const float* rgb, int w, int h) {
+    std::vector<unsigned char> raw((size_t)h * (1 + 3 * (size_t)w));
+    for (int y = 0; y < h; y++) {
+        unsigned char* r = &raw[(size_t)y * (1 + 3 * (size_t)w)];
+        r[0] = 0;
+        for (int i = 0; i < 3 * w; i++) r[1 + i] = (unsigned char)to_byte(rgb[3 * (size_t)y * w + i]);
+    }
+    uLongf zlen = compressBound((uLong)raw.size());
+    std::vector<unsigned char> z(zlen);
+    if (compress2(z.data(), &zlen, raw.data(), (uLong)raw.size(), 6) != Z_OK) return false;
+    z.resize(zlen);
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    auto be32 = [](unsigned char* p, uint32_t v) { p[0] = v >> 24; p[1] = v >> 16; p[2] = v >> 8; p[3] = v; };
+    bool ok = true;
+    auto chunk = [&](const char* type, const unsigned char* data, size_t n) {
+        unsigned char head[8];
+        be32(head, (uint32_t)n);
+        std::memcpy(head + 4, type, 4);
+        uLong crc = crc32(0L, head + 4, 4);
+        if (n) crc = crc32(crc, data, (uInt)n);
+        unsigned char tail[4];
+        be32(tail, (uint32_t)crc);
+        ok = ok && std::fwrite(head, 1, 8, f) == 8 && (n == 0 || std::fwrite(data, 1, n, f) == n) &&
+             std::fwrite(tail, 1, 4, f) == 4;
+    };
+    static const unsigned char sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    ok = std::fwrite(sig, 1, 8, f) == 8;
+    unsigned char ihdr[13];
+    be32(ihdr, (uint32_t)w);
+    be32(ihdr + 4, (uint32_t)h);
+    ihdr[8] = 8; ihdr[9] = 2; ihdr[10] = 0; ihdr[11] = 0; ihdr[12] = 0;
+    chunk("IHDR", ihdr, 13);
+    chunk("IDAT", z.data(), z.size());
+    chunk("IEND", nullptr, 0);
+    std::fclose(f);
+    return ok;
+}
+
 }  // namespace lts

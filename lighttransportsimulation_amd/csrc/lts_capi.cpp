@@ -70,6 +70,11 @@ struct lts_ctx {
     int lastMode = 0;
     int renderTree = LTS_TREE_PLOC;  // tree of the next lts_load_scene
     DeviceScene scene;
+    // progressive film: the sampler parameters of the renders accumulated in `film`, and the
+    // sample the next render must start at to continue it
+    int32_t filmKey[9] = {};  // width, height, spp, seed, stratified, max_bounce, rr_start, integrator, scene load
+    int32_t filmNext = 0;
+    int32_t sceneLoads = 0;   // lts_load_scene calls that succeeded
     float* sampleL = nullptr;
     size_t sampleCap = 0;
     uint64_t sampleGen = 0;   // allocation generation of sampleL
@@ -208,7 +213,38 @@ int check_render_params(lts_ctx* c, const lts_render_params* p) {
     }
     if (p->count_traversal != 0 && (p->count_traversal != 1 || p->integrator != LTS_INTEGRATOR_MEGAKERNEL))
         return fail(c, LTS_E_ARG, "count_traversal is a megakernel option (0 or 1)");
+    const int end = p->sample_end ? p->sample_end : p->spp;
+    if (p->sample_begin < 0 || end <= p->sample_begin || end > p->spp)
+        return fail(c, LTS_E_ARG, "sample range outside [0, spp)");
     return LTS_OK;
+}
+
+// Samples [*b0, *b1) of the render.
+void sample_range(const lts_render_params* p, int* b0, int* b1) {
+    *b0 = p->sample_begin;
+    *b1 = p->sample_end ? p->sample_end : p->spp;
+}
+
+void film_key(const lts_ctx* c, const lts_render_params* p, int32_t* k) {
+    const CameraDesc& cam = c->scene.view.cam;
+    const int32_t v[9] = {cam.width, cam.height, p->spp, (int32_t)p->seed, p->stratified, p->max_bounce, p->rr_start,
+                          p->integrator, c->sceneLoads};
+    std::memcpy(k, v, sizeof v);
+}
+
+// A render of [b0, ...) with b0 > 0 must continue the film this context holds.
+int film_continues(lts_ctx* c, const lts_render_params* p, int b0) {
+    if (b0 == 0) return LTS_OK;
+    int32_t k[9];
+    film_key(c, p, k);
+    if (c->filmNext != b0 || std::memcmp(k, c->filmKey, sizeof k) != 0)
+        return fail(c, LTS_E_STATE, "progressive render does not continue the film (sample_begin or parameters differ)");
+    return LTS_OK;
+}
+
+void film_advance(lts_ctx* c, const lts_render_params* p, int b1) {
+    film_key(c, p, c->filmKey);
+    c->filmNext = b1;
 }
 
 int ensure_film(lts_ctx* c, size_t npix) {
@@ -413,9 +449,13 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
     if (cached && scenes.size() != 1) return fail(c, LTS_E_ARG, "the cached integrator renders one scene");
     int rc = LTS_OK;
     if (cached && (rc = nrc_ensure(c)) != LTS_OK) return rc;
+    int b0, b1;
+    sample_range(p, &b0, &b1);
+    if (b0 > 0 && scenes.size() != 1) return fail(c, LTS_E_ARG, "progressive rendering is for one scene");
+    if (b0 > 0 && (rc = film_continues(c, p, b0)) != LTS_OK) return rc;
     int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, ((size_t)1 << 24) / totalPix);
-    if (sample_rgb) pass = p->spp;
-    pass = std::min(pass, p->spp);
+    if (sample_rgb) pass = b1 - b0;
+    pass = std::min(pass, b1 - b0);
     const size_t cap = totalPix * (size_t)pass;
     if (cap > (size_t)INT32_MAX) return fail(c, LTS_E_ARG, "pass holds more than 2^31 paths");
     const bool realloc = cap > c->wf.cap || (int)scenes.size() > c->wf.maxScenes;
@@ -438,8 +478,9 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
             return fail(c, rc, x->err);
     }
     HIPCHK(c, hipMemcpyAsync(c->wf.scenes, views.data(), views.size() * sizeof(SceneView), hipMemcpyHostToDevice, s));
-    for (lts_ctx* x : scenes)
-        HIPCHK(c, launch_film_clear(x->film, x->scene.view.cam.width * x->scene.view.cam.height, s));
+    if (b0 == 0)
+        for (lts_ctx* x : scenes)
+            HIPCHK(c, launch_film_clear(x->film, x->scene.view.cam.width * x->scene.view.cam.height, s));
 
     WfTuning t;
     double tuneMs = 0;
@@ -476,8 +517,8 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         for (hipEvent_t& e : c->prof->ev) HIPCHK(c, hipEventCreate(&e));
     }
     const WfProfile* prof = p->profile_phases ? c->prof : nullptr;
-    for (int s0 = 0; s0 < p->spp; s0 += pass) {
-        const int s1 = std::min(p->spp, s0 + pass);
+    for (int s0 = b0; s0 < b1; s0 += pass) {
+        const int s1 = std::min(b1, s0 + pass);
         PassCache pc;
         if (cached) {
             // training paths: every stride-th path of the pass from an offset that moves from pass to pass
@@ -524,6 +565,7 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
     for (size_t i = 0; i < scenes.size(); i++) {
         lts_ctx* x = scenes[i];
         HIPCHK(c, launch_get_render(views[i], x->film, x->rgb, s));
+        film_advance(x, p, b1);
         if (rgb_out && rgb_out[i])
             HIPCHK(c, hipMemcpyAsync(rgb_out[i], x->rgb, (size_t)views[i].cam.width * views[i].cam.height * 3 * sizeof(float),
                                      hipMemcpyDeviceToHost, s));
@@ -535,7 +577,7 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         st->resolve_ms = resolveMs;
         st->launches = launches;
         st->kernel_avg_ms = launches ? renderMs / launches : 0;
-        st->samples = (int64_t)totalPix * p->spp;
+        st->samples = (int64_t)totalPix * (b1 - b0);
         st->msamples_per_s = (renderMs + resolveMs) > 0 ? st->samples / ((renderMs + resolveMs) * 1e3) : 0;
         st->trace_block = t.traceBlock;
         st->persistent_blocks = t.persistentBlocks;
@@ -546,7 +588,7 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         if (prof && p->max_bounce > 0) {
             const int64_t perBounce = launches * p->max_bounce;
             st->phase_launches[LTS_PHASE_EXTEND] = perBounce;
-            st->phase_launches[LTS_PHASE_SORT] = t.sortMaterials ? 3 * perBounce : 0;
+            st->phase_launches[LTS_PHASE_SORT] = t.sortMaterials ? 2 * perBounce : 0;
             st->phase_launches[LTS_PHASE_SHADE] = perBounce;
             st->phase_launches[LTS_PHASE_SHADOW] = perBounce;
             st->phase_launches[LTS_PHASE_MIS] = perBounce;
@@ -773,6 +815,8 @@ int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* s
     v.cam = H.cam;
     std::memcpy(v.filter, H.filter, sizeof(v.filter));
     S.loaded = true;
+    c->sceneLoads++;
+    c->filmNext = 0;
     return LTS_OK;
 }
 
@@ -862,13 +906,16 @@ int lts_render(lts_ctx* c, const lts_render_params* p, float* rgb_out, float* sa
     const SceneView& sv = c->scene.view;
     const int W = sv.cam.width, H = sv.cam.height;
     const size_t npix = (size_t)W * H;
-    int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, std::min<size_t>((size_t)p->spp, (size_t)(1u << 28) / (npix * 12)));
-    if (sample_rgb) pass = p->spp;  // all samples are returned: one pass
-    pass = std::min(pass, p->spp);
+    int b0, b1;
+    sample_range(p, &b0, &b1);
+    if ((rc = film_continues(c, p, b0)) != LTS_OK) return rc;
+    int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, std::min<size_t>((size_t)(b1 - b0), (size_t)(1u << 28) / (npix * 12)));
+    if (sample_rgb) pass = b1 - b0;  // all samples are returned: one pass
+    pass = std::min(pass, b1 - b0);
     if ((rc = ensure_samples(c, npix * (size_t)pass * 3)) != LTS_OK) return rc;
     if ((rc = ensure_film(c, npix)) != LTS_OK) return rc;
     hipStream_t s = c->stream;
-    HIPCHK(c, launch_film_clear(c->film, (int)npix, s));
+    if (b0 == 0) HIPCHK(c, launch_film_clear(c->film, (int)npix, s));
     RenderLaunch rl{};
     rl.spp = p->spp; rl.maxBounce = p->max_bounce; rl.rrStart = p->rr_start; rl.seed = p->seed;
     rl.stratified = p->stratified; rl.sampleL = c->sampleL; rl.film = c->film;
@@ -876,9 +923,9 @@ int lts_render(lts_ctx* c, const lts_render_params* p, float* rgb_out, float* sa
     if (rl.counters) HIPCHK(c, hipMemsetAsync(c->counters, 0, 2 * sizeof(unsigned long long), s));
     double renderMs = 0, resolveMs = 0;
     int64_t launches = 0;
-    for (int s0 = 0; s0 < p->spp; s0 += pass) {
+    for (int s0 = b0; s0 < b1; s0 += pass) {
         rl.s0 = s0;
-        rl.s1 = std::min(p->spp, s0 + pass);
+        rl.s1 = std::min(b1, s0 + pass);
         HIPCHK(c, hipEventRecord(c->ev[5], s));
         HIPCHK(c, launch_megakernel(sv, rl, s));
         HIPCHK(c, hipEventRecord(c->ev[6], s));
@@ -894,13 +941,14 @@ int lts_render(lts_ctx* c, const lts_render_params* p, float* rgb_out, float* sa
     HIPCHK(c, launch_get_render(sv, c->film, c->rgb, s));
     if (rgb_out) HIPCHK(c, hipMemcpyAsync(rgb_out, c->rgb, npix * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
+    film_advance(c, p, b1);
     if (st) {
         std::memset(st, 0, sizeof(*st));
         st->render_ms = renderMs;
         st->resolve_ms = resolveMs;
         st->launches = launches;
         st->kernel_avg_ms = launches ? renderMs / launches : 0;
-        st->samples = (int64_t)npix * p->spp;
+        st->samples = (int64_t)npix * (b1 - b0);
         st->msamples_per_s = (renderMs + resolveMs) > 0 ? st->samples / ((renderMs + resolveMs) * 1e3) : 0;
         if (rl.counters) {
             unsigned long long cnt[2];
@@ -927,6 +975,8 @@ int lts_render_batch(lts_ctx* const* ctxs, int32_t n, const lts_render_params* p
     }
     int rc = check_render_params(c, p);
     if (rc != LTS_OK) return rc;
+    if (p->sample_begin != 0 || (p->sample_end != 0 && p->sample_end != p->spp))
+        return fail(c, LTS_E_ARG, "batched rendering renders every sample");
     HIPCHK(c, hipSetDevice(c->device));
     for (lts_ctx* x : scenes) HIPCHK(c, hipStreamSynchronize(x->stream));
     // groups: scenes sorted by pixel count (largest first, stable), a group closes once it holds
@@ -1058,9 +1108,69 @@ int lts_device_copy_benchmark(lts_ctx* c, uint64_t bytes, int32_t repeat, double
     return LTS_OK;
 }
 
+// Film checkpoint file: "LTSFILM1", int32 width, height, next sample, spp, seed, stratified,
+// max_bounce, rr_start, integrator, then width * height * 4 floats (rgb sums, filter weight).
+int lts_film_save(lts_ctx* c, const char* path) {
+    if (!c || !path) return LTS_E_ARG;
+    if (!c->scene.loaded || c->filmNext == 0 || !c->film) return fail(c, LTS_E_STATE, "no rendered film to save");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t npix = (size_t)c->filmKey[0] * c->filmKey[1];
+    std::vector<float> film(npix * 4);
+    HIPCHK(c, hipMemcpy(film.data(), c->film, film.size() * sizeof(float), hipMemcpyDeviceToHost));
+    std::ofstream f(path, std::ios::binary);
+    const int32_t head[9] = {c->filmKey[0], c->filmKey[1], c->filmNext, c->filmKey[2], c->filmKey[3],
+                             c->filmKey[4], c->filmKey[5], c->filmKey[6], c->filmKey[7]};
+    f.write("LTSFILM1", 8);
+    f.write(reinterpret_cast<const char*>(head), sizeof head);
+    f.write(reinterpret_cast<const char*>(film.data()), (std::streamsize)(film.size() * sizeof(float)));
+    if (!f) return fail(c, LTS_E_IO, std::string("cannot write ") + path);
+    return LTS_OK;
+}
+
+int lts_film_load(lts_ctx* c, const char* path) {
+    if (!c || !path) return LTS_E_ARG;
+    if (!c->scene.loaded) return fail(c, LTS_E_STATE, "no scene loaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::ifstream f(path, std::ios::binary);
+    char magic[8];
+    int32_t head[9];
+    if (!f.read(magic, 8) || std::memcmp(magic, "LTSFILM1", 8) != 0 || !f.read(reinterpret_cast<char*>(head), sizeof head))
+        return fail(c, LTS_E_IO, std::string("not a film checkpoint: ") + path);
+    const CameraDesc& cam = c->scene.view.cam;
+    if (head[0] != cam.width || head[1] != cam.height) return fail(c, LTS_E_ARG, "checkpoint film size differs from the scene's");
+    if (head[2] < 1 || head[2] > head[3]) return fail(c, LTS_E_IO, "bad checkpoint sample count");
+    const size_t npix = (size_t)head[0] * head[1];
+    std::vector<float> film(npix * 4);
+    if (!f.read(reinterpret_cast<char*>(film.data()), (std::streamsize)(film.size() * sizeof(float))))
+        return fail(c, LTS_E_IO, std::string("truncated film checkpoint: ") + path);
+    int rc = ensure_film(c, npix);
+    if (rc != LTS_OK) return rc;
+    HIPCHK(c, hipMemcpy(c->film, film.data(), film.size() * sizeof(float), hipMemcpyHostToDevice));
+    const int32_t key[9] = {head[0], head[1], head[3], head[4], head[5], head[6], head[7], head[8], c->sceneLoads};
+    std::memcpy(c->filmKey, key, sizeof key);
+    c->filmNext = head[2];
+    return LTS_OK;
+}
+
+int lts_film_resolve(lts_ctx* c, float* rgb_out) {
+    if (!c || !rgb_out) return LTS_E_ARG;
+    if (!c->scene.loaded || c->filmNext == 0 || !c->film) return fail(c, LTS_E_STATE, "no film to resolve");
+    HIPCHK(c, hipSetDevice(c->device));
+    const SceneView& sv = c->scene.view;
+    const size_t npix = (size_t)sv.cam.width * sv.cam.height;
+    HIPCHK(c, launch_get_render(sv, c->film, c->rgb, c->stream));
+    HIPCHK(c, hipMemcpyAsync(rgb_out, c->rgb, npix * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LTS_OK;
+}
+
 int lts_write_image(const char* path, const float* rgb, int32_t w, int32_t h, int32_t format) {
     if (!path || !rgb || w < 1 || h < 1) return LTS_E_ARG;
-    bool ok = format == 1 ? write_pfm(path, rgb, w, h) : write_ppm(path, rgb, w, h);
+    if (format < 0 || format > 3) return LTS_E_ARG;
+    const bool ok = format == 0   ? write_ppm(path, rgb, w, h)
+                    : format == 1 ? write_pfm(path, rgb, w, h)
+                    : format == 2 ? write_hdr(path, rgb, w, h)
+                                  : write_png(path, rgb, w, h);
     return ok ? LTS_OK : LTS_E_IO;
 }
 
