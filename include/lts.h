@@ -158,6 +158,18 @@ int lts_scene_info(lts_ctx* ctx, int32_t* n_tris, int32_t* n_lights, int32_t* n_
 #define LTS_TREE_KARRAS 0
 #define LTS_TREE_PLOC 1
 int lts_set_render_tree(lts_ctx* ctx, int32_t kind);
+/* Texture pipeline of the scenes this context loads from now on. LTS_TEXTURE_REFERENCE (default)
+ * reproduces the reference's image textures: its blocked texel layout with the aliasing row index
+ * (core/MemoryHelper.cuh:88-96), pyramid levels filled only near the origin
+ * (Mipmap_kernel.cu:29-30) and the fixed lookup width 0.01 (ImageTexture.cuh:8). LTS_TEXTURE_FILTERED
+ * is the pipeline done right (no reference counterpart): a full box-filtered pyramid down to 1x1,
+ * and trilinear lookups whose width comes from the ray differentials of the primary hit
+ * (Interaction.cuh:118-160, PBRT's isotropic width); secondary hits, which have no differentials,
+ * read the finest level. */
+#define LTS_TEXTURE_REFERENCE 0
+#define LTS_TEXTURE_FILTERED 1
+int lts_set_texture_filter(lts_ctx* ctx, int32_t mode);
+
 /* Tree the loaded scene renders with, its depth in edges and its 4-wide node count. */
 int lts_render_tree_info(lts_ctx* ctx, int32_t* kind, int32_t* depth, int32_t* nodes4);
 

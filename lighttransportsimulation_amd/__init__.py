@@ -24,8 +24,11 @@ EXPORTED_SYMBOLS = (
     "lts_scene_info", "lts_bvh_build", "lts_bvh_build_device", "lts_bvh_export_last", "lts_render",
     "lts_write_image", "lts_device_copy_benchmark", "lts_render_batch", "lts_nrc_configure", "lts_nrc_set_weights",
     "lts_nrc_get_weights", "lts_nrc_infer", "lts_nrc_train_step", "lts_set_render_tree", "lts_render_tree_info",
-    "lts_obj_to_bin", "lts_film_save", "lts_film_load", "lts_film_resolve",
+    "lts_obj_to_bin", "lts_film_save", "lts_film_load", "lts_film_resolve", "lts_set_texture_filter",
 )
+
+TEXTURE_REFERENCE = 0
+TEXTURE_FILTERED = 1
 
 TREE_KARRAS = 0
 TREE_PLOC = 1
@@ -96,6 +99,7 @@ def load_library():
     L.lts_load_scene.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, i32, i32]
     L.lts_scene_info.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
     L.lts_set_render_tree.argtypes = [vp, i32]
+    L.lts_set_texture_filter.argtypes = [vp, i32]
     L.lts_render_tree_info.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
     L.lts_bvh_build.argtypes = [vp, vp, i32, i32, C.POINTER(BvhExport), C.POINTER(BvhTiming)]
     L.lts_bvh_build_device.argtypes = [vp, vp, i32, i32, i32, C.POINTER(BvhTiming)]
@@ -189,6 +193,10 @@ class Context:
         rgb = np.zeros((self.height, self.width, 3), np.float32)
         self._check(self.lib.lts_film_resolve(self.h, _ptr(rgb)), "lts_film_resolve")
         return rgb
+
+    def set_texture_filter(self, mode: int):
+        """Texture pipeline of the scenes loaded from now on: TEXTURE_REFERENCE (default) or TEXTURE_FILTERED."""
+        self._check(self.lib.lts_set_texture_filter(self.h, mode), "lts_set_texture_filter")
 
     def set_render_tree(self, kind: int):
         """Render tree of the scenes loaded from now on: TREE_PLOC (default) or TREE_KARRAS."""

@@ -69,6 +69,7 @@ struct lts_ctx {
     size_t buildSoupCap = 0;
     int lastMode = 0;
     int renderTree = LTS_TREE_PLOC;  // tree of the next lts_load_scene
+    int textureFilter = LTS_TEXTURE_REFERENCE;  // texture pipeline of the next lts_load_scene
     DeviceScene scene;
     // progressive film: the sampler parameters of the renders accumulated in `film`, and the
     // sample the next render must start at to continue it
@@ -762,7 +763,8 @@ int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* s
     std::string err;
     std::string d(dir);
     if (!d.empty() && d.back() != '/') d += '/';
-    if (!load_scene_files(d, scene, subject ? subject : "", width, height, S.host, err)) return fail(c, LTS_E_IO, err);
+    if (!load_scene_files(d, scene, subject ? subject : "", width, height, S.host, err, c->textureFilter == LTS_TEXTURE_FILTERED))
+        return fail(c, LTS_E_IO, err);
     const HostScene& H = S.host;
     const int n = (int)H.shade.size();
     for (int a = 0; a < 3; a++) {
@@ -833,6 +835,13 @@ int lts_set_render_tree(lts_ctx* c, int32_t kind) {
     if (!c) return LTS_E_ARG;
     if (kind != LTS_TREE_KARRAS && kind != LTS_TREE_PLOC) return fail(c, LTS_E_ARG, "unknown render tree kind");
     c->renderTree = kind;
+    return LTS_OK;
+}
+
+int lts_set_texture_filter(lts_ctx* c, int32_t mode) {
+    if (!c) return LTS_E_ARG;
+    if (mode != LTS_TEXTURE_REFERENCE && mode != LTS_TEXTURE_FILTERED) return fail(c, LTS_E_ARG, "unknown texture pipeline");
+    c->textureFilter = mode;
     return LTS_OK;
 }
 

@@ -86,7 +86,7 @@ struct MipDesc {
     int32_t wrap;    // 0 repeat, 1 black, 2 clamp
     int32_t levels;
     float lookupLevel;  // (levels - 1) + log2(max(width, 1e-8)) for the fixed width 0.01
-    int32_t pad;
+    int32_t filtered;   // 1: full box pyramid, row-major texels, width from the ray differentials
     MipLevelDesc lv[kMaxMipLevels];
 };
 

@@ -20,9 +20,10 @@ struct Hit {
 struct SurfacePoint {
     V3 p, pError, wo, n;
     V2 uv;
-    V3 dpdu;
+    V3 dpdu, dpdv;
     V3 sn, sdpdu;
     V3 dpdx, dpdy;
+    float dudx, dvdx, dudy, dvdy;  // uv differentials (primary hits; read by filtered textures)
     int prim;
 };
 
@@ -276,6 +277,8 @@ LTS_D SurfacePoint make_surface(const SceneView& sv, int leaf, const Ray& r, flo
     s.sdpdu = dpdu;
     s.prim = prim;
     s.dpdx = s.dpdy = v3(0, 0, 0);
+    s.dpdv = dpdv;
+    s.dudx = s.dvdx = s.dudy = s.dvdy = 0.0f;
     if (st.hasN) {
         V3 n0 = v3(st.n[0], st.n[1], st.n[2]), n1 = v3(st.n[3], st.n[4], st.n[5]), n2 = v3(st.n[6], st.n[7], st.n[8]);
         V3 ns = normalize(n0 * w + n1 * u + n2 * v);
@@ -622,6 +625,14 @@ LTS_D Spec marble_tex(const int* P, const TexDesc& X, V3 p, V3 dpdx, V3 dpdy) {
 // Mipmap texel / triangle / lookUp (materials/textures/Mipmap.cuh:14-72)
 LTS_D Spec mip_texel(const SceneView& sv, const MipDesc& m, int s, int t, int l) {
     const MipLevelDesc& L = m.lv[l];
+    if (m.filtered) {
+        // row-major texels; repeat wraps to [0, res), clamp clamps, black is zero outside
+        if (m.wrap == 0) { s = ((s % L.ures) + L.ures) % L.ures; t = ((t % L.vres) + L.vres) % L.vres; }
+        else if (m.wrap == 2) { s = min(max(s, 0), L.ures - 1); t = min(max(t, 0), L.vres - 1); }
+        else if (s < 0 || s >= L.ures || t < 0 || t >= L.vres) return spec(0.0f);
+        const float* px = sv.mipData + 3 * (L.offset + (long long)t * L.ures + s);
+        return spec(px[0], px[1], px[2]);
+    }
     if (m.wrap == 0) { s = (int)fmodf((float)s, (float)L.ures); t = (int)fmodf((float)t, (float)L.vres); }
     else if (m.wrap == 2) { s = (int)clampf((float)s, 0, (float)(L.ures - 1)); t = (int)clampf((float)t, 0, (float)(L.vres - 1)); }
     else if (s < 0 || s >= L.ures || t < 0 || t >= L.vres) return spec(0.0f);
@@ -639,10 +650,13 @@ LTS_D Spec mip_triangle(const SceneView& sv, const MipDesc& m, int l, V2 st) {
     return (1 - ds) * (1 - dt) * mip_texel(sv, m, s0, t0, l) + (1 - ds) * dt * mip_texel(sv, m, s0, t0 + 1, l) +
            ds * (1 - dt) * mip_texel(sv, m, s0 + 1, t0, l) + ds * dt * mip_texel(sv, m, s0 + 1, t0 + 1, l);
 }
-LTS_D Spec mip_lookup(const SceneView& sv, const MipDesc& m, V2 st) {
-    float levels = m.lookupLevel;
+// The reference's fixed width 0.01 gives a precomputed level; the filtered pipeline takes the
+// width of the lookup and reads the top level correctly (the reference swaps texel's arguments
+// there: texel (levels - 1, 0) of level 0).
+LTS_D Spec mip_lookup(const SceneView& sv, const MipDesc& m, V2 st, float width) {
+    float levels = m.filtered ? (float)(m.levels - 1) + (float)log2((double)fmaxf(width, (float)1e-8)) : m.lookupLevel;
     if (levels < 0) return mip_triangle(sv, m, 0, st);
-    if (levels >= m.levels - 1) return mip_texel(sv, m, m.levels - 1, 0, 0);
+    if (levels >= m.levels - 1) return m.filtered ? mip_texel(sv, m, 0, 0, m.levels - 1) : mip_texel(sv, m, m.levels - 1, 0, 0);
     int iLevel = (int)floorf(levels);
     float delta = levels - iLevel;
     return lerpS(delta, mip_triangle(sv, m, iLevel, st), mip_triangle(sv, m, iLevel + 1, st));
@@ -652,7 +666,12 @@ LTS_D Spec eval_tex(const SceneView& sv, int ti, const SurfacePoint& s) {
     const TexDesc& X = sv.texs[ti];
     switch (X.type) {
         case TEX_CONST: return spec(X.c[0], X.c[1], X.c[2]);
-        case TEX_IMAGE: return mip_lookup(sv, sv.mips[X.img], v2(1 * s.uv.x + 0, 1 * s.uv.y + 0));
+        case TEX_IMAGE: {
+            // filtered pipeline: PBRT's isotropic width 2 max(|ds/dx|, |dt/dx|, |ds/dy|, |dt/dy|) of
+            // UVMapping2D (st = uv, Mapping.cuh:19-37); the reference's lookup ignores it
+            const float width = 2 * fmaxf(fmaxf(fabsf(s.dudx), fabsf(s.dvdx)), fmaxf(fabsf(s.dudy), fabsf(s.dvdy)));
+            return mip_lookup(sv, sv.mips[X.img], v2(1 * s.uv.x + 0, 1 * s.uv.y + 0), width);
+        }
         case TEX_FBM: return spec(fbm(sv.perm, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy), X.omega, X.octaves));
         case TEX_WRINKLED: return spec(turbulence(sv.perm, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy), X.omega, X.octaves));
         case TEX_MARBLE: return marble_tex(sv.perm, X, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy));
@@ -894,6 +913,23 @@ LTS_D void differentials(SurfacePoint& s, const CamRays& cr) {
     V3 py = cr.ryO + cr.ryD * ty;
     s.dpdx = px - s.p;
     s.dpdy = py - s.p;
+    // du/dx, dv/dx, du/dy, dv/dy on the two axes other than the normal's dominant one
+    // (Interaction.cuh:134-153, solveLinearSystem2x2 GeneralHelper.cuh:127-134)
+    int d0, d1;
+    if (fabsf(s.n.x) > fabsf(s.n.y) && fabsf(s.n.x) > fabsf(s.n.z)) { d0 = 1; d1 = 2; }
+    else if (fabsf(s.n.y) > fabsf(s.n.z)) { d0 = 0; d1 = 2; }
+    else { d0 = 0; d1 = 1; }
+    auto c = [](V3 v, int k) { return k == 0 ? v.x : k == 1 ? v.y : v.z; };
+    const float a00 = c(s.dpdu, d0), a01 = c(s.dpdv, d0), a10 = c(s.dpdu, d1), a11 = c(s.dpdv, d1);
+    auto solve = [&](float b0, float b1, float& x0, float& x1) {
+        const float det = a00 * a11 - a01 * a10;
+        if (fabsf(det) < 1e-10f) { x0 = x1 = 0.0f; return; }
+        x0 = (a11 * b0 - a01 * b1) / det;
+        x1 = (a00 * b1 - a10 * b0) / det;
+        if (isnan(x0) || isnan(x1)) x0 = x1 = 0.0f;
+    };
+    solve(c(px, d0) - c(s.p, d0), c(px, d1) - c(s.p, d1), s.dudx, s.dvdx);
+    solve(c(py, d0) - c(s.p, d0), c(py, d1) - c(s.p, d1), s.dudy, s.dvdy);
 }
 
 }  // namespace lts

@@ -230,13 +230,53 @@ inline int64_t blocked_slot(int ublocks, int u, int v) {
     return 16LL * ((int64_t)ublocks * (v >> 2) + (u >> 2)) + 4LL * (v >> 2) + (u & 3);
 }
 
-// Appends one mip pyramid to `data` and fills `d`.
-bool build_mipmap(const std::string& path, int wrap, MipDesc& d, std::vector<float>& data, std::string& err) {
+// The filtered pyramid: levels down to 1x1 (1 + floor(log2(max(w, h))) of them), level l of
+// max(1, w >> l) x max(1, h >> l) row-major texels, each the mean of its 2x2 parent block (parent
+// indices clamped at odd edges), summed as ((a + b) + c) + d over (2s, 2t), (2s+1, 2t), (2s, 2t+1),
+// (2s+1, 2t+1).
+bool build_filtered_mipmap(int w, int h, const std::vector<float>& img, int wrap, MipDesc& d, std::vector<float>& data,
+                           std::string& err) {
+    int levels = 1;
+    while ((std::max(w, h) >> levels) > 0) levels++;
+    if (levels > kMaxMipLevels) { err = "image too large"; return false; }
+    d.wrap = wrap;
+    d.filtered = 1;
+    d.levels = levels;
+    for (int l = 0; l < levels; l++) {
+        MipLevelDesc& L = d.lv[l];
+        L.ures = std::max(1, w >> l);
+        L.vres = std::max(1, h >> l);
+        L.ublocks = 0;
+        L.offset = (int64_t)(data.size() / 3);
+        data.resize(data.size() + 3 * (size_t)L.ures * L.vres, 0.0f);
+    }
+    std::memcpy(&data[3 * (size_t)d.lv[0].offset], img.data(), img.size() * sizeof(float));
+    for (int l = 1; l < levels; l++) {
+        const MipLevelDesc& P = d.lv[l - 1];
+        const MipLevelDesc& L = d.lv[l];
+        auto src = [&](int s, int t) { return &data[3 * (size_t)(P.offset + (int64_t)t * P.ures + s)]; };
+        for (int t = 0; t < L.vres; t++)
+            for (int s = 0; s < L.ures; s++) {
+                const int s0 = std::min(2 * s, P.ures - 1), s1 = std::min(2 * s + 1, P.ures - 1);
+                const int t0 = std::min(2 * t, P.vres - 1), t1 = std::min(2 * t + 1, P.vres - 1);
+                float* o = &data[3 * (size_t)(L.offset + (int64_t)t * L.ures + s)];
+                for (int k = 0; k < 3; k++)
+                    o[k] = 0.25f * (((src(s0, t0)[k] + src(s1, t0)[k]) + src(s0, t1)[k]) + src(s1, t1)[k]);
+            }
+    }
+    return true;
+}
+
+// Appends one mip pyramid to `data` and fills `d`: the reference's (filtered == false) or the
+// filtered one.
+bool build_mipmap(const std::string& path, int wrap, bool filtered, MipDesc& d, std::vector<float>& data,
+                  std::string& err) {
     int w = 0, h = 0;
     std::vector<float> img;
     if (!load_image_linear(path, w, h, img, err)) return false;
-    if (w < 16 || h < 16) { err = "images smaller than 16x16 are not supported: " + path; return false; }
     std::memset(&d, 0, sizeof(d));
+    if (filtered) return build_filtered_mipmap(w, h, img, wrap, d, data, err);
+    if (w < 16 || h < 16) { err = "images smaller than 16x16 are not supported: " + path; return false; }
     d.wrap = wrap;
     d.levels = (int)(float)std::log2((double)(float)(w > h ? w : h));
     if (d.levels > kMaxMipLevels) { err = "image too large: " + path; return false; }
@@ -293,7 +333,7 @@ void gaussian_filter_table(float* out) {
 }
 
 bool load_scene_files(const std::string& dir, const std::string& sceneName, const std::string& subjectName, int W,
-                      int H, HostScene& S, std::string& err) {
+                      int H, HostScene& S, std::string& err, bool filteredTextures) {
     if (W < 2 || H < 2) { err = "film must be at least 2x2"; return false; }
     Tokens sc, sb;
     const bool hasSubject = !subjectName.empty();
@@ -324,7 +364,7 @@ bool load_scene_files(const std::string& dir, const std::string& sceneName, cons
             wrap = wrap < 0 ? 0 : (wrap > 2 ? 2 : wrap);
             std::string file = k.next();
             MipDesc md;
-            if (!build_mipmap(dir + file, wrap, md, S.mipData, err)) return false;
+            if (!build_mipmap(dir + file, wrap, filteredTextures, md, S.mipData, err)) return false;
             S.mips.push_back(md);
         }
         // meshes

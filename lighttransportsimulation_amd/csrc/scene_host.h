@@ -26,8 +26,9 @@ struct HostScene {
 };
 
 // parseScene (core/SceneLoader.cuh:195-360). Returns false with `err` set on failure.
+// filteredTextures: build the filtered mip pyramids instead of the reference's.
 bool load_scene_files(const std::string& dir, const std::string& scene, const std::string& subject, int width,
-                      int height, HostScene& out, std::string& err);
+                      int height, HostScene& out, std::string& err, bool filteredTextures = false);
 
 // OBJ import (core/FileIO.cuh:68-158, parseMeshFile with binMeshWrite): parses <stem>.obj and
 // returns the bytes of the equivalent <stem>.bin (header nTri, nP, nUV, nN; points, uvs, normals;

@@ -70,6 +70,10 @@ int32_t oracle_occluded(oracle_scene* s, const float o[3], const float d[3], flo
 
 // Gaussian filter table (Filter.cuh:32-50, 82-99): 16x16 floats.
 void oracle_filter_table(float* out256);
+// Texture pipeline of scenes loaded afterwards: 0 the reference's mipmap, 1 the filtered one.
+void oracle_set_texture_filter(int32_t filtered);
+void oracle_mip_lookup(const float* rgb, int32_t w, int32_t h, int32_t wrap, int32_t filtered, float s, float t,
+                       float width, float* out3);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,7 @@ struct SI {
     V3 dpdu, dpdv;
     V3 sn, sdpdu;
     V3 dpdx, dpdy;
+    float dudx = 0, dvdx = 0, dudy = 0, dvdy = 0;
     int tri = -1;
 };
 
@@ -550,7 +551,16 @@ static Spec marble(const Tex& X, V3 p, V3 dpdx, V3 dpdy) {
 static Spec evalTex(const Scene& s, const Tex& X, const SI& si) {
     switch (X.type) {
         case 'c': return X.c;
-        case 'i': return mipLookup(s.mips[X.img], V2(1 * si.uv.x + 0, 1 * si.uv.y + 0), 0.01f);
+        case 'i': {
+            // ImageTexture::evaluate uses the fixed width 0.01 (ImageTexture.cuh:8, 26); the filtered
+            // pipeline takes PBRT's isotropic width 2 max(|ds/dx|, |dt/dx|, |ds/dy|, |dt/dy|) of
+            // UVMapping2D (st = uv, Mapping.cuh:19-37)
+            const Mipmap& m = s.mips[X.img];
+            const float width = m.filtered ? 2 * std::fmax(std::fmax(std::fabs(si.dudx), std::fabs(si.dvdx)),
+                                                           std::fmax(std::fabs(si.dudy), std::fabs(si.dvdy)))
+                                           : 0.01f;
+            return mipLookup(m, V2(1 * si.uv.x + 0, 1 * si.uv.y + 0), width);
+        }
         case 'f': return Spec(fbm(identityMap(si.p), identityMap(si.dpdx), identityMap(si.dpdy), X.omega, X.octaves));
         case 'w': return Spec(turbulence(identityMap(si.p), identityMap(si.dpdx), identityMap(si.dpdy), X.omega, X.octaves));
         case 'm': return marble(X, identityMap(si.p), identityMap(si.dpdx), identityMap(si.dpdy));
@@ -568,7 +578,24 @@ static void computeDifferentials(SI& si, const Ray& r) {
     V3 py = r.ryO + r.ryD * ty;
     si.dpdx = px - si.p;
     si.dpdy = py - si.p;
-    // du/dx etc. only feed UVMapping2D's dstdx, which no texture in this scene model reads.
+    // du/dx, dv/dx, du/dy, dv/dy on the two axes other than the normal's dominant one
+    // (Interaction.cuh:134-153, solveLinearSystem2x2 GeneralHelper.cuh:127-134); only the filtered
+    // texture pipeline reads them
+    int d0, d1;
+    if (std::fabs(si.n.x) > std::fabs(si.n.y) && std::fabs(si.n.x) > std::fabs(si.n.z)) { d0 = 1; d1 = 2; }
+    else if (std::fabs(si.n.y) > std::fabs(si.n.z)) { d0 = 0; d1 = 2; }
+    else { d0 = 0; d1 = 1; }
+    auto c = [](V3 v, int k) { return k == 0 ? v.x : k == 1 ? v.y : v.z; };
+    const float a00 = c(si.dpdu, d0), a01 = c(si.dpdv, d0), a10 = c(si.dpdu, d1), a11 = c(si.dpdv, d1);
+    auto solve = [&](float b0, float b1, float& x0, float& x1) {
+        const float det = a00 * a11 - a01 * a10;
+        if (std::fabs(det) < 1e-10f) { x0 = x1 = 0; return; }
+        x0 = (a11 * b0 - a01 * b1) / det;
+        x1 = (a00 * b1 - a10 * b0) / det;
+        if (std::isnan(x0) || std::isnan(x1)) x0 = x1 = 0;
+    };
+    solve(c(px, d0) - c(si.p, d0), c(px, d1) - c(si.p, d1), si.dudx, si.dvdx);
+    solve(c(py, d0) - c(si.p, d0), c(py, d1) - c(si.p, d1), si.dudy, si.dvdy);
 }
 
 // Material::computeScatteringFunctions (Material.cuh:31-130)
@@ -876,6 +903,19 @@ extern "C" int32_t oracle_scene_info(const oracle_scene* s, int32_t* nt, int32_t
 }
 
 extern "C" void oracle_filter_table(float* out) { filterTable(out); }
+
+extern "C" void oracle_set_texture_filter(int32_t filtered) { g_textureFiltered = filtered != 0; }
+
+// Pyramid of a linear RGB image (row 0 first) and one lookup at (s, t) with the given width.
+extern "C" void oracle_mip_lookup(const float* rgb, int32_t w, int32_t h, int32_t wrap, int32_t filtered, float s, float t,
+                                  float width, float* out) {
+    std::vector<Spec> img((size_t)w * h);
+    for (size_t i = 0; i < img.size(); i++) img[i] = Spec(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+    Mipmap m;
+    buildMipmap(m, w, h, img, wrap, filtered != 0);
+    const Spec v = mipLookup(m, V2(s, t), width);
+    out[0] = v.r; out[1] = v.g; out[2] = v.b;
+}
 
 extern "C" void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
     U4 c; for (int i = 0; i < 4; i++) c.v[i] = ctr[i];

@@ -122,8 +122,17 @@ static bool loadImage(const std::string& path, int& w, int& h, std::vector<Spec>
     return true;
 }
 
+bool g_textureFiltered = false;
+
 static Spec texelAt(const Mipmap& m, int s, int t, int l, int wrap) {
     const MipLevel& L = m.lv[l];
+    if (m.filtered) {
+        // row-major texels; repeat wraps to [0, res), clamp clamps, black is zero outside
+        if (wrap == 0) { s = ((s % L.ures) + L.ures) % L.ures; t = ((t % L.vres) + L.vres) % L.vres; }
+        else if (wrap == 2) { s = std::min(std::max(s, 0), L.ures - 1); t = std::min(std::max(t, 0), L.vres - 1); }
+        else if (s < 0 || s >= L.ures || t < 0 || t >= L.vres) return Spec(0.0f);
+        return L.slots[(size_t)t * L.ures + s];
+    }
     switch (wrap) {
         case 0: s = (int)std::fmod((float)s, (float)L.ures); t = (int)std::fmod((float)t, (float)L.vres); break;
         case 2: s = (int)clampf((float)s, 0, (float)(L.ures - 1)); t = (int)clampf((float)t, 0, (float)(L.vres - 1)); break;
@@ -140,14 +149,39 @@ static void initLevel(MipLevel& L, int u, int v) {
     L.slots.assign(L.slot(u - 1, v - 1) + 1, Spec(0.0f));
 }
 
-// CreateMipMap (Mipmap_kernel.cu:39-73). Level l > 0 is filled only for s, t < 16 >> l
-// (Mipmap_kernel.cu:29-30); colliding writes resolve as a sequential loop would (last t wins).
-static bool createMipmap(Mipmap& m, const std::string& path, int wrap, std::string& err) {
-    int w, h;
-    std::vector<Spec> img;
-    if (!loadImage(path, w, h, img, err)) return false;
-    if (w < 16 || h < 16) { err = "images smaller than 16x16 are not supported: " + path; return false; }
+// The filtered pyramid: levels down to 1x1 (1 + floor(log2(max(w, h))) of them), level l of
+// max(1, w >> l) x max(1, h >> l) texels, each the mean of its 2x2 parent block, parent indices
+// clamped at odd edges, summed in the order (2s, 2t), (2s+1, 2t), (2s, 2t+1), (2s+1, 2t+1).
+void buildMipmap(Mipmap& m, int w, int h, const std::vector<Spec>& img, int wrap, bool filtered) {
     m.wrap = wrap;
+    m.filtered = filtered;
+    if (filtered) {
+        int levels = 1;
+        while ((std::max(w, h) >> levels) > 0) levels++;
+        m.levels = levels;
+        m.lv.assign(levels, MipLevel());
+        m.lv[0].ures = w; m.lv[0].vres = h;
+        m.lv[0].slots = img;
+        for (int l = 1; l < levels; l++) {
+            const MipLevel& P = m.lv[l - 1];
+            MipLevel& L = m.lv[l];
+            L.ures = std::max(1, w >> l); L.vres = std::max(1, h >> l);
+            L.slots.assign((size_t)L.ures * L.vres, Spec(0.0f));
+            for (int t = 0; t < L.vres; t++)
+                for (int s = 0; s < L.ures; s++) {
+                    const int s0 = std::min(2 * s, P.ures - 1), s1 = std::min(2 * s + 1, P.ures - 1);
+                    const int t0 = std::min(2 * t, P.vres - 1), t1 = std::min(2 * t + 1, P.vres - 1);
+                    const Spec& a = P.slots[(size_t)t0 * P.ures + s0];
+                    const Spec& b = P.slots[(size_t)t0 * P.ures + s1];
+                    const Spec& c = P.slots[(size_t)t1 * P.ures + s0];
+                    const Spec& d = P.slots[(size_t)t1 * P.ures + s1];
+                    L.slots[(size_t)t * L.ures + s] = 0.25f * (((a + b) + c) + d);
+                }
+        }
+        return;
+    }
+    // CreateMipMap (Mipmap_kernel.cu:39-73). Level l > 0 is filled only for s, t < 16 >> l
+    // (Mipmap_kernel.cu:29-30); colliding writes resolve as a sequential loop would (last t wins).
     m.levels = (int)(float)std::log2((double)(float)std::max(w, h));
     m.lv.resize(m.levels);
     initLevel(m.lv[0], w, h);
@@ -163,6 +197,14 @@ static bool createMipmap(Mipmap& m, const std::string& path, int wrap, std::stri
                 m.lv[l].slots[m.lv[l].slot(s, t)] = 0.25f * (a + b + c + d);
             }
     }
+}
+
+static bool createMipmap(Mipmap& m, const std::string& path, int wrap, std::string& err) {
+    int w, h;
+    std::vector<Spec> img;
+    if (!loadImage(path, w, h, img, err)) return false;
+    if (!g_textureFiltered && (w < 16 || h < 16)) { err = "images smaller than 16x16 are not supported: " + path; return false; }
+    buildMipmap(m, w, h, img, wrap, g_textureFiltered);
     return true;
 }
 
@@ -177,10 +219,15 @@ static Spec mipTriangle(const Mipmap& m, int l, V2 st) {
            ds * (1 - dt) * texelAt(m, s0 + 1, t0, l, m.wrap) + ds * dt * texelAt(m, s0 + 1, t0 + 1, l, m.wrap);
 }
 
+Spec mipTexel(const Mipmap& m, int s, int t, int l) { return texelAt(m, s, t, l, m.wrap); }
+
+// The filtered pipeline keeps this level selection and fixes the top branch, which the reference
+// calls with its arguments swapped (texel(levels - 1, 0, 0): texel (levels - 1, 0) of level 0).
 Spec mipLookup(const Mipmap& m, V2 st, float width) {
     float levels = (float)(m.levels - 1) + f_log2(std::fmax(width, (float)1e-8));
     if (levels < 0) return mipTriangle(m, 0, st);
-    else if (levels >= m.levels - 1) return texelAt(m, m.levels - 1, 0, 0, m.wrap);
+    else if (levels >= m.levels - 1)
+        return m.filtered ? texelAt(m, 0, 0, m.levels - 1, m.wrap) : texelAt(m, m.levels - 1, 0, 0, m.wrap);
     int iLevel = (int)std::floor(levels);
     float delta = levels - iLevel;
     return lerpS(delta, mipTriangle(m, iLevel, st), mipTriangle(m, iLevel + 1, st));

@@ -26,8 +26,15 @@ struct MipLevel {
 struct Mipmap {
     int wrap = 1;   // 0 Repeat, 1 Black, 2 Clamp (materials/textures/Mipmap.cuh:10-12)
     int levels = 0;
+    bool filtered = false;  // full box pyramid, row-major texels, width from ray differentials
     std::vector<MipLevel> lv;
 };
+
+// Texture pipeline of scenes loaded from now on: false = the reference's (bug-compatible) mipmap,
+// true = the filtered one. Test infrastructure state, set through oracle_set_texture_filter.
+extern bool g_textureFiltered;
+// Builds a pyramid from linear RGB texels (row 0 first) in either mode.
+void buildMipmap(Mipmap& m, int w, int h, const std::vector<Spec>& img, int wrap, bool filtered);
 
 struct Tex {
     char type = 'c';
@@ -75,5 +82,6 @@ bool loadScene(Scene& s, const std::string& dir, const std::string& scene, const
                int W, int H, std::string& err);
 void filterTable(float* out256);
 Spec mipLookup(const Mipmap& m, V2 st, float width);
+Spec mipTexel(const Mipmap& m, int s, int t, int l);
 
 } // namespace orc

@@ -57,6 +57,9 @@ def lib():
         L.oracle_occluded.restype = C.c_int32
         L.oracle_occluded.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
         L.oracle_filter_table.argtypes = [C.c_void_p]
+        L.oracle_set_texture_filter.argtypes = [C.c_int32]
+        L.oracle_mip_lookup.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float,
+                                        C.c_float, C.c_void_p]
         L.oracle_nrc_param_count.restype = C.c_int32
         L.oracle_nrc_encode.argtypes = [C.c_void_p, C.c_void_p]
         L.oracle_nrc_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
@@ -69,6 +72,19 @@ def lib():
 
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def set_texture_filter(filtered: bool):
+    """Texture pipeline of OracleScene objects created afterwards (False: the reference's)."""
+    lib().oracle_set_texture_filter(int(filtered))
+
+
+def mip_lookup(img: np.ndarray, s: float, t: float, width: float, wrap: int = 1, filtered: bool = True) -> np.ndarray:
+    """Pyramid of a linear [h][w][3] image and one lookup (Mipmap::lookUp, Mipmap.cuh:55-72)."""
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().oracle_mip_lookup(_p(img), img.shape[1], img.shape[0], wrap, int(filtered), s, t, width, _p(out))
+    return out
 
 
 def lbvh_build(tri: np.ndarray, mode: int = 0) -> dict:
