@@ -458,11 +458,11 @@ def main() -> int:
             progress("cpu baseline done")
             bytes_per_sample = 32.0 * cb["node_visits_per_sample"] + 36.0 * cb["tri_tests_per_sample"]
             if wavefront:
-                # traversal kernels (closest hit of path rays, any hit of shadow rays, closest hit of
-                # MIS rays) against all node and triangle tests of the frame
+                # the traversal kernel (closest hit of path rays, any hit of shadow rays, closest hit
+                # of MIS rays, traced together) against all node and triangle tests of the frame
                 trav_ms = (phase["extend"] + phase["shadow"] + phase["mis"]) / args.steps
                 achieved = bytes_per_sample * samples / (trav_ms * 1e-3) / 1e9
-                kernel = "k_wf_extend + k_wf_shadow + k_wf_mis (per frame)"
+                kernel = "k_wf_trace (per frame)"
                 kernel_ms = trav_ms
             else:
                 launch_s = kern_ms / launches / 1e3

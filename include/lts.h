@@ -98,11 +98,12 @@ int lts_film_load(lts_ctx* ctx, const char* path);
 int lts_film_resolve(lts_ctx* ctx, float* rgb_out);
 
 /* Wavefront phases timed when profile_phases is set (indices into lts_stats.phase_ms). */
-#define LTS_PHASE_EXTEND 0      /* closest-hit traversal of path rays                        */
+#define LTS_PHASE_EXTEND 0      /* all traversal: path rays with the shadow and MIS rays traced */
+                                /* in the same launches (one per bounce)                      */
 #define LTS_PHASE_SORT 1        /* material sort of the shading queue                        */
 #define LTS_PHASE_SHADE 2
-#define LTS_PHASE_SHADOW 3      /* any-hit traversal of shadow rays                          */
-#define LTS_PHASE_MIS 4         /* closest-hit traversal of MIS-BSDF rays                    */
+#define LTS_PHASE_SHADOW 3      /* 0: shadow rays are traced with the path rays (EXTEND)      */
+#define LTS_PHASE_MIS 4         /* 0: MIS-BSDF rays are traced with the path rays (EXTEND)    */
 #define LTS_PHASE_FINALIZE 5
 #define LTS_PHASE_CACHE_INFER 6 /* radiance-cache inference (cached integrator)              */
 #define LTS_PHASE_CACHE_TRAIN 7 /* training-batch compaction and Adam steps                   */

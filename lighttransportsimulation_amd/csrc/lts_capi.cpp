@@ -550,9 +550,14 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         resolveMs += ms_between(c->ev[6], c->ev[7]);
         launches++;
         if (prof && p->max_bounce > 0) {
+            // the pass's first trace, then per bounce: sort, shade, trace, finalize
+            phase[LTS_PHASE_EXTEND] += ms_between(prof->ev[8 * kWfMaxBounce + 3], prof->ev[0]);
             for (int b = 0; b < p->max_bounce; b++) {
                 const hipEvent_t* e = prof->ev + 8 * b;
-                for (int k = 0; k < 6; k++) phase[k] += ms_between(e[k], e[k + 1]);
+                phase[LTS_PHASE_SORT] += ms_between(e[0], e[1]);
+                phase[LTS_PHASE_SHADE] += ms_between(e[1], e[2]);
+                phase[LTS_PHASE_EXTEND] += ms_between(e[2], e[3]);
+                phase[LTS_PHASE_FINALIZE] += ms_between(e[3], e[4]);
             }
             if (cached) {
                 const hipEvent_t* e = prof->ev + 8 * kWfMaxBounce;
@@ -588,11 +593,11 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         for (int k = 0; k < 8; k++) st->phase_ms[k] = phase[k];
         if (prof && p->max_bounce > 0) {
             const int64_t perBounce = launches * p->max_bounce;
-            st->phase_launches[LTS_PHASE_EXTEND] = perBounce;
+            st->phase_launches[LTS_PHASE_EXTEND] = perBounce + launches;
             st->phase_launches[LTS_PHASE_SORT] = t.sortMaterials ? 2 * perBounce : 0;
             st->phase_launches[LTS_PHASE_SHADE] = perBounce;
-            st->phase_launches[LTS_PHASE_SHADOW] = perBounce;
-            st->phase_launches[LTS_PHASE_MIS] = perBounce;
+            st->phase_launches[LTS_PHASE_SHADOW] = 0;
+            st->phase_launches[LTS_PHASE_MIS] = 0;
             st->phase_launches[LTS_PHASE_FINALIZE] = perBounce;
             st->phase_launches[LTS_PHASE_CACHE_INFER] = cached ? launches : 0;
             st->phase_launches[LTS_PHASE_CACHE_TRAIN] = cached ? launches * c->nrc.steps : 0;

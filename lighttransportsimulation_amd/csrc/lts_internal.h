@@ -179,10 +179,11 @@ constexpr int kWfGlobalBase = (kWfCountsPerBounce + kWfMaxMatBins) * (kWfMaxBoun
 constexpr int kWfGlobalCounts = 8;  // [0] cache queries
 
 // Optional phase timing: events recorded between the kernels of a pass launched directly (not
-// captured). Bounce b uses ev[8b + 0..6]: before extend, after extend, sort, shade, shadow, mis and
-// finalize; the cached integrator's tail uses ev[8 kWfMaxBounce + 0..2]: before inference, after
-// inference, after the training steps.
-constexpr int kWfPhaseEvents = 8 * kWfMaxBounce + 3;
+// captured). ev[8 kWfMaxBounce + 3] precedes the pass's first trace; bounce b uses ev[8b + 0..4]:
+// before the sort, after the sort, after shading, after the trace of the next path rays with this
+// bounce's shadow and MIS rays, after finalize; the cached integrator's tail uses
+// ev[8 kWfMaxBounce + 0..2]: before inference, after inference, after the training steps.
+constexpr int kWfPhaseEvents = 8 * kWfMaxBounce + 4;
 struct WfProfile { hipEvent_t ev[kWfPhaseEvents]; };
 
 hipError_t wf_reserve(WfBuffers& b, size_t cap, int maxScenes);

@@ -87,47 +87,24 @@ LTS_D int wave_fetch(int32_t* counter) {
     return __shfl(base, 0) + (threadIdx.x & 63);
 }
 
-// With `hist` set, the kernel also files each hit under its material bin (misses in the last bin)
-// for the counting sort: bins[path] and a block histogram added to hist at the end.
-template <int BS>
-__global__ __launch_bounds__(BS) void k_wf_extend(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
-                                                  const WfRay* __restrict__ rays, WfHit* __restrict__ hits,
-                                                  const int32_t* __restrict__ queue, const int32_t* __restrict__ count,
-                                                  int32_t* __restrict__ work, int32_t* __restrict__ bins,
-                                                  int32_t* __restrict__ hist, int nBins) {
-    __shared__ int stk[kShortStack * BS];
-    __shared__ int32_t lh[kWfMaxMatBins];
-    const TraceStack ts = lds_stack(stk, threadIdx.x, BS);
-    if (hist) {
-        for (int b = threadIdx.x; b < kWfMaxMatBins; b += BS) lh[b] = 0;
-        __syncthreads();
-    }
-    const int n = *count;
-    for (;;) {
-        const int k = wave_fetch(work);
-        if (k - (int)(threadIdx.x & 63) >= n) break;
-        if (k >= n) continue;
-        const int pi = queue[k];
-        const WfRay rr = rays[pi];
-        Ray r;
-        r.o = v3(rr.o[0], rr.o[1], rr.o[2]);
-        r.d = v3(rr.d[0], rr.d[1], rr.d[2]);
-        r.tMax = INFINITY;
-        const SceneView& sv = scenes[paths[pi].scene];
-        Hit h = trace_closest(sv, r, ts, nullptr, nullptr);
-        WfHit o;
-        o.leaf = h.leaf; o.u = h.u; o.v = h.v; o.invDet = h.invDet;
-        hits[pi] = o;
-        if (hist) {
-            const int bin = h.leaf < 0 ? nBins - 1 : min(sv.tris[h.leaf].mat, nBins - 2);
-            bins[pi] = bin;
-            atomicAdd(&lh[bin], 1);
-        }
-    }
-    if (hist) {
-        __syncthreads();
-        for (int b = threadIdx.x; b < nBins; b += BS)
-            if (lh[b]) atomicAdd(&hist[b], lh[b]);
+// Closest hit of path ray `pi`; with `hist` set, the hit is also filed under its material bin
+// (misses in the last bin) for the counting sort: bins[path] and the block histogram lh.
+LTS_D void extend_item(const SceneView* scenes, const WfPath* paths, const WfRay* rays, WfHit* hits, int pi,
+                       TraceStack ts, int32_t* bins, int32_t* lh, bool filing, int nBins) {
+    const WfRay rr = rays[pi];
+    Ray r;
+    r.o = v3(rr.o[0], rr.o[1], rr.o[2]);
+    r.d = v3(rr.d[0], rr.d[1], rr.d[2]);
+    r.tMax = INFINITY;
+    const SceneView& sv = scenes[paths[pi].scene];
+    Hit h = trace_closest(sv, r, ts, nullptr, nullptr);
+    WfHit o;
+    o.leaf = h.leaf; o.u = h.u; o.v = h.v; o.invDet = h.invDet;
+    hits[pi] = o;
+    if (filing) {
+        const int bin = h.leaf < 0 ? nBins - 1 : min(sv.tris[h.leaf].mat, nBins - 2);
+        bins[pi] = bin;
+        atomicAdd(&lh[bin], 1);
     }
 }
 
@@ -467,63 +444,83 @@ __global__ __launch_bounds__(kCollectBlock) void k_wf_nrc_scatter(const WfParams
     }
 }
 
-template <int BS>
-__global__ __launch_bounds__(BS) void k_wf_shadow(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
-                                                  const WfShadow* __restrict__ shadows, const int32_t* __restrict__ count,
-                                                  WfNee* __restrict__ nee, int32_t* __restrict__ work) {
-    __shared__ int stk[kShortStack * BS];
-    const TraceStack ts = lds_stack(stk, threadIdx.x, BS);
-    const int n = *count;
-    for (;;) {
-        const int k = wave_fetch(work);
-        if (k - (int)(threadIdx.x & 63) >= n) break;
-        if (k >= n) continue;
-        const WfShadow s = shadows[k];
-        Ray r;
-        r.o = v3(s.o[0], s.o[1], s.o[2]);
-        r.d = v3(s.d[0], s.d[1], s.d[2]);
-        r.tMax = 1 - SHADOW_EPS;
-        if (!trace_any(scenes[paths[s.path].scene], r, ts)) {
-            nee[s.path].p1[0] = s.c[0];
-            nee[s.path].p1[1] = s.c[1];
-            nee[s.path].p1[2] = s.c[2];
-        }
+// Any hit of shadow ray k: the light sample's contribution when the segment is free.
+LTS_D void shadow_item(const SceneView* scenes, const WfPath* paths, const WfShadow* shadows, WfNee* nee, int k,
+                       TraceStack ts) {
+    const WfShadow s = shadows[k];
+    Ray r;
+    r.o = v3(s.o[0], s.o[1], s.o[2]);
+    r.d = v3(s.d[0], s.d[1], s.d[2]);
+    r.tMax = 1 - SHADOW_EPS;
+    if (!trace_any(scenes[paths[s.path].scene], r, ts)) {
+        nee[s.path].p1[0] = s.c[0];
+        nee[s.path].p1[1] = s.c[1];
+        nee[s.path].p1[2] = s.c[2];
     }
 }
 
+// Closest hit of MIS-BSDF ray k and the light identity test (IntegratorHelper.cuh:103-122).
+LTS_D void mis_item(const SceneView* scenes, const WfPath* paths, const WfMis* misRays, WfNee* nee, int k, TraceStack ts) {
+    const WfMis m = misRays[k];
+    const SceneView& sv = scenes[paths[m.path].scene];
+    const LightDesc& Lt = sv.lights[m.light];
+    Ray r;
+    r.o = v3(m.o[0], m.o[1], m.o[2]);
+    r.d = v3(m.d[0], m.d[1], m.d[2]);
+    r.tMax = INFINITY;
+    const V3 wi = r.d;
+    Hit h = trace_closest(sv, r, ts, nullptr, nullptr);
+    Spec Li2 = spec(0.f);
+    if (h.leaf >= 0) {
+        SurfacePoint lt = make_surface(sv, h.leaf, r, h.u, h.v, h.invDet);
+        if (sv.shade[lt.prim].light == m.light) Li2 = dot(lt.n, -wi) > 0.f ? light_le(Lt) : spec(0.0f);
+    } else if (Lt.kind == 1) {
+        Li2 = light_le(Lt);
+    }
+    if (!isBlack(Li2)) {
+        Spec c = spec(m.f[0], m.f[1], m.f[2]) * Li2 * spec(1.0f) * m.weight / m.pdf;
+        nee[m.path].p2[0] = c.r;
+        nee[m.path].p2[1] = c.g;
+        nee[m.path].p2[2] = c.b;
+    }
+}
+
+// One persistent launch traces every ray that is ready: the path rays of the next bounce (queue
+// `ext`), then the shadow rays and the MIS rays of the bounce just shaded. The three sets do not
+// depend on each other, so one work counter over their concatenation balances long rays against
+// short ones and leaves one tail per bounce instead of three. Counts may be null (empty set).
 template <int BS>
-__global__ __launch_bounds__(BS) void k_wf_mis(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
-                                               const WfMis* __restrict__ misRays, const int32_t* __restrict__ count,
-                                               WfNee* __restrict__ nee, int32_t* __restrict__ work) {
+__global__ __launch_bounds__(BS) void k_wf_trace(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
+                                                 const WfRay* __restrict__ rays, WfHit* __restrict__ hits,
+                                                 const int32_t* __restrict__ ext, const int32_t* __restrict__ extCount,
+                                                 int32_t* __restrict__ bins, int32_t* __restrict__ hist, int nBins,
+                                                 const WfShadow* __restrict__ shadows, const int32_t* __restrict__ shadowCount,
+                                                 const WfMis* __restrict__ misRays, const int32_t* __restrict__ misCount,
+                                                 WfNee* __restrict__ nee, int32_t* __restrict__ work) {
     __shared__ int stk[kShortStack * BS];
+    __shared__ int32_t lh[kWfMaxMatBins];
     const TraceStack ts = lds_stack(stk, threadIdx.x, BS);
-    const int n = *count;
+    const bool filing = hist != nullptr;
+    if (filing) {
+        for (int b = threadIdx.x; b < kWfMaxMatBins; b += BS) lh[b] = 0;
+        __syncthreads();
+    }
+    const int nE = extCount ? *extCount : 0;
+    const int nS = shadowCount ? *shadowCount : 0;
+    const int nM = misCount ? *misCount : 0;
+    const int n = nE + nS + nM;
     for (;;) {
         const int k = wave_fetch(work);
         if (k - (int)(threadIdx.x & 63) >= n) break;
         if (k >= n) continue;
-        const WfMis m = misRays[k];
-        const SceneView& sv = scenes[paths[m.path].scene];
-        const LightDesc& Lt = sv.lights[m.light];
-        Ray r;
-        r.o = v3(m.o[0], m.o[1], m.o[2]);
-        r.d = v3(m.d[0], m.d[1], m.d[2]);
-        r.tMax = INFINITY;
-        const V3 wi = r.d;
-        Hit h = trace_closest(sv, r, ts, nullptr, nullptr);
-        Spec Li2 = spec(0.f);
-        if (h.leaf >= 0) {
-            SurfacePoint lt = make_surface(sv, h.leaf, r, h.u, h.v, h.invDet);
-            if (sv.shade[lt.prim].light == m.light) Li2 = dot(lt.n, -wi) > 0.f ? light_le(Lt) : spec(0.0f);
-        } else if (Lt.kind == 1) {
-            Li2 = light_le(Lt);
-        }
-        if (!isBlack(Li2)) {
-            Spec c = spec(m.f[0], m.f[1], m.f[2]) * Li2 * spec(1.0f) * m.weight / m.pdf;
-            nee[m.path].p2[0] = c.r;
-            nee[m.path].p2[1] = c.g;
-            nee[m.path].p2[2] = c.b;
-        }
+        if (k < nE) extend_item(scenes, paths, rays, hits, ext[k], ts, bins, lh, filing, nBins);
+        else if (k < nE + nS) shadow_item(scenes, paths, shadows, nee, k - nE, ts);
+        else mis_item(scenes, paths, misRays, nee, k - nE - nS, ts);
+    }
+    if (filing) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < nBins; b += BS)
+            if (lh[b]) atomicAdd(&hist[b], lh[b]);
     }
 }
 
@@ -586,46 +583,53 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
     const bool cached = nrc != nullptr;
     int32_t* qCount = b.counts + kWfGlobalBase;
     k_wf_generate<<<gridAll, 256, 0, s>>>(b.scenes, b.params, b.slots, b.paths, b.rays, b.queue[0], b.counts, b.tRec);
+    auto histOf = [&](int bounce) {
+        return t.sortMaterials ? b.counts + kWfCountsPerBounce * (maxBounce + 1) + kWfMaxMatBins * bounce : nullptr;
+    };
+    // traces the path rays of `bounce` (queue q, count cnt[0]) and the shadow and MIS rays of the
+    // bounce before it (counts prev[1], prev[2]); work counter `work`
+    auto trace = [&](int bounce, const int32_t* extCount, const int32_t* shadowCount, const int32_t* misCount, int32_t* work) {
+        const int32_t* ext = b.queue[bounce & 1];
+        int32_t* hist = histOf(bounce);
+        switch (t.traceBlock) {
+            case 64:
+                k_wf_trace<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, ext, extCount, b.bins, hist,
+                                                                b.matBins, b.shadows, shadowCount, b.mis, misCount, b.nee, work);
+                break;
+            case 128:
+                k_wf_trace<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, ext, extCount, b.bins, hist,
+                                                                  b.matBins, b.shadows, shadowCount, b.mis, misCount, b.nee, work);
+                break;
+            default:
+                k_wf_trace<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, ext, extCount, b.bins, hist,
+                                                                  b.matBins, b.shadows, shadowCount, b.mis, misCount, b.nee, work);
+                break;
+        }
+    };
+    mark(8 * kWfMaxBounce + 3);
+    trace(0, b.counts, nullptr, nullptr, b.counts + 3);
     for (int bounce = 0; bounce < maxBounce; bounce++) {
         const int q = bounce & 1;
         int32_t* cnt = b.counts + kWfCountsPerBounce * bounce;
         int32_t* nxt = b.counts + kWfCountsPerBounce * (bounce + 1);
         mark(8 * bounce + 0);
-        int32_t* hist = t.sortMaterials ? b.counts + kWfCountsPerBounce * (maxBounce + 1) + kWfMaxMatBins * bounce : nullptr;
-        switch (t.traceBlock) {
-            case 64: k_wf_extend<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3,
-                                                                      b.bins, hist, b.matBins); break;
-            case 128: k_wf_extend<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3,
-                                                                         b.bins, hist, b.matBins); break;
-            default: k_wf_extend<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, b.queue[q], cnt, cnt + 3,
-                                                                        b.bins, hist, b.matBins); break;
-        }
-        mark(8 * bounce + 1);
         const int32_t* shadeQueue = b.queue[q];
         if (t.sortMaterials) {
+            int32_t* hist = histOf(bounce);
             k_wf_mat_scan<<<1, 64, 0, s>>>(hist, b.matBins);
             k_wf_mat_scatter<<<gridAll, 256, 0, s>>>(b.bins, b.queue[q], cnt, hist, b.matBins, b.sorted);
             shadeQueue = b.sorted;
         }
-        mark(8 * bounce + 2);
+        mark(8 * bounce + 1);
         k_wf_shade<<<gridAll, kShadeBlock, 0, s>>>(b.scenes, b.params, b.slots, bounce, b.paths, b.rays, b.hits, b.nee, shadeQueue,
                                                    cnt, b.queue[q ^ 1], nxt, b.shadows, cnt + 1, b.mis, cnt + 2,
                                                    cached ? b.qRaw : nullptr, b.qMeta, b.qQueue, qCount, b.tRec, b.tRaw);
+        mark(8 * bounce + 2);
+        // the next bounce's path rays (none after the last bounce) with this bounce's shadow and MIS rays
+        trace(bounce + 1, nxt, cnt + 1, cnt + 2, cnt + 4);
         mark(8 * bounce + 3);
-        switch (t.traceBlock) {
-            case 64: k_wf_shadow<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.shadows, cnt + 1, b.nee, cnt + 4); break;
-            case 128: k_wf_shadow<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.shadows, cnt + 1, b.nee, cnt + 4); break;
-            default: k_wf_shadow<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.shadows, cnt + 1, b.nee, cnt + 4); break;
-        }
-        mark(8 * bounce + 4);
-        switch (t.traceBlock) {
-            case 64: k_wf_mis<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.mis, cnt + 2, b.nee, cnt + 5); break;
-            case 128: k_wf_mis<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.mis, cnt + 2, b.nee, cnt + 5); break;
-            default: k_wf_mis<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.mis, cnt + 2, b.nee, cnt + 5); break;
-        }
-        mark(8 * bounce + 5);
         k_wf_finalize<<<gridAll, 256, 0, s>>>(b.params, b.slots, bounce, b.paths, b.nee, shadeQueue, cnt);
-        mark(8 * bounce + 6);
+        mark(8 * bounce + 4);
     }
     if (cached) {
         mark(8 * kWfMaxBounce + 0);

@@ -139,8 +139,9 @@ def test_phase_profiling_keeps_the_image_and_accounts_for_the_pass(gpu_ctx, disp
     img, _, st = gpu_ctx.render(8, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT, spp_per_pass=4, profile_phases=1)
     assert _identical(ref, img)
     ph = st["phase_ms"]
-    assert ph["extend"] > 0 and ph["shade"] > 0 and ph["shadow"] > 0 and ph["mis"] > 0
-    assert ph["cache_infer"] == 0 and st["phase_launches"]["extend"] == 2 * 8
+    # one trace launch per bounce (path, shadow and MIS rays together) plus the first of each pass
+    assert ph["extend"] > 0 and ph["shade"] > 0 and ph["finalize"] > 0 and ph["shadow"] == 0 and ph["mis"] == 0
+    assert ph["cache_infer"] == 0 and st["phase_launches"]["extend"] == 2 * 8 + 2
     total = sum(ph.values())
     assert 0.5 * st["render_ms"] < total <= 1.01 * st["render_ms"]
 
