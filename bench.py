@@ -327,6 +327,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--integrator", default="wavefront", choices=("megakernel", "wavefront"))
     ap.add_argument("--autotune", type=int, default=1, help="wavefront: tune block/grid/sort during warm-up")
+    ap.add_argument("--spp-per-pass", type=int, default=0, help="samples per pixel per pass (0: the library's choice)")
     ap.add_argument("--no-quality", action="store_true", help="skip the cached-vs-plain equal-time comparison")
     ap.add_argument("--batch-scenes", type=int, default=0, help="batch64: number of scenes (default 64)")
     ap.add_argument("--cached-frame", action="store_true",
@@ -378,7 +379,8 @@ def main() -> int:
             kw = dict(spp=spp, max_bounce=cfg["max_bounce"], rr_start=cfg["rr_start"], seed=cfg["seed"],
                       stratified=cfg["stratified"], want_image=False,
                       integrator=INTEGRATOR_WAVEFRONT if wavefront else INTEGRATOR_MEGAKERNEL,
-                      autotune=int(wavefront and args.autotune), profile_phases=int(wavefront))
+                      autotune=int(wavefront and args.autotune), profile_phases=int(wavefront),
+                      spp_per_pass=args.spp_per_pass)
             tuning = None
             progress(f"scene loaded: {info['tris']} triangles")
             for _ in range(max(args.warmup, 1 if wavefront and args.autotune else 0)):

@@ -399,18 +399,20 @@ int time_pass(lts_ctx* c, const WfTuning& t, int maxBounce, int reps, double* ms
 }
 
 // Coordinate search over the traversal block size, the persistent grid and material sorting, on
-// a pass of at most two samples per pixel (the parameters of that pass are uploaded here).
-int autotune(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_render_params* p, WfTuning* best) {
-    const int trialSamples = std::min(p->spp, 2);
+// one pass of the render's own size (the parameters of that pass are uploaded here): the end of
+// every trace launch is a tail whose share depends on the pass size, so smaller trial passes pick
+// other settings than the render needs.
+int autotune(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_render_params* p, int pass, WfTuning* best) {
+    const int trialSamples = std::max(1, std::min(p->spp, pass));
     int64_t paths = 0;
     int rc = wf_upload_pass(c, scenes, p, 0, trialSamples, &paths);
     if (rc != LTS_OK) return rc;
     WfTuning cur = *best;
     double bestMs = 0;
-    if ((rc = time_pass(c, cur, p->max_bounce, 2, &bestMs)) != LTS_OK) return rc;
+    if ((rc = time_pass(c, cur, p->max_bounce, 1, &bestMs)) != LTS_OK) return rc;
     auto consider = [&](WfTuning t) -> int {
         double ms = 0;
-        int r = time_pass(c, t, p->max_bounce, 2, &ms);
+        int r = time_pass(c, t, p->max_bounce, 1, &ms);
         if (r != LTS_OK) return r;
         if (ms < bestMs) { bestMs = ms; cur = t; }
         return LTS_OK;
@@ -454,7 +456,12 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
     sample_range(p, &b0, &b1);
     if (b0 > 0 && scenes.size() != 1) return fail(c, LTS_E_ARG, "progressive rendering is for one scene");
     if (b0 > 0 && (rc = film_continues(c, p, b0)) != LTS_OK) return rc;
-    int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, ((size_t)1 << 24) / totalPix);
+    // automatic pass: 2^26 paths (about 19 GB of path state; each trace launch ends in a tail, so
+    // fewer, larger passes keep the GPU busy: 2^24 -> 2^26 paths took the 1080p frame from 585 to
+    // 756 Msamples/s); the cached integrator keeps 2^24, as its warm-up and training steps are
+    // counted per pass
+    const size_t passPaths = cached ? (size_t)1 << 24 : (size_t)1 << 26;
+    int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, passPaths / totalPix);
     if (sample_rgb) pass = b1 - b0;
     pass = std::min(pass, b1 - b0);
     const size_t cap = totalPix * (size_t)pass;
@@ -494,7 +501,7 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
             c->tuned[key] = t;
         } else {
             HIPCHK(c, hipEventRecord(c->ev[3], s));
-            if ((rc = autotune(c, scenes, p, &t)) != LTS_OK) return rc;
+            if ((rc = autotune(c, scenes, p, pass, &t)) != LTS_OK) return rc;
             HIPCHK(c, hipEventRecord(c->ev[4], s));
             HIPCHK(c, hipEventSynchronize(c->ev[4]));
             tuneMs = ms_between(c->ev[3], c->ev[4]);
