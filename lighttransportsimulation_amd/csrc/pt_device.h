@@ -141,11 +141,12 @@ LTS_D void cswap(float& ta, int& ca, float& tb, int& cb) {
     }
 }
 
-// Closest hit (BVH::intersect, acceleration/BVH.cuh:365-398) over the 4-wide collapse of the same
-// Karras tree, children visited nearest first. The 4-wide node tests the boxes of the binary
-// tree's grandchildren and skips the children's boxes, which never prunes a candidate the binary
-// traversal would keep (a ray that passes a grandchild box passes its parent's box); the reported
-// hit is therefore the binary traversal's, except for exactly equal distances.
+// Closest hit (BVH::intersect, acceleration/BVH.cuh:365-398) over the 4-wide collapse of the render
+// tree (PLOC or the LBVH itself, over the same sorted leaves), children visited nearest first. A
+// 4-wide node tests the boxes of the binary tree's grandchildren and skips the children's boxes,
+// which never prunes a candidate a binary traversal would keep (a ray that passes a box passes its
+// ancestors' boxes). Every triangle whose box the ray passes before the current closest distance
+// is tested, so the reported hit is the reference traversal's, except for exactly equal distances.
 LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned long long* nodeCount, unsigned long long* triCount) {
     Hit h; h.leaf = -1; h.t = h.u = h.v = h.invDet = 0;
     const int n = sv.nPrim;
