@@ -788,7 +788,6 @@ int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* s
     if (n > kMaxPrims) return fail(c, LTS_E_ARG, "scene has more than 2^26 triangles");
     hipStream_t s = c->stream;
     HIPCHK(c, upload(&S.soup, H.soup.data(), H.soup.size(), s));
-    HIPCHK(c, upload(&S.shade, H.shade.data(), H.shade.size(), s));
     HIPCHK(c, upload(&S.mats, H.mats.data(), H.mats.size(), s));
     HIPCHK(c, upload(&S.texs, H.texs.data(), H.texs.size(), s));
     HIPCHK(c, upload(&S.mips, H.mips.data(), H.mips.size(), s));
@@ -806,7 +805,6 @@ int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* s
     }
     int depth = 0, nodes4 = 0;
     HIPCHK(c, bvh4_enqueue(S.bvh, s, &depth, &nodes4));
-    HIPCHK(c, launch_tri_materials(S.bvh.tris, S.shade, n, s));
     HIPCHK(c, hipStreamSynchronize(s));
     // the Karras tree of at most 2^26 leaves is at most 57 deep; the check guards the stack bound
     if (depth > kMaxRenderDepth) return fail(c, LTS_E_STATE, "render tree deeper than the traversal stack allows");
@@ -817,6 +815,11 @@ int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* s
     HIPCHK(c, hipMemcpy(perm.data(), S.bvh.valsA, n * sizeof(int32_t), hipMemcpyDeviceToHost));
     std::vector<int32_t> leafOf(n);
     for (int k = 0; k < n; k++) leafOf[perm[k]] = k;
+    // shading records in leaf order, so a hit reads its record alongside its triangle
+    std::vector<ShadeTri> shadeLeaf(n);
+    for (int k = 0; k < n; k++) shadeLeaf[k] = H.shade[perm[k]];
+    HIPCHK(c, upload(&S.shade, shadeLeaf.data(), shadeLeaf.size(), s));
+    HIPCHK(c, launch_tri_materials(S.bvh.tris, S.shade, n, s));
     std::vector<LightDesc> lights = H.lights;
     for (LightDesc& L : lights)
         if (L.kind == 0) L.leaf = leafOf[L.prim];

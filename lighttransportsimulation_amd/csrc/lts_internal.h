@@ -87,7 +87,7 @@ struct RenderLaunch {
 hipError_t launch_megakernel(const SceneView& sv, const RenderLaunch& rl, hipStream_t stream);
 hipError_t launch_film_gather(const SceneView& sv, const RenderLaunch& rl, hipStream_t stream);
 hipError_t launch_film_clear(float* film, int npix, hipStream_t stream);
-// tris[k].mat = shade[tris[k].prim].mat for the n leaves of a loaded scene.
+// tris[k].mat = shade[k].mat for the n leaves of a loaded scene (shade is in leaf order).
 hipError_t launch_tri_materials(TriLeaf* tris, const ShadeTri* shade, int n, hipStream_t stream);
 hipError_t launch_get_render(const SceneView& sv, const float* film, float* rgb, hipStream_t stream);
 // plain copy of `bytes` (a multiple of 16) with 16-byte vector loads and stores

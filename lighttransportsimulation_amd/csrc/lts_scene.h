@@ -3,7 +3,7 @@
 // HBM layout of one loaded scene:
 //   tris     TriLeaf[n]   48 B  vertices in BVH leaf order (leaf k = k-th primitive in Morton order)
 //   nodes    BvhNode[n-1] 64 B  both child boxes + child indices (unified index space)
-//   shade    ShadeTri[n]  80 B  per primitive: normals, uvs, material, light (read once per hit)
+//   shade    ShadeTri[n]  80 B  per leaf: normals, uvs, material, light (read once per hit)
 //   lights   LightDesc[L]       area lights in global triangle order, infinite light last
 //   mats/texs/mips              material, texture and mip-pyramid descriptors
 //   mipData  float[3*]          all pyramid levels, in the reference's blocked-array slot order
@@ -112,7 +112,7 @@ struct SceneView {
     const TriLeaf* tris;
     const BvhNode* nodes;
     const BvhQNode* nodes4;  // 4-wide collapse of `nodes`, root at 0 (used by the traversal)
-    const ShadeTri* shade;
+    const ShadeTri* shade;   // per BVH leaf: shade[k] belongs to the triangle tris[k]
     const MatDesc* mats;
     const TexDesc* texs;
     const MipDesc* mips;

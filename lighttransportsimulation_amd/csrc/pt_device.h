@@ -24,7 +24,7 @@ struct SurfacePoint {
     V3 sn, sdpdu;
     V3 dpdx, dpdy;
     float dudx, dvdx, dudy, dvdy;  // uv differentials (primary hits; read by filtered textures)
-    int prim;
+    int leaf;  // BVH leaf of the triangle, which indexes its ShadeTri record
 };
 
 LTS_D void load_tri(const TriLeaf* __restrict__ tris, int k, V3& p0, V3& p1, V3& p2, int& prim) {
@@ -251,7 +251,7 @@ LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts) {
 LTS_D SurfacePoint make_surface(const SceneView& sv, int leaf, const Ray& r, float u, float v, float invDet) {
     V3 p0, p1, p2; int prim;
     load_tri(sv.tris, leaf, p0, p1, p2, prim);
-    const ShadeTri& st = sv.shade[prim];
+    const ShadeTri& st = sv.shade[leaf];
     V3 edge1 = p1 - p0, edge2 = p2 - p0;
     float w = 1 - u - v;
     V2 uv0 = v2(st.uv[0], st.uv[1]), uv1 = v2(st.uv[2], st.uv[3]), uv2 = v2(st.uv[4], st.uv[5]);
@@ -276,7 +276,7 @@ LTS_D SurfacePoint make_surface(const SceneView& sv, int leaf, const Ray& r, flo
     s.n = normalize(cross(dpdu, dpdv));
     s.sn = s.n;
     s.sdpdu = dpdu;
-    s.prim = prim;
+    s.leaf = leaf;
     s.dpdx = s.dpdy = v3(0, 0, 0);
     s.dpdv = dpdv;
     s.dudx = s.dvdx = s.dudy = s.dvdy = 0.0f;
@@ -689,7 +689,7 @@ LTS_D Bsdf make_bsdf(const SceneView& sv, const SurfacePoint& s) {
     b.kind = -1; b.type = 0;
     b.R = b.T = spec(0.0f);
     b.A = b.B = b.eta = b.alpha = 0;
-    const MatDesc& M = sv.mats[sv.shade[s.prim].mat];
+    const MatDesc& M = sv.mats[sv.shade[s.leaf].mat];
     switch (M.type) {
         case MAT_MATTE: {
             Spec kd = eval_tex(sv, M.tex0, s);
@@ -726,7 +726,7 @@ LTS_D LightPoint tri_sample(const SceneView& sv, const LightDesc& L, V2 u) {
     V2 b = v2(1 - t, u.y * t);
     V3 p0, p1, p2; int prim;
     load_tri(sv.tris, L.leaf, p0, p1, p2, prim);
-    const ShadeTri& st = sv.shade[L.prim];
+    const ShadeTri& st = sv.shade[L.leaf];
     LightPoint lp;
     lp.p = p0 * b.x + p1 * b.y + p2 * (1 - b.x - b.y);
     if (st.hasN)
@@ -837,7 +837,7 @@ LTS_D Spec estimate_direct(const SceneView& sv, const SurfacePoint& si, const Bs
         Spec Li2 = spec(0.f);
         if (h.leaf >= 0) {
             SurfacePoint lt = make_surface(sv, h.leaf, r, h.u, h.v, h.invDet);
-            if (sv.shade[lt.prim].light == li) Li2 = dot(lt.n, -wi) > 0.f ? light_le(L) : spec(0.0f);
+            if (sv.shade[lt.leaf].light == li) Li2 = dot(lt.n, -wi) > 0.f ? light_le(L) : spec(0.0f);
         } else if (L.kind == 1) {
             Li2 = light_le(L);
         }

@@ -28,8 +28,7 @@ LTS_D Spec radiance(const SceneView& sv, int px, int py, int sample, const Rende
         Hit h = trace_closest(sv, r, ts, cnt, cnt ? cnt + 1 : nullptr);
         if (bounce == 0 || specularBounce) {
             if (h.leaf >= 0) {
-                int prim = sv.tris[h.leaf].prim;
-                int l = sv.shade[prim].light;
+                int l = sv.shade[h.leaf].light;
                 // the geometric normal decides emission; it needs the full interaction
                 if (l >= 0) {
                     SurfacePoint e = make_surface(sv, h.leaf, r, h.u, h.v, h.invDet);
@@ -74,7 +73,7 @@ LTS_D Spec radiance(const SceneView& sv, int px, int py, int sample, const Rende
 
 __global__ __launch_bounds__(256) void k_tri_materials(TriLeaf* __restrict__ tris, const ShadeTri* __restrict__ shade, int n) {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k < n) tris[k].mat = shade[tris[k].prim].mat;
+    if (k < n) tris[k].mat = shade[k].mat;
 }
 
 __global__ __launch_bounds__(256) void k_megakernel(SceneView sv, RenderLaunch rl) {

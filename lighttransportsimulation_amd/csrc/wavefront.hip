@@ -222,7 +222,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
             bool alive = true;
             if (bounce == 0 || specularBounce) {
                 if (hh.leaf >= 0) {
-                    const int l = sv.shade[sv.tris[hh.leaf].prim].light;
+                    const int l = sv.shade[hh.leaf].light;
                     if (l >= 0) {
                         SurfacePoint e = make_surface(sv, hh.leaf, r, hh.u, hh.v, hh.invDet);
                         Spec le = dot(e.n, -r.d) > 0.f ? light_le(sv.lights[l]) : spec(0.0f);
@@ -473,7 +473,7 @@ LTS_D void mis_item(const SceneView* scenes, const WfPath* paths, const WfMis* m
     Spec Li2 = spec(0.f);
     if (h.leaf >= 0) {
         SurfacePoint lt = make_surface(sv, h.leaf, r, h.u, h.v, h.invDet);
-        if (sv.shade[lt.prim].light == m.light) Li2 = dot(lt.n, -wi) > 0.f ? light_le(Lt) : spec(0.0f);
+        if (sv.shade[lt.leaf].light == m.light) Li2 = dot(lt.n, -wi) > 0.f ? light_le(Lt) : spec(0.0f);
     } else if (Lt.kind == 1) {
         Li2 = light_le(Lt);
     }
