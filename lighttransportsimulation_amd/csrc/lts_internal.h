@@ -95,7 +95,11 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t str
 
 // ---------------------------------------------------------------- wavefront tracer (wavefront.hip)
 constexpr int kWfAlive = 1, kWfSpecular = 2;
-constexpr int kWfCountsPerBounce = 8;  // queue, shadow, mis counts; extend, shadow, mis work counters
+// per bounce b: [0] path queue, [1] shadow rays, [2] MIS rays; the 8 work counters of the trace
+// launch of bounce b's path rays at kWfWorkBase + kWfWorkStride * j (one 64-byte line each)
+constexpr int kWfCountsPerBounce = 160;
+constexpr int kWfWorkBase = 32, kWfWorkStride = 16, kWfWorkSegments = 8;
+static_assert(kWfWorkBase + kWfWorkStride * kWfWorkSegments <= kWfCountsPerBounce, "work counters of a bounce");
 constexpr int kWfMaxMatBins = 64;
 constexpr int kWfMaxBounce = 64;
 

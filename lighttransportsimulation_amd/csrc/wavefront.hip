@@ -80,11 +80,23 @@ __global__ __launch_bounds__(256) void k_wf_generate(const SceneView* __restrict
     }
 }
 
-// Wave-batched work fetch: lane 0 takes 64 items from the launch's work counter.
-LTS_D int wave_fetch(int32_t* counter) {
-    int base = 0;
-    if ((threadIdx.x & 63) == 0) base = atomicAdd(counter, 64);
-    return __shfl(base, 0) + (threadIdx.x & 63);
+// Wave-batched work fetch (lane 0 takes 64 items at a time) over kWfWorkSegments contiguous segments of segLen items (a multiple of
+// 64), one counter each. Workgroups are dealt to the 8 XCDs round robin, so the waves of XCD
+// blockIdx % 8 start on their own segment and keep neighbouring rays in one L2; a wave whose
+// segment has run dry moves on to the next ones. Returns the item of this lane, with `j` (uniform
+// over the wave) advanced past the segments found empty; j == kWfWorkSegments means no work is left.
+LTS_D int wave_fetch_segmented(int32_t* work, int segLen, int& j) {
+    const int lane = threadIdx.x & 63;
+    const int home = blockIdx.x % kWfWorkSegments;
+    while (j < kWfWorkSegments) {
+        const int seg = (home + j) % kWfWorkSegments;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&work[kWfWorkStride * seg], 64);
+        base = __shfl(base, 0);
+        if (base < segLen) return seg * segLen + base + lane;
+        j++;
+    }
+    return 0;
 }
 
 // Closest hit of path ray `pi`; with `hist` set, the hit is also filed under its material bin
@@ -109,8 +121,9 @@ LTS_D void extend_item(const SceneView* scenes, const WfPath* paths, const WfRay
 }
 
 // Counting sort of a queue by the material bin the extend kernel filed each hit under: one block
-// turns the histogram into bin offsets, then each block ranks its entries per bin in LDS and
-// reserves its range of every bin with one atomic per bin.
+// turns the histogram into bin offsets; each scatter block then takes one contiguous chunk of the
+// queue, counts its entries per bin, reserves its range of every bin with one atomic per bin and
+// writes the chunk's entries into those ranges, ranked within each wave by ballots.
 __global__ void k_wf_mat_scan(int32_t* __restrict__ hist, int nBins) {
     if (threadIdx.x == 0) {
         int run = 0;
@@ -118,28 +131,51 @@ __global__ void k_wf_mat_scan(int32_t* __restrict__ hist, int nBins) {
     }
 }
 
+// Rank of each lane among the lanes of its wave holding the same bin (bin < 0: none), offset by
+// the running count h[bin], which grows by the number of such lanes (one LDS atomic per bin).
+LTS_D int wave_bin_rank(int bin, int32_t* h) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = lane ? ~0ull >> (64 - lane) : 0ull;
+    int slot = 0;
+    uint64_t todo = __ballot(bin >= 0);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int b = __shfl(bin, leader);
+        const uint64_t peers = __ballot(bin == b);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&h[b], (int)__popcll(peers));
+        base = __shfl(base, leader);
+        if (bin == b) slot = base + (int)__popcll(peers & below);
+        todo &= ~peers;
+    }
+    return slot;
+}
+
 __global__ __launch_bounds__(256) void k_wf_mat_scatter(const int32_t* __restrict__ bins, const int32_t* __restrict__ queue,
                                                         const int32_t* __restrict__ count, int32_t* __restrict__ offs, int nBins,
                                                         int32_t* __restrict__ sorted) {
     __shared__ int32_t h[kWfMaxMatBins], base[kWfMaxMatBins];
     const int n = *count;
-    const int stride = gridDim.x * 256;
-    const int rounds = (n + stride - 1) / stride;
-    for (int round = 0; round < rounds; round++) {
-        if (threadIdx.x < kWfMaxMatBins) h[threadIdx.x] = 0;
-        __syncthreads();
-        const int k = round * stride + blockIdx.x * 256 + threadIdx.x;
-        int pi = -1, bin = 0, slot = 0;
-        if (k < n) {
-            pi = queue[k];
-            bin = bins[pi];
-            slot = atomicAdd(&h[bin], 1);
-        }
-        __syncthreads();
-        if (threadIdx.x < nBins) base[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&offs[threadIdx.x], h[threadIdx.x]) : 0;
-        __syncthreads();
+    const int per = ((n + (int)gridDim.x - 1) / (int)gridDim.x + 255) / 256 * 256;
+    const int begin = min((int)blockIdx.x * per, n), end = min(begin + per, n);
+    if (threadIdx.x < kWfMaxMatBins) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int k0 = begin; k0 < end; k0 += 256) {
+        const int k = k0 + threadIdx.x;
+        (void)wave_bin_rank(k < end ? bins[queue[k]] : -1, h);
+    }
+    __syncthreads();
+    if (threadIdx.x < nBins) {
+        base[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&offs[threadIdx.x], h[threadIdx.x]) : 0;
+        h[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (int k0 = begin; k0 < end; k0 += 256) {
+        const int k = k0 + threadIdx.x;
+        const int pi = k < end ? queue[k] : -1;
+        const int bin = pi >= 0 ? bins[pi] : -1;
+        const int slot = wave_bin_rank(bin, h);
         if (pi >= 0) sorted[base[bin] + slot] = pi;
-        __syncthreads();
     }
 }
 
@@ -196,8 +232,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
     const bool cached = qRaw != nullptr;
     const int stride = gridDim.x * kShadeBlock;
     const int rounds = (n + stride - 1) / stride;  // uniform trip count: the appends below are wave-collective
+    // chunk of the block within a round: workgroups are dealt to the 8 XCDs round robin, so the
+    // blocks of one XCD take neighbouring chunks (neighbouring paths hit neighbouring geometry)
+    const int chunk = gridDim.x % 8 == 0 ? (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8 : blockIdx.x;
     for (int round = 0; round < rounds; round++) {
-        const int k = round * stride + blockIdx.x * kShadeBlock + threadIdx.x;
+        const int k = round * stride + chunk * kShadeBlock + threadIdx.x;
         const bool active = k < n;
         const int pi = active ? queue[k] : 0;
         bool wantShadow = false, wantMis = false, wantNext = false, wantQuery = false;
@@ -509,9 +548,11 @@ __global__ __launch_bounds__(BS) void k_wf_trace(const SceneView* __restrict__ s
     const int nS = shadowCount ? *shadowCount : 0;
     const int nM = misCount ? *misCount : 0;
     const int n = nE + nS + nM;
+    const int segLen = ((n + kWfWorkSegments - 1) / kWfWorkSegments + 63) / 64 * 64;
+    int j = 0;
     for (;;) {
-        const int k = wave_fetch(work);
-        if (k - (int)(threadIdx.x & 63) >= n) break;
+        const int k = wave_fetch_segmented(work, segLen, j);
+        if (j >= kWfWorkSegments) break;
         if (k >= n) continue;
         if (k < nE) extend_item(scenes, paths, rays, hits, ext[k], ts, bins, lh, filing, nBins);
         else if (k < nE + nS) shadow_item(scenes, paths, shadows, nee, k - nE, ts);
@@ -587,9 +628,10 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
         return t.sortMaterials ? b.counts + kWfCountsPerBounce * (maxBounce + 1) + kWfMaxMatBins * bounce : nullptr;
     };
     // traces the path rays of `bounce` (queue q, count cnt[0]) and the shadow and MIS rays of the
-    // bounce before it (counts prev[1], prev[2]); work counter `work`
-    auto trace = [&](int bounce, const int32_t* extCount, const int32_t* shadowCount, const int32_t* misCount, int32_t* work) {
+    // bounce before it (counts prev[1], prev[2]), with the work counters of `bounce`
+    auto trace = [&](int bounce, const int32_t* extCount, const int32_t* shadowCount, const int32_t* misCount) {
         const int32_t* ext = b.queue[bounce & 1];
+        int32_t* work = b.counts + kWfCountsPerBounce * bounce + kWfWorkBase;
         int32_t* hist = histOf(bounce);
         switch (t.traceBlock) {
             case 64:
@@ -607,7 +649,7 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
         }
     };
     mark(8 * kWfMaxBounce + 3);
-    trace(0, b.counts, nullptr, nullptr, b.counts + 3);
+    trace(0, b.counts, nullptr, nullptr);
     for (int bounce = 0; bounce < maxBounce; bounce++) {
         const int q = bounce & 1;
         int32_t* cnt = b.counts + kWfCountsPerBounce * bounce;
@@ -626,7 +668,7 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
                                                    cached ? b.qRaw : nullptr, b.qMeta, b.qQueue, qCount, b.tRec, b.tRaw);
         mark(8 * bounce + 2);
         // the next bounce's path rays (none after the last bounce) with this bounce's shadow and MIS rays
-        trace(bounce + 1, nxt, cnt + 1, cnt + 2, cnt + 4);
+        trace(bounce + 1, nxt, cnt + 1, cnt + 2);
         mark(8 * bounce + 3);
         k_wf_finalize<<<gridAll, 256, 0, s>>>(b.params, b.slots, bounce, b.paths, b.nee, shadeQueue, cnt);
         mark(8 * bounce + 4);
