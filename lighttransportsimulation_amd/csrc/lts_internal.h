@@ -95,7 +95,8 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t str
 
 // ---------------------------------------------------------------- wavefront tracer (wavefront.hip)
 constexpr int kWfAlive = 1, kWfSpecular = 2;
-// per bounce b: [0] path queue, [1] shadow rays, [2] MIS rays; the 8 work counters of the trace
+// per bounce b: [0] path queue, [1] shadow rays, [2] MIS rays, [3] ended paths awaiting their
+// light sample; the 8 work counters of the trace
 // launch of bounce b's path rays at kWfWorkBase + kWfWorkStride * j (one 64-byte line each)
 constexpr int kWfCountsPerBounce = 160;
 constexpr int kWfWorkBase = 32, kWfWorkStride = 16, kWfWorkSegments = 8;
@@ -161,6 +162,7 @@ struct WfBuffers {
     WfNee* nee = nullptr;
     int32_t* queue[2] = {nullptr, nullptr};
     int32_t* sorted = nullptr;
+    int32_t* done = nullptr;      // paths that ended with a light sample still in flight
     int32_t* bins = nullptr;      // material bin of each path's hit, written by the extend kernel
     WfShadow* shadows = nullptr;
     WfMis* mis = nullptr;

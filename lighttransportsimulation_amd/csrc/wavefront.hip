@@ -226,7 +226,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
                                                           WfMis* __restrict__ misRays, int32_t* __restrict__ misCount,
                                                           float* __restrict__ qRaw, NrcQuery* __restrict__ qMeta,
                                                           int32_t* __restrict__ qQueue, int32_t* __restrict__ qCount,
-                                                          WfTrainRec* __restrict__ tRec, float* __restrict__ tRaw) {
+                                                          WfTrainRec* __restrict__ tRec, float* __restrict__ tRaw,
+                                                          int32_t* __restrict__ doneQueue, int32_t* __restrict__ doneCount) {
     const int n = *count;
     const int maxBounce = prm->maxBounce, rrStart = prm->rrStart, queryBounce = prm->queryBounce;
     const bool cached = qRaw != nullptr;
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
         const int k = round * stride + chunk * kShadeBlock + threadIdx.x;
         const bool active = k < n;
         const int pi = active ? queue[k] : 0;
-        bool wantShadow = false, wantMis = false, wantNext = false, wantQuery = false;
+        bool wantShadow = false, wantMis = false, wantNext = false, wantQuery = false, wantDone = false;
         WfShadow sh;
         WfMis ms;
         if (active) {
@@ -253,6 +254,15 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
             r.d = v3(rr.d[0], rr.d[1], rr.d[2]);
             r.tMax = INFINITY;
             Spec L = spec(p.L[0], p.L[1], p.L[2]), beta = spec(p.beta[0], p.beta[1], p.beta[2]);
+            if (bounce > 0) {
+                // the light sample of the previous vertex, traced since: added before anything of
+                // this vertex, as the reference's loop adds it before the next intersection
+                const WfNee pn = nee[pi];
+                if (pn.used) {
+                    Spec Ld = spec(pn.p1[0], pn.p1[1], pn.p1[2]) + spec(pn.p2[0], pn.p2[1], pn.p2[2]);
+                    L = L + spec(pn.nb[0], pn.nb[1], pn.nb[2]) * (Ld / pn.lpdf);
+                }
+            }
             const bool specularBounce = (p.flags & kWfSpecular) != 0;
             WfNee ne;
             ne.used = 0;
@@ -393,6 +403,15 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
             paths[pi] = p;
             nee[pi] = ne;
             wantNext = alive && bounce + 1 < maxBounce;
+            if (!wantNext) {
+                // the path ends here: its sample is final now, or once its light sample is traced
+                if (ne.used) {
+                    wantDone = true;
+                } else {
+                    float* o = sl.sampleL + 3 * ((size_t)(p.sample - sl.s0) * sl.width * sl.height + p.pixel);
+                    o[0] = L.r; o[1] = L.g; o[2] = L.b;
+                }
+            }
         }
         const int si = wave_append(shadowCount, wantShadow);
         if (wantShadow) shadows[si] = sh;
@@ -400,6 +419,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
         if (wantMis) misRays[mi] = ms;
         const int ni = wave_append(nextCount, wantNext);
         if (wantNext) nextQueue[ni] = pi;
+        const int di = wave_append(doneCount, wantDone);
+        if (wantDone) doneQueue[di] = pi;
         if (cached) {
             const int qi = wave_append(qCount, wantQuery);
             if (wantQuery) qQueue[qi] = pi;
@@ -566,27 +587,22 @@ __global__ __launch_bounds__(BS, 8) void k_wf_trace(const SceneView* __restrict_
     }
 }
 
-__global__ __launch_bounds__(256) void k_wf_finalize(const WfParams* __restrict__ prm, const WfSceneSlot* __restrict__ slots,
-                                                     int bounce, WfPath* __restrict__ paths, const WfNee* __restrict__ nee,
-                                                     const int32_t* __restrict__ queue, const int32_t* __restrict__ count) {
+// Paths that ended at this bounce with a light sample in flight: the traced sample completes their
+// radiance (a continuing path takes its sample at the start of the next bounce's shading).
+__global__ __launch_bounds__(256) void k_wf_finalize(const WfSceneSlot* __restrict__ slots, const WfPath* __restrict__ paths,
+                                                     const WfNee* __restrict__ nee, const int32_t* __restrict__ done,
+                                                     const int32_t* __restrict__ count) {
     const int n = *count;
-    const bool last = bounce + 1 >= prm->maxBounce;
     for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
-        const int pi = queue[k];
-        WfPath p = paths[pi];
+        const int pi = done[k];
+        const WfPath p = paths[pi];
         const WfNee ne = nee[pi];
         Spec L = spec(p.L[0], p.L[1], p.L[2]);
-        if (ne.used) {
-            Spec Ld = spec(ne.p1[0], ne.p1[1], ne.p1[2]) + spec(ne.p2[0], ne.p2[1], ne.p2[2]);
-            L = L + spec(ne.nb[0], ne.nb[1], ne.nb[2]) * (Ld / ne.lpdf);
-            p.L[0] = L.r; p.L[1] = L.g; p.L[2] = L.b;
-            paths[pi] = p;
-        }
-        if (!(p.flags & kWfAlive) || last) {
-            const WfSceneSlot& sl = slots[p.scene];
-            float* o = sl.sampleL + 3 * ((size_t)(p.sample - sl.s0) * sl.width * sl.height + p.pixel);
-            o[0] = L.r; o[1] = L.g; o[2] = L.b;
-        }
+        Spec Ld = spec(ne.p1[0], ne.p1[1], ne.p1[2]) + spec(ne.p2[0], ne.p2[1], ne.p2[2]);
+        L = L + spec(ne.nb[0], ne.nb[1], ne.nb[2]) * (Ld / ne.lpdf);
+        const WfSceneSlot& sl = slots[p.scene];
+        float* o = sl.sampleL + 3 * ((size_t)(p.sample - sl.s0) * sl.width * sl.height + p.pixel);
+        o[0] = L.r; o[1] = L.g; o[2] = L.b;
     }
 }
 
@@ -666,12 +682,13 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
         mark(8 * bounce + 1);
         k_wf_shade<<<gridAll, kShadeBlock, 0, s>>>(b.scenes, b.params, b.slots, bounce, b.paths, b.rays, b.hits, b.nee, shadeQueue,
                                                    cnt, b.queue[q ^ 1], nxt, b.shadows, cnt + 1, b.mis, cnt + 2,
-                                                   cached ? b.qRaw : nullptr, b.qMeta, b.qQueue, qCount, b.tRec, b.tRaw);
+                                                   cached ? b.qRaw : nullptr, b.qMeta, b.qQueue, qCount, b.tRec, b.tRaw,
+                                                   b.done, cnt + 3);
         mark(8 * bounce + 2);
         // the next bounce's path rays (none after the last bounce) with this bounce's shadow and MIS rays
         trace(bounce + 1, nxt, cnt + 1, cnt + 2);
         mark(8 * bounce + 3);
-        k_wf_finalize<<<gridAll, 256, 0, s>>>(b.params, b.slots, bounce, b.paths, b.nee, shadeQueue, cnt);
+        k_wf_finalize<<<gridAll, 256, 0, s>>>(b.slots, b.paths, b.nee, b.done, cnt + 3);
         mark(8 * bounce + 4);
     }
     if (cached) {
@@ -717,6 +734,7 @@ hipError_t wf_reserve(WfBuffers& b, size_t cap, int maxScenes) {
     WTRY(wf_alloc(&b.queue[0], cap));
     WTRY(wf_alloc(&b.queue[1], cap));
     WTRY(wf_alloc(&b.sorted, cap));
+    WTRY(wf_alloc(&b.done, cap));
     WTRY(wf_alloc(&b.bins, cap));
     WTRY(wf_alloc(&b.shadows, cap));
     WTRY(wf_alloc(&b.mis, cap));
@@ -754,7 +772,7 @@ void wf_release(WfBuffers& b) {
     void* cs[] = {b.qRaw, b.qMeta, b.qQueue, b.tRec, b.tRaw, b.tCount};
     for (void* p : cs)
         if (p) (void)hipFree(p);
-    void* ps[] = {b.scenes, b.params, b.slots, b.paths, b.rays, b.hits, b.nee, b.queue[0], b.queue[1], b.sorted, b.bins, b.shadows, b.mis, b.counts};
+    void* ps[] = {b.scenes, b.params, b.slots, b.paths, b.rays, b.hits, b.nee, b.queue[0], b.queue[1], b.sorted, b.done, b.bins, b.shadows, b.mis, b.counts};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     b = WfBuffers();
