@@ -666,7 +666,8 @@ LTS_D Spec mip_lookup(const SceneView& sv, const MipDesc& m, V2 st, float width)
     return lerpS(delta, mip_triangle(sv, m, iLevel, st), mip_triangle(sv, m, iLevel + 1, st));
 }
 
-LTS_D Spec eval_tex(const SceneView& sv, int ti, const SurfacePoint& s) {
+// `perm` is the Perlin permutation (sv.perm or a copy of it in LDS).
+LTS_D Spec eval_tex(const SceneView& sv, int ti, const SurfacePoint& s, const int* perm) {
     const TexDesc& X = sv.texs[ti];
     switch (X.type) {
         case TEX_CONST: return spec(X.c[0], X.c[1], X.c[2]);
@@ -676,15 +677,15 @@ LTS_D Spec eval_tex(const SceneView& sv, int ti, const SurfacePoint& s) {
             const float width = 2 * fmaxf(fmaxf(fabsf(s.dudx), fabsf(s.dvdx)), fmaxf(fabsf(s.dudy), fabsf(s.dvdy)));
             return mip_lookup(sv, sv.mips[X.img], v2(1 * s.uv.x + 0, 1 * s.uv.y + 0), width);
         }
-        case TEX_FBM: return spec(fbm(sv.perm, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy), X.omega, X.octaves));
-        case TEX_WRINKLED: return spec(turbulence(sv.perm, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy), X.omega, X.octaves));
-        case TEX_MARBLE: return marble_tex(sv.perm, X, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy));
+        case TEX_FBM: return spec(fbm(perm, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy), X.omega, X.octaves));
+        case TEX_WRINKLED: return spec(turbulence(perm, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy), X.omega, X.octaves));
+        case TEX_MARBLE: return marble_tex(perm, X, identityMap(s.p), identityMap(s.dpdx), identityMap(s.dpdy));
     }
     return spec(0.0f);
 }
 
 // Material::computeScatteringFunctions (materials/Material.cuh:31-130)
-LTS_D Bsdf make_bsdf(const SceneView& sv, const SurfacePoint& s) {
+LTS_D Bsdf make_bsdf(const SceneView& sv, const SurfacePoint& s, const int* perm) {
     Bsdf b;
     b.ns = s.sn; b.ng = s.n;
     b.ss = normalize(s.sdpdu);
@@ -695,24 +696,24 @@ LTS_D Bsdf make_bsdf(const SceneView& sv, const SurfacePoint& s) {
     const MatDesc& M = sv.mats[sv.shade[s.leaf].mat];
     switch (M.type) {
         case MAT_MATTE: {
-            Spec kd = eval_tex(sv, M.tex0, s);
+            Spec kd = eval_tex(sv, M.tex0, s, perm);
             if (!isBlack(kd)) { b.R = kd; b.type = B_REFL | B_DIFF; b.kind = M.extra == 0 ? 0 : 1; b.A = M.onA; b.B = M.onB; }
             break;
         }
         case MAT_GLASS: {
-            Spec r = eval_tex(sv, M.tex0, s), t = eval_tex(sv, M.tex1, s);
+            Spec r = eval_tex(sv, M.tex0, s, perm), t = eval_tex(sv, M.tex1, s, perm);
             if (isBlack(r) && isBlack(t)) break;
             b.kind = 3; b.type = B_REFL | B_TRANS | B_SPEC; b.R = r; b.T = t; b.eta = M.extra;
             break;
         }
         case MAT_MIRROR: {
-            Spec r = eval_tex(sv, M.tex0, s);
+            Spec r = eval_tex(sv, M.tex0, s, perm);
             if (isBlack(r)) break;
             b.kind = 2; b.type = B_REFL | B_SPEC; b.R = r;
             break;
         }
         case MAT_MICROFACET: {
-            Spec r = eval_tex(sv, M.tex0, s);
+            Spec r = eval_tex(sv, M.tex0, s, perm);
             if (isBlack(r)) break;
             b.kind = 4; b.type = B_REFL | B_GLOSSY; b.R = r; b.alpha = M.alpha;
             break;

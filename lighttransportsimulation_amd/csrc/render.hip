@@ -15,7 +15,7 @@ constexpr int kTileSide = 16;
 
 // One camera sample of PathTracingIntegrator::evaluate.
 LTS_D Spec radiance(const SceneView& sv, int px, int py, int sample, const RenderLaunch& rl, unsigned long long* cnt,
-                     TraceStack ts) {
+                     TraceStack ts, const int* perm) {
     Sampler smp;
     smp.seed = rl.seed;
     smp.pixel = (uint32_t)(py * sv.cam.width + px);
@@ -44,7 +44,7 @@ LTS_D Spec radiance(const SceneView& sv, int px, int py, int sample, const Rende
         if (h.leaf < 0) break;
         SurfacePoint si = make_surface(sv, h.leaf, r, h.u, h.v, h.invDet);
         if (bounce == 0) differentials(si, cr);
-        Bsdf bsdf = make_bsdf(sv, si);
+        Bsdf bsdf = make_bsdf(sv, si, perm);
         float u0[4], u1[4];
         smp.block(1 + 2 * bounce, u0);
         smp.block(2 + 2 * bounce, u1);
@@ -78,13 +78,16 @@ __global__ __launch_bounds__(256) void k_tri_materials(TriLeaf* __restrict__ tri
 
 __global__ __launch_bounds__(256) void k_megakernel(SceneView sv, RenderLaunch rl) {
     __shared__ int stk[kShortStack * 256];
+    __shared__ int perm[512];  // the procedural textures' permutation lookups chain three deep
+    for (int i = threadIdx.x; i < 512; i += 256) perm[i] = sv.perm[i];
+    __syncthreads();
     const int W = sv.cam.width, H = sv.cam.height;
     const int tilesX = (W + kTileSide - 1) / kTileSide;
     const int px = (blockIdx.x % tilesX) * kTileSide + (threadIdx.x & (kTileSide - 1));
     const int py = (blockIdx.x / tilesX) * kTileSide + (threadIdx.x / kTileSide);
     const int sample = rl.s0 + blockIdx.y;
     if (px >= W || py >= H || sample >= rl.s1) return;
-    Spec L = radiance(sv, px, py, sample, rl, rl.counters, lds_stack(stk, threadIdx.x, 256));
+    Spec L = radiance(sv, px, py, sample, rl, rl.counters, lds_stack(stk, threadIdx.x, 256), perm);
     float* o = rl.sampleL + 3 * ((size_t)blockIdx.y * W * H + (size_t)py * W + px);
     o[0] = L.r; o[1] = L.g; o[2] = L.b;
 }

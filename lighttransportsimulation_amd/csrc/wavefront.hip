@@ -231,6 +231,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
     const int n = *count;
     const int maxBounce = prm->maxBounce, rrStart = prm->rrStart, queryBounce = prm->queryBounce;
     const bool cached = qRaw != nullptr;
+    // the procedural textures' permutation lookups chain three deep: served from LDS (every scene
+    // view points at its context's one table)
+    __shared__ int perm[512];
+    for (int i = threadIdx.x; i < 512; i += kShadeBlock) perm[i] = scenes[0].perm[i];
+    __syncthreads();
     const int stride = gridDim.x * kShadeBlock;
     const int rounds = (n + stride - 1) / stride;  // uniform trip count: the appends below are wave-collective
     // chunk of the block within a round: workgroups are dealt to the 8 XCDs round robin, so the
@@ -291,7 +296,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
                     const int px = p.pixel % sv.cam.width, py = p.pixel / sv.cam.width;
                     differentials(si, camera_sample(sv, px, py, p.sample, sl.spp, sl.stratified, sl.seed));
                 }
-                Bsdf bsdf = make_bsdf(sv, si);
+                Bsdf bsdf = make_bsdf(sv, si, perm);
                 const int slot = (p.sample - sl.s0) * sl.width * sl.height + p.pixel;  // sample index of the path
                 if (cached && (bsdf.kind == 0 || bsdf.kind == 1 || bsdf.kind == 4)) {
                     if (p.trainSlot >= 0) {
