@@ -137,7 +137,7 @@ struct WfSceneSlot {
 struct alignas(16) WfTrainRec { float Lb[3]; float beta[3]; float a[3]; int32_t slot; int32_t valid; int32_t pad; };
 
 struct WfTuning {
-    int traceBlock = 128;         // block size of the persistent traversal kernels (64, 128, 256)
+    int traceBlock = 256;         // block size of the persistent traversal kernels (64, 128, 256)
     int persistentBlocks = 2048;  // grid of the traversal kernels
     int sweepBlocks = 2048;       // grid of the generate / shade / finalize sweeps
     int sortMaterials = 1;        // counting-sort the shade queue by material
