@@ -145,8 +145,9 @@ struct Sampler {
 };
 
 // Decoded plane of a quantized render-node box: origin + q * 2^e. The product is exact (q has 8
-// bits and e is at least -126), so the only rounding is the addition; the builder evaluates this
-// same expression when it picks q, which keeps decoded boxes conservative.
-LTS_D float qdecode(float origin, int e, unsigned q) { return origin + (float)q * __int_as_float((e + 127) << 23); }
+// bits and e is at least -126), so the only rounding is the addition, and the fused form is one
+// instruction; the builder evaluates this same expression when it picks q, which keeps decoded
+// boxes conservative.
+LTS_D float qdecode(float origin, int e, unsigned q) { return __builtin_fmaf((float)q, __int_as_float((e + 127) << 23), origin); }
 
 }  // namespace lts

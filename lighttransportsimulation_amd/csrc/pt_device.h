@@ -114,6 +114,8 @@ LTS_D TraceStack lds_stack(int* shared, int tid, int blockSize) { return TraceSt
 // Child boxes of a 4-wide node: hit distance per slot (INFINITY for a miss or an empty slot). The
 // quantized boxes are decoded with qdecode, as the builder did, and contain the exact child boxes;
 // the slab test is monotone in the planes, so a ray that passes an exact box passes its decoded box.
+// The test is box_hit's, step for step (NaN cases included); the ray's direction signs pick the
+// near and far plane bytes once per axis for all four children, so only those planes are decoded.
 LTS_D void node4_hits(const BvhQNode* nodes4, int cur, const Ray& r, V3 inv, int nx, int ny, int nz, float (&t)[4],
                       int (&c)[4], int& count) {
     const uint4* q = reinterpret_cast<const uint4*>(nodes4 + cur);
@@ -122,17 +124,31 @@ LTS_D void node4_hits(const BvhQNode* nodes4, int cur, const Ray& r, V3 inv, int
     const int ex = (int)(int8_t)(h.w & 0xffu), ey = (int)(int8_t)((h.w >> 8) & 0xffu), ez = (int)(int8_t)((h.w >> 16) & 0xffu);
     count = (int)(h.w >> 24);
     // lo = (qlo x, qlo y, qlo z, qhi x), hi = (qhi y, qhi z, pad, pad); byte k belongs to child k
+    const unsigned nearX = nx ? lo.w : lo.x, farX = nx ? lo.x : lo.w;
+    const unsigned nearY = ny ? hi.x : lo.y, farY = ny ? lo.y : hi.x;
+    const unsigned nearZ = nz ? hi.y : lo.z, farZ = nz ? lo.z : hi.y;
+    const float g3 = 1 + 2 * gammaN(3);
     const int cc[4] = {(int)ch.x, (int)ch.y, (int)ch.z, (int)ch.w};
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const unsigned s = 8 * k;
-        const float mn[3] = {qdecode(ox, ex, (lo.x >> s) & 0xffu), qdecode(oy, ey, (lo.y >> s) & 0xffu),
-                             qdecode(oz, ez, (lo.z >> s) & 0xffu)};
-        const float mx[3] = {qdecode(ox, ex, (lo.w >> s) & 0xffu), qdecode(oy, ey, (hi.x >> s) & 0xffu),
-                             qdecode(oz, ez, (hi.y >> s) & 0xffu)};
-        float tk;
-        const bool hit = k < count && box_hit(mn, mx, r, inv, nx, ny, nz, tk);
-        t[k] = hit ? tk : INFINITY;
+        float tMin = (qdecode(ox, ex, (nearX >> s) & 0xffu) - r.o.x) * inv.x;
+        float tMax = (qdecode(ox, ex, (farX >> s) & 0xffu) - r.o.x) * inv.x;
+        const float tyMin = (qdecode(oy, ey, (nearY >> s) & 0xffu) - r.o.y) * inv.y;
+        float tyMax = (qdecode(oy, ey, (farY >> s) & 0xffu) - r.o.y) * inv.y;
+        tMax *= g3;
+        tyMax *= g3;
+        bool hit = k < count && !(tMin > tyMax || tyMin > tMax);
+        if (tyMin > tMin) tMin = tyMin;
+        if (tyMax < tMax) tMax = tyMax;
+        const float tzMin = (qdecode(oz, ez, (nearZ >> s) & 0xffu) - r.o.z) * inv.z;
+        float tzMax = (qdecode(oz, ez, (farZ >> s) & 0xffu) - r.o.z) * inv.z;
+        tzMax *= g3;
+        hit = hit && !(tMin > tzMax || tzMin > tMax);
+        if (tzMin > tMin) tMin = tzMin;
+        if (tzMax < tMax) tMax = tzMax;
+        hit = hit && tMin < r.tMax && tMax > 0;
+        t[k] = hit ? tMin : INFINITY;
         c[k] = cc[k];
     }
 }
