@@ -114,8 +114,13 @@ LTS_D TraceStack lds_stack(int* shared, int tid, int blockSize) { return TraceSt
 // Child boxes of a 4-wide node: hit distance per slot (INFINITY for a miss or an empty slot). The
 // quantized boxes are decoded with qdecode, as the builder did, and contain the exact child boxes;
 // the slab test is monotone in the planes, so a ray that passes an exact box passes its decoded box.
-// The test is box_hit's, step for step (NaN cases included); the ray's direction signs pick the
-// near and far plane bytes once per axis for all four children, so only those planes are decoded.
+// The ray's direction signs pick the near and far plane bytes once per axis for all four children,
+// so only those planes are decoded. The slab test is box_hit's in closed form: its pairwise
+// comparisons pass exactly when the largest near distance is at most the smallest far distance
+// (far distances scaled by 1 + 2 gamma(3); rounding is monotone, so scaling the minimum equals the
+// minimum of the scaled values), then the entry lies before tMax and the exit after 0. The forms
+// agree on every input without NaN; a NaN slab distance (a zero direction component with the
+// origin exactly on a plane) is ignored here on every axis, where box_hit culls on x only.
 LTS_D void node4_hits(const BvhQNode* nodes4, int cur, const Ray& r, V3 inv, int nx, int ny, int nz, float (&t)[4],
                       int (&c)[4], int& count) {
     const uint4* q = reinterpret_cast<const uint4*>(nodes4 + cur);
@@ -132,22 +137,15 @@ LTS_D void node4_hits(const BvhQNode* nodes4, int cur, const Ray& r, V3 inv, int
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const unsigned s = 8 * k;
-        float tMin = (qdecode(ox, ex, (nearX >> s) & 0xffu) - r.o.x) * inv.x;
-        float tMax = (qdecode(ox, ex, (farX >> s) & 0xffu) - r.o.x) * inv.x;
+        const float txMin = (qdecode(ox, ex, (nearX >> s) & 0xffu) - r.o.x) * inv.x;
+        const float txMax = (qdecode(ox, ex, (farX >> s) & 0xffu) - r.o.x) * inv.x;
         const float tyMin = (qdecode(oy, ey, (nearY >> s) & 0xffu) - r.o.y) * inv.y;
-        float tyMax = (qdecode(oy, ey, (farY >> s) & 0xffu) - r.o.y) * inv.y;
-        tMax *= g3;
-        tyMax *= g3;
-        bool hit = k < count && !(tMin > tyMax || tyMin > tMax);
-        if (tyMin > tMin) tMin = tyMin;
-        if (tyMax < tMax) tMax = tyMax;
+        const float tyMax = (qdecode(oy, ey, (farY >> s) & 0xffu) - r.o.y) * inv.y;
         const float tzMin = (qdecode(oz, ez, (nearZ >> s) & 0xffu) - r.o.z) * inv.z;
-        float tzMax = (qdecode(oz, ez, (farZ >> s) & 0xffu) - r.o.z) * inv.z;
-        tzMax *= g3;
-        hit = hit && !(tMin > tzMax || tzMin > tMax);
-        if (tzMin > tMin) tMin = tzMin;
-        if (tzMax < tMax) tMax = tzMax;
-        hit = hit && tMin < r.tMax && tMax > 0;
+        const float tzMax = (qdecode(oz, ez, (farZ >> s) & 0xffu) - r.o.z) * inv.z;
+        const float tMin = fmaxf(fmaxf(txMin, tyMin), tzMin);
+        const float tMax = fminf(fminf(txMax, tyMax), tzMax) * g3;
+        const bool hit = k < count && tMin <= tMax && tMin < r.tMax && tMax > 0;
         t[k] = hit ? tMin : INFINITY;
         c[k] = cc[k];
     }
