@@ -62,7 +62,7 @@ typedef struct lts_render_params {
     int32_t stratified;   /* 1: film sample per Sampler::stratifiedSample          */
     int32_t integrator;   /* LTS_INTEGRATOR_*                                      */
     int32_t spp_per_pass; /* samples per pass; 0 = automatic (wavefront: 2^26 paths */
-                          /* per pass, cached: 2^24; megakernel: 2^28 / 12 B)       */
+                          /* per pass, cached: 2^25; megakernel: 2^28 / 12 B)       */
     /* wavefront only (0 / -1 = default or the autotuned value) */
     int32_t trace_block;        /* block size of the persistent traversal kernels: 64, 128, 256 */
     int32_t persistent_blocks;  /* grid of the persistent traversal kernels                     */
