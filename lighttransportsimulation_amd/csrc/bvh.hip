@@ -601,6 +601,34 @@ __device__ unsigned q_ceil(float o, int e, float hi) {
     return (unsigned)q;
 }
 
+// The quantized 4-wide node of `cnt` children with exact boxes mn / mx: the origin is the minimum
+// corner of their union, per axis the smallest exponent that reaches the maximum corner, then each
+// child's planes rounded outwards (q_floor / q_ceil).
+__device__ BvhQNode quantize_node(const int (&ch)[4], const float (&mn)[4][3], const float (&mx)[4][3], int cnt) {
+    BvhQNode o;
+    float org[3];
+    int ex[3];
+    for (int a = 0; a < 3; a++) {
+        float lo = mn[0][a], hi = mx[0][a];
+        for (int k = 1; k < cnt; k++) { lo = fminf(lo, mn[k][a]); hi = fmaxf(hi, mx[k][a]); }
+        org[a] = lo;
+        ex[a] = q_exponent(lo, hi);
+    }
+    o.ox = org[0]; o.oy = org[1]; o.oz = org[2];
+    o.ex = (int8_t)ex[0]; o.ey = (int8_t)ex[1]; o.ez = (int8_t)ex[2];
+    o.count = (uint8_t)cnt;
+    for (int k = 0; k < 4; k++) {
+        const bool valid = k < cnt;
+        o.child[k] = valid ? ch[k] : 0;
+        for (int a = 0; a < 3; a++) {
+            o.qlo[a][k] = valid ? (uint8_t)q_floor(org[a], ex[a], mn[k][a]) : 0;
+            o.qhi[a][k] = valid ? (uint8_t)q_ceil(org[a], ex[a], mx[k][a]) : 0;
+        }
+    }
+    o.pad[0] = o.pad[1] = 0;
+    return o;
+}
+
 __global__ __launch_bounds__(kBlock) void k_build_bvh4(const BvhNode* __restrict__ nodes, const int32_t* __restrict__ flag,
                                                        const int32_t* __restrict__ index4, int n, BvhQNode* __restrict__ out) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -626,28 +654,74 @@ __global__ __launch_bounds__(kBlock) void k_build_bvh4(const BvhNode* __restrict
             add(g.right, g.rmin, g.rmax);
         }
     }
-    BvhQNode o;
-    float org[3];
-    int ex[3];
-    for (int a = 0; a < 3; a++) {
-        float lo = mn[0][a], hi = mx[0][a];
-        for (int k = 1; k < cnt; k++) { lo = fminf(lo, mn[k][a]); hi = fmaxf(hi, mx[k][a]); }
-        org[a] = lo;
-        ex[a] = q_exponent(lo, hi);
-    }
-    o.ox = org[0]; o.oy = org[1]; o.oz = org[2];
-    o.ex = (int8_t)ex[0]; o.ey = (int8_t)ex[1]; o.ez = (int8_t)ex[2];
-    o.count = (uint8_t)cnt;
-    for (int k = 0; k < 4; k++) {
-        const bool valid = k < cnt;
-        o.child[k] = valid ? ch[k] : 0;
-        for (int a = 0; a < 3; a++) {
-            o.qlo[a][k] = valid ? (uint8_t)q_floor(org[a], ex[a], mn[k][a]) : 0;
-            o.qhi[a][k] = valid ? (uint8_t)q_ceil(org[a], ex[a], mx[k][a]) : 0;
+    out[index4[i]] = quantize_node(ch, mn, mx, cnt);
+}
+
+// Greedy surface-area collapse, one level of the 4-wide tree per launch (breadth first). Frontier
+// entry t is binary interior node front[t]; it becomes 4-wide node levelStart + t, whose children
+// start as the binary node's two children, then the interior child of largest surface area is
+// replaced by its two children until there are four or only leaves are left. Leaf children are
+// stored as ~leaf; the interior ones are listed in tmp[4 t + slot] (-1 elsewhere) and counted in
+// cnt[t], and k_collapse_emit numbers them once cnt holds their exclusive prefix.
+__global__ __launch_bounds__(kBlock) void k_collapse_count(const int32_t* __restrict__ front, int count,
+                                                           const BvhNode* __restrict__ nodes, int n, int levelStart,
+                                                           int32_t* __restrict__ tmp, int32_t* __restrict__ cnt,
+                                                           BvhQNode* __restrict__ out) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const BvhNode nd = nodes[front[t]];
+    int id[4];
+    float mn[4][3], mx[4][3];
+    auto put = [&](int k, int c, const float* lo, const float* hi) {
+        id[k] = c;
+        for (int a = 0; a < 3; a++) { mn[k][a] = lo[a]; mx[k][a] = hi[a]; }
+    };
+    put(0, nd.left, nd.lmin, nd.lmax);
+    put(1, nd.right, nd.rmin, nd.rmax);
+    int c = 2;
+    while (c < 4) {
+        int best = -1;
+        float bestArea = -1.0f;
+        for (int k = 0; k < c; k++) {
+            if (id[k] >= n - 1) continue;
+            const float dx = mx[k][0] - mn[k][0], dy = mx[k][1] - mn[k][1], dz = mx[k][2] - mn[k][2];
+            const float area = dx * dy + dy * dz + dz * dx;
+            if (area > bestArea) { bestArea = area; best = k; }
         }
+        if (best < 0) break;
+        const BvhNode g = nodes[id[best]];
+        put(best, g.left, g.lmin, g.lmax);
+        put(c, g.right, g.rmin, g.rmax);
+        c++;
     }
-    o.pad[0] = o.pad[1] = 0;
-    out[index4[i]] = o;
+    int ch[4];
+    int interior = 0;
+    for (int k = 0; k < 4; k++) {
+        const bool leaf = k < c && id[k] >= n - 1;
+        ch[k] = leaf ? ~(id[k] - (n - 1)) : 0;
+        const bool inner = k < c && !leaf;
+        tmp[4 * t + k] = inner ? id[k] : -1;
+        interior += inner;
+    }
+    out[levelStart + t] = quantize_node(ch, mn, mx, c);
+    cnt[t] = interior;
+}
+
+// Numbers the interior children of the level's nodes (cnt now holds exclusive offsets): the j-th
+// interior child of entry t becomes node nextStart + cnt[t] + j and frontier entry cnt[t] + j.
+__global__ __launch_bounds__(kBlock) void k_collapse_emit(int count, const int32_t* __restrict__ tmp,
+                                                          const int32_t* __restrict__ cnt, int levelStart, int nextStart,
+                                                          BvhQNode* __restrict__ out, int32_t* __restrict__ next) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    int j = cnt[t];
+    for (int k = 0; k < 4; k++) {
+        const int c = tmp[4 * t + k];
+        if (c < 0) continue;
+        out[levelStart + t].child[k] = nextStart + j;
+        next[j] = c;
+        j++;
+    }
 }
 
 
@@ -827,6 +901,44 @@ hipError_t bvh_reserve(BvhBuffers& b, int n) {
     return hipSuccess;
 }
 
+// Greedy surface-area collapse (k_collapse_count / k_collapse_emit), level by level from the root.
+// Returns the number of 4-wide nodes in *nodes4, or 0 when the tree has more than kTraceStack / 3
+// levels (a 4-wide node leaves at most 3 entries on the traversal stack).
+static hipError_t collapse_greedy(BvhBuffers& b, hipStream_t s, int nb, int* nodes4) {
+    const int n = b.n, m = n - 1;
+    *nodes4 = 0;
+    int32_t *fa = nullptr, *fb = nullptr, *cnt = nullptr, *tmp = nullptr;
+    hipError_t e = dalloc(&fa, m);
+    if (e == hipSuccess) e = dalloc(&fb, m);
+    if (e == hipSuccess) e = dalloc(&cnt, m);
+    if (e == hipSuccess) e = dalloc(&tmp, 4 * (size_t)m);
+    const int32_t root = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(fa, &root, sizeof(root), hipMemcpyHostToDevice, s);
+    int levelStart = 0, count = 1, levels = 0;
+    while (e == hipSuccess && count > 0) {
+        if (3 * ++levels > kTraceStack) { levelStart = 0; break; }
+        const int lb = (count + kScanBlock - 1) / kScanBlock;
+        k_collapse_count<<<blocks_for(count), kBlock, 0, s>>>(fa, count, b.nodes, n, levelStart, tmp, cnt, b.nodes4);
+        k_scan_local<<<lb, kScanBlock, 0, s>>>(cnt, count, b.scanTotals);
+        k_scan_totals<<<1, 64, 0, s>>>(b.scanTotals, lb);
+        k_scan_add<<<lb, kScanBlock, 0, s>>>(cnt, count, b.scanTotals);
+        k_collapse_emit<<<blocks_for(count), kBlock, 0, s>>>(count, tmp, cnt, levelStart, levelStart + count, b.nodes4, fb);
+        int32_t next = 0;
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(&next, b.scanTotals + lb, sizeof(next), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        levelStart += count;
+        count = next;
+        std::swap(fa, fb);
+    }
+    (void)nb;
+    void* ps[] = {fa, fb, cnt, tmp};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    if (e == hipSuccess) *nodes4 = levelStart;
+    return e;
+}
+
 hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t s, int* depth, int* nodes4) {
     const int n = b.n;
     *depth = 0;
@@ -845,11 +957,19 @@ hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t s, int* depth, int* nodes4) {
         if (e != hipSuccess) return e;
         b.cap4 = n;
     }
-    // flags (reusing b.flags) and the depth -> exclusive prefix in index4 -> 4-wide nodes
+    // the binary tree's depth (and its even-depth flags, for the fallback collapse)
     int32_t* depthDev = b.scanTotals + nb + 1;
     hipError_t e = hipMemsetAsync(depthDev, 0, sizeof(int32_t), s);
     if (e != hipSuccess) return e;
     k_even_depth<<<blocks_for(m), kBlock, 0, s>>>(b.parent, n, b.flags, depthDev);
+    e = hipMemcpyAsync(depth, depthDev, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    const int binaryDepth = *depth;
+    if ((e = collapse_greedy(b, s, nb, nodes4)) != hipSuccess) return e;
+    if (*nodes4 > 0) return hipSuccess;
+    // fallback: every interior node of even depth becomes a 4-wide node (ceil(depth / 2) levels)
+    // flags -> exclusive prefix in index4 -> 4-wide nodes
     e = hipMemcpyAsync(b.index4, b.flags, (size_t)m * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return e;
     k_scan_local<<<nb, kScanBlock, 0, s>>>(b.index4, m, b.scanTotals);
@@ -858,12 +978,12 @@ hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t s, int* depth, int* nodes4) {
     k_build_bvh4<<<blocks_for(m), kBlock, 0, s>>>(b.nodes, b.flags, b.index4, n, b.nodes4);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    int32_t tail[2];  // scanTotals[nb] = 4-wide node count, then the depth
-    e = hipMemcpyAsync(tail, b.scanTotals + nb, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    int32_t total = 0;  // scanTotals[nb] = 4-wide node count
+    e = hipMemcpyAsync(&total, b.scanTotals + nb, sizeof(int32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
-    *nodes4 = tail[0];
-    *depth = tail[1];
+    *nodes4 = total;
+    *depth = binaryDepth;
     return hipSuccess;
 }
 
