@@ -94,10 +94,10 @@ LTS_D bool box_hit(const float* mn, const float* mx, const Ray& r, V3 inv, int n
 }
 
 constexpr int kStack = kTraceStack;
-// Stack entries kept in LDS; deeper entries go to private memory. 19 entries of a 256-lane block
-// (19456 B) plus the 256 B material histogram let 8 trace blocks (8 waves per SIMD) share a CU's
+// Stack entries kept in LDS; deeper entries go to private memory. 22 entries of a 256-lane block
+// (22528 B) plus the 256 B material histogram let 7 trace blocks (7 waves per SIMD) share a CU's
 // 160 KB of LDS.
-constexpr int kShortStack = 19;
+constexpr int kShortStack = 22;
 
 // Traversal stack of one lane: entries [0, kShortStack) in LDS (entry e at lds[e * stride], so the
 // lanes of a wave touch consecutive words), deeper entries in a private array that only deep

@@ -554,9 +554,10 @@ LTS_D void mis_item(const SceneView* scenes, const WfPath* paths, const WfMis* m
 // `ext`), then the shadow rays and the MIS rays of the bounce just shaded. The three sets do not
 // depend on each other, so one work counter over their concatenation balances long rays against
 // short ones and leaves one tail per bounce instead of three. Counts may be null (empty set).
-// The traversal is latency-bound: the launch asks for 8 waves per SIMD (at most 64 VGPRs).
+// The launch asks for 7 waves per SIMD (at most 72 VGPRs): enough waves to cover the load latency
+// of the traversal, which then runs at the VALU issue limit; at 8 the 64-register cap spills.
 template <int BS>
-__global__ __launch_bounds__(BS, 8) void k_wf_trace(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
+__global__ __launch_bounds__(BS, 7) void k_wf_trace(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
                                                  const WfRay* __restrict__ rays, WfHit* __restrict__ hits,
                                                  const int32_t* __restrict__ ext, const int32_t* __restrict__ extCount,
                                                  int32_t* __restrict__ bins, int32_t* __restrict__ hist, int nBins,
