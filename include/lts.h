@@ -155,10 +155,14 @@ int lts_scene_info(lts_ctx* ctx, int32_t* n_tris, int32_t* n_lights, int32_t* n_
  * LTS_TREE_PLOC (default) clusters the sorted leaves by PLOC (surface-area nearest neighbours,
  * radius 8) and falls back to the Karras tree when the PLOC tree is too deep for the traversal
  * stack; LTS_TREE_KARRAS renders with the LBVH itself (BVH::intersect's tree, BVH.cuh:365-398).
+ * The binary tree is collapsed into 4-wide nodes greedily (each node opens its largest-area
+ * interior child until it has four children) or, with LTS_TREE_COLLAPSE_EVEN or when the greedy
+ * tree has too many levels for the traversal stack, by taking every interior node of even depth.
  * The exported LBVH (lts_bvh_build*) is always the bit-exact Karras tree. No reference
  * counterpart; the reference renders with its LBVH. */
 #define LTS_TREE_KARRAS 0
 #define LTS_TREE_PLOC 1
+#define LTS_TREE_COLLAPSE_EVEN 0x100 /* or-ed into the kind */
 int lts_set_render_tree(lts_ctx* ctx, int32_t kind);
 /* Texture pipeline of the scenes this context loads from now on. LTS_TEXTURE_REFERENCE (default)
  * reproduces the reference's image textures: its blocked texel layout with the aliasing row index
@@ -172,7 +176,8 @@ int lts_set_render_tree(lts_ctx* ctx, int32_t kind);
 #define LTS_TEXTURE_FILTERED 1
 int lts_set_texture_filter(lts_ctx* ctx, int32_t mode);
 
-/* Tree the loaded scene renders with, its depth in edges and its 4-wide node count. */
+/* Tree the loaded scene renders with (LTS_TREE_COLLAPSE_EVEN set when its 4-wide nodes come from
+ * the even-depth collapse), its depth in edges and its 4-wide node count. */
 int lts_render_tree_info(lts_ctx* ctx, int32_t* kind, int32_t* depth, int32_t* nodes4);
 
 /* Replaces CreateBVHTree (acceleration/BVH.cuh:411-452) for a triangle soup of n triangles

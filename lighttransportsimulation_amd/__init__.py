@@ -32,6 +32,7 @@ TEXTURE_FILTERED = 1
 
 TREE_KARRAS = 0
 TREE_PLOC = 1
+TREE_COLLAPSE_EVEN = 0x100  # or-ed into the kind: even-depth 4-wide collapse
 
 NRC_PARAMS = 21504
 NRC_RAW = 16
@@ -199,13 +200,15 @@ class Context:
         self._check(self.lib.lts_set_texture_filter(self.h, mode), "lts_set_texture_filter")
 
     def set_render_tree(self, kind: int):
-        """Render tree of the scenes loaded from now on: TREE_PLOC (default) or TREE_KARRAS."""
+        """Render tree of the scenes loaded from now on: TREE_PLOC (default) or TREE_KARRAS, with
+        TREE_COLLAPSE_EVEN or-ed in for the even-depth 4-wide collapse instead of the greedy one."""
         self._check(self.lib.lts_set_render_tree(self.h, kind), "lts_set_render_tree")
 
     def render_tree_info(self) -> dict:
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
         self._check(self.lib.lts_render_tree_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "lts_render_tree_info")
-        return dict(kind="ploc" if a.value == TREE_PLOC else "karras", depth=b.value, nodes4=c.value)
+        return dict(kind="ploc" if (a.value & ~TREE_COLLAPSE_EVEN) == TREE_PLOC else "karras",
+                    collapse="even" if a.value & TREE_COLLAPSE_EVEN else "greedy", depth=b.value, nodes4=c.value)
 
     # ------------------------------------------------------------------ LBVH
     def bvh_build(self, tri: np.ndarray, mode: int = 0, export: bool = True):

@@ -939,7 +939,7 @@ static hipError_t collapse_greedy(BvhBuffers& b, hipStream_t s, int nb, int* nod
     return e;
 }
 
-hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t s, int* depth, int* nodes4) {
+hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t s, int* depth, int* nodes4, bool* even) {
     const int n = b.n;
     *depth = 0;
     *nodes4 = 0;
@@ -966,8 +966,11 @@ hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t s, int* depth, int* nodes4) {
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
     const int binaryDepth = *depth;
-    if ((e = collapse_greedy(b, s, nb, nodes4)) != hipSuccess) return e;
-    if (*nodes4 > 0) return hipSuccess;
+    if (!*even) {
+        if ((e = collapse_greedy(b, s, nb, nodes4)) != hipSuccess) return e;
+        if (*nodes4 > 0) return hipSuccess;
+        *even = true;
+    }
     // fallback: every interior node of even depth becomes a 4-wide node (ceil(depth / 2) levels)
     // flags -> exclusive prefix in index4 -> 4-wide nodes
     e = hipMemcpyAsync(b.index4, b.flags, (size_t)m * sizeof(int32_t), hipMemcpyDeviceToDevice, s);

@@ -799,13 +799,15 @@ int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* s
     int rc = run_build(c, S.bvh, S.soup, n, 1, true, nullptr);
     if (rc != LTS_OK) return rc;
     S.treeKind = LTS_TREE_KARRAS;
-    if (c->renderTree == LTS_TREE_PLOC && n > 1) {
+    if ((c->renderTree & ~LTS_TREE_COLLAPSE_EVEN) == LTS_TREE_PLOC && n > 1) {
         int depth = 0;
         HIPCHK(c, ploc_enqueue(S.bvh, s, kMaxRenderDepth, &depth));
         if (depth <= kMaxRenderDepth) S.treeKind = LTS_TREE_PLOC;
     }
     int depth = 0, nodes4 = 0;
-    HIPCHK(c, bvh4_enqueue(S.bvh, s, &depth, &nodes4));
+    bool even = (c->renderTree & LTS_TREE_COLLAPSE_EVEN) != 0;
+    HIPCHK(c, bvh4_enqueue(S.bvh, s, &depth, &nodes4, &even));
+    if (even) S.treeKind |= LTS_TREE_COLLAPSE_EVEN;
     HIPCHK(c, hipStreamSynchronize(s));
     // the Karras tree of at most 2^26 leaves is at most 57 deep; the check guards the stack bound
     if (depth > kMaxRenderDepth) return fail(c, LTS_E_STATE, "render tree deeper than the traversal stack allows");
@@ -849,7 +851,8 @@ int lts_scene_info(lts_ctx* c, int32_t* nt, int32_t* nl, int32_t* nm) {
 
 int lts_set_render_tree(lts_ctx* c, int32_t kind) {
     if (!c) return LTS_E_ARG;
-    if (kind != LTS_TREE_KARRAS && kind != LTS_TREE_PLOC) return fail(c, LTS_E_ARG, "unknown render tree kind");
+    const int tree = kind & ~LTS_TREE_COLLAPSE_EVEN;
+    if (tree != LTS_TREE_KARRAS && tree != LTS_TREE_PLOC) return fail(c, LTS_E_ARG, "unknown render tree kind");
     c->renderTree = kind;
     return LTS_OK;
 }

@@ -56,7 +56,9 @@ void bvh_release(BvhBuffers& b);
 // nodes of even depth become 4-wide nodes (numbered in interior order, so the root is node 0) and
 // take the boxes of their children's children. Deterministic. Waits for the stream and returns
 // the tree's depth (edges down to the deepest leaf) and the 4-wide node count.
-hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t stream, int* depth, int* nodes4);
+// 4-wide render nodes of the binary tree in b (greedy surface-area collapse, or the even-depth
+// collapse when *even is set on entry or the greedy tree is too deep; *even tells which was used).
+hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t stream, int* depth, int* nodes4, bool* even);
 // Render tree of a scene: PLOC over the sorted leaves of the last bvh_enqueue (which must have run
 // with pack). When the PLOC tree is at most maxDepth deep it replaces b.nodes and b.parent (the
 // 4-wide collapse then takes it); otherwise the Karras tree stays. *depth = the PLOC tree's depth.
