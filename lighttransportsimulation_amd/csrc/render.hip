@@ -83,12 +83,22 @@ __global__ __launch_bounds__(256) void k_megakernel(SceneView sv, RenderLaunch r
     __syncthreads();
     const int W = sv.cam.width, H = sv.cam.height;
     const int tilesX = (W + kTileSide - 1) / kTileSide;
-    const int px = (blockIdx.x % tilesX) * kTileSide + (threadIdx.x & (kTileSide - 1));
-    const int py = (blockIdx.x / tilesX) * kTileSide + (threadIdx.x / kTileSide);
-    const int sample = rl.s0 + blockIdx.y;
+    // Workgroups are dealt to the 8 XCDs round robin in dispatch order. Tiles stay dealt round robin
+    // (every XCD gets every eighth tile, which balances sky against geometry), but an XCD runs all
+    // samples of a tile back to back, so the blocks it holds at a time cover a few tiles.
+    const int dispatch = blockIdx.y * gridDim.x + blockIdx.x;
+    int tile = blockIdx.x, sy = blockIdx.y;
+    if (gridDim.x % 8 == 0) {
+        const int i = dispatch / 8;  // the XCD's i-th block
+        tile = dispatch % 8 + 8 * (i / gridDim.y);
+        sy = i % gridDim.y;
+    }
+    const int px = (tile % tilesX) * kTileSide + (threadIdx.x & (kTileSide - 1));
+    const int py = (tile / tilesX) * kTileSide + (threadIdx.x / kTileSide);
+    const int sample = rl.s0 + sy;
     if (px >= W || py >= H || sample >= rl.s1) return;
     Spec L = radiance(sv, px, py, sample, rl, rl.counters, lds_stack(stk, threadIdx.x, 256), perm);
-    float* o = rl.sampleL + 3 * ((size_t)blockIdx.y * W * H + (size_t)py * W + px);
+    float* o = rl.sampleL + 3 * ((size_t)sy * W * H + (size_t)py * W + px);
     o[0] = L.r; o[1] = L.g; o[2] = L.b;
 }
 
