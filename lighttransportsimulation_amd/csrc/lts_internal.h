@@ -165,9 +165,11 @@ struct WfBuffers {
     int32_t* queue[2] = {nullptr, nullptr};
     int32_t* sorted = nullptr;
     int32_t* done = nullptr;      // paths that ended with a light sample still in flight
+    int32_t* shadowQ = nullptr;   // paths with a shadow ray (their records sit at the path index)
+    int32_t* misQ = nullptr;      // paths with an MIS-BSDF ray (likewise)
     int32_t* bins = nullptr;      // material bin of each path's hit, written by the extend kernel
-    WfShadow* shadows = nullptr;
-    WfMis* mis = nullptr;
+    WfShadow* shadows = nullptr;  // per path
+    WfMis* mis = nullptr;         // per path
     int32_t* counts = nullptr;
     // cached integrator (allocated by wf_reserve_cache)
     size_t cacheCap = 0;          // paths per pass covered by the query arrays

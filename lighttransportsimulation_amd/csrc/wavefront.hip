@@ -204,18 +204,9 @@ LTS_D void nrc_raw_record(const SurfacePoint& si, const Bsdf& bsdf, const WfScen
     float4* q = reinterpret_cast<float4*>(o);
     q[0] = r0; q[1] = r1; q[2] = r2; q[3] = r3;
 }
-
-// Wave-aggregated append: one atomic per wave.
-LTS_D int wave_append(int32_t* counter, bool want) {
-    const uint64_t mask = __ballot(want);
-    const int lane = threadIdx.x & 63;
-    int base = 0;
-    const int leader = __ffsll((long long)mask) - 1;
-    if (mask && lane == leader) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader < 0 ? 0 : leader);
-    return base + (int)__popcll(mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-}
-
+// One shading kernel per (integrator, first bounce or not), so that each carries only its own code
+// and registers: the radiance cache's records, and the camera-ray differentials of bounce 0.
+template <bool kCached, bool kFirst>
 __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __restrict__ scenes, const WfParams* __restrict__ prm,
                                                           const WfSceneSlot* __restrict__ slots, int bounce,
                                                           WfPath* __restrict__ paths, WfRay* __restrict__ rays,
@@ -227,13 +218,15 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
                                                           float* __restrict__ qRaw, NrcQuery* __restrict__ qMeta,
                                                           int32_t* __restrict__ qQueue, int32_t* __restrict__ qCount,
                                                           WfTrainRec* __restrict__ tRec, float* __restrict__ tRaw,
-                                                          int32_t* __restrict__ doneQueue, int32_t* __restrict__ doneCount) {
+                                                          int32_t* __restrict__ doneQueue, int32_t* __restrict__ doneCount,
+                                                          int32_t* __restrict__ shadowQueue, int32_t* __restrict__ misQueue) {
     const int n = *count;
     const int maxBounce = prm->maxBounce, rrStart = prm->rrStart, queryBounce = prm->queryBounce;
-    const bool cached = qRaw != nullptr;
+    constexpr bool cached = kCached;
     // the procedural textures' permutation lookups chain three deep: served from LDS (every scene
     // view points at its context's one table)
     __shared__ int perm[512];
+    __shared__ int32_t appendLds[2][5 * (kShadeBlock / 64) + 5];
     for (int i = threadIdx.x; i < 512; i += kShadeBlock) perm[i] = scenes[0].perm[i];
     __syncthreads();
     const int stride = gridDim.x * kShadeBlock;
@@ -259,7 +252,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
             r.d = v3(rr.d[0], rr.d[1], rr.d[2]);
             r.tMax = INFINITY;
             Spec L = spec(p.L[0], p.L[1], p.L[2]), beta = spec(p.beta[0], p.beta[1], p.beta[2]);
-            if (bounce > 0) {
+            if (!kFirst) {
                 // the light sample of the previous vertex, traced since: added before anything of
                 // this vertex, as the reference's loop adds it before the next intersection
                 const WfNee pn = nee[pi];
@@ -274,7 +267,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
             ne.p1[0] = ne.p1[1] = ne.p1[2] = 0.0f;
             ne.p2[0] = ne.p2[1] = ne.p2[2] = 0.0f;
             bool alive = true;
-            if (bounce == 0 || specularBounce) {
+            if (kFirst || specularBounce) {
                 if (hh.leaf >= 0) {
                     const int l = sv.shade[hh.leaf].light;
                     if (l >= 0) {
@@ -292,7 +285,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
                 alive = false;
             } else {
                 SurfacePoint si = make_surface(sv, hh.leaf, r, hh.u, hh.v, hh.invDet);
-                if (bounce == 0) {
+                if (kFirst) {
                     const int px = p.pixel % sv.cam.width, py = p.pixel / sv.cam.width;
                     differentials(si, camera_sample(sv, px, py, p.sample, sl.spp, sl.stratified, sl.seed));
                 }
@@ -353,6 +346,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
                             sh.path = pi;
                             sh.c[0] = c.r; sh.c[1] = c.g; sh.c[2] = c.b;
                             wantShadow = true;
+                            shadows[pi] = sh;  // per path; the queue below lists the paths
                         }
                     }
                     // BSDF sample with MIS (IntegratorHelper.cuh:96-122)
@@ -376,6 +370,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
                             ms.f[0] = f2.r; ms.f[1] = f2.g; ms.f[2] = f2.b;
                             ms.weight = weight; ms.pdf = scatteringPdf;
                             wantMis = true;
+                            misRays[pi] = ms;
                         }
                     }
                 }
@@ -418,18 +413,34 @@ __global__ __launch_bounds__(kShadeBlock) void k_wf_shade(const SceneView* __res
                 }
             }
         }
-        const int si = wave_append(shadowCount, wantShadow);
-        if (wantShadow) shadows[si] = sh;
-        const int mi = wave_append(misCount, wantMis);
-        if (wantMis) misRays[mi] = ms;
-        const int ni = wave_append(nextCount, wantNext);
-        if (wantNext) nextQueue[ni] = pi;
-        const int di = wave_append(doneCount, wantDone);
-        if (wantDone) doneQueue[di] = pi;
-        if (cached) {
-            const int qi = wave_append(qCount, wantQuery);
-            if (wantQuery) qQueue[qi] = pi;
+        // The round's appends, aggregated per block: each wave files its counts in LDS, one lane per
+        // queue reserves the block's range with a single device atomic (one per wave on a handful of
+        // shared counters serialised), and every lane writes at its offset. The buffers alternate
+        // between rounds so that one barrier pair per round suffices.
+        constexpr int kQueues = 5, kWaves = kShadeBlock / 64;
+        const bool want[kQueues] = {wantShadow, wantMis, wantNext, wantDone, cached && wantQuery};
+        int32_t* const counter[kQueues] = {shadowCount, misCount, nextCount, doneCount, qCount};
+        int32_t* const lds = appendLds[round & 1];
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const uint64_t below = lane ? ~0ull >> (64 - lane) : 0ull;
+        uint64_t mask[kQueues];
+#pragma unroll
+        for (int q = 0; q < kQueues; q++) {
+            mask[q] = __ballot(want[q]);
+            if (lane == 0) lds[q * kWaves + wave] = (int)__popcll(mask[q]);
         }
+        __syncthreads();
+        if (threadIdx.x < kQueues && (threadIdx.x < 4 || cached)) {
+            const int q = threadIdx.x;
+            int run = 0;
+            for (int w = 0; w < kWaves; w++) { const int c = lds[q * kWaves + w]; lds[q * kWaves + w] = run; run += c; }
+            lds[kQueues * kWaves + q] = run ? atomicAdd(counter[q], run) : 0;
+        }
+        __syncthreads();
+        int32_t* const out[kQueues] = {shadowQueue, misQueue, nextQueue, doneQueue, qQueue};
+#pragma unroll
+        for (int q = 0; q < kQueues; q++)
+            if (want[q]) out[q][lds[kQueues * kWaves + q] + lds[q * kWaves + wave] + (int)__popcll(mask[q] & below)] = pi;
     }
 }
 
@@ -509,7 +520,7 @@ __global__ __launch_bounds__(kCollectBlock) void k_wf_nrc_scatter(const WfParams
     }
 }
 
-// Any hit of shadow ray k: the light sample's contribution when the segment is free.
+// Any hit of the shadow ray of path k: the light sample's contribution when the segment is free.
 LTS_D void shadow_item(const SceneView* scenes, const WfPath* paths, const WfShadow* shadows, WfNee* nee, int k,
                        TraceStack ts) {
     const WfShadow s = shadows[k];
@@ -524,7 +535,7 @@ LTS_D void shadow_item(const SceneView* scenes, const WfPath* paths, const WfSha
     }
 }
 
-// Closest hit of MIS-BSDF ray k and the light identity test (IntegratorHelper.cuh:103-122).
+// Closest hit of the MIS-BSDF ray of path k and the light identity test (IntegratorHelper.cuh:103-122).
 LTS_D void mis_item(const SceneView* scenes, const WfPath* paths, const WfMis* misRays, WfNee* nee, int k, TraceStack ts) {
     const WfMis m = misRays[k];
     const SceneView& sv = scenes[paths[m.path].scene];
@@ -563,6 +574,7 @@ __global__ __launch_bounds__(BS, 7) void k_wf_trace(const SceneView* __restrict_
                                                  int32_t* __restrict__ bins, int32_t* __restrict__ hist, int nBins,
                                                  const WfShadow* __restrict__ shadows, const int32_t* __restrict__ shadowCount,
                                                  const WfMis* __restrict__ misRays, const int32_t* __restrict__ misCount,
+                                                 const int32_t* __restrict__ shadowQueue, const int32_t* __restrict__ misQueue,
                                                  WfNee* __restrict__ nee, int32_t* __restrict__ work) {
     __shared__ int stk[kShortStack * BS];
     __shared__ int32_t lh[kWfMaxMatBins];
@@ -583,8 +595,8 @@ __global__ __launch_bounds__(BS, 7) void k_wf_trace(const SceneView* __restrict_
         if (j >= kWfWorkSegments) break;
         if (k >= n) continue;
         if (k < nE) extend_item(scenes, paths, rays, hits, ext[k], ts, bins, lh, filing, nBins);
-        else if (k < nE + nS) shadow_item(scenes, paths, shadows, nee, k - nE, ts);
-        else mis_item(scenes, paths, misRays, nee, k - nE - nS, ts);
+        else if (k < nE + nS) shadow_item(scenes, paths, shadows, nee, shadowQueue[k - nE], ts);
+        else mis_item(scenes, paths, misRays, nee, misQueue[k - nE - nS], ts);
     }
     if (filing) {
         __syncthreads();
@@ -659,15 +671,15 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
         switch (t.traceBlock) {
             case 64:
                 k_wf_trace<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, ext, extCount, b.bins, hist,
-                                                                b.matBins, b.shadows, shadowCount, b.mis, misCount, b.nee, work);
+                                                                b.matBins, b.shadows, shadowCount, b.mis, misCount, b.shadowQ, b.misQ, b.nee, work);
                 break;
             case 128:
                 k_wf_trace<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, ext, extCount, b.bins, hist,
-                                                                  b.matBins, b.shadows, shadowCount, b.mis, misCount, b.nee, work);
+                                                                  b.matBins, b.shadows, shadowCount, b.mis, misCount, b.shadowQ, b.misQ, b.nee, work);
                 break;
             default:
                 k_wf_trace<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, ext, extCount, b.bins, hist,
-                                                                  b.matBins, b.shadows, shadowCount, b.mis, misCount, b.nee, work);
+                                                                  b.matBins, b.shadows, shadowCount, b.mis, misCount, b.shadowQ, b.misQ, b.nee, work);
                 break;
         }
     };
@@ -686,10 +698,14 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
             shadeQueue = b.sorted;
         }
         mark(8 * bounce + 1);
-        k_wf_shade<<<gridAll, kShadeBlock, 0, s>>>(b.scenes, b.params, b.slots, bounce, b.paths, b.rays, b.hits, b.nee, shadeQueue,
-                                                   cnt, b.queue[q ^ 1], nxt, b.shadows, cnt + 1, b.mis, cnt + 2,
-                                                   cached ? b.qRaw : nullptr, b.qMeta, b.qQueue, qCount, b.tRec, b.tRaw,
-                                                   b.done, cnt + 3);
+        {
+            auto shade = cached ? (bounce == 0 ? k_wf_shade<true, true> : k_wf_shade<true, false>)
+                                : (bounce == 0 ? k_wf_shade<false, true> : k_wf_shade<false, false>);
+            shade<<<gridAll, kShadeBlock, 0, s>>>(b.scenes, b.params, b.slots, bounce, b.paths, b.rays, b.hits, b.nee, shadeQueue,
+                                                  cnt, b.queue[q ^ 1], nxt, b.shadows, cnt + 1, b.mis, cnt + 2,
+                                                  cached ? b.qRaw : nullptr, b.qMeta, b.qQueue, qCount, b.tRec, b.tRaw,
+                                                  b.done, cnt + 3, b.shadowQ, b.misQ);
+        }
         mark(8 * bounce + 2);
         // the next bounce's path rays (none after the last bounce) with this bounce's shadow and MIS rays
         trace(bounce + 1, nxt, cnt + 1, cnt + 2);
@@ -741,6 +757,8 @@ hipError_t wf_reserve(WfBuffers& b, size_t cap, int maxScenes) {
     WTRY(wf_alloc(&b.queue[1], cap));
     WTRY(wf_alloc(&b.sorted, cap));
     WTRY(wf_alloc(&b.done, cap));
+    WTRY(wf_alloc(&b.shadowQ, cap));
+    WTRY(wf_alloc(&b.misQ, cap));
     WTRY(wf_alloc(&b.bins, cap));
     WTRY(wf_alloc(&b.shadows, cap));
     WTRY(wf_alloc(&b.mis, cap));
@@ -778,7 +796,7 @@ void wf_release(WfBuffers& b) {
     void* cs[] = {b.qRaw, b.qMeta, b.qQueue, b.tRec, b.tRaw, b.tCount};
     for (void* p : cs)
         if (p) (void)hipFree(p);
-    void* ps[] = {b.scenes, b.params, b.slots, b.paths, b.rays, b.hits, b.nee, b.queue[0], b.queue[1], b.sorted, b.done, b.bins, b.shadows, b.mis, b.counts};
+    void* ps[] = {b.scenes, b.params, b.slots, b.paths, b.rays, b.hits, b.nee, b.queue[0], b.queue[1], b.sorted, b.done, b.shadowQ, b.misQ, b.bins, b.shadows, b.mis, b.counts};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     b = WfBuffers();
