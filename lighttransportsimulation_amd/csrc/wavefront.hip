@@ -204,6 +204,7 @@ LTS_D void nrc_raw_record(const SurfacePoint& si, const Bsdf& bsdf, const WfScen
     float4* q = reinterpret_cast<float4*>(o);
     q[0] = r0; q[1] = r1; q[2] = r2; q[3] = r3;
 }
+
 // One shading kernel per (integrator, first bounce or not), so that each carries only its own code
 // and registers: the radiance cache's records, and the camera-ray differentials of bounce 0.
 template <bool kCached, bool kFirst>
