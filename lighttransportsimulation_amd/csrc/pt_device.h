@@ -221,8 +221,22 @@ LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned lon
     return h;
 }
 
+// The triangle test of an any-hit traversal: Triangle::simpleIntersect for shadow rays, or the
+// closest-hit test (Triangle::intersect) when the question is whether any triangle lies before r.tMax
+// by the closest-hit traversal's own predicate.
+template <bool kHitTest>
+LTS_D bool leaf_blocks(V3 p0, V3 p1, V3 p2, const Ray& r) {
+    if constexpr (kHitTest) {
+        float t, u, v, id;
+        return tri_hit(p0, p1, p2, r, t, u, v, id);
+    } else {
+        return tri_occludes(p0, p1, p2, r);
+    }
+}
+
 // Any hit (BVH::simpleIntersect, acceleration/BVH.cuh:331-363) over the 4-wide tree: leaf children
 // are tested as soon as their box is hit, interior children are stacked.
+template <bool kHitTest = false>
 LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts) {
     const int n = sv.nPrim;
     V3 inv = v3((float)1 / r.d.x, (float)1 / r.d.y, (float)1 / r.d.z);
@@ -230,7 +244,7 @@ LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts) {
     if (n == 1) {
         V3 p0, p1, p2; int prim;
         load_tri(sv.tris, 0, p0, p1, p2, prim);
-        return tri_occludes(p0, p1, p2, r);
+        return leaf_blocks<kHitTest>(p0, p1, p2, r);
     }
     int spill[kStack - kShortStack];
     int sp = 0;
@@ -248,7 +262,7 @@ LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts) {
                     if (leaf >= n) return false;
                     V3 p0, p1, p2; int prim;
                     load_tri(sv.tris, leaf, p0, p1, p2, prim);
-                    if (tri_occludes(p0, p1, p2, r)) return true;
+                    if (leaf_blocks<kHitTest>(p0, p1, p2, r)) return true;
                 } else if (sp < kStack) {
                     if (sp < kShortStack) ts.lds[sp * ts.stride] = c[k];
                     else spill[sp - kShortStack] = c[k];
@@ -818,6 +832,28 @@ LTS_D int sample_light(const SceneView& sv, float u, float* pdf) {
     return off;
 }
 
+// The light test of the MIS-BSDF ray (IntegratorHelper.cuh:103-122): whether the closest hit of r
+// (spawned with tMax = infinity) is light li's triangle, seen from its emitting side, or for the
+// infinite light whether r escapes. The closest-hit traversal is not needed for that. A ray that
+// misses the light's triangle, or meets its back, contributes nothing whatever lies in front. A ray
+// that meets its front at t reaches it unless some triangle passes the closest-hit test before t,
+// which an any-hit traversal bounded at t answers; the same t, u, v and 1/det come out of the
+// triangle test whatever tMax is. The only difference from the closest-hit form is a second
+// triangle at exactly the same distance, which that form may visit first.
+LTS_D bool mis_reaches_light(const SceneView& sv, const LightDesc& L, int li, Ray r, TraceStack ts) {
+    if (L.kind == 0) {
+        if (L.leaf < 0 || sv.shade[L.leaf].light != li) return false;
+        V3 p0, p1, p2; int prim;
+        load_tri(sv.tris, L.leaf, p0, p1, p2, prim);
+        float t, u, v, id;
+        if (!tri_hit(p0, p1, p2, r, t, u, v, id)) return false;
+        const SurfacePoint lt = make_surface(sv, L.leaf, r, u, v, id);
+        if (!(dot(lt.n, -r.d) > 0.f)) return false;
+        r.tMax = t;
+    }
+    return !trace_any<true>(sv, r, ts);
+}
+
 // estimateDirect (intergrators/IntegratorHelper.cuh:51-125)
 LTS_D Spec estimate_direct(const SceneView& sv, const SurfacePoint& si, const Bsdf& bsdf, V2 uScatter, int li, V2 uLight, bool specular,
                            TraceStack ts) {
@@ -850,15 +886,7 @@ LTS_D Spec estimate_direct(const SceneView& sv, const SurfacePoint& si, const Bs
             if (lightPdf == 0) return Ld;
             weight = power_heuristic(scatteringPdf, lightPdf);
         }
-        Ray r = spawn_ray(si, wi);
-        Hit h = trace_closest(sv, r, ts, nullptr, nullptr);
-        Spec Li2 = spec(0.f);
-        if (h.leaf >= 0) {
-            SurfacePoint lt = make_surface(sv, h.leaf, r, h.u, h.v, h.invDet);
-            if (sv.shade[lt.leaf].light == li) Li2 = dot(lt.n, -wi) > 0.f ? light_le(L) : spec(0.0f);
-        } else if (L.kind == 1) {
-            Li2 = light_le(L);
-        }
+        const Spec Li2 = mis_reaches_light(sv, L, li, spawn_ray(si, wi), ts) ? light_le(L) : spec(0.f);
         if (!isBlack(Li2)) Ld = Ld + f * Li2 * spec(1.0f) * weight / scatteringPdf;
     }
     return Ld;

@@ -538,7 +538,7 @@ LTS_D void shadow_item(const SceneView* scenes, const WfPath* paths, const WfSha
     }
 }
 
-// Closest hit of the MIS-BSDF ray of path k and the light identity test (IntegratorHelper.cuh:103-122).
+// The MIS-BSDF ray of path k: does it reach the light it was cast for (IntegratorHelper.cuh:103-122).
 LTS_D void mis_item(const SceneView* scenes, const WfPath* paths, const WfMis* misRays, WfNee* nee, int k, TraceStack ts) {
     const WfMis m = misRays[k];
     const SceneView& sv = scenes[paths[m.path].scene];
@@ -547,15 +547,7 @@ LTS_D void mis_item(const SceneView* scenes, const WfPath* paths, const WfMis* m
     r.o = v3(m.o[0], m.o[1], m.o[2]);
     r.d = v3(m.d[0], m.d[1], m.d[2]);
     r.tMax = INFINITY;
-    const V3 wi = r.d;
-    Hit h = trace_closest(sv, r, ts, nullptr, nullptr);
-    Spec Li2 = spec(0.f);
-    if (h.leaf >= 0) {
-        SurfacePoint lt = make_surface(sv, h.leaf, r, h.u, h.v, h.invDet);
-        if (sv.shade[lt.leaf].light == m.light) Li2 = dot(lt.n, -wi) > 0.f ? light_le(Lt) : spec(0.0f);
-    } else if (Lt.kind == 1) {
-        Li2 = light_le(Lt);
-    }
+    const Spec Li2 = mis_reaches_light(sv, Lt, m.light, r, ts) ? light_le(Lt) : spec(0.f);
     if (!isBlack(Li2)) {
         Spec c = spec(m.f[0], m.f[1], m.f[2]) * Li2 * spec(1.0f) * m.weight / m.pdf;
         nee[m.path].p2[0] = c.r;
