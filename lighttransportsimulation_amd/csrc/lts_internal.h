@@ -107,11 +107,11 @@ constexpr int kWfMaxMatBins = 64;
 constexpr int kWfMaxBounce = 64;
 
 struct alignas(16) WfPath { float beta[3]; float L[3]; int32_t pixel, sample, bounce, flags, scene, trainSlot; };
-struct alignas(16) WfRay { float o[3]; float d[3]; float pad[2]; };
+struct alignas(16) WfRay { float o[3]; float d[3]; int32_t scene; int32_t pad; };
 struct alignas(16) WfHit { int32_t leaf; float u, v, invDet; };
 struct alignas(16) WfNee { float nb[3]; float lpdf; float p1[3]; float p2[3]; int32_t used; int32_t pad; };
-struct alignas(16) WfShadow { float o[3]; float d[3]; int32_t path; int32_t pad; float c[3]; float pad2; };
-struct alignas(16) WfMis { float o[3]; float d[3]; int32_t path; int32_t light; float f[3]; float weight; float pdf; int32_t pad[3]; };
+struct alignas(16) WfShadow { float o[3]; float d[3]; int32_t path; int32_t scene; float c[3]; float pad2; };
+struct alignas(16) WfMis { float o[3]; float d[3]; int32_t path; int32_t light; float f[3]; float weight; float pdf; int32_t scene; int32_t pad[2]; };
 
 // Read by the kernels from device memory, so a captured pass graph can be replayed for every pass.
 struct WfParams {

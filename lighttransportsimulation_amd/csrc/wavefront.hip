@@ -43,10 +43,22 @@ LTS_D void path_origin(const WfParams& P, const WfSceneSlot* slots, long i, int&
     pixel = band_pixel((int)(local % npix), sl.width, sl.height);
 }
 
+// Training paths of the radiance cache: every trainStride-th path of scene 0 from trainOffset; the
+// slot of path k, or -1.
+LTS_D int train_slot(const WfParams& P, int scene, long k) {
+    if (P.trainSlots > 0 && scene == 0 && k >= P.trainOffset) {
+        const long d = k - P.trainOffset;
+        if (d % P.trainStride == 0 && d / P.trainStride < P.trainSlots) return (int)(d / P.trainStride);
+    }
+    return -1;
+}
+
+// The camera rays of the pass. The path records are not written here: at bounce 0 every field
+// follows from the path index, and the first shading builds them.
 __global__ __launch_bounds__(256) void k_wf_generate(const SceneView* __restrict__ scenes, const WfParams* __restrict__ prm,
-                                                     const WfSceneSlot* __restrict__ slots, WfPath* __restrict__ paths,
-                                                     WfRay* __restrict__ rays, int32_t* __restrict__ queue,
-                                                     int32_t* __restrict__ counts, WfTrainRec* __restrict__ tRec) {
+                                                     const WfSceneSlot* __restrict__ slots, WfRay* __restrict__ rays,
+                                                     int32_t* __restrict__ queue, int32_t* __restrict__ counts,
+                                                     WfTrainRec* __restrict__ tRec) {
     const WfParams P = *prm;
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i == 0) counts[0] = (int)P.nPaths;  // bounce-0 queue holds every path of the pass
@@ -57,24 +69,13 @@ __global__ __launch_bounds__(256) void k_wf_generate(const SceneView* __restrict
         const WfSceneSlot& sl = slots[sc];
         const int px = pixel % sv.cam.width, py = pixel / sv.cam.width;
         CamRays cr = camera_sample(sv, px, py, sample, sl.spp, sl.stratified, sl.seed);
-        WfPath p;
-        p.beta[0] = p.beta[1] = p.beta[2] = 1.0f;
-        p.L[0] = p.L[1] = p.L[2] = 0.0f;
-        p.pixel = pixel; p.sample = sample; p.flags = kWfAlive; p.scene = sc;
-        // training paths of the radiance cache: every trainStride-th path of scene 0 from trainOffset
-        p.trainSlot = -1;
-        if (P.trainSlots > 0 && sc == 0 && k >= P.trainOffset) {
-            const long d = k - P.trainOffset;
-            if (d % P.trainStride == 0 && d / P.trainStride < P.trainSlots) {
-                p.trainSlot = d / P.trainStride;
-                for (int b = 0; b < P.maxBounce; b++) tRec[(size_t)p.trainSlot * kWfMaxBounce + b].valid = 0;
-            }
-        }
-        paths[k] = p;
+        const int ts = train_slot(P, sc, k);
+        if (ts >= 0)
+            for (int b = 0; b < P.maxBounce; b++) tRec[(size_t)ts * kWfMaxBounce + b].valid = 0;
         WfRay r;
         r.o[0] = cr.r.o.x; r.o[1] = cr.r.o.y; r.o[2] = cr.r.o.z;
         r.d[0] = cr.r.d.x; r.d[1] = cr.r.d.y; r.d[2] = cr.r.d.z;
-        r.pad[0] = r.pad[1] = 0;
+        r.scene = sc; r.pad = 0;
         rays[k] = r;
         queue[k] = (int)k;
     }
@@ -101,14 +102,14 @@ LTS_D int wave_fetch_segmented(int32_t* work, int segLen, int& j) {
 
 // Closest hit of path ray `pi`; with `hist` set, the hit is also filed under its material bin
 // (misses in the last bin) for the counting sort: bins[path] and the block histogram lh.
-LTS_D void extend_item(const SceneView* scenes, const WfPath* paths, const WfRay* rays, WfHit* hits, int pi,
+LTS_D void extend_item(const SceneView* scenes, const WfRay* rays, WfHit* hits, int pi,
                        TraceStack ts, int32_t* bins, int32_t* lh, bool filing, int nBins) {
     const WfRay rr = rays[pi];
     Ray r;
     r.o = v3(rr.o[0], rr.o[1], rr.o[2]);
     r.d = v3(rr.d[0], rr.d[1], rr.d[2]);
     r.tMax = INFINITY;
-    const SceneView& sv = scenes[paths[pi].scene];
+    const SceneView& sv = scenes[rr.scene];
     Hit h = trace_closest(sv, r, ts, nullptr, nullptr);
     WfHit o;
     o.leaf = h.leaf; o.u = h.u; o.v = h.v; o.invDet = h.invDet;
@@ -245,8 +246,20 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
         WfShadow sh;
         WfMis ms;
         if (active) {
-            WfPath p = paths[pi];
             const WfRay rr = rays[pi];
+            WfPath p;
+            if (kFirst) {
+                // bounce 0: the path record follows from the path index
+                p.beta[0] = p.beta[1] = p.beta[2] = 1.0f;
+                p.L[0] = p.L[1] = p.L[2] = 0.0f;
+                const WfParams P = *prm;
+                path_origin(P, slots, pi, p.scene, p.pixel, p.sample);
+                p.bounce = 0;
+                p.flags = kWfAlive;
+                p.trainSlot = train_slot(P, p.scene, pi);
+            } else {
+                p = paths[pi];
+            }
             const WfHit hh = hits[pi];
             const SceneView& sv = scenes[p.scene];
             const WfSceneSlot& sl = slots[p.scene];
@@ -346,7 +359,7 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                             V3 d = target - origin;
                             sh.o[0] = origin.x; sh.o[1] = origin.y; sh.o[2] = origin.z;
                             sh.d[0] = d.x; sh.d[1] = d.y; sh.d[2] = d.z;
-                            sh.path = pi;
+                            sh.path = pi; sh.scene = p.scene;
                             sh.c[0] = c.r; sh.c[1] = c.g; sh.c[2] = c.b;
                             wantShadow = true;
                             shadows[pi] = sh;  // per path; the queue below lists the paths
@@ -369,7 +382,7 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                             Ray mr = spawn_ray(si, wi);
                             ms.o[0] = mr.o.x; ms.o[1] = mr.o.y; ms.o[2] = mr.o.z;
                             ms.d[0] = wi.x; ms.d[1] = wi.y; ms.d[2] = wi.z;
-                            ms.path = pi; ms.light = li;
+                            ms.path = pi; ms.light = li; ms.scene = p.scene;
                             ms.f[0] = f2.r; ms.f[1] = f2.g; ms.f[2] = f2.b;
                             ms.weight = weight; ms.pdf = scatteringPdf;
                             wantMis = true;
@@ -391,7 +404,7 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                     WfRay o;
                     o.o[0] = nr.o.x; o.o[1] = nr.o.y; o.o[2] = nr.o.z;
                     o.d[0] = wi.x; o.d[1] = wi.y; o.d[2] = wi.z;
-                    o.pad[0] = o.pad[1] = 0;
+                    o.scene = p.scene; o.pad = 0;
                     rays[pi] = o;
                     if (bounce >= rrStart) {
                         float q = fmaxf(0.05f, 1 - lum(beta));
@@ -403,9 +416,12 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
             p.L[0] = L.r; p.L[1] = L.g; p.L[2] = L.b;
             p.beta[0] = beta.r; p.beta[1] = beta.g; p.beta[2] = beta.b;
             if (!alive) p.flags &= ~kWfAlive;
-            paths[pi] = p;
-            nee[pi] = ne;
             wantNext = alive && bounce + 1 < maxBounce;
+            // the records are read again only by the next shading or by finalize
+            if (wantNext || ne.used) {
+                paths[pi] = p;
+                nee[pi] = ne;
+            }
             if (!wantNext) {
                 // the path ends here: its sample is final now, or once its light sample is traced
                 if (ne.used) {
@@ -524,14 +540,14 @@ __global__ __launch_bounds__(kCollectBlock) void k_wf_nrc_scatter(const WfParams
 }
 
 // Any hit of the shadow ray of path k: the light sample's contribution when the segment is free.
-LTS_D void shadow_item(const SceneView* scenes, const WfPath* paths, const WfShadow* shadows, WfNee* nee, int k,
+LTS_D void shadow_item(const SceneView* scenes, const WfShadow* shadows, WfNee* nee, int k,
                        TraceStack ts) {
     const WfShadow s = shadows[k];
     Ray r;
     r.o = v3(s.o[0], s.o[1], s.o[2]);
     r.d = v3(s.d[0], s.d[1], s.d[2]);
     r.tMax = 1 - SHADOW_EPS;
-    if (!trace_any(scenes[paths[s.path].scene], r, ts)) {
+    if (!trace_any(scenes[s.scene], r, ts)) {
         nee[s.path].p1[0] = s.c[0];
         nee[s.path].p1[1] = s.c[1];
         nee[s.path].p1[2] = s.c[2];
@@ -539,9 +555,9 @@ LTS_D void shadow_item(const SceneView* scenes, const WfPath* paths, const WfSha
 }
 
 // The MIS-BSDF ray of path k: does it reach the light it was cast for (IntegratorHelper.cuh:103-122).
-LTS_D void mis_item(const SceneView* scenes, const WfPath* paths, const WfMis* misRays, WfNee* nee, int k, TraceStack ts) {
+LTS_D void mis_item(const SceneView* scenes, const WfMis* misRays, WfNee* nee, int k, TraceStack ts) {
     const WfMis m = misRays[k];
-    const SceneView& sv = scenes[paths[m.path].scene];
+    const SceneView& sv = scenes[m.scene];
     const LightDesc& Lt = sv.lights[m.light];
     Ray r;
     r.o = v3(m.o[0], m.o[1], m.o[2]);
@@ -563,8 +579,7 @@ LTS_D void mis_item(const SceneView* scenes, const WfPath* paths, const WfMis* m
 // The launch asks for 7 waves per SIMD (at most 72 VGPRs): enough waves to cover the load latency
 // of the traversal, which then runs at the VALU issue limit; at 8 the 64-register cap spills.
 template <int BS>
-__global__ __launch_bounds__(BS, 7) void k_wf_trace(const SceneView* __restrict__ scenes, const WfPath* __restrict__ paths,
-                                                 const WfRay* __restrict__ rays, WfHit* __restrict__ hits,
+__global__ __launch_bounds__(BS, 7) void k_wf_trace(const SceneView* __restrict__ scenes, const WfRay* __restrict__ rays, WfHit* __restrict__ hits,
                                                  const int32_t* __restrict__ ext, const int32_t* __restrict__ extCount,
                                                  int32_t* __restrict__ bins, int32_t* __restrict__ hist, int nBins,
                                                  const WfShadow* __restrict__ shadows, const int32_t* __restrict__ shadowCount,
@@ -589,9 +604,9 @@ __global__ __launch_bounds__(BS, 7) void k_wf_trace(const SceneView* __restrict_
         const int k = wave_fetch_segmented(work, segLen, j);
         if (j >= kWfWorkSegments) break;
         if (k >= n) continue;
-        if (k < nE) extend_item(scenes, paths, rays, hits, ext[k], ts, bins, lh, filing, nBins);
-        else if (k < nE + nS) shadow_item(scenes, paths, shadows, nee, shadowQueue[k - nE], ts);
-        else mis_item(scenes, paths, misRays, nee, misQueue[k - nE - nS], ts);
+        if (k < nE) extend_item(scenes, rays, hits, ext[k], ts, bins, lh, filing, nBins);
+        else if (k < nE + nS) shadow_item(scenes, shadows, nee, shadowQueue[k - nE], ts);
+        else mis_item(scenes, misRays, nee, misQueue[k - nE - nS], ts);
     }
     if (filing) {
         __syncthreads();
@@ -653,7 +668,7 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
     }
     const bool cached = nrc != nullptr;
     int32_t* qCount = b.counts + kWfGlobalBase;
-    k_wf_generate<<<gridAll, 256, 0, s>>>(b.scenes, b.params, b.slots, b.paths, b.rays, b.queue[0], b.counts, b.tRec);
+    k_wf_generate<<<gridAll, 256, 0, s>>>(b.scenes, b.params, b.slots, b.rays, b.queue[0], b.counts, b.tRec);
     auto histOf = [&](int bounce) {
         return t.sortMaterials ? b.counts + kWfCountsPerBounce * (maxBounce + 1) + kWfMaxMatBins * bounce : nullptr;
     };
@@ -665,15 +680,15 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
         int32_t* hist = histOf(bounce);
         switch (t.traceBlock) {
             case 64:
-                k_wf_trace<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, ext, extCount, b.bins, hist,
+                k_wf_trace<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.rays, b.hits, ext, extCount, b.bins, hist,
                                                                 b.matBins, b.shadows, shadowCount, b.mis, misCount, b.shadowQ, b.misQ, b.nee, work);
                 break;
             case 128:
-                k_wf_trace<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, ext, extCount, b.bins, hist,
+                k_wf_trace<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.rays, b.hits, ext, extCount, b.bins, hist,
                                                                   b.matBins, b.shadows, shadowCount, b.mis, misCount, b.shadowQ, b.misQ, b.nee, work);
                 break;
             default:
-                k_wf_trace<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.paths, b.rays, b.hits, ext, extCount, b.bins, hist,
+                k_wf_trace<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.rays, b.hits, ext, extCount, b.bins, hist,
                                                                   b.matBins, b.shadows, shadowCount, b.mis, misCount, b.shadowQ, b.misQ, b.nee, work);
                 break;
         }
