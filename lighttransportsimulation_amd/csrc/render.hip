@@ -104,24 +104,41 @@ __global__ __launch_bounds__(256) void k_megakernel(SceneView sv, RenderLaunch r
 
 // Pixel (x, y) gathers, for each sample in order and each owner pixel of the 3x3 window above
 // and left of it (row major), the owner's filtered contribution when its addSample footprint
-// covers (x, y).
+// covers (x, y). A 16x16 block stages, per sample, the film positions and radiance of the 18x18
+// owners its pixels read in LDS, so each owner's film position is drawn once per block instead of
+// once per covered pixel; the sums keep their order.
 __global__ __launch_bounds__(256) void k_film_gather(SceneView sv, RenderLaunch rl) {
+    constexpr int T = 16, R = T + 2;
+    __shared__ float own[R * R][5];  // film position x, y and radiance r, g, b of each owner
     const int W = sv.cam.width, H = sv.cam.height;
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= W || y >= H) return;
-    float4 acc = reinterpret_cast<float4*>(rl.film)[(size_t)y * W + x];
+    const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
+    const int x = x0 + (threadIdx.x & 15), y = y0 + (threadIdx.x >> 4);
+    const bool inside = x < W && y < H;
+    float4 acc = inside ? reinterpret_cast<float4*>(rl.film)[(size_t)y * W + x] : make_float4(0.f, 0.f, 0.f, 0.f);
     Spec rgb = spec(acc.x, acc.y, acc.z);
     float wsum = acc.w;
     for (int s = rl.s0; s < rl.s1; s++) {
+        __syncthreads();  // the previous sample's owners have been read
+        for (int e = threadIdx.x; e < R * R; e += 256) {
+            const int ox = x0 - 2 + e % R, oy = y0 - 2 + e / R;
+            if (ox < 0 || oy < 0 || ox >= W || oy >= H) continue;
+            Sampler smp;
+            smp.seed = rl.seed; smp.pixel = (uint32_t)(oy * W + ox); smp.sample = (uint32_t)s;
+            float d[4];
+            smp.block(0, d);
+            const V2 pf = film_sample(ox, oy, s, rl.spp, rl.stratified, d);
+            const float* l = rl.sampleL + 3 * ((size_t)(s - rl.s0) * W * H + (size_t)oy * W + ox);
+            own[e][0] = pf.x; own[e][1] = pf.y;
+            own[e][2] = l[0]; own[e][3] = l[1]; own[e][4] = l[2];
+        }
+        __syncthreads();
+        if (!inside) continue;
         for (int oy = y - 2; oy <= y; oy++) {
             if (oy < 0) continue;
             for (int ox = x - 2; ox <= x; ox++) {
                 if (ox < 0) continue;
-                Sampler smp;
-                smp.seed = rl.seed; smp.pixel = (uint32_t)(oy * W + ox); smp.sample = (uint32_t)s;
-                float d[4];
-                smp.block(0, d);
-                V2 pf = film_sample(ox, oy, s, rl.spp, rl.stratified, d);
+                const float* o = own[(oy - y0 + 2) * R + (ox - x0 + 2)];
+                const V2 pf = v2(o[0], o[1]);
                 int p0x = max((int)ceilf(pf.x - 1.0f), 0), p0y = max((int)ceilf(pf.y - 1.0f), 0);
                 int p1x = min((int)floorf(pf.x + 1.0f) + 1, W), p1y = min((int)floorf(pf.y + 1.0f) + 1, H);
                 if (x < p0x || x >= p1x || y < p0y || y >= p1y) continue;
@@ -130,13 +147,12 @@ __global__ __launch_bounds__(256) void k_film_gather(SceneView sv, RenderLaunch 
                 int ix = (int)fminf((float)(int)floorf(dx), 15.0f);
                 int iy = (int)fminf((float)(int)floorf(dy), 15.0f);
                 float w = sv.filter[iy * 16 + ix];
-                const float* l = rl.sampleL + 3 * ((size_t)(s - rl.s0) * W * H + (size_t)oy * W + ox);
-                rgb = rgb + spec(l[0], l[1], l[2]) * w;
+                rgb = rgb + spec(o[2], o[3], o[4]) * w;
                 wsum += w;
             }
         }
     }
-    reinterpret_cast<float4*>(rl.film)[(size_t)y * W + x] = make_float4(rgb.r, rgb.g, rgb.b, wsum);
+    if (inside) reinterpret_cast<float4*>(rl.film)[(size_t)y * W + x] = make_float4(rgb.r, rgb.g, rgb.b, wsum);
 }
 
 __global__ void k_film_clear(float4* film, int npix) {
