@@ -55,7 +55,7 @@ typedef struct lts_bvh_timing {
 #define LTS_INTEGRATOR_CACHED 2
 
 typedef struct lts_render_params {
-    int32_t spp;          /* samples per pixel                                     */
+    int32_t spp;          /* samples per pixel, 1 .. 2^29                          */
     int32_t max_bounce;   /* PathTracingIntegrator MAX_BOUNCE                      */
     int32_t rr_start;     /* Russian roulette from this bounce                     */
     uint32_t seed;        /* sampler seed (replaces time(0))                       */

@@ -199,7 +199,7 @@ uint64_t splitmix64(uint64_t& x) {
 int nrc_ensure(lts_ctx* c);
 
 int check_render_params(lts_ctx* c, const lts_render_params* p) {
-    if (p->spp < 1 || p->max_bounce < 0 || p->spp_per_pass < 0) return fail(c, LTS_E_ARG, "bad render parameters");
+    if (p->spp < 1 || p->spp > kWfMaxSpp || p->max_bounce < 0 || p->spp_per_pass < 0) return fail(c, LTS_E_ARG, "bad render parameters");
     if (p->stratified) {
         int s = (int)std::floor(std::sqrt((float)p->spp));
         if (s * s != p->spp) return fail(c, LTS_E_ARG, "stratified film sampling needs a square sample count");

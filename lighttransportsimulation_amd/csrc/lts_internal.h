@@ -106,7 +106,11 @@ static_assert(kWfWorkBase + kWfWorkStride * kWfWorkSegments <= kWfCountsPerBounc
 constexpr int kWfMaxMatBins = 64;
 constexpr int kWfMaxBounce = 64;
 
-struct alignas(16) WfPath { float beta[3]; float L[3]; int32_t pixel, sample, bounce, flags, scene, trainSlot; };
+// 32 bytes, so that a record read in material order touches one 64-byte segment; the scene and the
+// training slot follow from the path index, the flags share a word with the sample number.
+struct alignas(32) WfPath { float beta[3]; float L[3]; int32_t pixel; uint32_t sampleFlags; };
+constexpr int kWfFlagBits = 2;
+constexpr int kWfMaxSpp = 1 << (31 - kWfFlagBits);
 struct alignas(16) WfRay { float o[3]; float d[3]; int32_t scene; int32_t pad; };
 struct alignas(16) WfHit { int32_t leaf; float u, v, invDet; };
 struct alignas(16) WfNee { float nb[3]; float lpdf; float p1[3]; float p2[3]; int32_t used; int32_t pad; };

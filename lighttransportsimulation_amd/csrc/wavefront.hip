@@ -31,10 +31,20 @@ LTS_D int band_pixel(int j, int W, int H) {
     return (8 * band + jj % rows) * W + jj / rows;
 }
 
+// Scene of path i of the pass: the last slot whose path range starts at or before i.
+LTS_D int path_scene(const WfParams& P, const WfSceneSlot* slots, long i) {
+    int lo = 0, hi = P.nScenes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (slots[mid].pathBase <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Path i of the pass -> (scene, pixel, sample).
 LTS_D void path_origin(const WfParams& P, const WfSceneSlot* slots, long i, int& scene, int& pixel, int& sample) {
-    int s = 0;
-    while (s + 1 < P.nScenes && i >= slots[s + 1].pathBase) s++;
+    const int s = path_scene(P, slots, i);
     const WfSceneSlot& sl = slots[s];
     const long local = i - sl.pathBase;
     const long npix = (long)sl.width * sl.height;
@@ -246,26 +256,40 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
         WfShadow sh;
         WfMis ms;
         if (active) {
-            const WfRay rr = rays[pi];
+            const WfParams P = *prm;
             WfPath p;
+            int scene, pixel, sample, pflags;
             if (kFirst) {
                 // bounce 0: the path record follows from the path index
                 p.beta[0] = p.beta[1] = p.beta[2] = 1.0f;
                 p.L[0] = p.L[1] = p.L[2] = 0.0f;
-                const WfParams P = *prm;
-                path_origin(P, slots, pi, p.scene, p.pixel, p.sample);
-                p.bounce = 0;
-                p.flags = kWfAlive;
-                p.trainSlot = train_slot(P, p.scene, pi);
+                path_origin(P, slots, pi, scene, pixel, sample);
+                pflags = kWfAlive;
             } else {
                 p = paths[pi];
+                scene = path_scene(P, slots, pi);
+                pixel = p.pixel;
+                sample = (int)(p.sampleFlags >> kWfFlagBits);
+                pflags = (int)(p.sampleFlags & ((1u << kWfFlagBits) - 1));
             }
+            const int trainSlot = cached ? train_slot(P, scene, pi) : -1;
             const WfHit hh = hits[pi];
-            const SceneView& sv = scenes[p.scene];
-            const WfSceneSlot& sl = slots[p.scene];
+            const SceneView& sv = scenes[scene];
+            const WfSceneSlot& sl = slots[scene];
             Ray r;
-            r.o = v3(rr.o[0], rr.o[1], rr.o[2]);
-            r.d = v3(rr.d[0], rr.d[1], rr.d[2]);
+            CamRays cam;
+            if (kFirst) {
+                // the camera ray and its differentials, drawn again as the generate kernel drew
+                // them (a path that escapes needs neither)
+                if (hh.leaf >= 0) {
+                    cam = camera_sample(sv, pixel % sv.cam.width, pixel / sv.cam.width, sample, sl.spp, sl.stratified, sl.seed);
+                    r = cam.r;
+                }
+            } else {
+                const WfRay rr = rays[pi];
+                r.o = v3(rr.o[0], rr.o[1], rr.o[2]);
+                r.d = v3(rr.d[0], rr.d[1], rr.d[2]);
+            }
             r.tMax = INFINITY;
             Spec L = spec(p.L[0], p.L[1], p.L[2]), beta = spec(p.beta[0], p.beta[1], p.beta[2]);
             if (!kFirst) {
@@ -277,7 +301,7 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                     L = L + spec(pn.nb[0], pn.nb[1], pn.nb[2]) * (Ld / pn.lpdf);
                 }
             }
-            const bool specularBounce = (p.flags & kWfSpecular) != 0;
+            const bool specularBounce = (pflags & kWfSpecular) != 0;
             WfNee ne;
             ne.used = 0;
             ne.p1[0] = ne.p1[1] = ne.p1[2] = 0.0f;
@@ -301,16 +325,13 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                 alive = false;
             } else {
                 SurfacePoint si = make_surface(sv, hh.leaf, r, hh.u, hh.v, hh.invDet);
-                if (kFirst) {
-                    const int px = p.pixel % sv.cam.width, py = p.pixel / sv.cam.width;
-                    differentials(si, camera_sample(sv, px, py, p.sample, sl.spp, sl.stratified, sl.seed));
-                }
+                if (kFirst) differentials(si, cam);
                 Bsdf bsdf = make_bsdf(sv, si, perm);
-                const int slot = (p.sample - sl.s0) * sl.width * sl.height + p.pixel;  // sample index of the path
+                const int slot = (sample - sl.s0) * sl.width * sl.height + pixel;  // sample index of the path
                 if (cached && (bsdf.kind == 0 || bsdf.kind == 1 || bsdf.kind == 4)) {
-                    if (p.trainSlot >= 0) {
+                    if (trainSlot >= 0) {
                         // training vertex: radiance so far and throughput, for the suffix target
-                        const size_t r = (size_t)p.trainSlot * kWfMaxBounce + bounce;
+                        const size_t r = (size_t)trainSlot * kWfMaxBounce + bounce;
                         nrc_raw_record(si, bsdf, sl, tRaw + 16 * r);
                         WfTrainRec tr;
                         tr.Lb[0] = L.r; tr.Lb[1] = L.g; tr.Lb[2] = L.b;
@@ -331,7 +352,7 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                     }
                 }
                 Sampler smp;
-                smp.seed = sl.seed; smp.pixel = (uint32_t)p.pixel; smp.sample = (uint32_t)p.sample;
+                smp.seed = sl.seed; smp.pixel = (uint32_t)pixel; smp.sample = (uint32_t)sample;
                 float u0[4], u1[4];
                 smp.block(1 + 2 * bounce, u0);
                 smp.block(2 + 2 * bounce, u1);
@@ -359,7 +380,7 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                             V3 d = target - origin;
                             sh.o[0] = origin.x; sh.o[1] = origin.y; sh.o[2] = origin.z;
                             sh.d[0] = d.x; sh.d[1] = d.y; sh.d[2] = d.z;
-                            sh.path = pi; sh.scene = p.scene;
+                            sh.path = pi; sh.scene = scene;
                             sh.c[0] = c.r; sh.c[1] = c.g; sh.c[2] = c.b;
                             wantShadow = true;
                             shadows[pi] = sh;  // per path; the queue below lists the paths
@@ -382,7 +403,7 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                             Ray mr = spawn_ray(si, wi);
                             ms.o[0] = mr.o.x; ms.o[1] = mr.o.y; ms.o[2] = mr.o.z;
                             ms.d[0] = wi.x; ms.d[1] = wi.y; ms.d[2] = wi.z;
-                            ms.path = pi; ms.light = li; ms.scene = p.scene;
+                            ms.path = pi; ms.light = li; ms.scene = scene;
                             ms.f[0] = f2.r; ms.f[1] = f2.g; ms.f[2] = f2.b;
                             ms.weight = weight; ms.pdf = scatteringPdf;
                             wantMis = true;
@@ -398,13 +419,13 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                 if (wantQuery || isBlack(f) || pdf == 0) {
                     alive = false;
                 } else {
-                    p.flags = (flags & B_SPEC) ? (p.flags | kWfSpecular) : (p.flags & ~kWfSpecular);
+                    pflags = (flags & B_SPEC) ? (pflags | kWfSpecular) : (pflags & ~kWfSpecular);
                     beta = beta * (f * absDot(wi, si.sn) / pdf);
                     Ray nr = spawn_ray(si, wi);
                     WfRay o;
                     o.o[0] = nr.o.x; o.o[1] = nr.o.y; o.o[2] = nr.o.z;
                     o.d[0] = wi.x; o.d[1] = wi.y; o.d[2] = wi.z;
-                    o.scene = p.scene; o.pad = 0;
+                    o.scene = scene; o.pad = 0;
                     rays[pi] = o;
                     if (bounce >= rrStart) {
                         float q = fmaxf(0.05f, 1 - lum(beta));
@@ -415,7 +436,9 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
             }
             p.L[0] = L.r; p.L[1] = L.g; p.L[2] = L.b;
             p.beta[0] = beta.r; p.beta[1] = beta.g; p.beta[2] = beta.b;
-            if (!alive) p.flags &= ~kWfAlive;
+            if (!alive) pflags &= ~kWfAlive;
+            p.pixel = pixel;
+            p.sampleFlags = ((uint32_t)sample << kWfFlagBits) | (uint32_t)pflags;
             wantNext = alive && bounce + 1 < maxBounce;
             // the records are read again only by the next shading or by finalize
             if (wantNext || ne.used) {
@@ -427,7 +450,7 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                 if (ne.used) {
                     wantDone = true;
                 } else {
-                    float* o = sl.sampleL + 3 * ((size_t)(p.sample - sl.s0) * sl.width * sl.height + p.pixel);
+                    float* o = sl.sampleL + 3 * ((size_t)(sample - sl.s0) * sl.width * sl.height + pixel);
                     o[0] = L.r; o[1] = L.g; o[2] = L.b;
                 }
             }
@@ -617,10 +640,11 @@ __global__ __launch_bounds__(BS, 7) void k_wf_trace(const SceneView* __restrict_
 
 // Paths that ended at this bounce with a light sample in flight: the traced sample completes their
 // radiance (a continuing path takes its sample at the start of the next bounce's shading).
-__global__ __launch_bounds__(256) void k_wf_finalize(const WfSceneSlot* __restrict__ slots, const WfPath* __restrict__ paths,
-                                                     const WfNee* __restrict__ nee, const int32_t* __restrict__ done,
-                                                     const int32_t* __restrict__ count) {
+__global__ __launch_bounds__(256) void k_wf_finalize(const WfParams* __restrict__ prm, const WfSceneSlot* __restrict__ slots,
+                                                     const WfPath* __restrict__ paths, const WfNee* __restrict__ nee,
+                                                     const int32_t* __restrict__ done, const int32_t* __restrict__ count) {
     const int n = *count;
+    const WfParams P = *prm;
     for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
         const int pi = done[k];
         const WfPath p = paths[pi];
@@ -628,8 +652,9 @@ __global__ __launch_bounds__(256) void k_wf_finalize(const WfSceneSlot* __restri
         Spec L = spec(p.L[0], p.L[1], p.L[2]);
         Spec Ld = spec(ne.p1[0], ne.p1[1], ne.p1[2]) + spec(ne.p2[0], ne.p2[1], ne.p2[2]);
         L = L + spec(ne.nb[0], ne.nb[1], ne.nb[2]) * (Ld / ne.lpdf);
-        const WfSceneSlot& sl = slots[p.scene];
-        float* o = sl.sampleL + 3 * ((size_t)(p.sample - sl.s0) * sl.width * sl.height + p.pixel);
+        const WfSceneSlot& sl = slots[path_scene(P, slots, pi)];
+        const int sample = (int)(p.sampleFlags >> kWfFlagBits);
+        float* o = sl.sampleL + 3 * ((size_t)(sample - sl.s0) * sl.width * sl.height + p.pixel);
         o[0] = L.r; o[1] = L.g; o[2] = L.b;
     }
 }
@@ -720,7 +745,7 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
         // the next bounce's path rays (none after the last bounce) with this bounce's shadow and MIS rays
         trace(bounce + 1, nxt, cnt + 1, cnt + 2);
         mark(8 * bounce + 3);
-        k_wf_finalize<<<gridAll, 256, 0, s>>>(b.slots, b.paths, b.nee, b.done, cnt + 3);
+        k_wf_finalize<<<gridAll, 256, 0, s>>>(b.params, b.slots, b.paths, b.nee, b.done, cnt + 3);
         mark(8 * bounce + 4);
     }
     if (cached) {
