@@ -117,10 +117,13 @@ LTS_D TraceStack lds_stack(int* shared, int tid, int blockSize) { return TraceSt
 // The ray's direction signs pick the near and far plane bytes once per axis for all four children,
 // so only those planes are decoded. The slab test is box_hit's in closed form: its pairwise
 // comparisons pass exactly when the largest near distance is at most the smallest far distance
-// (far distances scaled by 1 + 2 gamma(3); rounding is monotone, so scaling the minimum equals the
-// minimum of the scaled values), then the entry lies before tMax and the exit after 0. The forms
-// agree on every input without NaN; a NaN slab distance (a zero direction component with the
-// origin exactly on a plane) is ignored here on every axis, where box_hit culls on x only.
+// (rounding is monotone, so scaling the minimum equals the minimum of the scaled values), then the
+// entry lies before tMax and the exit after 0. A NaN slab distance (a zero direction component with
+// the origin exactly on a plane) is ignored here on every axis, where box_hit culls on x only.
+// The inverse direction comes from the hardware reciprocal (at most 1 ulp, ray_inverse), so a slab
+// distance (p - o) * inv carries at most about 4 roundings instead of box_hit's 3, and the far
+// distances are scaled by 1 + 2 gamma(4) = (1 + 4u) / (1 - 4u) instead of 1 + 2 gamma(3): every box
+// the exact arithmetic passes still passes, so the test stays conservative.
 LTS_D void node4_hits(const BvhQNode* nodes4, int cur, const Ray& r, V3 inv, int nx, int ny, int nz, float (&t)[4],
                       int (&c)[4], int& count) {
     const uint4* q = reinterpret_cast<const uint4*>(nodes4 + cur);
@@ -132,7 +135,7 @@ LTS_D void node4_hits(const BvhQNode* nodes4, int cur, const Ray& r, V3 inv, int
     const unsigned nearX = nx ? lo.w : lo.x, farX = nx ? lo.x : lo.w;
     const unsigned nearY = ny ? hi.x : lo.y, farY = ny ? lo.y : hi.x;
     const unsigned nearZ = nz ? hi.y : lo.z, farZ = nz ? lo.z : hi.y;
-    const float g3 = 1 + 2 * gammaN(3);
+    const float g4 = 1 + 2 * gammaN(4);
     const int cc[4] = {(int)ch.x, (int)ch.y, (int)ch.z, (int)ch.w};
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -144,12 +147,15 @@ LTS_D void node4_hits(const BvhQNode* nodes4, int cur, const Ray& r, V3 inv, int
         const float tzMin = (qdecode(oz, ez, (nearZ >> s) & 0xffu) - r.o.z) * inv.z;
         const float tzMax = (qdecode(oz, ez, (farZ >> s) & 0xffu) - r.o.z) * inv.z;
         const float tMin = fmaxf(fmaxf(txMin, tyMin), tzMin);
-        const float tMax = fminf(fminf(txMax, tyMax), tzMax) * g3;
+        const float tMax = fminf(fminf(txMax, tyMax), tzMax) * g4;
         const bool hit = k < count && tMin <= tMax && tMin < r.tMax && tMax > 0;
         t[k] = hit ? tMin : INFINITY;
         c[k] = cc[k];
     }
 }
+
+// 1 / d per component by v_rcp_f32 (within 1 ulp; 1 / ±0 = ±inf), for the 4-wide node test only.
+LTS_D V3 ray_inverse(V3 d) { return v3(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rcpf(d.y), __builtin_amdgcn_rcpf(d.z)); }
 
 LTS_D void cswap(float& ta, int& ca, float& tb, int& cb) {
     if (tb < ta) {
@@ -167,7 +173,7 @@ LTS_D void cswap(float& ta, int& ca, float& tb, int& cb) {
 LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned long long* nodeCount, unsigned long long* triCount) {
     Hit h; h.leaf = -1; h.t = h.u = h.v = h.invDet = 0;
     const int n = sv.nPrim;
-    V3 inv = v3((float)1 / r.d.x, (float)1 / r.d.y, (float)1 / r.d.z);
+    const V3 inv = ray_inverse(r.d);
     int nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
     unsigned long long nodes = 0, tests = 0;
     if (n == 1) {
@@ -239,7 +245,7 @@ LTS_D bool leaf_blocks(V3 p0, V3 p1, V3 p2, const Ray& r) {
 template <bool kHitTest = false>
 LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts) {
     const int n = sv.nPrim;
-    V3 inv = v3((float)1 / r.d.x, (float)1 / r.d.y, (float)1 / r.d.z);
+    const V3 inv = ray_inverse(r.d);
     int nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
     if (n == 1) {
         V3 p0, p1, p2; int prim;
