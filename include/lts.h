@@ -61,7 +61,7 @@ typedef struct lts_render_params {
     uint32_t seed;        /* sampler seed (replaces time(0))                       */
     int32_t stratified;   /* 1: film sample per Sampler::stratifiedSample          */
     int32_t integrator;   /* LTS_INTEGRATOR_*                                      */
-    int32_t spp_per_pass; /* samples per pass; 0 = automatic (wavefront: 2^26 paths */
+    int32_t spp_per_pass; /* samples per pass; 0 = automatic (wavefront: 2^27 paths */
                           /* per pass, cached: 2^25; megakernel: 2^28 / 12 B)       */
     /* wavefront only (0 / -1 = default or the autotuned value) */
     int32_t trace_block;        /* block size of the persistent traversal kernels: 64, 128, 256 */

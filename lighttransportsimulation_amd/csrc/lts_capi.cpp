@@ -456,12 +456,14 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
     sample_range(p, &b0, &b1);
     if (b0 > 0 && scenes.size() != 1) return fail(c, LTS_E_ARG, "progressive rendering is for one scene");
     if (b0 > 0 && (rc = film_continues(c, p, b0)) != LTS_OK) return rc;
-    // automatic pass: 2^26 paths (about 19 GB of path state; each trace launch ends in a tail, so
-    // fewer, larger passes keep the GPU busy: 2^24 -> 2^26 paths took the 1080p frame from 585 to
-    // 756 Msamples/s); the cached integrator takes 2^25, as its warm-up and training steps are
-    // counted per pass (at 1080p: 4 passes of 16 spp for 64 spp, two of them querying the cache;
-    // measured against 8 passes of 8 spp and 2 of 32, it reaches a given error soonest)
-    const size_t passPaths = cached ? (size_t)1 << 25 : (size_t)1 << 26;
+    // automatic pass: 2^27 paths (about 36 GB of path state at 268 B per path). Each pass runs its
+    // bounces as separate launches whose late ones hold few paths, so fewer, larger passes keep the
+    // GPU busy: 2^24 -> 2^26 paths took the 1080p 64-spp frame from 585 to 756 Msamples/s, and
+    // 2^26 -> 2^27 (the whole frame in one pass) from 1392 to 1524. The cached integrator takes
+    // 2^25, as its warm-up and training steps are counted per pass (at 1080p: 4 passes of 16 spp for
+    // 64 spp, two of them querying the cache; measured against 8 passes of 8 spp and 2 of 32, it
+    // reaches a given error soonest)
+    const size_t passPaths = cached ? (size_t)1 << 25 : (size_t)1 << 27;
     int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, passPaths / totalPix);
     if (sample_rgb) pass = b1 - b0;
     pass = std::min(pass, b1 - b0);
