@@ -95,6 +95,9 @@ def test_wavefront_rejects_bad_tuning(gpu_ctx, cornell):
         gpu_ctx.render(4, 5, 3, SEED, stratified=1, integrator=INTEGRATOR_WAVEFRONT, trace_block=100)
     with pytest.raises(LtsError):
         gpu_ctx.render(4, 65, 3, SEED, stratified=1, integrator=INTEGRATOR_WAVEFRONT)
+    # the path record keeps the sample number in 29 bits: larger sample counts are refused up front
+    with pytest.raises(LtsError):
+        gpu_ctx.render(2**29 + 1, 5, 3, SEED, stratified=0, integrator=INTEGRATOR_WAVEFRONT)
 
 
 @pytest.mark.gpu
