@@ -36,19 +36,27 @@ LTS_D void load_tri(const TriLeaf* __restrict__ tris, int k, V3& p0, V3& p1, V3&
     prim = __float_as_int(c.y);
 }
 
+// The reference's triangle tests take 1.0 / det in double and compare float barycentrics with the
+// double 1 -/+ SHADOW_EPS. Both have exact float forms: a double quotient of float operands rounded
+// to float is the correctly rounded float quotient (53 >= 2 * 24 + 2 bits: the double rounding is
+// innocuous), which the IEEE float division gives; and for a float x, (double)x > c holds exactly
+// when x exceeds the largest float not above c, the two constants below.
+constexpr float kOneMinusEpsF = 0x1.fff2e4p-1f;  // largest float <= 1.0 - (double)SHADOW_EPS
+constexpr float kOnePlusEpsF = 0x1.00068cp+0f;   // largest float <= 1.0 + (double)SHADOW_EPS
+
 // Triangle::intersect hit test (shapes/Triangle.cu:94-128)
 LTS_D bool tri_hit(V3 p0, V3 p1, V3 p2, const Ray& r, float& t, float& u, float& v, float& invDetOut) {
     V3 edge1 = p1 - p0, edge2 = p2 - p0;
     V3 pVec = cross(r.d, edge2);
     float det = dot(edge1, pVec);
     if (det == 0) return false;
-    float invDet = (float)(1.0 / (double)det);
+    float invDet = 1.0f / det;
     V3 tVec = r.o - p0;
     u = invDet * dot(tVec, pVec);
-    if (u < SHADOW_EPS || (double)u > 1.0 - (double)SHADOW_EPS) return false;
+    if (u < SHADOW_EPS || u > kOneMinusEpsF) return false;
     V3 qVec = cross(tVec, edge1);
     v = invDet * dot(r.d, qVec);
-    if (v < SHADOW_EPS || (double)(u + v) > 1.0 - (double)SHADOW_EPS) return false;
+    if (v < SHADOW_EPS || u + v > kOneMinusEpsF) return false;
     t = invDet * dot(edge2, qVec);
     invDetOut = invDet;
     return t > 0 && t < r.tMax;
@@ -60,13 +68,13 @@ LTS_D bool tri_occludes(V3 p0, V3 p1, V3 p2, const Ray& r) {
     V3 pVec = cross(r.d, edge1);
     float det = dot(edge2, pVec);
     if (det == 0) return false;
-    float invDet = (float)(1.0 / (double)det);
+    float invDet = 1.0f / det;
     V3 tVec = r.o - p0;
     float u = invDet * dot(tVec, pVec);
-    if (u < SHADOW_EPS || (double)u > 1.0 + (double)SHADOW_EPS) return false;
+    if (u < SHADOW_EPS || u > kOnePlusEpsF) return false;
     V3 qVec = cross(tVec, edge2);
     float v = invDet * dot(r.d, qVec);
-    if (v < SHADOW_EPS || (double)(u + v) > 1.0 + (double)SHADOW_EPS) return false;
+    if (v < SHADOW_EPS || u + v > kOnePlusEpsF) return false;
     float t = invDet * dot(edge1, qVec);
     return t > 0 && t < r.tMax;
 }
