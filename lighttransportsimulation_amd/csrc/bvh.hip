@@ -904,7 +904,7 @@ hipError_t bvh_reserve(BvhBuffers& b, int n) {
 // Greedy surface-area collapse (k_collapse_count / k_collapse_emit), level by level from the root.
 // Returns the number of 4-wide nodes in *nodes4, or 0 when the tree has more than kTraceStack / 3
 // levels (a 4-wide node leaves at most 3 entries on the traversal stack).
-static hipError_t collapse_greedy(BvhBuffers& b, hipStream_t s, int nb, int* nodes4) {
+static hipError_t collapse_greedy(BvhBuffers& b, hipStream_t s, int* nodes4) {
     const int n = b.n, m = n - 1;
     *nodes4 = 0;
     int32_t *fa = nullptr, *fb = nullptr, *cnt = nullptr, *tmp = nullptr;
@@ -931,7 +931,6 @@ static hipError_t collapse_greedy(BvhBuffers& b, hipStream_t s, int nb, int* nod
         count = next;
         std::swap(fa, fb);
     }
-    (void)nb;
     void* ps[] = {fa, fb, cnt, tmp};
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -967,7 +966,7 @@ hipError_t bvh4_enqueue(BvhBuffers& b, hipStream_t s, int* depth, int* nodes4, b
     if (e != hipSuccess) return e;
     const int binaryDepth = *depth;
     if (!*even) {
-        if ((e = collapse_greedy(b, s, nb, nodes4)) != hipSuccess) return e;
+        if ((e = collapse_greedy(b, s, nodes4)) != hipSuccess) return e;
         if (*nodes4 > 0) return hipSuccess;
         *even = true;
     }
