@@ -25,6 +25,7 @@ EXPORTED_SYMBOLS = (
     "lts_write_image", "lts_device_copy_benchmark", "lts_render_batch", "lts_nrc_configure", "lts_nrc_set_weights",
     "lts_nrc_get_weights", "lts_nrc_infer", "lts_nrc_train_step", "lts_set_render_tree", "lts_render_tree_info",
     "lts_obj_to_bin", "lts_film_save", "lts_film_load", "lts_film_resolve", "lts_set_texture_filter",
+    "lts_load_image",
 )
 
 TEXTURE_REFERENCE = 0
@@ -109,6 +110,7 @@ def load_library():
     L.lts_render_batch.argtypes = [C.POINTER(vp), i32, C.POINTER(RenderParams), C.POINTER(vp), C.POINTER(Stats)]
     L.lts_write_image.argtypes = [C.c_char_p, vp, i32, i32, i32]
     L.lts_obj_to_bin.argtypes = [C.c_char_p, C.POINTER(i32)]
+    L.lts_load_image.argtypes = [C.c_char_p, C.POINTER(i32), C.POINTER(i32), vp, C.c_int64]
     L.lts_film_save.argtypes = [vp, C.c_char_p]
     L.lts_film_load.argtypes = [vp, C.c_char_p]
     L.lts_film_resolve.argtypes = [vp, vp]
@@ -366,3 +368,18 @@ def write_image(path: str, rgb: np.ndarray, fmt: int) -> None:
     rc = load_library().lts_write_image(path.encode(), _ptr(rgb), w, h, fmt)
     if rc != 0:
         raise LtsError(f"lts_write_image({path}) failed ({rc})")
+
+
+def load_image(path: str) -> np.ndarray:
+    """Decodes an 8-bit image to linear RGB, bottom row first (include/lts.h: lts_load_image,
+    the texture loader's decode); returns an [h][w][3] float32 array."""
+    L = load_library()
+    w, h = C.c_int32(), C.c_int32()
+    rc = L.lts_load_image(path.encode(), C.byref(w), C.byref(h), None, 0)
+    if rc != 0:
+        raise LtsError(f"lts_load_image({path}) failed ({rc})")
+    out = np.zeros((h.value, w.value, 3), np.float32)
+    rc = L.lts_load_image(path.encode(), C.byref(w), C.byref(h), _ptr(out), out.size)
+    if rc != 0:
+        raise LtsError(f"lts_load_image({path}) failed ({rc})")
+    return out

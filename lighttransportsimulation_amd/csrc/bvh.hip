@@ -44,9 +44,10 @@ __device__ __forceinline__ void tri_box(const float* t, float* mn, float* mx) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_prim_bounds(const float* __restrict__ soup, int n, float* __restrict__ primBox,
-                                                        uint32_t* __restrict__ partial) {
+                                                        uint32_t* __restrict__ partial, int32_t* __restrict__ status) {
     __shared__ uint32_t red[6][kBlock / 64];
     int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) *status = 0;  // the build's first kernel: later kernels on the stream see the cleared flags
     uint32_t k[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
     if (i < n) {
         float t[9];
@@ -337,6 +338,13 @@ constexpr int kChunk2 = 64 * kChunk;     // sorted leaves per level-2 workgroup
 // level-1 roots filed per level-2 chunk: each is a child of a node containing one of the
 // kChunk2 / kChunk + 1 chunk boundaries of that range, and at most 58 nodes contain a boundary
 constexpr int kBucketCap = 2 * (kChunk2 / kChunk + 1) * 58;
+// Build status bits (BvhBuffers::status). The capacities above follow from the depth bound of a
+// Karras tree over at most 2^26 leaves (58 nodes on a root-to-leaf path); a violated bound is
+// reported instead of leaving boxes unrefitted.
+constexpr int kStatusBucket = 1;  // a level-2 chunk received more than kBucketCap level-1 roots
+constexpr int kStatusRoots = 2;   // more level-3 roots than primitives
+constexpr int kStatusLevel3 = 4;  // more level-3 roots than level-3 climbers were launched
+constexpr int kStatusClimb = 8;   // a climb took more than 128 steps
 
 __device__ __forceinline__ void load_box(const float* box, int node, float* o) {
 #pragma unroll
@@ -355,7 +363,8 @@ __global__ __launch_bounds__(kChunk) void k_refit_l1(const float* __restrict__ p
                                                      int mode, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
                                                      const int32_t* __restrict__ parent, const int32_t* __restrict__ rangeLo,
                                                      const int32_t* __restrict__ rangeHi, float* __restrict__ box,
-                                                     int32_t* __restrict__ bucket, int32_t* __restrict__ bucketCount) {
+                                                     int32_t* __restrict__ bucket, int32_t* __restrict__ bucketCount,
+                                                     int32_t* __restrict__ status) {
     __shared__ float lbox[kChunk * 6];
     __shared__ float ibox[kChunk * 6];
     __shared__ int32_t sl[kChunk], sr[kChunk], sp[kChunk];
@@ -381,7 +390,8 @@ __global__ __launch_bounds__(kChunk) void k_refit_l1(const float* __restrict__ p
     if (k < ce && n > 1) {
         int cur = (n - 1) + k;
         int node = parent[cur];
-        for (int step = 0; step < 128; step++) {
+        int step = 0;
+        for (; step < 128; step++) {
             const int j = node - cs;
             const bool isLocal = node >= cs && node < ce && node < n - 1 && ((local[j >> 5] >> (j & 31)) & 1u);
             if (!isLocal) {
@@ -389,6 +399,7 @@ __global__ __launch_bounds__(kChunk) void k_refit_l1(const float* __restrict__ p
                     int b = cs / kChunk2;
                     int slot = atomicAdd(&bucketCount[b], 1);
                     if (slot < kBucketCap) bucket[(size_t)b * kBucketCap + slot] = cur;
+                    else atomicOr(status, kStatusBucket);
                 }
                 break;
             }
@@ -410,6 +421,7 @@ __global__ __launch_bounds__(kChunk) void k_refit_l1(const float* __restrict__ p
             cur = node;
             node = sp[j];
         }
+        if (step == 128) atomicOr(status, kStatusClimb);
     }
     __syncthreads();
     if (k < ce && k < n - 1 && ((local[t >> 5] >> (t & 31)) & 1u)) {
@@ -426,7 +438,7 @@ __global__ __launch_bounds__(1024) void k_refit_l2(int n, const int32_t* __restr
                                                    const int32_t* __restrict__ parent, const int32_t* __restrict__ rangeLo,
                                                    const int32_t* __restrict__ rangeHi, float* box, const int32_t* __restrict__ bucket,
                                                    const int32_t* __restrict__ bucketCount, int32_t* __restrict__ roots,
-                                                   int32_t* __restrict__ rootCount) {
+                                                   int32_t* __restrict__ rootCount, int32_t* __restrict__ status) {
     __shared__ uint32_t arrive[kChunk2 / 32];
     const int cs = blockIdx.x * kChunk2, ce = min(cs + kChunk2, n);
     for (int w = threadIdx.x; w < kChunk2 / 32; w += 1024) arrive[w] = 0;
@@ -435,10 +447,12 @@ __global__ __launch_bounds__(1024) void k_refit_l2(int n, const int32_t* __restr
     for (int e = threadIdx.x; e < cnt; e += 1024) {
         int cur = bucket[(size_t)blockIdx.x * kBucketCap + e];
         int node = parent[cur];
-        for (int step = 0; step < 128 && node >= 0 && node < n - 1; step++) {
+        int step = 0;
+        for (; step < 128 && node >= 0 && node < n - 1; step++) {
             if (rangeLo[node] < cs || rangeHi[node] >= ce) {
                 int slot = atomicAdd(rootCount, 1);
-                roots[slot] = cur;
+                if (slot < n) roots[slot] = cur;
+                else atomicOr(status, kStatusRoots);
                 break;
             }
             const int j = node - cs;
@@ -455,6 +469,7 @@ __global__ __launch_bounds__(1024) void k_refit_l2(int n, const int32_t* __restr
             cur = node;
             node = parent[node];
         }
+        if (step == 128) atomicOr(status, kStatusClimb);
     }
 }
 
@@ -462,11 +477,16 @@ __global__ __launch_bounds__(1024) void k_refit_l2(int n, const int32_t* __restr
 // with device-scope ordering.
 __global__ __launch_bounds__(kBlock) void k_refit_global(int n, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
                                                          const int32_t* __restrict__ parent, float* box, int32_t* flags,
-                                                         const int32_t* __restrict__ roots, const int32_t* __restrict__ rootCount) {
+                                                         const int32_t* __restrict__ roots, const int32_t* __restrict__ rootCount,
+                                                         int32_t* __restrict__ status) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= *rootCount) return;
+    const int count = *rootCount;
+    if (t == 0 && count > (int)(gridDim.x * kBlock)) atomicOr(status, kStatusLevel3);
+    if (t >= count || t >= n) return;
     int node = parent[roots[t]];
-    for (int step = 0; step < 128 && node >= 0 && node < n - 1; step++) {
+    for (int step = 0;; step++) {
+        if (node < 0 || node >= n - 1) return;
+        if (step == 128) { atomicOr(status, kStatusClimb); return; }
         __threadfence();
         int old = __hip_atomic_fetch_add(&flags[node], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (old == 0) return;
@@ -894,6 +914,7 @@ hipError_t bvh_reserve(BvhBuffers& b, int n) {
     TRY(dalloc(&b.bucket, (size_t)((n + kChunk2 - 1) / kChunk2) * kBucketCap));
     TRY(dalloc(&b.partial, 6 * (size_t)blocks_for(n)));
     TRY(dalloc(&b.sceneKeys, 8));
+    TRY(dalloc(&b.status, 1));
     TRY(dalloc(&b.nodes, n));
     TRY(dalloc(&b.tris, n));
 #undef TRY
@@ -1055,7 +1076,7 @@ hipError_t ploc_enqueue(BvhBuffers& b, hipStream_t s, int maxDepth, int* depthOu
 void bvh_release(BvhBuffers& b) {
     void* ps[] = {b.primBox, b.box, b.keysA, b.keysB, b.valsA, b.valsB, b.counts, b.digitBase, b.left, b.right,
                   b.parent, b.flags, b.sceneKeys, b.nodes, b.tris, b.rangeLo, b.rangeHi, b.roots, b.rootCount, b.bucket, b.partial,
-                  b.nodes4, b.index4, b.scanTotals};
+                  b.nodes4, b.index4, b.scanTotals, b.status};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     b = BvhBuffers();
@@ -1066,7 +1087,7 @@ hipError_t bvh_enqueue(BvhBuffers& b, const float* soup, int n, int mode, bool p
     b.n = n;
     b.tiles = (n + kTile - 1) / kTile;
     if (ev) (void)hipEventRecord(ev->e[0], s);
-    k_prim_bounds<<<blocks_for(n), kBlock, 0, s>>>(soup, n, b.primBox, b.partial);
+    k_prim_bounds<<<blocks_for(n), kBlock, 0, s>>>(soup, n, b.primBox, b.partial, b.status);
     k_reduce_scene_keys<<<1, 1024, 0, s>>>(b.partial, blocks_for(n), b.sceneKeys);
     k_morton<<<blocks_for(n), kBlock, 0, s>>>(b.primBox, n, b.sceneKeys, b.keysA, b.valsA);
     if (ev) (void)hipEventRecord(ev->e[1], s);
@@ -1093,15 +1114,16 @@ hipError_t bvh_enqueue(BvhBuffers& b, const float* soup, int n, int mode, bool p
     const int chunks = (n + kChunk - 1) / kChunk, chunks2 = (n + kChunk2 - 1) / kChunk2;
     k_fill_i32<<<blocks_for(chunks2 + 1), kBlock, 0, s>>>(b.rootCount, chunks2 + 1, 0);  // level-3 count + buckets
     k_refit_l1<<<chunks, kChunk, 0, s>>>(b.primBox, b.valsA, n, mode, b.left, b.right, b.parent, b.rangeLo, b.rangeHi, b.box,
-                                         b.bucket, b.rootCount + 1);
+                                         b.bucket, b.rootCount + 1, b.status);
     if (chunks > 1)
         k_refit_l2<<<chunks2, 1024, 0, s>>>(n, b.left, b.right, b.parent, b.rangeLo, b.rangeHi, b.box, b.bucket,
-                                            b.rootCount + 1, b.roots, b.rootCount);
+                                            b.rootCount + 1, b.roots, b.rootCount, b.status);
     if (chunks2 > 1) {
         // level-3 roots: at most two per level-2-crossing node, and each of the chunks2 - 1
         // boundaries is crossed by at most 58 nodes
         const long bound = std::min<long>((long)n, 2L * chunks2 * 58);
-        k_refit_global<<<blocks_for(bound), kBlock, 0, s>>>(n, b.left, b.right, b.parent, b.box, b.flags, b.roots, b.rootCount);
+        k_refit_global<<<blocks_for(bound), kBlock, 0, s>>>(n, b.left, b.right, b.parent, b.box, b.flags, b.roots, b.rootCount,
+                                                            b.status);
     }
     if (ev) (void)hipEventRecord(ev->e[3], s);
     if (pack) {

@@ -21,6 +21,11 @@ static bool slurp(const std::string& path, std::vector<uint8_t>& buf) {
     return true;
 }
 
+// Largest image accepted: 32768 texels a side and 2^28 texels (a 16384^2 image), so that the
+// decoded bytes and the pyramid stay far below the sizes where allocation sizes could overflow.
+constexpr uint32_t kMaxImageSide = 1u << 15;
+constexpr uint64_t kMaxImageTexels = 1ull << 28;
+
 static inline uint32_t rd_be32(const uint8_t* p) {
     return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | (uint32_t)p[3];
 }
@@ -41,7 +46,13 @@ static bool png_to_rgb8(const std::vector<uint8_t>& b, int& w, int& h, std::vect
         if (at + 12 + (size_t)n > b.size()) { err = "truncated PNG chunk"; return false; }
         const uint8_t* d = &b[at + 8];
         if (!std::memcmp(&b[at + 4], "IHDR", 4)) {
-            w = (int)rd_be32(d); h = (int)rd_be32(d + 4); depth = d[8]; color = d[9]; interlace = d[12];
+            if (n != 13) { err = "bad PNG header chunk"; return false; }
+            const uint32_t uw = rd_be32(d), uh = rd_be32(d + 4);
+            if (uw == 0 || uh == 0 || uw > kMaxImageSide || uh > kMaxImageSide || (uint64_t)uw * uh > kMaxImageTexels) {
+                err = "PNG dimensions out of range";
+                return false;
+            }
+            w = (int)uw; h = (int)uh; depth = d[8]; color = d[9]; interlace = d[12];
         } else if (!std::memcmp(&b[at + 4], "PLTE", 4)) {
             pal.assign(d, d + n);
         } else if (!std::memcmp(&b[at + 4], "IDAT", 4)) {
@@ -104,11 +115,16 @@ static bool ppm_to_rgb8(const std::vector<uint8_t>& b, int& w, int& h, std::vect
         }
         if (at >= b.size() || !std::isdigit(b[at])) { err = "bad PPM header"; return false; }
         v[k] = 0;
-        while (at < b.size() && std::isdigit(b[at])) v[k] = v[k] * 10 + (b[at++] - '0');
+        while (at < b.size() && std::isdigit(b[at])) {
+            if (v[k] < (1L << 40)) v[k] = v[k] * 10 + (b[at] - '0');  // larger values are rejected below
+            at++;
+        }
     }
     at++;  // single whitespace after maxval
+    if (v[0] <= 0 || v[1] <= 0 || v[0] > (long)kMaxImageSide || v[1] > (long)kMaxImageSide ||
+        (uint64_t)v[0] * (uint64_t)v[1] > kMaxImageTexels) { err = "PPM dimensions out of range"; return false; }
     w = (int)v[0]; h = (int)v[1];
-    if (v[2] != 255 || w <= 0 || h <= 0 || at + (size_t)w * h * 3 > b.size()) { err = "unsupported PPM (need P6, maxval 255)"; return false; }
+    if (v[2] != 255 || at + (size_t)w * h * 3 > b.size()) { err = "unsupported PPM (need P6, maxval 255)"; return false; }
     rgb.assign(b.begin() + (long)at, b.begin() + (long)(at + (size_t)w * h * 3));
     return true;
 }

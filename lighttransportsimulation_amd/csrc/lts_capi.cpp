@@ -180,6 +180,11 @@ int run_build(lts_ctx* c, BvhBuffers& b, const float* dsoup, int n, int mode, bo
     for (int i = 0; i < 5; i++) ev.e[i] = c->ev[i];
     HIPCHK(c, bvh_enqueue(b, dsoup, n, mode, pack, c->stream, &ev));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    int32_t status = 0;
+    HIPCHK(c, hipMemcpy(&status, b.status, sizeof(status), hipMemcpyDeviceToHost));
+    if (status != 0)
+        return fail(c, LTS_E_STATE, "LBVH refit exceeded its capacity (status " + std::to_string(status) +
+                                        "): the tree is deeper than the 58 levels the refit provides for");
     if (t) {
         t->bounds_ms = ms_between(ev.e[0], ev.e[1]);
         t->sort_ms = ms_between(ev.e[1], ev.e[2]);
@@ -693,6 +698,19 @@ int nrc_setup(lts_ctx* c, const lts_nrc_config* in) {
 
 int nrc_ensure(lts_ctx* c) { return c->nrcReady ? LTS_OK : nrc_setup(c, nullptr); }
 
+// Runs an entry point's body; a failed host allocation returns LTS_E_NOMEM and any other C++
+// exception LTS_E_IO, with the message kept in the context.
+template <class F>
+int guarded(lts_ctx* c, F&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return fail(c, LTS_E_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(c, LTS_E_IO, std::string("unexpected error: ") + e.what());
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -770,7 +788,7 @@ int lts_read_app_params(const char* path, int32_t* w, int32_t* h, int32_t* spp, 
     return LTS_OK;
 }
 
-int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* subject, int32_t width, int32_t height) {
+static int load_scene_impl(lts_ctx* c, const char* dir, const char* scene, const char* subject, int32_t width, int32_t height) {
     if (!c || !dir || !scene) return fail(c, LTS_E_ARG, "null argument");
     HIPCHK(c, hipSetDevice(c->device));
     free_scene(c->scene);
@@ -875,7 +893,7 @@ int lts_render_tree_info(lts_ctx* c, int32_t* kind, int32_t* depth, int32_t* nod
     return LTS_OK;
 }
 
-int lts_bvh_build(lts_ctx* c, const float* tri, int32_t n, int32_t mode, lts_bvh_export* out, lts_bvh_timing* t) {
+static int bvh_build_impl(lts_ctx* c, const float* tri, int32_t n, int32_t mode, lts_bvh_export* out, lts_bvh_timing* t) {
     if (!c || !tri || n < 1 || n > kMaxPrims || (mode != 0 && mode != 1)) return fail(c, LTS_E_ARG, "bad build arguments");
     HIPCHK(c, hipSetDevice(c->device));
     size_t bytes = 9 * (size_t)n * sizeof(float);
@@ -894,7 +912,7 @@ int lts_bvh_build(lts_ctx* c, const float* tri, int32_t n, int32_t mode, lts_bvh
     return out ? copy_export(c, c->build, mode, out) : LTS_OK;
 }
 
-int lts_bvh_build_device(lts_ctx* c, const float* dtri, int32_t n, int32_t mode, int32_t repeat, lts_bvh_timing* t) {
+static int bvh_build_device_impl(lts_ctx* c, const float* dtri, int32_t n, int32_t mode, int32_t repeat, lts_bvh_timing* t) {
     if (!c || n < 1 || n > kMaxPrims || (mode != 0 && mode != 1) || repeat < 1) return fail(c, LTS_E_ARG, "bad build arguments");
     if (!dtri) {
         if (!c->buildSoup || c->buildSoupCap < 9 * (size_t)n * sizeof(float)) return fail(c, LTS_E_STATE, "no uploaded soup of that size");
@@ -917,13 +935,13 @@ int lts_bvh_build_device(lts_ctx* c, const float* dtri, int32_t n, int32_t mode,
     return LTS_OK;
 }
 
-int lts_bvh_export_last(lts_ctx* c, lts_bvh_export* out) {
+static int bvh_export_last_impl(lts_ctx* c, lts_bvh_export* out) {
     if (!c || !out) return LTS_E_ARG;
     if (c->build.n < 1) return fail(c, LTS_E_STATE, "no build to export");
     return copy_export(c, c->build, c->lastMode, out);
 }
 
-int lts_render(lts_ctx* c, const lts_render_params* p, float* rgb_out, float* sample_rgb, lts_stats* st) {
+static int render_impl(lts_ctx* c, const lts_render_params* p, float* rgb_out, float* sample_rgb, lts_stats* st) {
     if (!c || !p) return fail(c, LTS_E_ARG, "null argument");
     if (!c->scene.loaded) return fail(c, LTS_E_STATE, "no scene loaded");
     int rc = check_render_params(c, p);
@@ -990,7 +1008,7 @@ int lts_render(lts_ctx* c, const lts_render_params* p, float* rgb_out, float* sa
     return LTS_OK;
 }
 
-int lts_render_batch(lts_ctx* const* ctxs, int32_t n, const lts_render_params* p, float* const* rgb_out, lts_stats* st) {
+static int render_batch_impl(lts_ctx* const* ctxs, int32_t n, const lts_render_params* p, float* const* rgb_out, lts_stats* st) {
     if (!ctxs || n < 1 || !ctxs[0] || !p) return LTS_E_ARG;
     lts_ctx* c = ctxs[0];
     if (p->integrator != LTS_INTEGRATOR_WAVEFRONT) return fail(c, LTS_E_ARG, "batched rendering uses the wavefront integrator");
@@ -1052,12 +1070,12 @@ int lts_render_batch(lts_ctx* const* ctxs, int32_t n, const lts_render_params* p
     return LTS_OK;
 }
 
-int lts_nrc_configure(lts_ctx* c, const lts_nrc_config* cfg) {
+static int nrc_configure_impl(lts_ctx* c, const lts_nrc_config* cfg) {
     if (!c) return LTS_E_ARG;
     return nrc_setup(c, cfg);
 }
 
-int lts_nrc_set_weights(lts_ctx* c, const float* w) {
+static int nrc_set_weights_impl(lts_ctx* c, const float* w) {
     if (!c || !w) return fail(c, LTS_E_ARG, "null argument");
     int rc = nrc_ensure(c);
     if (rc != LTS_OK) return rc;
@@ -1065,7 +1083,7 @@ int lts_nrc_set_weights(lts_ctx* c, const float* w) {
     return nrc_upload_weights(c, w);
 }
 
-int lts_nrc_get_weights(lts_ctx* c, float* w) {
+static int nrc_get_weights_impl(lts_ctx* c, float* w) {
     if (!c || !w) return fail(c, LTS_E_ARG, "null argument");
     int rc = nrc_ensure(c);
     if (rc != LTS_OK) return rc;
@@ -1074,7 +1092,7 @@ int lts_nrc_get_weights(lts_ctx* c, float* w) {
     return LTS_OK;
 }
 
-int lts_nrc_infer(lts_ctx* c, const float* raw, int32_t n, float* out) {
+static int nrc_infer_impl(lts_ctx* c, const float* raw, int32_t n, float* out) {
     if (!c || n < 0 || (n > 0 && (!raw || !out))) return fail(c, LTS_E_ARG, "bad nrc_infer arguments");
     int rc = nrc_ensure(c);
     if (rc != LTS_OK) return rc;
@@ -1098,7 +1116,7 @@ int lts_nrc_infer(lts_ctx* c, const float* raw, int32_t n, float* out) {
     return LTS_OK;
 }
 
-int lts_nrc_train_step(lts_ctx* c, const float* raw, const float* target, int32_t n, float* loss, float* grad) {
+static int nrc_train_step_impl(lts_ctx* c, const float* raw, const float* target, int32_t n, float* loss, float* grad) {
     if (!c || n < 0 || (n > 0 && (!raw || !target))) return fail(c, LTS_E_ARG, "bad nrc_train_step arguments");
     int rc = nrc_ensure(c);
     if (rc != LTS_OK) return rc;
@@ -1140,7 +1158,7 @@ int lts_device_copy_benchmark(lts_ctx* c, uint64_t bytes, int32_t repeat, double
 
 // Film checkpoint file: "LTSFILM1", int32 width, height, next sample, spp, seed, stratified,
 // max_bounce, rr_start, integrator, then width * height * 4 floats (rgb sums, filter weight).
-int lts_film_save(lts_ctx* c, const char* path) {
+static int film_save_impl(lts_ctx* c, const char* path) {
     if (!c || !path) return LTS_E_ARG;
     if (!c->scene.loaded || c->filmNext == 0 || !c->film) return fail(c, LTS_E_STATE, "no rendered film to save");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1157,7 +1175,7 @@ int lts_film_save(lts_ctx* c, const char* path) {
     return LTS_OK;
 }
 
-int lts_film_load(lts_ctx* c, const char* path) {
+static int film_load_impl(lts_ctx* c, const char* path) {
     if (!c || !path) return LTS_E_ARG;
     if (!c->scene.loaded) return fail(c, LTS_E_STATE, "no scene loaded");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1182,7 +1200,7 @@ int lts_film_load(lts_ctx* c, const char* path) {
     return LTS_OK;
 }
 
-int lts_film_resolve(lts_ctx* c, float* rgb_out) {
+static int film_resolve_impl(lts_ctx* c, float* rgb_out) {
     if (!c || !rgb_out) return LTS_E_ARG;
     if (!c->scene.loaded || c->filmNext == 0 || !c->film) return fail(c, LTS_E_STATE, "no film to resolve");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1194,7 +1212,7 @@ int lts_film_resolve(lts_ctx* c, float* rgb_out) {
     return LTS_OK;
 }
 
-int lts_write_image(const char* path, const float* rgb, int32_t w, int32_t h, int32_t format) {
+static int write_image_impl(const char* path, const float* rgb, int32_t w, int32_t h, int32_t format) {
     if (!path || !rgb || w < 1 || h < 1) return LTS_E_ARG;
     if (format < 0 || format > 3) return LTS_E_ARG;
     const bool ok = format == 0   ? write_ppm(path, rgb, w, h)
@@ -1204,7 +1222,20 @@ int lts_write_image(const char* path, const float* rgb, int32_t w, int32_t h, in
     return ok ? LTS_OK : LTS_E_IO;
 }
 
-int lts_obj_to_bin(const char* stem, int32_t* nTris) {
+static int load_image_impl(const char* path, int32_t* w, int32_t* h, float* rgb_out, int64_t cap) {
+    if (!path || !w || !h) return LTS_E_ARG;
+    int iw = 0, ih = 0;
+    std::vector<float> rgb;
+    std::string err;
+    if (!load_image_linear(path, iw, ih, rgb, err)) return LTS_E_IO;
+    *w = iw; *h = ih;
+    if (!rgb_out) return LTS_OK;
+    if (cap < (int64_t)rgb.size()) return LTS_E_ARG;
+    std::memcpy(rgb_out, rgb.data(), rgb.size() * sizeof(float));
+    return LTS_OK;
+}
+
+static int obj_to_bin_impl(const char* stem, int32_t* nTris) {
     if (!stem) return LTS_E_ARG;
     std::vector<char> bin;
     std::string err;
@@ -1214,6 +1245,75 @@ int lts_obj_to_bin(const char* stem, int32_t* nTris) {
     if (!out || !out.write(bin.data(), (std::streamsize)bin.size())) return LTS_E_IO;
     if (nTris) *nTris = n;
     return LTS_OK;
+}
+
+// Entry points: host allocation failures and other exceptions become error codes, never
+// an exception crossing the C boundary.
+int lts_load_scene(lts_ctx* c, const char* dir, const char* scene, const char* subject, int32_t width, int32_t height) {
+    return guarded(c, [&] { return load_scene_impl(c, dir, scene, subject, width, height); });
+}
+
+int lts_bvh_build(lts_ctx* c, const float* tri, int32_t n, int32_t mode, lts_bvh_export* out, lts_bvh_timing* t) {
+    return guarded(c, [&] { return bvh_build_impl(c, tri, n, mode, out, t); });
+}
+
+int lts_bvh_build_device(lts_ctx* c, const float* dtri, int32_t n, int32_t mode, int32_t repeat, lts_bvh_timing* t) {
+    return guarded(c, [&] { return bvh_build_device_impl(c, dtri, n, mode, repeat, t); });
+}
+
+int lts_bvh_export_last(lts_ctx* c, lts_bvh_export* out) {
+    return guarded(c, [&] { return bvh_export_last_impl(c, out); });
+}
+
+int lts_render(lts_ctx* c, const lts_render_params* p, float* rgb_out, float* sample_rgb, lts_stats* st) {
+    return guarded(c, [&] { return render_impl(c, p, rgb_out, sample_rgb, st); });
+}
+
+int lts_render_batch(lts_ctx* const* ctxs, int32_t n, const lts_render_params* p, float* const* rgb_out, lts_stats* st) {
+    return guarded((ctxs && n > 0) ? ctxs[0] : nullptr, [&] { return render_batch_impl(ctxs, n, p, rgb_out, st); });
+}
+
+int lts_nrc_configure(lts_ctx* c, const lts_nrc_config* cfg) {
+    return guarded(c, [&] { return nrc_configure_impl(c, cfg); });
+}
+
+int lts_nrc_set_weights(lts_ctx* c, const float* w) {
+    return guarded(c, [&] { return nrc_set_weights_impl(c, w); });
+}
+
+int lts_nrc_get_weights(lts_ctx* c, float* w) {
+    return guarded(c, [&] { return nrc_get_weights_impl(c, w); });
+}
+
+int lts_nrc_infer(lts_ctx* c, const float* raw, int32_t n, float* out) {
+    return guarded(c, [&] { return nrc_infer_impl(c, raw, n, out); });
+}
+
+int lts_nrc_train_step(lts_ctx* c, const float* raw, const float* target, int32_t n, float* loss, float* grad) {
+    return guarded(c, [&] { return nrc_train_step_impl(c, raw, target, n, loss, grad); });
+}
+
+int lts_film_save(lts_ctx* c, const char* path) {
+    return guarded(c, [&] { return film_save_impl(c, path); });
+}
+
+int lts_film_load(lts_ctx* c, const char* path) {
+    return guarded(c, [&] { return film_load_impl(c, path); });
+}
+
+int lts_film_resolve(lts_ctx* c, float* rgb_out) {
+    return guarded(c, [&] { return film_resolve_impl(c, rgb_out); });
+}
+
+int lts_write_image(const char* path, const float* rgb, int32_t w, int32_t h, int32_t format) {
+    return guarded(nullptr, [&] { return write_image_impl(path, rgb, w, h, format); });
+}
+
+int lts_load_image(const char* path, int32_t* w, int32_t* h, float* rgb_out, int64_t cap) {
+    return guarded(nullptr, [&] { return load_image_impl(path, w, h, rgb_out, cap); });
+}
+int lts_obj_to_bin(const char* stem, int32_t* nTris) {
+    return guarded(nullptr, [&] { return obj_to_bin_impl(stem, nTris); });
 }
 
 }  // extern "C"

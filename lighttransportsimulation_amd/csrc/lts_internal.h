@@ -41,6 +41,7 @@ struct BvhBuffers {
     int32_t* bucket = nullptr;     // level-1 roots per 64K-leaf chunk
     uint32_t* partial = nullptr;   // per-block scene-bound keys
     uint32_t* sceneKeys = nullptr;// 6 ordered keys of the scene bounds
+    int32_t* status = nullptr;    // refit capacity violations of the last build (bits; 0 = none)
     BvhNode* nodes = nullptr;     // n-1 (or 1 for n == 1)
     TriLeaf* tris = nullptr;      // n
     BvhQNode* nodes4 = nullptr;   // 4-wide collapse for rendering (allocated by bvh4_enqueue)

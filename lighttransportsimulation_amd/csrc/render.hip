@@ -102,13 +102,16 @@ __global__ __launch_bounds__(256) void k_megakernel(SceneView sv, RenderLaunch r
     o[0] = L.r; o[1] = L.g; o[2] = L.b;
 }
 
-// Pixel (x, y) gathers, for each sample in order and each owner pixel of the 3x3 window above
-// and left of it (row major), the owner's filtered contribution when its addSample footprint
-// covers (x, y). A 16x16 block stages, per sample, the film positions and radiance of the 18x18
-// owners its pixels read in LDS, so each owner's film position is drawn once per block instead of
-// once per covered pixel; the sums keep their order.
+// Pixel (x, y) gathers, for each sample in order and each owner pixel (ox, oy) with ox in
+// x-2..x+1 and oy in y-2..y+1 (row major), the owner's filtered contribution when its addSample
+// footprint [ceil(p - 1), floor(p + 1) + 1) covers (x, y). A film position p = owner + f with f in
+// (0, 1) covers owner..owner+1; f = 1 adds owner+2; and p rounding down to the owner itself (f
+// below half an ulp of the owner coordinate, possible from 1024 pixels on) adds owner-1, which is
+// why the window reaches one owner past the pixel. A 16x16 block stages, per sample, the film
+// positions and radiance of the 19x19 owners its pixels read in LDS, so each owner's film position
+// is drawn once per block instead of once per covered pixel; the sums keep their order.
 __global__ __launch_bounds__(256) void k_film_gather(SceneView sv, RenderLaunch rl) {
-    constexpr int T = 16, R = T + 2;
+    constexpr int T = 16, R = T + 3;
     __shared__ float own[R * R][5];  // film position x, y and radiance r, g, b of each owner
     const int W = sv.cam.width, H = sv.cam.height;
     const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
@@ -133,10 +136,10 @@ __global__ __launch_bounds__(256) void k_film_gather(SceneView sv, RenderLaunch 
         }
         __syncthreads();
         if (!inside) continue;
-        for (int oy = y - 2; oy <= y; oy++) {
-            if (oy < 0) continue;
-            for (int ox = x - 2; ox <= x; ox++) {
-                if (ox < 0) continue;
+        for (int oy = y - 2; oy <= y + 1; oy++) {
+            if (oy < 0 || oy >= H) continue;
+            for (int ox = x - 2; ox <= x + 1; ox++) {
+                if (ox < 0 || ox >= W) continue;
                 const float* o = own[(oy - y0 + 2) * R + (ox - x0 + 2)];
                 const V2 pf = v2(o[0], o[1]);
                 int p0x = max((int)ceilf(pf.x - 1.0f), 0), p0y = max((int)ceilf(pf.y - 1.0f), 0);

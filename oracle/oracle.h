@@ -70,6 +70,10 @@ int32_t oracle_occluded(oracle_scene* s, const float o[3], const float d[3], flo
 
 // Gaussian filter table (Filter.cuh:32-50, 82-99): 16x16 floats.
 void oracle_filter_table(float* out256);
+// The deterministic film gather (Camera::addSample, Camera.cu:192-218, as a gather): pfilm holds
+// spp x h x w film positions (x, y), L the radiance of each (3 floats); film receives h x w x
+// {r, g, b, weight} sums.
+void oracle_film_gather(int32_t w, int32_t h, int32_t spp, const float* pfilm, const float* L, float* film);
 // Texture pipeline of scenes loaded afterwards: 0 the reference's mipmap, 1 the filtered one.
 void oracle_set_texture_filter(int32_t filtered);
 void oracle_mip_lookup(const float* rgb, int32_t w, int32_t h, int32_t wrap, int32_t filtered, float s, float t,

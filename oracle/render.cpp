@@ -854,23 +854,46 @@ static Spec Li(const Scene& s, int px, int py, int sample, const oracle_render_p
     return L;
 }
 
-// Film gather: pixel (x, y) accumulates, for each sample s and each owner pixel
-// (ox, oy) in x-2..x, y-2..y (row-major), the owner's sample if its 2x2 (or 3x3 when a
-// film sample is exactly 1) footprint covers (x, y) (Camera::addSample, Camera.cu:192-218).
+// Film gather: pixel (x, y) accumulates, for each sample s and each owner pixel (ox, oy) in
+// x-2..x+1, y-2..y+1 (row-major), the owner's sample if its footprint covers (x, y)
+// (Camera::addSample, Camera.cu:192-218). The footprint [ceil(p - 1), floor(p + 1) + 1) of a film
+// position p = owner + f spans owner..owner+1 for f in (0, 1), adds owner+2 when f is exactly 1,
+// and adds owner-1 when owner + f rounds to the owner itself (f below half an ulp, from 1024 on).
 struct FilmPix { Spec rgb; float w = 0; };
 
-static inline bool footprintWeight(const Scene& s, V2 pFilm, int x, int y, float* w) {
+static inline bool footprintWeight(int W, int H, const float* table, V2 pFilm, int x, int y, float* w) {
     int p0x = (int)std::ceil(pFilm.x - 1.0f), p0y = (int)std::ceil(pFilm.y - 1.0f);
     int p1x = (int)std::floor(pFilm.x + 1.0f) + 1, p1y = (int)std::floor(pFilm.y + 1.0f) + 1;
     p0x = std::max(p0x, 0); p0y = std::max(p0y, 0);
-    p1x = std::min(p1x, s.W); p1y = std::min(p1y, s.H);
+    p1x = std::min(p1x, W); p1y = std::min(p1y, H);
     if (x < p0x || x >= p1x || y < p0y || y >= p1y) return false;
     float dx = std::fabs((x - pFilm.x - 0.5f) * 1.0f * 16);
     float dy = std::fabs((y - pFilm.y - 0.5f) * 1.0f * 16);
     int ix = (int)std::fmin((float)(int)std::floor(dx), (float)(16 - 1));
     int iy = (int)std::fmin((float)(int)std::floor(dy), (float)(16 - 1));
-    *w = s.filterTable[iy * 16 + ix];
+    *w = table[iy * 16 + ix];
     return true;
+}
+
+// The gather over a film of W x H pixels: pf and L hold each sample's film position and radiance,
+// [sample][y][x]; film receives the rgb and weight sums.
+static void gatherFilm(int W, int H, int spp, const V2* pf, const float* L, const float* table, FilmPix* film) {
+    const size_t np = (size_t)W * H;
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            FilmPix& fp = film[(size_t)y * W + x];
+            for (int smp = 0; smp < spp; smp++)
+                for (int oy = y - 2; oy <= y + 1; oy++)
+                    for (int ox = x - 2; ox <= x + 1; ox++) {
+                        if (ox < 0 || oy < 0 || ox >= W || oy >= H) continue;
+                        size_t oi = (size_t)smp * np + (size_t)oy * W + ox;
+                        float w;
+                        if (!footprintWeight(W, H, table, pf[oi], x, y, &w)) continue;
+                        const float* l = &L[3 * oi];
+                        fp.rgb += Spec(l[0], l[1], l[2]) * w;
+                        fp.w += w;
+                    }
+        }
 }
 
 } // namespace orc
@@ -903,6 +926,16 @@ extern "C" int32_t oracle_scene_info(const oracle_scene* s, int32_t* nt, int32_t
 }
 
 extern "C" void oracle_filter_table(float* out) { filterTable(out); }
+
+extern "C" void oracle_film_gather(int32_t w, int32_t h, int32_t spp, const float* pfilm, const float* L, float* film) {
+    float table[256];
+    filterTable(table);
+    std::vector<FilmPix> f((size_t)w * h);
+    gatherFilm(w, h, spp, reinterpret_cast<const V2*>(pfilm), L, table, f.data());
+    for (size_t i = 0; i < f.size(); i++) {
+        film[4 * i] = f[i].rgb.r; film[4 * i + 1] = f[i].rgb.g; film[4 * i + 2] = f[i].rgb.b; film[4 * i + 3] = f[i].w;
+    }
+}
 
 extern "C" void oracle_set_texture_filter(int32_t filtered) { g_textureFiltered = filtered != 0; }
 
@@ -996,21 +1029,7 @@ extern "C" int oracle_render(oracle_scene* sc, const oracle_render_params* p, fl
                     }
                     pf[(size_t)smp * np + (size_t)y * W + x] = V2((float)(x + film2.x), (float)(y + film2.y));
                 }
-        for (int y = 0; y < H; y++)
-            for (int x = 0; x < W; x++) {
-                FilmPix& fp = film[(size_t)y * W + x];
-                for (int smp = 0; smp < spp; smp++)
-                    for (int oy = y - 2; oy <= y; oy++)
-                        for (int ox = x - 2; ox <= x; ox++) {
-                            if (ox < 0 || oy < 0) continue;
-                            size_t oi = (size_t)smp * np + (size_t)oy * W + ox;
-                            float w;
-                            if (!footprintWeight(s, pf[oi], x, y, &w)) continue;
-                            const float* l = &L[3 * oi];
-                            fp.rgb += Spec(l[0], l[1], l[2]) * w;
-                            fp.w += w;
-                        }
-            }
+        gatherFilm(W, H, spp, pf.data(), L, s.filterTable, film.data());
         for (int j = 0; j < H; j++)
             for (int i = 0; i < W; i++) {
                 const FilmPix& fp = film[(size_t)(H - 1 - j) * W + i];
