@@ -965,8 +965,22 @@ LTS_D CamRays camera_sample(const SceneView& sv, int px, int py, int sample, int
     return camera_rays(sv.cam, pFilm, lens, spp);
 }
 
-// SurfaceInteraction::computeDifferentials, dpdx/dpdy part (Interaction.cuh:118-133)
-LTS_D void differentials(SurfacePoint& s, const CamRays& cr) {
+// invDet * determinant of a hit: Triangle::intersect scales dpdu and dpdv by the ray's invDet where
+// the uv parametrisation's 1 / determinant belongs (Triangle.cu:147-149), so the interaction's dpdu,
+// dpdv are k times the true derivatives (k = 1 when the uv determinant is 0 and dpdu, dpdv come
+// from coordinateSystem). The shading frame only uses their directions; uv differentials need k.
+LTS_D float uv_derivative_scale(const SceneView& sv, int leaf, float invDet) {
+    const ShadeTri& st = sv.shade[leaf];
+    V2 uv0 = v2(st.uv[0], st.uv[1]), uv1 = v2(st.uv[2], st.uv[3]), uv2 = v2(st.uv[4], st.uv[5]);
+    V2 duv02 = uv0 - uv2, duv12 = uv1 - uv2;
+    float determinant = duv02.x * duv12.y - duv02.y * duv12.x;
+    return determinant == 0 ? 1.0f : invDet * determinant;
+}
+
+// SurfaceInteraction::computeDifferentials (Interaction.cuh:118-160). The uv differentials, read
+// only by the filtered texture pipeline, are those of the true uv parametrisation: solved against
+// the interaction's dpdu, dpdv (k times too long, see uv_derivative_scale) and multiplied by k.
+LTS_D void differentials(SurfacePoint& s, const CamRays& cr, float k) {
     float d = -dot(s.n, s.p);
     float tx = (-dot(s.n, cr.rxO) - d) / dot(s.n, cr.rxD);
     V3 px = cr.rxO + cr.rxD * tx;
@@ -991,6 +1005,7 @@ LTS_D void differentials(SurfacePoint& s, const CamRays& cr) {
     };
     solve(c(px, d0) - c(s.p, d0), c(px, d1) - c(s.p, d1), s.dudx, s.dvdx);
     solve(c(py, d0) - c(s.p, d0), c(py, d1) - c(s.p, d1), s.dudy, s.dvdy);
+    s.dudx *= k; s.dvdx *= k; s.dudy *= k; s.dvdy *= k;
 }
 
 }  // namespace lts

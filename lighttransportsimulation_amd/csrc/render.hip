@@ -43,7 +43,7 @@ LTS_D Spec radiance(const SceneView& sv, int px, int py, int sample, const Rende
         }
         if (h.leaf < 0) break;
         SurfacePoint si = make_surface(sv, h.leaf, r, h.u, h.v, h.invDet);
-        if (bounce == 0) differentials(si, cr);
+        if (bounce == 0) differentials(si, cr, uv_derivative_scale(sv, h.leaf, h.invDet));
         Bsdf bsdf = make_bsdf(sv, si, perm);
         float u0[4], u1[4];
         smp.block(1 + 2 * bounce, u0);

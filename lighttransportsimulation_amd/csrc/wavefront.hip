@@ -325,7 +325,7 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                 alive = false;
             } else {
                 SurfacePoint si = make_surface(sv, hh.leaf, r, hh.u, hh.v, hh.invDet);
-                if (kFirst) differentials(si, cam);
+                if (kFirst) differentials(si, cam, uv_derivative_scale(sv, hh.leaf, hh.invDet));
                 Bsdf bsdf = make_bsdf(sv, si, perm);
                 const int slot = (sample - sl.s0) * sl.width * sl.height + pixel;  // sample index of the path
                 if (cached && (bsdf.kind == 0 || bsdf.kind == 1 || bsdf.kind == 4)) {

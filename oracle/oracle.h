@@ -68,6 +68,20 @@ int oracle_render(oracle_scene* s, const oracle_render_params* p, float* rgb_out
 int32_t oracle_intersect(oracle_scene* s, const float o[3], const float d[3], float tmax, float* t_out);
 int32_t oracle_occluded(oracle_scene* s, const float o[3], const float d[3], float tmax);
 
+// Camera ray at film position (px, py) with a centred lens sample, differentials scaled for spp
+// samples: out = o, d, rxO, rxD, ryO, ryD (18 floats).
+void oracle_camera_ray(oracle_scene* s, float px, float py, int32_t spp, float* out);
+// Primary hit of that ray and its differentials: out = p, n, uv, dpdx, dpdy, du/dx, dv/dx, du/dy,
+// dv/dy (18 floats); returns the triangle or -1.
+int32_t oracle_primary_hit(oracle_scene* s, float px, float py, int32_t spp, float* out);
+// InfiniteLight::sampleLi from point p with sample u: out = wi, pdf. -1 without an infinite light.
+int32_t oracle_infinite_sample(oracle_scene* s, const float p[3], const float u[2], float* out);
+float oracle_infinite_pdf(const float wi[3]);
+// Light l: out = kind, power entry, centre (3), radius, area, triangle (8 floats).
+int32_t oracle_light_info(oracle_scene* s, int32_t l, float* out);
+// Light pick by Distribution1D::sampleDiscrete over the light powers.
+int32_t oracle_sample_light(oracle_scene* s, float u, float* pdf);
+
 // Gaussian filter table (Filter.cuh:32-50, 82-99): 16x16 floats.
 void oracle_filter_table(float* out256);
 // The deterministic film gather (Camera::addSample, Camera.cu:192-218, as a gather): pfilm holds

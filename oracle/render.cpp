@@ -36,6 +36,7 @@ struct SI {
     V3 sn, sdpdu;
     V3 dpdx, dpdy;
     float dudx = 0, dvdx = 0, dudy = 0, dvdy = 0;
+    float uvScale = 1;  // invDet * uv determinant: dpdu, dpdv are uvScale times the true derivatives
     int tri = -1;
 };
 
@@ -112,6 +113,9 @@ static SI triInteraction(const Scene& s, int g, const Ray& r, float u, float v, 
     float ys = std::fabs(w * p0.y) + std::fabs(u * p1.y) + std::fabs(v * p2.y);
     float zs = std::fabs(w * p0.z) + std::fabs(u * p1.z) + std::fabs(v * p2.z);
     SI si;
+    // Triangle.cu:147-149 scales by the ray's invDet instead of 1 / determinant (Q11); the uv
+    // differentials undo it (computeDifferentials)
+    si.uvScale = determinant == 0 ? 1.0f : invDet * determinant;
     si.pError = V3(xs, ys, zs) * gammaN(7);
     si.p = p0 * w + p1 * u + p2 * v;
     si.uv = uvs[0] * w + uvs[1] * u + uvs[2] * v;
@@ -580,7 +584,8 @@ static void computeDifferentials(SI& si, const Ray& r) {
     si.dpdy = py - si.p;
     // du/dx, dv/dx, du/dy, dv/dy on the two axes other than the normal's dominant one
     // (Interaction.cuh:134-153, solveLinearSystem2x2 GeneralHelper.cuh:127-134); only the filtered
-    // texture pipeline reads them
+    // texture pipeline reads them, as the derivatives of the true uv parametrisation: solved against
+    // the interaction's dpdu, dpdv (uvScale times too long) and multiplied by uvScale
     int d0, d1;
     if (std::fabs(si.n.x) > std::fabs(si.n.y) && std::fabs(si.n.x) > std::fabs(si.n.z)) { d0 = 1; d1 = 2; }
     else if (std::fabs(si.n.y) > std::fabs(si.n.z)) { d0 = 0; d1 = 2; }
@@ -596,6 +601,7 @@ static void computeDifferentials(SI& si, const Ray& r) {
     };
     solve(c(px, d0) - c(si.p, d0), c(px, d1) - c(si.p, d1), si.dudx, si.dvdx);
     solve(c(py, d0) - c(si.p, d0), c(py, d1) - c(si.p, d1), si.dudy, si.dvdy);
+    si.dudx *= si.uvScale; si.dvdx *= si.uvScale; si.dudy *= si.uvScale; si.dvdy *= si.uvScale;
 }
 
 // Material::computeScatteringFunctions (Material.cuh:31-130)
@@ -775,6 +781,27 @@ static Spec estimateDirect(const Scene& s, const SI& si, const BSDF& bsdf, V2 uS
     return Ld;
 }
 
+// Camera::generateRay / generateDifferentialRay (Camera.cu:51-81) at a film position and lens
+// sample, the differentials scaled by scaleDifferentials(1 / sqrt(spp)) (PathTracingIntegrator.cuh:46).
+static Ray cameraRay(const Scene& s, V2 pFilm, V2 lens, int spp) {
+    auto gen = [&](V2 pf, V3* o, V3* d) {
+        V3 offset = (s.cu * lens.x + s.cv * lens.y) * s.lensRadius;
+        V2 pr(pf.x / (s.W - 1), pf.y / (s.H - 1));
+        *o = s.o + offset;
+        *d = normalize(s.llc + s.right * pr.x + s.up * pr.y - s.o - offset);
+    };
+    Ray r;
+    gen(pFilm, &r.o, &r.d);
+    gen(pFilm + V2(1, 0), &r.rxO, &r.rxD);
+    gen(pFilm + V2(0, 1), &r.ryO, &r.ryD);
+    r.hasDiff = true;
+    float sc = quake((float)spp);
+    r.rxO = r.o + (r.rxO - r.o) * sc;
+    r.ryO = r.o + (r.ryO - r.o) * sc;
+    r.rxD = r.d + (r.rxD - r.d) * sc;
+    r.ryD = r.d + (r.ryD - r.d) * sc;
+    return r;
+}
 // PathTracingIntegrator::evaluate, one camera sample (PathTracingIntegrator.cuh:38-104)
 static Spec Li(const Scene& s, int px, int py, int sample, const oracle_render_params& prm, V2* pFilmOut, Counters& c) {
     Sampler smp;
@@ -793,22 +820,7 @@ static Spec Li(const Scene& s, int px, int py, int sample, const oracle_render_p
     V2 pFilm((float)(px + film.x), (float)(py + film.y));
     *pFilmOut = pFilm;
     // Camera::generateRay / generateDifferentialRay (Camera.cu:51-81)
-    auto gen = [&](V2 pf, V3* o, V3* d) {
-        V3 offset = (s.cu * lens.x + s.cv * lens.y) * s.lensRadius;
-        V2 pr(pf.x / (s.W - 1), pf.y / (s.H - 1));
-        *o = s.o + offset;
-        *d = normalize(s.llc + s.right * pr.x + s.up * pr.y - s.o - offset);
-    };
-    Ray r;
-    gen(pFilm, &r.o, &r.d);
-    gen(pFilm + V2(1, 0), &r.rxO, &r.rxD);
-    gen(pFilm + V2(0, 1), &r.ryO, &r.ryD);
-    r.hasDiff = true;
-    float sc = quake((float)prm.spp);
-    r.rxO = r.o + (r.rxO - r.o) * sc;
-    r.ryO = r.o + (r.ryO - r.o) * sc;
-    r.rxD = r.d + (r.rxD - r.d) * sc;
-    r.ryD = r.d + (r.ryD - r.d) * sc;
+    Ray r = cameraRay(s, pFilm, lens, prm.spp);
 
     Spec L(0.0f), beta(1.0f);
     bool specularBounce = false;
@@ -949,6 +961,52 @@ extern "C" void oracle_mip_lookup(const float* rgb, int32_t w, int32_t h, int32_
     const Spec v = mipLookup(m, V2(s, t), width);
     out[0] = v.r; out[1] = v.g; out[2] = v.b;
 }
+
+// The camera ray at film position (px, py) with a centred lens sample: out = o, d, rxO, rxD, ryO,
+// ryD (18 floats).
+extern "C" void oracle_camera_ray(oracle_scene* sc, float px, float py, int32_t spp, float* out) {
+    Ray r = cameraRay(sc->s, V2(px, py), V2(0, 0), spp);
+    const V3 v[6] = {r.o, r.d, r.rxO, r.rxD, r.ryO, r.ryD};
+    for (int i = 0; i < 6; i++) { out[3 * i] = v[i].x; out[3 * i + 1] = v[i].y; out[3 * i + 2] = v[i].z; }
+}
+// The primary hit of that ray with its differentials: out = p, n, uv, dpdx, dpdy, du/dx, dv/dx,
+// du/dy, dv/dy (18 floats). Returns the hit triangle or -1.
+extern "C" int32_t oracle_primary_hit(oracle_scene* sc, float px, float py, int32_t spp, float* out) {
+    Ray r = cameraRay(sc->s, V2(px, py), V2(0, 0), spp);
+    Counters c;
+    SI si;
+    if (!traverse(sc->s, r, false, &si, c)) return -1;
+    computeDifferentials(si, r);
+    const float v[18] = {si.p.x, si.p.y, si.p.z, si.n.x, si.n.y, si.n.z, si.uv.x, si.uv.y, si.dpdx.x, si.dpdx.y,
+                         si.dpdx.z, si.dpdy.x, si.dpdy.y, si.dpdy.z, si.dudx, si.dvdx, si.dudy, si.dvdy};
+    std::memcpy(out, v, sizeof v);
+    return si.tri;
+}
+// InfiniteLight::sampleLi from a point p (Light.cu:68-90): out = wi, pdf; returns -1 without an
+// infinite light.
+extern "C" int32_t oracle_infinite_sample(oracle_scene* sc, const float p[3], const float u[2], float* out) {
+    const Scene& s = sc->s;
+    if (s.infiniteLight < 0) return -1;
+    SI it; it.p = V3(p[0], p[1], p[2]);
+    V3 wi; float pdf; LightSample tgt;
+    sampleLi(s, s.lights[s.infiniteLight], it, V2(u[0], u[1]), &wi, &pdf, &tgt);
+    out[0] = wi.x; out[1] = wi.y; out[2] = wi.z; out[3] = pdf;
+    return 0;
+}
+// InfiniteLight::PdfLi (Light.cuh:154-158)
+extern "C" float oracle_infinite_pdf(const float wi[3]) { return infPdf(V3(wi[0], wi[1], wi[2])); }
+// Light l: out = kind, power distribution entry, centre (3), radius (infinite light), area (area
+// light), triangle.
+extern "C" int32_t oracle_light_info(oracle_scene* sc, int32_t l, float* out) {
+    const Scene& s = sc->s;
+    if (l < 0 || l >= (int)s.lights.size()) return -1;
+    const Light& L = s.lights[l];
+    const float v[8] = {(float)L.kind, s.lfunc[l], L.center.x, L.center.y, L.center.z, L.radius, L.area, (float)L.tri};
+    std::memcpy(out, v, sizeof v);
+    return 0;
+}
+// Distribution1D::sampleDiscrete over the light powers (Sampling.cuh:155-188).
+extern "C" int32_t oracle_sample_light(oracle_scene* sc, float u, float* pdf) { return sampleLight(sc->s, u, pdf); }
 
 extern "C" void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
     U4 c; for (int i = 0; i < 4; i++) c.v[i] = ctr[i];

@@ -66,12 +66,29 @@ def lib():
         L.oracle_nrc_grad.restype = C.c_double
         L.oracle_nrc_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                       C.c_void_p]
+        L.oracle_camera_ray.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int32, C.c_void_p]
+        L.oracle_primary_hit.restype = C.c_int32
+        L.oracle_primary_hit.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int32, C.c_void_p]
+        L.oracle_infinite_sample.restype = C.c_int32
+        L.oracle_infinite_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_infinite_pdf.restype = C.c_float
+        L.oracle_infinite_pdf.argtypes = [C.c_void_p]
+        L.oracle_light_info.restype = C.c_int32
+        L.oracle_light_info.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.oracle_sample_light.restype = C.c_int32
+        L.oracle_sample_light.argtypes = [C.c_void_p, C.c_float, C.POINTER(C.c_float)]
+        L.oracle_film_gather.argtypes = [C.c_int32] * 3 + [C.c_void_p] * 3
         _lib = L
     return _lib
 
 
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def infinite_pdf(wi) -> float:
+    w = np.asarray(wi, np.float32)
+    return lib().oracle_infinite_pdf(_p(w))
 
 
 def set_texture_filter(filtered: bool):
@@ -151,6 +168,36 @@ class OracleScene:
     def occluded(self, o, d, tmax):
         oo, dd = np.asarray(o, np.float32), np.asarray(d, np.float32)
         return bool(lib().oracle_occluded(self.h, _p(oo), _p(dd), np.float32(tmax)))
+
+    def camera_ray(self, px, py, spp):
+        out = np.zeros(18, np.float32)
+        lib().oracle_camera_ray(self.h, np.float32(px), np.float32(py), spp, _p(out))
+        return out.reshape(6, 3)
+
+    def primary_hit(self, px, py, spp):
+        out = np.zeros(18, np.float32)
+        tri = lib().oracle_primary_hit(self.h, np.float32(px), np.float32(py), spp, _p(out))
+        return tri, dict(p=out[0:3], n=out[3:6], uv=out[6:8], dpdx=out[8:11], dpdy=out[11:14], dudx=out[14],
+                         dvdx=out[15], dudy=out[16], dvdy=out[17])
+
+    def infinite_sample(self, p, u):
+        out = np.zeros(4, np.float32)
+        pp, uu = np.asarray(p, np.float32), np.asarray(u, np.float32)
+        if lib().oracle_infinite_sample(self.h, _p(pp), _p(uu), _p(out)) != 0:
+            raise RuntimeError("scene has no infinite light")
+        return out[:3], float(out[3])
+
+    def light_info(self, l):
+        out = np.zeros(8, np.float32)
+        if lib().oracle_light_info(self.h, l, _p(out)) != 0:
+            raise IndexError(l)
+        return dict(kind=int(out[0]), power=float(out[1]), center=out[2:5].copy(), radius=float(out[5]),
+                    area=float(out[6]), tri=int(out[7]))
+
+    def sample_light(self, u):
+        pdf = C.c_float()
+        k = lib().oracle_sample_light(self.h, np.float32(u), C.byref(pdf))
+        return k, pdf.value
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -90,3 +90,58 @@ def test_truncated_files_are_errors(tmp_path):
     ppm.write_bytes(b"P6\n99999999999999999999 2\n255\n" + bytes(12))
     with pytest.raises(LtsError):
         load_image(str(ppm))
+
+
+def _decode_png_py(data: bytes) -> np.ndarray:
+    """A plain PNG decoder (8-bit, non-interlaced) written from the PNG specification: chunks,
+    zlib, the five row filters; returns [h][w][channels] bytes, top row first."""
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, hdr = 8, b"", None
+    while pos < len(data):
+        n = struct.unpack(">I", data[pos:pos + 4])[0]
+        kind, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        if kind == b"IHDR":
+            hdr = struct.unpack(">IIBBBBB", body)
+        elif kind == b"IDAT":
+            idat += body
+        elif kind == b"IEND":
+            break
+        pos += 12 + n
+    w, h, depth, color, _, _, interlace = hdr
+    assert depth == 8 and interlace == 0
+    ch = {0: 1, 2: 3, 4: 2, 6: 4}[color]
+    raw = zlib.decompress(idat)
+    stride = w * ch
+    out = np.zeros((h, stride), np.int64)
+    prev = np.zeros(stride, np.int64)
+    for y in range(h):
+        ft = raw[y * (stride + 1)]
+        line = np.frombuffer(raw, np.uint8, stride, y * (stride + 1) + 1).astype(np.int64)
+        cur = np.zeros(stride, np.int64)
+        for x in range(stride):
+            a = cur[x - ch] if x >= ch else 0
+            b = prev[x]
+            c = prev[x - ch] if x >= ch else 0
+            if ft == 0:
+                p = 0
+            elif ft == 1:
+                p = a
+            elif ft == 2:
+                p = b
+            elif ft == 3:
+                p = (a + b) // 2
+            else:
+                pa, pb, pc = abs(b - c), abs(a - c), abs(a + b - 2 * c)
+                p = a if (pa <= pb and pa <= pc) else (b if pb <= pc else c)
+            cur[x] = (line[x] + p) & 255
+        out[y] = cur
+        prev = cur
+    return out.reshape(h, w, ch).astype(np.uint8)
+
+
+def test_reference_png_fixture_against_a_python_decoder():
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "res", "kernel_box_scene", "kernel.png")
+    px = _decode_png_py(open(path, "rb").read())
+    assert px.shape == (64, 64, 3)
+    assert np.array_equal(load_image(path), _linear(px))
