@@ -61,8 +61,8 @@ typedef struct lts_render_params {
     uint32_t seed;        /* sampler seed (replaces time(0))                       */
     int32_t stratified;   /* 1: film sample per Sampler::stratifiedSample          */
     int32_t integrator;   /* LTS_INTEGRATOR_*                                      */
-    int32_t spp_per_pass; /* samples per pass; 0 = automatic (wavefront: 2^27 paths */
-                          /* per pass, cached: 2^25; megakernel: 2^28 / 12 B)       */
+    int32_t spp_per_pass; /* samples per pass; 0 = automatic (wavefront and cached: */
+                          /* 2^27 paths per pass; megakernel: 2^28 / 12 B)          */
     /* wavefront only (0 / -1 = default or the autotuned value) */
     int32_t trace_block;        /* block size of the persistent traversal kernels: 64, 128, 256 */
     int32_t persistent_blocks;  /* grid of the persistent traversal kernels                     */
@@ -76,8 +76,10 @@ typedef struct lts_render_params {
     int32_t batch_group_pixels; /* lts_render_batch: scenes are grouped, largest first, until a   */
                                 /*    group holds this many pixels; a group shares every launch   */
                                 /*    of its passes. 0: 2^20; -1: all scenes in one group         */
-    int32_t count_traversal;    /* megakernel only, 1: count the render-node fetches and triangle */
-                                /*    tests of the path rays into stats.node_visits / tri_tests    */
+    int32_t count_traversal;    /* 1: count render-node fetches and triangle tests (a separate,    */
+                                /*    untimed instantiation of the traversal): stats.node_visits / */
+                                /*    tri_tests for the path rays; the wavefront integrators also   */
+                                /*    count shadow and MIS rays (stats.ray_nodes / ray_tris)       */
     int32_t sample_begin;       /* progressive rendering: samples [sample_begin, sample_end) of  */
     int32_t sample_end;         /*    the spp-sample image (sample_end 0: spp). sample_begin > 0  */
                                 /*    keeps the film: the previous render of this context (or    */
@@ -131,6 +133,8 @@ typedef struct lts_stats {
     int64_t phase_launches[8]; /* kernel launches per phase                          */
     int32_t batch_groups;   /* lts_render_batch: groups the scenes were rendered in    */
     int32_t pad;
+    int64_t ray_nodes[3];   /* count_traversal: 4-wide render-node fetches of the path, shadow and  */
+    int64_t ray_tris[3];    /*    MIS rays (megakernel: path rays only), and their triangle tests  */
 } lts_stats;
 
 /* Context. */

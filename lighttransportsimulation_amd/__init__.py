@@ -76,8 +76,11 @@ class Stats(C.Structure):
                 ("persistent_blocks", C.c_int32), ("sort_materials", C.c_int32), ("use_graph", C.c_int32),
                 ("autotune_ms", C.c_double), ("nrc_loss", C.c_double), ("nrc_queries", C.c_int64),
                 ("nrc_train_samples", C.c_int32), ("nrc_steps", C.c_int32), ("phase_ms", C.c_double * 8),
-                ("phase_launches", C.c_int64 * 8), ("batch_groups", C.c_int32), ("pad", C.c_int32)]
+                ("phase_launches", C.c_int64 * 8), ("batch_groups", C.c_int32), ("pad", C.c_int32),
+                ("ray_nodes", C.c_int64 * 3), ("ray_tris", C.c_int64 * 3)]
 
+
+RAY_KINDS = ("path", "shadow", "mis")
 
 _lib = None
 
@@ -318,7 +321,9 @@ def _stats_dict(st: Stats) -> dict:
                 autotune_ms=st.autotune_ms, nrc_loss=st.nrc_loss, nrc_queries=st.nrc_queries,
                 nrc_train_samples=st.nrc_train_samples, nrc_steps=st.nrc_steps,
                 phase_ms={k: st.phase_ms[i] for i, k in enumerate(PHASES)},
-                phase_launches={k: st.phase_launches[i] for i, k in enumerate(PHASES)}, batch_groups=st.batch_groups)
+                phase_launches={k: st.phase_launches[i] for i, k in enumerate(PHASES)}, batch_groups=st.batch_groups,
+                ray_nodes={k: st.ray_nodes[i] for i, k in enumerate(RAY_KINDS)},
+                ray_tris={k: st.ray_tris[i] for i, k in enumerate(RAY_KINDS)})
 
 
 def render_batch(contexts, spp: int, max_bounce: int, rr_start: int, seed: int, stratified: int = 0,

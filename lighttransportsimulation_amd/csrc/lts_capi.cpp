@@ -217,8 +217,7 @@ int check_render_params(lts_ctx* c, const lts_render_params* p) {
     } else if (p->integrator != LTS_INTEGRATOR_MEGAKERNEL) {
         return fail(c, LTS_E_ARG, "unsupported integrator");
     }
-    if (p->count_traversal != 0 && (p->count_traversal != 1 || p->integrator != LTS_INTEGRATOR_MEGAKERNEL))
-        return fail(c, LTS_E_ARG, "count_traversal is a megakernel option (0 or 1)");
+    if (p->count_traversal != 0 && p->count_traversal != 1) return fail(c, LTS_E_ARG, "count_traversal is 0 or 1");
     const int end = p->sample_end ? p->sample_end : p->spp;
     if (p->sample_begin < 0 || end <= p->sample_begin || end > p->spp)
         return fail(c, LTS_E_ARG, "sample range outside [0, spp)");
@@ -283,10 +282,11 @@ bool same_tuning(const WfTuning& a, const WfTuning& b) {
 
 // Runs the pass whose parameters are resident in c->wf: replays the captured graph, capturing it
 // first when the buffers, bounce count or tuning changed since the last capture.
-int wf_run(lts_ctx* c, const WfTuning& t, int maxBounce, const NrcBuffers* nrc, const WfProfile* prof = nullptr) {
+int wf_run(lts_ctx* c, const WfTuning& t, int maxBounce, const NrcBuffers* nrc, const WfProfile* prof = nullptr,
+           unsigned long long* traceCounts = nullptr) {
     hipStream_t s = c->stream;
-    if (!t.useGraph) {
-        HIPCHK(c, wf_enqueue_pass(c->wf, t, maxBounce, nrc, s, prof));
+    if (!t.useGraph || traceCounts) {
+        HIPCHK(c, wf_enqueue_pass(c->wf, t, maxBounce, nrc, s, prof, traceCounts));
         return LTS_OK;
     }
     const uint64_t nrcGen = nrc ? nrc->gen : 0, cacheGen = nrc ? c->wf.cacheGen : 0;
@@ -464,11 +464,12 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
     // automatic pass: 2^27 paths (about 36 GB of path state at 268 B per path). Each pass runs its
     // bounces as separate launches whose late ones hold few paths, so fewer, larger passes keep the
     // GPU busy: 2^24 -> 2^26 paths took the 1080p 64-spp frame from 585 to 756 Msamples/s, and
-    // 2^26 -> 2^27 (the whole frame in one pass) from 1392 to 1524. The cached integrator takes
-    // 2^25, as its warm-up and training steps are counted per pass (at 1080p: 4 passes of 16 spp for
-    // 64 spp, two of them querying the cache; measured against 8 passes of 8 spp and 2 of 32, it
-    // reaches a given error soonest)
-    const size_t passPaths = cached ? (size_t)1 << 25 : (size_t)1 << 27;
+    // 2^26 -> 2^27 (the whole frame in one pass) from 1392 to 1524. The cached integrator takes the
+    // same passes: a 64-spp 1080p frame is one pass that queries the cache and trains it once, and
+    // the cache is carried from frame to frame (its warm-up passes are the first frames after
+    // lts_nrc_configure). Passes of 2^25 paths, which let one fresh frame warm up and then query,
+    // cost the cached frame 112 ms against 81 ms in one pass.
+    const size_t passPaths = (size_t)1 << 27;
     int pass = p->spp_per_pass > 0 ? p->spp_per_pass : (int)std::max<size_t>(1, passPaths / totalPix);
     if (sample_rgb) pass = b1 - b0;
     pass = std::min(pass, b1 - b0);
@@ -522,6 +523,12 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
     if (p->sort_materials >= 0) t.sortMaterials = p->sort_materials != 0;
     if (p->use_graph >= 0) t.useGraph = p->use_graph != 0;
     if (p->profile_phases) t.useGraph = 0;  // phase events are recorded between direct launches
+    // counting runs launch the counting instantiation of the traversal kernel directly
+    unsigned long long* traceCounts = p->count_traversal ? c->counters : nullptr;
+    if (traceCounts) {
+        t.useGraph = 0;
+        HIPCHK(c, hipMemsetAsync(traceCounts, 0, kTraceCounters * sizeof(unsigned long long), s));
+    }
 
     double renderMs = 0, resolveMs = 0;
     int64_t launches = 0;
@@ -550,7 +557,7 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         int64_t paths = 0;
         if ((rc = wf_upload_pass(c, scenes, p, s0, s1, &paths, pc)) != LTS_OK) return rc;
         HIPCHK(c, hipEventRecord(c->ev[5], s));
-        if ((rc = wf_run(c, t, p->max_bounce, nrc, prof)) != LTS_OK) return rc;
+        if ((rc = wf_run(c, t, p->max_bounce, nrc, prof, traceCounts)) != LTS_OK) return rc;
         HIPCHK(c, hipEventRecord(c->ev[6], s));
         for (size_t i = 0; i < scenes.size(); i++) {
             RenderLaunch rl{};
@@ -616,6 +623,16 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
             st->phase_launches[LTS_PHASE_FINALIZE] = perBounce;
             st->phase_launches[LTS_PHASE_CACHE_INFER] = cached ? launches : 0;
             st->phase_launches[LTS_PHASE_CACHE_TRAIN] = cached ? launches * c->nrc.steps : 0;
+        }
+        if (traceCounts) {
+            unsigned long long cnt[kTraceCounters];
+            HIPCHK(c, hipMemcpy(cnt, traceCounts, sizeof cnt, hipMemcpyDeviceToHost));
+            for (int k = 0; k < 3; k++) {
+                st->ray_nodes[k] = (int64_t)cnt[2 * k];
+                st->ray_tris[k] = (int64_t)cnt[2 * k + 1];
+            }
+            st->node_visits = st->ray_nodes[0];
+            st->tri_tests = st->ray_tris[0];
         }
         if (cached) {
             NrcState ns;
@@ -734,7 +751,7 @@ int lts_create(lts_ctx** out, int dev) {
     for (int i = 0; i < 512; i++) table[i] = kPerm256[i & 255];
     if (hipMalloc((void**)&c->perm, sizeof(table)) != hipSuccess ||
         hipMemcpy(c->perm, table, sizeof(table), hipMemcpyHostToDevice) != hipSuccess) { lts_destroy(c); return LTS_E_HIP; }
-    if (hipMalloc((void**)&c->counters, 2 * sizeof(unsigned long long)) != hipSuccess) { delete c; return LTS_E_HIP; }
+    if (hipMalloc((void**)&c->counters, kTraceCounters * sizeof(unsigned long long)) != hipSuccess) { lts_destroy(c); return LTS_E_HIP; }
     *out = c;
     return LTS_OK;
 }
@@ -1003,6 +1020,8 @@ static int render_impl(lts_ctx* c, const lts_render_params* p, float* rgb_out, f
             HIPCHK(c, hipMemcpy(cnt, c->counters, sizeof cnt, hipMemcpyDeviceToHost));
             st->node_visits = (int64_t)cnt[0];
             st->tri_tests = (int64_t)cnt[1];
+            st->ray_nodes[0] = st->node_visits;
+            st->ray_tris[0] = st->tri_tests;
         }
     }
     return LTS_OK;
@@ -1054,6 +1073,12 @@ static int render_batch_impl(lts_ctx* const* ctxs, int32_t n, const lts_render_p
         total.launches += gs.launches;
         total.samples += gs.samples;
         total.autotune_ms += gs.autotune_ms;
+        total.node_visits += gs.node_visits;
+        total.tri_tests += gs.tri_tests;
+        for (int q = 0; q < 3; q++) {
+            total.ray_nodes[q] += gs.ray_nodes[q];
+            total.ray_tris[q] += gs.ray_tris[q];
+        }
         for (int q = 0; q < 8; q++) {
             total.phase_ms[q] += gs.phase_ms[q];
             total.phase_launches[q] += gs.phase_launches[q];

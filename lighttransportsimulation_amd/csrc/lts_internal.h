@@ -200,6 +200,9 @@ constexpr int kWfGlobalCounts = 8;  // [0] cache queries
 // ev[8 kWfMaxBounce + 0..2]: before inference, after inference, after the training steps.
 constexpr int kWfPhaseEvents = 8 * kWfMaxBounce + 4;
 struct WfProfile { hipEvent_t ev[kWfPhaseEvents]; };
+// Traversal counters of a context: render-node fetches and triangle tests of path, shadow and MIS
+// rays (the megakernel uses the first two, for its path rays).
+constexpr int kTraceCounters = 6;
 
 hipError_t wf_reserve(WfBuffers& b, size_t cap, int maxScenes);
 hipError_t wf_reserve_cache(WfBuffers& b, size_t cap, int trainCap);
@@ -207,8 +210,10 @@ void wf_release(WfBuffers& b);
 // Enqueues one pass: generate, then per bounce extend / (sort) / shade / shadow / mis / finalize.
 // With nrc (cached integrator): queries, training records, then cache inference into the sample
 // buffer, the training batch of path-suffix radiance and one Adam step.
+// traceCounts (6 counters, or null): launches the counting instantiation of the traversal kernel,
+// which adds render-node fetches and triangle tests of path, shadow and MIS rays.
 hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce, const NrcBuffers* nrc, hipStream_t stream,
-                           const WfProfile* prof = nullptr);
+                           const WfProfile* prof = nullptr, unsigned long long* traceCounts = nullptr);
 
 // ---------------------------------------------------------------- learned radiance cache (nrc.hip)
 // Network: 16-float raw record -> 64-dim encoding -> 5 hidden layers of 64 (ReLU) -> 3 outputs.

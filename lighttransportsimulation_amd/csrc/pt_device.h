@@ -44,8 +44,9 @@ LTS_D void load_tri(const TriLeaf* __restrict__ tris, int k, V3& p0, V3& p1, V3&
 constexpr float kOneMinusEpsF = 0x1.fff2e4p-1f;  // largest float <= 1.0 - (double)SHADOW_EPS
 constexpr float kOnePlusEpsF = 0x1.00068cp+0f;   // largest float <= 1.0 + (double)SHADOW_EPS
 
-// Triangle::intersect hit test (shapes/Triangle.cu:94-128)
-LTS_D bool tri_hit(V3 p0, V3 p1, V3 p2, const Ray& r, float& t, float& u, float& v, float& invDetOut) {
+// Triangle::intersect hit test (shapes/Triangle.cu:94-128) without its distance test: true when
+// the ray meets the triangle inside the barycentric margins, at distance t of either sign.
+LTS_D bool tri_mt(V3 p0, V3 p1, V3 p2, const Ray& r, float& t, float& u, float& v, float& invDetOut) {
     V3 edge1 = p1 - p0, edge2 = p2 - p0;
     V3 pVec = cross(r.d, edge2);
     float det = dot(edge1, pVec);
@@ -59,7 +60,22 @@ LTS_D bool tri_hit(V3 p0, V3 p1, V3 p2, const Ray& r, float& t, float& u, float&
     if (v < SHADOW_EPS || u + v > kOneMinusEpsF) return false;
     t = invDet * dot(edge2, qVec);
     invDetOut = invDet;
-    return t > 0 && t < r.tMax;
+    return true;
+}
+
+// Triangle::intersect hit test (shapes/Triangle.cu:94-128): t in (0, tMax).
+LTS_D bool tri_hit(V3 p0, V3 p1, V3 p2, const Ray& r, float& t, float& u, float& v, float& invDetOut) {
+    return tri_mt(p0, p1, p2, r, t, u, v, invDetOut) && t > 0 && t < r.tMax;
+}
+
+// The closest-hit acceptance of leaf `leaf` against the current closest hit (distance r.tMax, leaf
+// best): nearer, or exactly as near with a lower leaf index. BVH::intersect (BVH.cuh:365-398) keeps
+// the first triangle it meets at the closest distance, and its depth-first, left-first walk meets
+// the leaves in index order, so of several triangles at exactly that distance the lowest leaf wins;
+// this rule gives the same winner in any visiting order.
+LTS_D bool tri_hit_closest(V3 p0, V3 p1, V3 p2, const Ray& r, int leaf, int best, float& t, float& u, float& v,
+                           float& invDetOut) {
+    return tri_mt(p0, p1, p2, r, t, u, v, invDetOut) && t > 0 && (t < r.tMax || (t == r.tMax && leaf < best));
 }
 
 // Triangle::simpleIntersect (shapes/Triangle.cu:50-84)
@@ -156,7 +172,7 @@ LTS_D void node4_hits(const BvhQNode* nodes4, int cur, const Ray& r, V3 inv, int
         const float tzMax = (qdecode(oz, ez, (farZ >> s) & 0xffu) - r.o.z) * inv.z;
         const float tMin = fmaxf(fmaxf(txMin, tyMin), tzMin);
         const float tMax = fminf(fminf(txMax, tyMax), tzMax) * g4;
-        const bool hit = k < count && tMin <= tMax && tMin < r.tMax && tMax > 0;
+        const bool hit = k < count && tMin <= tMax && tMin <= r.tMax && tMax > 0;
         t[k] = hit ? tMin : INFINITY;
         c[k] = cc[k];
     }
@@ -178,7 +194,14 @@ LTS_D void cswap(float& ta, int& ca, float& tb, int& cb) {
 // which never prunes a candidate a binary traversal would keep (a ray that passes a box passes its
 // ancestors' boxes). Every triangle whose box the ray passes before the current closest distance
 // is tested, so the reported hit is the reference traversal's, except for exactly equal distances.
-LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned long long* nodeCount, unsigned long long* triCount) {
+// Render-node fetches and triangle tests of one lane's traversals (counting launches only; the
+// timed kernels are instantiated without it).
+struct TraceTally {
+    unsigned long long nodes = 0, tris = 0;
+};
+
+LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned long long* nodeCount, unsigned long long* triCount,
+                        TraceTally* tally = nullptr) {
     Hit h; h.leaf = -1; h.t = h.u = h.v = h.invDet = 0;
     const int n = sv.nPrim;
     const V3 inv = ray_inverse(r.d);
@@ -189,7 +212,7 @@ LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned lon
         load_tri(sv.tris, 0, p0, p1, p2, prim);
         float t, u, v, id;
         tests++;
-        if (tri_hit(p0, p1, p2, r, t, u, v, id)) { r.tMax = t; h.leaf = 0; h.t = t; h.u = u; h.v = v; h.invDet = id; }
+        if (tri_hit_closest(p0, p1, p2, r, 0, h.leaf, t, u, v, id)) { r.tMax = t; h.leaf = 0; h.t = t; h.u = u; h.v = v; h.invDet = id; }
     } else {
         int spill[kStack - kShortStack];
         int sp = 0;
@@ -224,7 +247,9 @@ LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned lon
                 load_tri(sv.tris, k, p0, p1, p2, prim);
                 float t, u, v, id;
                 tests++;
-                if (tri_hit(p0, p1, p2, r, t, u, v, id)) { r.tMax = t; h.leaf = k; h.t = t; h.u = u; h.v = v; h.invDet = id; }
+                if (tri_hit_closest(p0, p1, p2, r, k, h.leaf, t, u, v, id)) {
+                    r.tMax = t; h.leaf = k; h.t = t; h.u = u; h.v = v; h.invDet = id;
+                }
             }
             if (sp == 0) break;
             --sp;
@@ -232,17 +257,18 @@ LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned lon
         }
     }
     if (nodeCount) { atomicAdd(nodeCount, nodes); atomicAdd(triCount, tests); }
+    if (tally) { tally->nodes += nodes; tally->tris += tests; }
     return h;
 }
 
 // The triangle test of an any-hit traversal: Triangle::simpleIntersect for shadow rays, or the
-// closest-hit test (Triangle::intersect) when the question is whether any triangle lies before r.tMax
-// by the closest-hit traversal's own predicate.
+// closest-hit test (Triangle::intersect) when the question is whether leaf `leaf` would win the
+// closest hit against leaf `tieLeaf` at distance r.tMax (nearer, or as near with a lower index).
 template <bool kHitTest>
-LTS_D bool leaf_blocks(V3 p0, V3 p1, V3 p2, const Ray& r) {
+LTS_D bool leaf_blocks(V3 p0, V3 p1, V3 p2, const Ray& r, int leaf, int tieLeaf) {
     if constexpr (kHitTest) {
         float t, u, v, id;
-        return tri_hit(p0, p1, p2, r, t, u, v, id);
+        return tri_hit_closest(p0, p1, p2, r, leaf, tieLeaf, t, u, v, id);
     } else {
         return tri_occludes(p0, p1, p2, r);
     }
@@ -250,15 +276,17 @@ LTS_D bool leaf_blocks(V3 p0, V3 p1, V3 p2, const Ray& r) {
 
 // Any hit (BVH::simpleIntersect, acceleration/BVH.cuh:331-363) over the 4-wide tree: leaf children
 // are tested as soon as their box is hit, interior children are stacked.
+// kHitTest: the closest-hit question of an MIS ray, whether any leaf beats tieLeaf at r.tMax.
 template <bool kHitTest = false>
-LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts) {
+LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts, TraceTally* tally = nullptr, int tieLeaf = -1) {
     const int n = sv.nPrim;
     const V3 inv = ray_inverse(r.d);
     int nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
     if (n == 1) {
         V3 p0, p1, p2; int prim;
         load_tri(sv.tris, 0, p0, p1, p2, prim);
-        return leaf_blocks<kHitTest>(p0, p1, p2, r);
+        if (tally) tally->tris++;
+        return leaf_blocks<kHitTest>(p0, p1, p2, r, 0, tieLeaf);
     }
     int spill[kStack - kShortStack];
     int sp = 0;
@@ -268,6 +296,7 @@ LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts) {
         float t[4];
         int c[4], count;
         node4_hits(sv.nodes4, cur, r, inv, nx, ny, nz, t, c, count);
+        if (tally) tally->nodes++;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (t[k] < INFINITY) {
@@ -276,7 +305,8 @@ LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts) {
                     if (leaf >= n) return false;
                     V3 p0, p1, p2; int prim;
                     load_tri(sv.tris, leaf, p0, p1, p2, prim);
-                    if (leaf_blocks<kHitTest>(p0, p1, p2, r)) return true;
+                    if (tally) tally->tris++;
+                    if (leaf_blocks<kHitTest>(p0, p1, p2, r, leaf, tieLeaf)) return true;
                 } else if (sp < kStack) {
                     if (sp < kShortStack) ts.lds[sp * ts.stride] = c[k];
                     else spill[sp - kShortStack] = c[k];
@@ -850,22 +880,24 @@ LTS_D int sample_light(const SceneView& sv, float u, float* pdf) {
 // (spawned with tMax = infinity) is light li's triangle, seen from its emitting side, or for the
 // infinite light whether r escapes. The closest-hit traversal is not needed for that. A ray that
 // misses the light's triangle, or meets its back, contributes nothing whatever lies in front. A ray
-// that meets its front at t reaches it unless some triangle passes the closest-hit test before t,
-// which an any-hit traversal bounded at t answers; the same t, u, v and 1/det come out of the
-// triangle test whatever tMax is. The only difference from the closest-hit form is a second
-// triangle at exactly the same distance, which that form may visit first.
-LTS_D bool mis_reaches_light(const SceneView& sv, const LightDesc& L, int li, Ray r, TraceStack ts) {
+// that meets its front at t reaches it unless some triangle would win the closest hit against it
+// (nearer, or at exactly t with a lower leaf index, tri_hit_closest), which an any-hit traversal
+// bounded at t answers; the same t, u, v and 1/det come out of the triangle test whatever tMax is.
+LTS_D bool mis_reaches_light(const SceneView& sv, const LightDesc& L, int li, Ray r, TraceStack ts,
+                             TraceTally* tally = nullptr) {
     if (L.kind == 0) {
         if (L.leaf < 0 || sv.shade[L.leaf].light != li) return false;
         V3 p0, p1, p2; int prim;
         load_tri(sv.tris, L.leaf, p0, p1, p2, prim);
+        if (tally) tally->tris++;
         float t, u, v, id;
         if (!tri_hit(p0, p1, p2, r, t, u, v, id)) return false;
         const SurfacePoint lt = make_surface(sv, L.leaf, r, u, v, id);
         if (!(dot(lt.n, -r.d) > 0.f)) return false;
         r.tMax = t;
+        return !trace_any<true>(sv, r, ts, tally, L.leaf);
     }
-    return !trace_any<true>(sv, r, ts);
+    return !trace_any<true>(sv, r, ts, tally, -1);
 }
 
 // estimateDirect (intergrators/IntegratorHelper.cuh:51-125)

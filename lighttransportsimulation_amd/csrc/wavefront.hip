@@ -113,14 +113,14 @@ LTS_D int wave_fetch_segmented(int32_t* work, int segLen, int& j) {
 // Closest hit of path ray `pi`; with `hist` set, the hit is also filed under its material bin
 // (misses in the last bin) for the counting sort: bins[path] and the block histogram lh.
 LTS_D void extend_item(const SceneView* scenes, const WfRay* rays, WfHit* hits, int pi,
-                       TraceStack ts, int32_t* bins, int32_t* lh, bool filing, int nBins) {
+                       TraceStack ts, int32_t* bins, int32_t* lh, bool filing, int nBins, TraceTally* tally = nullptr) {
     const WfRay rr = rays[pi];
     Ray r;
     r.o = v3(rr.o[0], rr.o[1], rr.o[2]);
     r.d = v3(rr.d[0], rr.d[1], rr.d[2]);
     r.tMax = INFINITY;
     const SceneView& sv = scenes[rr.scene];
-    Hit h = trace_closest(sv, r, ts, nullptr, nullptr);
+    Hit h = trace_closest(sv, r, ts, nullptr, nullptr, tally);
     WfHit o;
     o.leaf = h.leaf; o.u = h.u; o.v = h.v; o.invDet = h.invDet;
     hits[pi] = o;
@@ -564,13 +564,13 @@ __global__ __launch_bounds__(kCollectBlock) void k_wf_nrc_scatter(const WfParams
 
 // Any hit of the shadow ray of path k: the light sample's contribution when the segment is free.
 LTS_D void shadow_item(const SceneView* scenes, const WfShadow* shadows, WfNee* nee, int k,
-                       TraceStack ts) {
+                       TraceStack ts, TraceTally* tally = nullptr) {
     const WfShadow s = shadows[k];
     Ray r;
     r.o = v3(s.o[0], s.o[1], s.o[2]);
     r.d = v3(s.d[0], s.d[1], s.d[2]);
     r.tMax = 1 - SHADOW_EPS;
-    if (!trace_any(scenes[s.scene], r, ts)) {
+    if (!trace_any(scenes[s.scene], r, ts, tally)) {
         nee[s.path].p1[0] = s.c[0];
         nee[s.path].p1[1] = s.c[1];
         nee[s.path].p1[2] = s.c[2];
@@ -578,7 +578,8 @@ LTS_D void shadow_item(const SceneView* scenes, const WfShadow* shadows, WfNee* 
 }
 
 // The MIS-BSDF ray of path k: does it reach the light it was cast for (IntegratorHelper.cuh:103-122).
-LTS_D void mis_item(const SceneView* scenes, const WfMis* misRays, WfNee* nee, int k, TraceStack ts) {
+LTS_D void mis_item(const SceneView* scenes, const WfMis* misRays, WfNee* nee, int k, TraceStack ts,
+                    TraceTally* tally = nullptr) {
     const WfMis m = misRays[k];
     const SceneView& sv = scenes[m.scene];
     const LightDesc& Lt = sv.lights[m.light];
@@ -586,7 +587,7 @@ LTS_D void mis_item(const SceneView* scenes, const WfMis* misRays, WfNee* nee, i
     r.o = v3(m.o[0], m.o[1], m.o[2]);
     r.d = v3(m.d[0], m.d[1], m.d[2]);
     r.tMax = INFINITY;
-    const Spec Li2 = mis_reaches_light(sv, Lt, m.light, r, ts) ? light_le(Lt) : spec(0.f);
+    const Spec Li2 = mis_reaches_light(sv, Lt, m.light, r, ts, tally) ? light_le(Lt) : spec(0.f);
     if (!isBlack(Li2)) {
         Spec c = spec(m.f[0], m.f[1], m.f[2]) * Li2 * spec(1.0f) * m.weight / m.pdf;
         nee[m.path].p2[0] = c.r;
@@ -601,14 +602,17 @@ LTS_D void mis_item(const SceneView* scenes, const WfMis* misRays, WfNee* nee, i
 // short ones and leaves one tail per bounce instead of three. Counts may be null (empty set).
 // The launch asks for 7 waves per SIMD (at most 72 VGPRs): enough waves to cover the load latency
 // of the traversal, which then runs at the VALU issue limit; at 8 the 64-register cap spills.
-template <int BS>
+// kCount: the counting instantiation, which also adds every lane's render-node fetches and
+// triangle tests per ray kind (path, shadow, MIS) to traceCounts[0..5] (nodes, tests per kind).
+template <int BS, bool kCount>
 __global__ __launch_bounds__(BS, 7) void k_wf_trace(const SceneView* __restrict__ scenes, const WfRay* __restrict__ rays, WfHit* __restrict__ hits,
                                                  const int32_t* __restrict__ ext, const int32_t* __restrict__ extCount,
                                                  int32_t* __restrict__ bins, int32_t* __restrict__ hist, int nBins,
                                                  const WfShadow* __restrict__ shadows, const int32_t* __restrict__ shadowCount,
                                                  const WfMis* __restrict__ misRays, const int32_t* __restrict__ misCount,
                                                  const int32_t* __restrict__ shadowQueue, const int32_t* __restrict__ misQueue,
-                                                 WfNee* __restrict__ nee, int32_t* __restrict__ work) {
+                                                 WfNee* __restrict__ nee, int32_t* __restrict__ work,
+                                                 unsigned long long* __restrict__ traceCounts) {
     __shared__ int stk[kShortStack * BS];
     __shared__ int32_t lh[kWfMaxMatBins];
     const TraceStack ts = lds_stack(stk, threadIdx.x, BS);
@@ -623,13 +627,20 @@ __global__ __launch_bounds__(BS, 7) void k_wf_trace(const SceneView* __restrict_
     const int n = nE + nS + nM;
     const int segLen = ((n + kWfWorkSegments - 1) / kWfWorkSegments + 63) / 64 * 64;
     int j = 0;
+    TraceTally tally[3];
     for (;;) {
         const int k = wave_fetch_segmented(work, segLen, j);
         if (j >= kWfWorkSegments) break;
         if (k >= n) continue;
-        if (k < nE) extend_item(scenes, rays, hits, ext[k], ts, bins, lh, filing, nBins);
-        else if (k < nE + nS) shadow_item(scenes, shadows, nee, shadowQueue[k - nE], ts);
-        else mis_item(scenes, misRays, nee, misQueue[k - nE - nS], ts);
+        if (k < nE) extend_item(scenes, rays, hits, ext[k], ts, bins, lh, filing, nBins, kCount ? &tally[0] : nullptr);
+        else if (k < nE + nS) shadow_item(scenes, shadows, nee, shadowQueue[k - nE], ts, kCount ? &tally[1] : nullptr);
+        else mis_item(scenes, misRays, nee, misQueue[k - nE - nS], ts, kCount ? &tally[2] : nullptr);
+    }
+    if constexpr (kCount) {
+        for (int q = 0; q < 3; q++) {
+            atomicAdd(&traceCounts[2 * q], tally[q].nodes);
+            atomicAdd(&traceCounts[2 * q + 1], tally[q].tris);
+        }
     }
     if (filing) {
         __syncthreads();
@@ -679,7 +690,7 @@ __global__ void k_wf_fill(int32_t* p, int n, int32_t v) {
 }  // namespace
 
 hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce, const NrcBuffers* nrc, hipStream_t s,
-                           const WfProfile* prof) {
+                           const WfProfile* prof, unsigned long long* traceCounts) {
     auto mark = [&](int i) {
         if (prof) (void)hipEventRecord(prof->ev[i], s);
     };
@@ -703,19 +714,15 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
         const int32_t* ext = b.queue[bounce & 1];
         int32_t* work = b.counts + kWfCountsPerBounce * bounce + kWfWorkBase;
         int32_t* hist = histOf(bounce);
+        auto launch = [&](auto kernel, int bs) {
+            kernel<<<t.persistentBlocks, bs, 0, s>>>(b.scenes, b.rays, b.hits, ext, extCount, b.bins, hist, b.matBins, b.shadows,
+                                                     shadowCount, b.mis, misCount, b.shadowQ, b.misQ, b.nee, work, traceCounts);
+        };
+        const bool counting = traceCounts != nullptr;
         switch (t.traceBlock) {
-            case 64:
-                k_wf_trace<64><<<t.persistentBlocks, 64, 0, s>>>(b.scenes, b.rays, b.hits, ext, extCount, b.bins, hist,
-                                                                b.matBins, b.shadows, shadowCount, b.mis, misCount, b.shadowQ, b.misQ, b.nee, work);
-                break;
-            case 128:
-                k_wf_trace<128><<<t.persistentBlocks, 128, 0, s>>>(b.scenes, b.rays, b.hits, ext, extCount, b.bins, hist,
-                                                                  b.matBins, b.shadows, shadowCount, b.mis, misCount, b.shadowQ, b.misQ, b.nee, work);
-                break;
-            default:
-                k_wf_trace<256><<<t.persistentBlocks, 256, 0, s>>>(b.scenes, b.rays, b.hits, ext, extCount, b.bins, hist,
-                                                                  b.matBins, b.shadows, shadowCount, b.mis, misCount, b.shadowQ, b.misQ, b.nee, work);
-                break;
+            case 64: launch(counting ? k_wf_trace<64, true> : k_wf_trace<64, false>, 64); break;
+            case 128: launch(counting ? k_wf_trace<128, true> : k_wf_trace<128, false>, 128); break;
+            default: launch(counting ? k_wf_trace<256, true> : k_wf_trace<256, false>, 256); break;
         }
     };
     mark(8 * kWfMaxBounce + 3);

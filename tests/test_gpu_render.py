@@ -15,14 +15,26 @@ SEED = 0x5EED
 
 
 def _compare(g_smp, o_smp, g_rgb, o_rgb, label):
+    """Per-sample radiance identical except for at most 1 sample in 10^4 (exact-distance ties); the
+    filtered image equal to 1e-5 relative at every pixel whose film footprint holds no differing
+    sample (a sample of owner (x, y) reaches pixels x-1..x+2, y-1..y+2; images are top row first,
+    samples film row first)."""
     same = np.all(g_smp.view(np.uint32) == o_smp.view(np.uint32), axis=-1)
     frac = same.mean()
+    H, W = g_rgb.shape[:2]
+    touched = np.zeros((H + 3, W + 3), bool)
+    bad = np.argwhere(~same)
+    for s_, y, x in bad:
+        touched[y:y + 4, x:x + 4] = True  # rows y-1..y+2 shifted by one
+    mask = touched[1:H + 1, 1:W + 1][::-1]
     rel = np.abs(g_rgb - o_rgb) / np.maximum(np.abs(o_rgb), 1e-3)
-    print(f"[{label}] identical samples {frac:.6f}  max image rel err {rel.max():.3e}", flush=True)
-    # at most 1 sample in 10^4 may differ (exact-distance ties); the filtered image agrees to 1e-5
-    # relative on 99.9% of pixel channels
+    clean = rel[~mask]
+    print(f"[{label}] identical samples {frac:.6f}  max image rel err {rel.max():.3e}, "
+          f"{clean.max() if clean.size else 0:.3e} away from differing samples", flush=True)
+    for s_, y, x in bad[:5]:
+        print(f"    differs: sample {s_} pixel ({x}, {y}): gpu {g_smp[s_, y, x]} oracle {o_smp[s_, y, x]}", flush=True)
     assert frac >= 1 - 1e-4, f"{label}: only {frac:.6f} of samples identical"
-    assert np.quantile(rel, 0.999) <= 1e-5, label
+    assert clean.size == 0 or clean.max() <= 1e-5, label
 
 
 @pytest.mark.gpu

@@ -69,8 +69,28 @@ def test_ploc_tree_fetches_fewer_nodes(gpu_ctx, scenes):
     print(f"  node fetches per sample: karras {fetches[TREE_KARRAS]:.2f}, ploc {fetches[TREE_PLOC]:.2f}", flush=True)
     assert fetches[TREE_PLOC] < 0.9 * fetches[TREE_KARRAS]
     gpu_ctx.set_render_tree(TREE_PLOC)
+    gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], w, h)
     with pytest.raises(LtsError):
-        gpu_ctx.render(4, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT, count_traversal=1)
+        gpu_ctx.render(4, 8, 3, SEED, count_traversal=2)
+
+
+@pytest.mark.gpu
+def test_wavefront_counts_every_ray_kind(gpu_ctx, scenes):
+    """The wavefront's counting traversal (a separate instantiation of the trace kernel) counts the
+    same path-ray work as the megakernel (the same rays on the same tree), plus its shadow and MIS
+    rays, and leaves the image unchanged."""
+    d, cfg, w, h = scenes[1]
+    gpu_ctx.set_render_tree(TREE_PLOC)
+    gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], w, h)
+    _, _, mk = gpu_ctx.render(4, 8, 3, SEED, count_traversal=1)
+    img, _, st = gpu_ctx.render(4, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT, count_traversal=1)
+    ref, _, _ = gpu_ctx.render(4, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT)
+    assert _identical(img, ref)
+    assert st["ray_nodes"]["path"] == mk["node_visits"] and st["ray_tris"]["path"] == mk["tri_tests"]
+    assert st["node_visits"] == mk["node_visits"]
+    assert st["ray_nodes"]["shadow"] > 0 and st["ray_nodes"]["mis"] > 0 and st["ray_tris"]["mis"] > 0
+    print("  per sample: " + ", ".join(f"{k} {st['ray_nodes'][k] / st['samples']:.2f} nodes "
+                                       f"{st['ray_tris'][k] / st['samples']:.2f} tris" for k in st["ray_nodes"]), flush=True)
 
 
 @pytest.mark.gpu

@@ -481,3 +481,26 @@ def test_lambert_plane_under_triangle_emitter(tmp_path):
     expect = albedo / math.pi * Le * _polygon_irradiance(x0, np.array([0, 1.0, 0]), tri_v)
     se = est.std() / math.sqrt(est.size)
     assert abs(est.mean() - expect) < 4 * se + 1e-4 * expect, (est.mean(), expect, se)
+
+
+def test_exact_distance_ties_go_to_the_lowest_leaf(tmp_path):
+    """BVH::intersect keeps the first triangle it meets at the closest distance (BVH.cuh:365-398, with
+    Triangle::intersect's strict t < tMax) and walks depth first, left child first, so of triangles at
+    exactly the same distance the one at the lowest leaf (Morton order, ties by triangle number)
+    wins. Coincident triangles (the same mesh loaded twice, as in the reference's folding-box scene)
+    meet every ray at the same t; the first mesh's copy must win wherever it sits in the tree."""
+    d = str(tmp_path)
+    A = np.array([[-1.0, -1.0, 3.0], [1.0, -1.0, 3.0], [-1.0, 1.0, 3.0]])
+    C = np.array([[2.0, 2.0, 5.0], [3.0, 2.0, 5.0], [2.0, 3.0, 5.0]])  # elsewhere, shifts the Morton grid
+    n = np.tile([0, 0, -1.0], (3, 1))
+    f = np.array([[0, 1, 2]])
+    for extra_first in (False, True):
+        meshes = [("a0", (A, f, n)), ("a1", (A, f, n))]
+        if extra_first:
+            meshes = [("c", (C, f, n))] + meshes
+        sc = _analytic_scene(d, meshes, ["L c 0.5 0.5 0.5 0"] * len(meshes), [], (0, 0, 0, 0, 0, 1, 0, 1, 0, 30, 1))
+        s = O.OracleScene(d, sc, "", 4, 4)
+        first = 1 if extra_first else 0  # global number of the first copy
+        for x, y in ((-0.2, -0.3), (-0.4, 0.1), (0.3, -0.5), (-0.7, -0.6)):
+            prim, t = s.intersect([0, 0, 0], _unit([x, y, 3.0]))
+            assert prim == first, (extra_first, x, y, prim)
