@@ -198,6 +198,24 @@ int lts_bvh_build_device(lts_ctx* ctx, const float* d_tri_xyz, int32_t n, int32_
 /* Copies the arrays of the most recent build to host memory. */
 int lts_bvh_export_last(lts_ctx* ctx, lts_bvh_export* out);
 
+/* No counterpart in the reference: the LBVH build for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Builds the
+ * tree lts_bvh_build_device builds (mode 0 or 1) for a device soup of n triangles, 1 <= n <=
+ * LTS_BVH_STATIC_MAX_PRIMS, without allocating anything: the temporaries live in one static device
+ * buffer of the library (LTS_BVH_STATIC_SCRATCH_BYTES per device, held until the process ends) and
+ * the results go to the caller's device arrays: d_codes_sorted (n), d_perm (n), d_left (n-1),
+ * d_right (n-1), d_parent (2n-1) and d_bounds (6(2n-1) floats {min xyz, max xyz} in the unified
+ * index space: interior i at 6i, leaf k at 6(n-1+k); leaf k holds primitive k in mode 0 and
+ * d_perm[k] in mode 1). scene_bounds (host, may be NULL) receives the global bounds of the Morton
+ * quantisation. Runs on the current device and `hip_stream` (a hipStream_t; NULL for the default
+ * stream) and returns when the build is done. Every call on one device shares its buffer: calls
+ * that overlap on one device race. LTS_E_ARG beyond LTS_BVH_STATIC_MAX_PRIMS. */
+#define LTS_BVH_STATIC_MAX_PRIMS (1 << 18)
+#define LTS_BVH_STATIC_SCRATCH_BYTES (16u << 20)
+int lts_bvh_build_static(const float* d_tri_xyz, int32_t n, int32_t mode, uint32_t* d_codes_sorted, int32_t* d_perm,
+                         int32_t* d_left, int32_t* d_right, int32_t* d_parent, float* d_bounds, float* scene_bounds,
+                         void* hip_stream);
+
 /* Replaces PathTracingKernel + Camera::getRender (intergrators/IntegratorKernels.cu:17-23,
  * rendering/Camera.cu:256-289). rgb_out (may be NULL) receives W*H*3 floats in getRender row
  * order; sample_rgb (may be NULL) receives every sample's radiance, [spp][H][W][3]. */

@@ -25,7 +25,7 @@ EXPORTED_SYMBOLS = (
     "lts_write_image", "lts_device_copy_benchmark", "lts_render_batch", "lts_nrc_configure", "lts_nrc_set_weights",
     "lts_nrc_get_weights", "lts_nrc_infer", "lts_nrc_train_step", "lts_set_render_tree", "lts_render_tree_info",
     "lts_obj_to_bin", "lts_film_save", "lts_film_load", "lts_film_resolve", "lts_set_texture_filter",
-    "lts_load_image",
+    "lts_load_image", "lts_bvh_build_static",
 )
 
 TEXTURE_REFERENCE = 0
@@ -114,6 +114,7 @@ def load_library():
     L.lts_write_image.argtypes = [C.c_char_p, vp, i32, i32, i32]
     L.lts_obj_to_bin.argtypes = [C.c_char_p, C.POINTER(i32)]
     L.lts_load_image.argtypes = [C.c_char_p, C.POINTER(i32), C.POINTER(i32), vp, C.c_int64]
+    L.lts_bvh_build_static.argtypes = [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]
     L.lts_film_save.argtypes = [vp, C.c_char_p]
     L.lts_film_load.argtypes = [vp, C.c_char_p]
     L.lts_film_resolve.argtypes = [vp, vp]

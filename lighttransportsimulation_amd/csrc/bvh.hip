@@ -12,6 +12,8 @@
 // The scene-bound reduction uses a total order on floats (-0 below +0); it equals the reference's
 // float atomics (GeneralHelper.cuh:150-160) on every input without -0.0.
 #include <algorithm>
+#include <mutex>
+#include <type_traits>
 
 #include "lts_device.h"
 #include "lts_internal.h"
@@ -1132,6 +1134,91 @@ hipError_t bvh_enqueue(BvhBuffers& b, const float* soup, int n, int mode, bool p
     }
     if (ev) (void)hipEventRecord(ev->e[4], s);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- build without allocation
+// Every temporary array of a build of at most kStaticMaxPrims primitives, carved from one static
+// device buffer (each device holds its own copy from the moment the library's code object is loaded
+// until the process ends). Host code never passes `scratch` itself to a kernel: its device address
+// comes from hipGetSymbolAddress, looked up once per device.
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+
+std::mutex g_scratchLock;
+void* g_scratchPtr[kStaticMaxDevices] = {};
+
+size_t carve(size_t& at, size_t bytes) {
+    const size_t o = at;
+    at += (bytes + 255) & ~size_t(255);
+    return o;
+}
+
+// The scratch layout of an n-primitive build; `base` null measures it (the returned size).
+size_t static_layout(BvhBuffers& b, unsigned char* base, int n) {
+    const int tiles = (n + kTile - 1) / kTile, chunks2 = (n + kChunk2 - 1) / kChunk2;
+    size_t at = 0;
+    auto take = [&](auto*& ptr, size_t count) {
+        const size_t o = carve(at, count * sizeof(*ptr));
+        ptr = base ? reinterpret_cast<std::remove_reference_t<decltype(ptr)>>(base + o) : nullptr;
+    };
+    take(b.primBox, 6 * (size_t)n);
+    take(b.keysB, n);
+    take(b.valsB, n);
+    take(b.counts, 256 * (size_t)tiles);
+    take(b.digitBase, 256);
+    take(b.flags, n);
+    take(b.rangeLo, n);
+    take(b.rangeHi, n);
+    take(b.roots, n);
+    take(b.rootCount, chunks2 + 1);
+    take(b.bucket, (size_t)chunks2 * kBucketCap);
+    take(b.partial, 6 * (size_t)blocks_for(n));
+    take(b.sceneKeys, 8);
+    take(b.status, 1);
+    return at;
+}
+
+}  // namespace
+
+// The largest build (kStaticMaxPrims) must fit the buffer.
+static_assert(kStaticScratchBytes >= 256 * 20 + 6 * 4 * (size_t)kStaticMaxPrims + 8 * 4 * (size_t)kStaticMaxPrims +
+                                         256 * 4 * ((kStaticMaxPrims + kTile - 1) / kTile) + 4 * 256 +
+                                         4 * ((kStaticMaxPrims + kChunk2 - 1) / kChunk2 + 1) +
+                                         4 * (size_t)((kStaticMaxPrims + kChunk2 - 1) / kChunk2) * kBucketCap +
+                                         4 * 6 * ((kStaticMaxPrims + kBlock - 1) / kBlock) + 4 * 8 + 4,
+              "static LBVH scratch too small for kStaticMaxPrims");
+
+hipError_t bvh_static_scratch(int device, void** ptr) {
+    if (device < 0 || device >= kStaticMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratchLock);
+    if (!g_scratchPtr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratchPtr[device] = p;
+    }
+    *ptr = g_scratchPtr[device];
+    return hipSuccess;
+}
+
+hipError_t bvh_enqueue_static(void* scratchPtr, const float* soup, int n, int mode, uint32_t* codesSorted, int32_t* perm,
+                              int32_t* left, int32_t* right, int32_t* parent, float* bounds, hipStream_t s,
+                              int32_t** status, uint32_t** sceneKeys) {
+    if (n < 1 || n > kStaticMaxPrims) return hipErrorInvalidValue;
+    BvhBuffers b;
+    if (static_layout(b, nullptr, n) > kStaticScratchBytes) return hipErrorInvalidValue;
+    static_layout(b, static_cast<unsigned char*>(scratchPtr), n);
+    b.cap = n;
+    b.keysA = codesSorted;
+    b.valsA = perm;
+    b.left = left;
+    b.right = right;
+    b.parent = parent;
+    b.box = bounds;
+    *status = b.status;
+    *sceneKeys = b.sceneKeys;
+    return bvh_enqueue(b, soup, n, mode, false, s, nullptr);
 }
 
 }  // namespace lts

@@ -1334,6 +1334,39 @@ int lts_write_image(const char* path, const float* rgb, int32_t w, int32_t h, in
     return guarded(nullptr, [&] { return write_image_impl(path, rgb, w, h, format); });
 }
 
+int lts_bvh_build_static(const float* d_tri_xyz, int32_t n, int32_t mode, uint32_t* d_codes_sorted, int32_t* d_perm,
+                         int32_t* d_left, int32_t* d_right, int32_t* d_parent, float* d_bounds, float* scene_bounds,
+                         void* hip_stream) {
+    static_assert(LTS_BVH_STATIC_MAX_PRIMS == kStaticMaxPrims && LTS_BVH_STATIC_SCRATCH_BYTES == kStaticScratchBytes,
+                  "header limits of the allocation-free build");
+    if (!d_tri_xyz || n < 1 || n > kStaticMaxPrims || (mode != 0 && mode != 1) || !d_codes_sorted || !d_perm ||
+        !d_parent || !d_bounds || (n > 1 && (!d_left || !d_right)))
+        return LTS_E_ARG;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return LTS_E_HIP;
+    void* scratch = nullptr;
+    if (bvh_static_scratch(dev, &scratch) != hipSuccess) return LTS_E_HIP;
+    hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    int32_t* dStatus = nullptr;
+    uint32_t* dKeys = nullptr;
+    if (bvh_enqueue_static(scratch, d_tri_xyz, n, mode, d_codes_sorted, d_perm, d_left, d_right, d_parent, d_bounds, s,
+                           &dStatus, &dKeys) != hipSuccess)
+        return LTS_E_HIP;
+    int32_t status = 0;
+    uint32_t keys[6];
+    if (hipMemcpyAsync(&status, dStatus, sizeof(status), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        (scene_bounds && hipMemcpyAsync(keys, dKeys, sizeof(keys), hipMemcpyDeviceToHost, s) != hipSuccess) ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return LTS_E_HIP;
+    if (status != 0) return LTS_E_STATE;
+    if (scene_bounds)
+        for (int a = 0; a < 6; a++) {
+            const uint32_t u = (keys[a] & 0x80000000u) ? (keys[a] & 0x7fffffffu) : ~keys[a];
+            std::memcpy(&scene_bounds[a], &u, 4);
+        }
+    return LTS_OK;
+}
+
 int lts_load_image(const char* path, int32_t* w, int32_t* h, float* rgb_out, int64_t cap) {
     return guarded(nullptr, [&] { return load_image_impl(path, w, h, rgb_out, cap); });
 }

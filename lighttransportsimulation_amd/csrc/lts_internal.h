@@ -69,6 +69,21 @@ struct BvhStageEvents {
     hipEvent_t e[5];  // start, after bounds+morton, after sort, after tree, after pack
 };
 
+// Build without allocation (lts_bvh_build_static): all temporaries in one static device buffer of
+// kStaticScratchBytes per device, enough for kStaticMaxPrims primitives.
+constexpr int kStaticMaxPrims = 1 << 18;
+constexpr size_t kStaticScratchBytes = (size_t)16 << 20;
+constexpr int kStaticMaxDevices = 64;
+// The buffer's device address on `device` (the current device), looked up once per device.
+hipError_t bvh_static_scratch(int device, void** ptr);
+// Enqueues a build of n <= kStaticMaxPrims primitives whose results go to the caller's device arrays
+// (sorted codes, permutation, children, parents, unified node bounds) and whose temporaries live
+// in the static buffer; *status and *sceneKeys point at the build's status word and scene-bound
+// keys inside that buffer.
+hipError_t bvh_enqueue_static(void* scratch, const float* soup, int n, int mode, uint32_t* codesSorted, int32_t* perm,
+                              int32_t* left, int32_t* right, int32_t* parent, float* bounds, hipStream_t s,
+                              int32_t** status, uint32_t** sceneKeys);
+
 // Enqueues the whole build on `stream`. `soup` is a device pointer to n*9 floats.
 // mode 0: leaf k holds primitive k (reference); mode 1: leaf k holds perm[k].
 // When pack is set, also writes nodes / tris for the traversal kernels (mode 1 layout).
