@@ -582,7 +582,7 @@ def main() -> int:
             n_launch = phase_launches.get("extend", 0) / args.steps
             achieved = bytes_frame / (trav_ms * 1e-3) / 1e9
             traffic = profile_figure(TRAFFIC_PROFILE, "traversal_hbm_bytes_per_frame")
-            valu = profile_figure(ISSUE_PROFILE, "traversal_valu_issue_frac")
+            valu = profile_figure(ISSUE_PROFILE, "traversal_valu_issue_rate")
             out["roofline"] = {
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_wf_trace",
@@ -596,9 +596,9 @@ def main() -> int:
                 "algorithmic_bytes_per_step": round(bytes_frame),
                 "hbm_traffic_frac": (round(traffic / (trav_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
                                      if traffic and args.workload == "displaced_1080p" else None),
-                "valu_issue_frac": valu,
+                "valu_issue_rate": valu,
                 "sources": f"traffic: {TRAFFIC_PROFILE} (FETCH_SIZE + WRITE_SIZE per frame, copy-kernel calibrated); "
-                           f"valu_issue_frac: {ISSUE_PROFILE} (SQ_INSTS_VALU over 1024 SIMDs x clock / 4)"}
+                           f"valu_issue_rate: {ISSUE_PROFILE} (wave64 VALU instructions per SIMD per 4 cycles: SQ_INSTS_VALU over 1024 SIMDs x 2.4 GHz / 4)"}
         if not args.no_cpu:
             progress("cpu baseline starts")
             cb = cpu_baseline(d, cfg, args.cpu_budget)
