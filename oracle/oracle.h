@@ -106,6 +106,8 @@ float oracle_fbm(const float p[3], const float dpdx[3], const float dpdy[3], flo
 float oracle_turbulence(const float p[3], const float dpdx[3], const float dpdy[3], float omega, int32_t octaves);
 void oracle_marble(const float p[3], int32_t octaves, float omega, float scale, float variation, float rgb[3]);
 float oracle_fr_dielectric(float cos_theta_i, float eta_i, float eta_t);
+void oracle_transform(const float t[3], const float axis[3], float angle, float scale, const float p[3], const float n[3],
+                      float* out);
 // BxDF in its local frame: kind 0 Lambert, 1 Oren-Nayar (param = sigma in degrees),
 // 3 FresnelSpecular (param = eta), 4 microfacet reflection (param = roughness); R = 1.
 // mode 0: f(wo, wi)[0] -> out[0]; 1: pdf(wo, wi) -> out[0];

@@ -1121,6 +1121,14 @@ extern "C" void oracle_marble(const float p[3], int32_t oct, float omega, float 
     Spec s = marble(X, V3(p[0], p[1], p[2]), V3(0, 0, 0), V3(0, 0, 0));
     rgb[0] = s.r; rgb[1] = s.g; rgb[2] = s.b;
 }
+// The subject transform of a .scene file (SceneLoader.cuh:59-77, Transform.cuh:119-240): out = the
+// transformed point (3), then the transformed, renormalised normal (3).
+extern "C" void oracle_transform(const float t[3], const float axis[3], float angle, float scale, const float p[3],
+                                 const float n[3], float* out) {
+    Xform x = makeXform(V3(t[0], t[1], t[2]), V3(axis[0], axis[1], axis[2]), angle, scale);
+    V3 a = x.point(V3(p[0], p[1], p[2])), b = x.normal(V3(n[0], n[1], n[2]));
+    out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = b.x; out[4] = b.y; out[5] = b.z;
+}
 extern "C" float oracle_fr_dielectric(float c, float ei, float et) { return frDielectric(c, ei, et); }
 extern "C" float oracle_roughness_to_alpha(float r) { return roughnessToAlpha(r); }
 extern "C" void oracle_bxdf(int32_t kind, float param, int32_t mode, const float wo_[3], const float wi_[3], const float u_[2], float* out) {

@@ -159,3 +159,82 @@ def test_turbulence_matches_pbrt():
             got = L.oracle_turbulence(O._p(p), O._p(dx), O._p(dy), omega, oct_)
             ref = _turbulence_ref(p, dx.astype(np.float64), dy.astype(np.float64), omega, oct_, perm)
             assert abs(got - ref) < 2e-4 * max(1.0, abs(ref)), (p, dx, omega, oct_, got, ref)
+
+
+# ----------------------------------------------------------------------------- glass, marble, transform
+def test_fresnel_specular_sampling_closed_form():
+    """FresnelSpecular::sampleF (Specular.cuh:84-136) with the clamp quirk: F = ((eta - 1) / (eta + 1))^2
+    whatever the angle; u < F keeps -wo (the 'reflection' passes straight on) with pdf F and f = F R / |cos|;
+    otherwise Snell refraction about the normal facing wo, pdf 1 - F, f = T (1 - F) (eta_i / eta_t)^2 / |cos|."""
+    eta = 1.5
+    F = ((eta - 1) / (eta + 1)) ** 2
+    rng = np.random.Generator(np.random.PCG64(41))
+    for _ in range(100):
+        wo = _hemi(rng, 1)[0] * (1 if rng.random() < 0.5 else -1)
+        r = _bxdf(3, eta, 2, wo, u=(F * 0.5, 0.3))
+        assert np.allclose(r[:3], -wo, atol=0) and abs(r[3] - F) < 1e-7
+        assert abs(r[4] - F / abs(wo[2])) < 1e-5 * r[4]
+        r = _bxdf(3, eta, 2, wo, u=(0.5, 0.3))
+        entering = wo[2] > 0
+        ei, et = (1.0, eta) if entering else (eta, 1.0)
+        n = np.array([0, 0, 1.0 if entering else -1.0])
+        e = ei / et
+        ci = wo @ n
+        if 1 - e * e * (1 - ci * ci) <= 0:  # total internal reflection: no transmitted direction
+            assert r[4] == 0
+            continue
+        ct = math.sqrt(1 - e * e * (1 - ci * ci))
+        wt = -wo * e + n * (e * ci - ct)
+        assert np.allclose(r[:3], wt, atol=2e-6), (wo, r[:3], wt)
+        assert abs(r[3] - (1 - F)) < 1e-6
+        assert abs(r[4] - (1 - F) * e * e / abs(wt[2])) < 2e-5 * r[4]
+
+
+MARBLE_COLOURS = np.array([[.58, .58, .6], [.58, .58, .6], [.58, .58, .6], [.5, .5, .5], [.6, .59, .58],
+                           [.58, .58, .6], [.58, .58, .6], [.2, .2, .33], [.58, .58, .6]])
+
+
+def test_marble_is_a_cubic_bezier_over_its_colour_table():
+    """MarbleTexture (Procedural.cuh:267-311) without variation: t = 0.5 + 0.5 sin(scale p.y), segment
+    floor(6 t) of the 9-colour table, and inside it the de Casteljau evaluation, i.e. the Bernstein
+    form (1-s)^3 c0 + 3 (1-s)^2 s c1 + 3 (1-s) s^2 c2 + s^3 c3, times 1.5."""
+    L = O.lib()
+    L.oracle_marble.argtypes = [O.C.c_void_p, O.C.c_int32, O.C.c_float, O.C.c_float, O.C.c_float, O.C.c_void_p]
+    out = np.zeros(3, np.float32)
+    for y in np.linspace(-3.0, 3.0, 97):
+        scale = 1.7
+        p = np.array([0.3, y, -0.8], np.float32)
+        L.oracle_marble(O._p(p), 4, 0.5, scale, 0.0, O._p(out))
+        t = 0.5 + 0.5 * math.sin(float(np.float32(p[1] * np.float32(scale))))
+        first = min(int(math.floor(t * 6)), 6)
+        s = t * 6 - first
+        c = [MARBLE_COLOURS[first + k] if first + k < 9 else np.zeros(3) for k in range(4)]
+        ref = 1.5 * ((1 - s) ** 3 * c[0] + 3 * (1 - s) ** 2 * s * c[1] + 3 * (1 - s) * s * s * c[2] + s ** 3 * c[3])
+        assert np.allclose(out, ref, atol=2e-6), (y, out, ref)
+
+
+def test_subject_transform_is_a_rotation_about_the_axis():
+    """The .scene subject transform (translation, unit axis, angle in radians, scale; SceneLoader.cuh:59-77)
+    applied by quaternion conjugation (Transform.cuh:131-240): p -> R(axis, angle) p * scale + t by
+    Rodrigues' formula. Normals are conjugated the other way round, conj(q) n q (Transform.cuh:218-223):
+    they turn by the inverse rotation R^T (and are renormalised), without the scale or translation, a
+    quirk of the reference that the oracle keeps."""
+    fn = O.lib().oracle_transform
+    fn.argtypes = [O.C.c_void_p, O.C.c_void_p, O.C.c_float, O.C.c_float, O.C.c_void_p, O.C.c_void_p, O.C.c_void_p]
+    rng = np.random.Generator(np.random.PCG64(43))
+    out = np.zeros(6, np.float32)
+    for _ in range(50):
+        t = rng.uniform(-3, 3, 3).astype(np.float32)
+        ax = rng.normal(size=3)
+        ax = (ax / np.linalg.norm(ax)).astype(np.float32)
+        ang = np.float32(rng.uniform(-3, 3))
+        sc = np.float32(rng.uniform(0.5, 2))
+        p = rng.uniform(-2, 2, 3).astype(np.float32)
+        n = rng.normal(size=3).astype(np.float32)
+        fn(O._p(t), O._p(ax), ang, sc, O._p(p), O._p(n), O._p(out))
+        a = ax.astype(np.float64)
+        K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+        R = np.eye(3) + math.sin(ang) * K + (1 - math.cos(ang)) * K @ K
+        assert np.allclose(out[:3], R @ p * sc + t, atol=2e-5 * (1 + np.abs(p).max() * sc)), (out[:3], R @ p * sc + t)
+        rn = R.T @ n
+        assert np.allclose(out[3:], rn / np.linalg.norm(rn), atol=3e-6)
