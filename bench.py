@@ -262,7 +262,7 @@ def quality(ctx, cfg, tuning, spps=(16, 32, 64, 128), ref_spp=4096) -> dict:
     """Equal-time relMSE of the cached and the plain wavefront tracer against a plain reference."""
     from lighttransportsimulation_amd import INTEGRATOR_CACHED, INTEGRATOR_WAVEFRONT
     base = dict(max_bounce=cfg["max_bounce"], rr_start=cfg["rr_start"], stratified=cfg["stratified"])
-    tune = {k: tuning[k] for k in ("trace_block", "persistent_blocks", "sort_materials")} if tuning else {}
+    tune = {k: tuning[k] for k in ("trace_block", "persistent_blocks", "sort_materials", "tile_rows")} if tuning else {}
     t0 = time.perf_counter()
     ref, _, _ = ctx.render(ref_spp, seed=cfg["seed"] + 1000, integrator=INTEGRATOR_WAVEFRONT, **base, **tune)
     ref_s = time.perf_counter() - t0
@@ -431,6 +431,9 @@ def reference_scene_block() -> dict:
     return out
 
 
+TUNING_KEYS = ("trace_block", "persistent_blocks", "sort_materials", "use_graph", "tile_rows", "spp_per_pass")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -504,7 +507,7 @@ def main() -> int:
             for _ in range(max(args.warmup, 1 if wavefront and args.autotune else 0)):
                 _, _, st = ctx.render(**kw)
                 if wavefront:
-                    tuning = {k: st[k] for k in ("trace_block", "persistent_blocks", "sort_materials", "use_graph")}
+                    tuning = {k: st[k] for k in TUNING_KEYS}
                     tuning["autotune_ms"] = round(st["autotune_ms"], 1)
             t0 = time.perf_counter()
             dev_ms, kern_ms, launches = 0.0, 0.0, 0
@@ -521,12 +524,13 @@ def main() -> int:
             wall = time.perf_counter() - t0
             progress(f"{args.steps} timed frames: {wall / args.steps * 1e3:.1f} ms per frame")
             if wavefront and tuning is None:
-                tuning = {k: st[k] for k in ("trace_block", "persistent_blocks", "sort_materials", "use_graph")}
+                tuning = {k: st[k] for k in TUNING_KEYS}
             graph_rate = mega = tree = counts = None
             if wavefront:
                 # the same frames replayed as captured graphs, without the phase events
                 gk = dict(kw, autotune=0, profile_phases=0, trace_block=tuning["trace_block"],
-                          persistent_blocks=tuning["persistent_blocks"], sort_materials=tuning["sort_materials"])
+                          persistent_blocks=tuning["persistent_blocks"], sort_materials=tuning["sort_materials"],
+                          tile_rows=tuning["tile_rows"], spp_per_pass=tuning["spp_per_pass"])
                 ctx.render(**gk)
                 t1 = time.perf_counter()
                 for _ in range(args.steps):

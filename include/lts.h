@@ -68,8 +68,10 @@ typedef struct lts_render_params {
     int32_t persistent_blocks;  /* grid of the persistent traversal kernels                     */
     int32_t sort_materials;     /* -1 default (on), 0 off, 1 counting-sort shading by material  */
     int32_t use_graph;          /* -1 default (on), 0 off: capture each pass in a hipGraph       */
-    int32_t autotune;           /* 1: time candidate tunings on this scene before rendering;    */
-                                /*    the choice is kept per scene, film and pass size in the   */
+    int32_t autotune;           /* 1: time candidate tunings on this scene before rendering:    */
+                                /*    trace block, persistent grid, material sort, tile rows and */
+                                /*    (with spp_per_pass 0) the pass size, whole or half; the    */
+                                /*    choice is kept per scene, film, pass size and spp in the   */
                                 /*    context, and in the file named by $LTS_TUNE_CACHE if set  */
     int32_t profile_phases;     /* 1: time the wavefront phases with HIP events (stats.phase_ms); */
                                 /*    passes are then launched directly, not replayed as graphs  */
@@ -87,7 +89,10 @@ typedef struct lts_render_params {
                                 /*    scene, spp, seed, stratification, bounces, roulette and      */
                                 /*    integrator. The image then equals one render of             */
                                 /*    [0, sample_end) bit for bit. lts_render_batch: full range.  */
-    int32_t reserved[1];
+    int32_t tile_rows;          /* wavefront: path order within a sample, bands of this many rows  */
+                                /*    walked column by column (a wave of 64 paths covers a        */
+                                /*    tile_rows x 64/tile_rows tile); 0: 8 or the autotuned value; */
+                                /*    1 .. 64. The image does not depend on it.                   */
 } lts_render_params;
 
 /* Film checkpoint (no reference counterpart: the reference renders all samples in one launch).
@@ -135,6 +140,8 @@ typedef struct lts_stats {
     int32_t pad;
     int64_t ray_nodes[3];   /* count_traversal: 4-wide render-node fetches of the path, shadow and  */
     int64_t ray_tris[3];    /*    MIS rays (megakernel: path rays only), and their triangle tests  */
+    int32_t tile_rows;      /* wavefront: band height of the path order that was used            */
+    int32_t spp_per_pass;   /* wavefront: samples per pass that were used                         */
 } lts_stats;
 
 /* Context. */

@@ -60,7 +60,7 @@ class RenderParams(C.Structure):
                 ("trace_block", C.c_int32), ("persistent_blocks", C.c_int32), ("sort_materials", C.c_int32),
                 ("use_graph", C.c_int32), ("autotune", C.c_int32), ("profile_phases", C.c_int32),
                 ("batch_group_pixels", C.c_int32), ("count_traversal", C.c_int32), ("sample_begin", C.c_int32),
-                ("sample_end", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("sample_end", C.c_int32), ("tile_rows", C.c_int32)]
 
 
 class NrcConfig(C.Structure):
@@ -77,7 +77,8 @@ class Stats(C.Structure):
                 ("autotune_ms", C.c_double), ("nrc_loss", C.c_double), ("nrc_queries", C.c_int64),
                 ("nrc_train_samples", C.c_int32), ("nrc_steps", C.c_int32), ("phase_ms", C.c_double * 8),
                 ("phase_launches", C.c_int64 * 8), ("batch_groups", C.c_int32), ("pad", C.c_int32),
-                ("ray_nodes", C.c_int64 * 3), ("ray_tris", C.c_int64 * 3)]
+                ("ray_nodes", C.c_int64 * 3), ("ray_tris", C.c_int64 * 3), ("tile_rows", C.c_int32),
+                ("spp_per_pass", C.c_int32)]
 
 
 RAY_KINDS = ("path", "shadow", "mis")
@@ -291,12 +292,13 @@ class Context:
                integrator: int = INTEGRATOR_MEGAKERNEL, spp_per_pass: int = 0, want_image: bool = True,
                want_samples: bool = False, trace_block: int = 0, persistent_blocks: int = 0, sort_materials: int = -1,
                use_graph: int = -1, autotune: int = 0, profile_phases: int = 0, count_traversal: int = 0,
-               sample_begin: int = 0, sample_end: int = 0):
+               sample_begin: int = 0, sample_end: int = 0, tile_rows: int = 0):
         """Renders the loaded scene (include/lts.h: lts_render). Returns (rgb, samples, stats): rgb is
         [H][W][3] in getRender row order, samples [spp][H][W][3] per-sample radiance when asked for."""
         p = render_params(spp, max_bounce, rr_start, seed, stratified, integrator, spp_per_pass, trace_block,
                           persistent_blocks, sort_materials, use_graph, autotune, profile_phases,
-                          count_traversal=count_traversal, sample_begin=sample_begin, sample_end=sample_end)
+                          count_traversal=count_traversal, sample_begin=sample_begin, sample_end=sample_end,
+                          tile_rows=tile_rows)
         rgb = np.zeros((self.height, self.width, 3), np.float32) if want_image else None
         n_smp = (sample_end or spp) - sample_begin
         smp = np.zeros((n_smp, self.height, self.width, 3), np.float32) if want_samples else None
@@ -308,10 +310,10 @@ class Context:
 def render_params(spp, max_bounce, rr_start, seed, stratified=0, integrator=INTEGRATOR_MEGAKERNEL, spp_per_pass=0,
                   trace_block=0, persistent_blocks=0, sort_materials=-1, use_graph=-1, autotune=0,
                   profile_phases=0, batch_group_pixels=0, count_traversal=0, sample_begin=0,
-                  sample_end=0) -> RenderParams:
+                  sample_end=0, tile_rows=0) -> RenderParams:
     return RenderParams(spp, max_bounce, rr_start, seed, stratified, integrator, spp_per_pass, trace_block,
                         persistent_blocks, sort_materials, use_graph, autotune, profile_phases, batch_group_pixels,
-                        count_traversal, sample_begin, sample_end)
+                        count_traversal, sample_begin, sample_end, tile_rows)
 
 
 def _stats_dict(st: Stats) -> dict:
@@ -324,7 +326,8 @@ def _stats_dict(st: Stats) -> dict:
                 phase_ms={k: st.phase_ms[i] for i, k in enumerate(PHASES)},
                 phase_launches={k: st.phase_launches[i] for i, k in enumerate(PHASES)}, batch_groups=st.batch_groups,
                 ray_nodes={k: st.ray_nodes[i] for i, k in enumerate(RAY_KINDS)},
-                ray_tris={k: st.ray_tris[i] for i, k in enumerate(RAY_KINDS)})
+                ray_tris={k: st.ray_tris[i] for i, k in enumerate(RAY_KINDS)}, tile_rows=st.tile_rows,
+                spp_per_pass=st.spp_per_pass)
 
 
 def render_batch(contexts, spp: int, max_bounce: int, rr_start: int, seed: int, stratified: int = 0,

@@ -214,6 +214,7 @@ int check_render_params(lts_ctx* c, const lts_render_params* p) {
         if (p->trace_block != 0 && p->trace_block != 64 && p->trace_block != 128 && p->trace_block != 256)
             return fail(c, LTS_E_ARG, "trace_block must be 64, 128 or 256");
         if (p->persistent_blocks < 0 || p->persistent_blocks > (1 << 20)) return fail(c, LTS_E_ARG, "bad persistent_blocks");
+        if (p->tile_rows < 0 || p->tile_rows > kMaxTileRows) return fail(c, LTS_E_ARG, "tile_rows must be 0 .. 64");
     } else if (p->integrator != LTS_INTEGRATOR_MEGAKERNEL) {
         return fail(c, LTS_E_ARG, "unsupported integrator");
     }
@@ -326,7 +327,7 @@ struct PassCache {
 
 // Writes the pass parameters for samples [s0, s1) of every scene to the device.
 int wf_upload_pass(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_render_params* p, int s0, int s1,
-                   int64_t* paths, const PassCache& pc = PassCache()) {
+                   int64_t* paths, int tileRows, const PassCache& pc = PassCache()) {
     const int n = (int)scenes.size();
     c->wfSlots.resize(n);
     int64_t base = 0;
@@ -351,7 +352,7 @@ int wf_upload_pass(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_re
     c->wfParams.trainStride = pc.trainStride;
     c->wfParams.trainOffset = pc.trainOffset;
     c->wfParams.trainSlots = pc.trainSlots;
-    c->wfParams.pad = 0;
+    c->wfParams.tileRows = tileRows;
     HIPCHK(c, hipMemcpyAsync(c->wf.params, &c->wfParams, sizeof(WfParams), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->wf.slots, c->wfSlots.data(), n * sizeof(WfSceneSlot), hipMemcpyHostToDevice, c->stream));
     *paths = base;
@@ -359,10 +360,11 @@ int wf_upload_pass(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_re
 }
 
 std::string tune_key(const std::vector<lts_ctx*>& scenes, const lts_render_params* p, int pass) {
+    // the automatic pass and the spp bound the pass sizes the tuner may choose
     std::ostringstream k;
     k << "gfx950";
     for (lts_ctx* x : scenes) k << ":" << x->scene.view.nPrim << "/" << x->scene.view.cam.width << "x" << x->scene.view.cam.height;
-    k << ":pass" << pass << ":b" << p->max_bounce << ":rr" << p->rr_start;
+    k << ":pass" << pass << ":spp" << p->spp << ":b" << p->max_bounce << ":rr" << p->rr_start;
     return k.str();
 }
 
@@ -370,14 +372,20 @@ bool tune_cache_read(const std::string& key, WfTuning* t) {
     const char* path = std::getenv("LTS_TUNE_CACHE");
     if (!path || !*path) return false;
     std::ifstream in(path);
-    std::string k;
-    int tb, pb, so;
+    std::string line;
     bool found = false;
-    while (in >> k >> tb >> pb >> so)
-        if (k == key && (tb == 64 || tb == 128 || tb == 256) && pb > 0) {
+    while (std::getline(in, line)) {
+        std::istringstream f(line);
+        std::string k;
+        int tb = 0, pb = 0, so = 0, rows = 8, pass = 0;
+        if (!(f >> k >> tb >> pb >> so)) continue;
+        f >> rows >> pass;  // absent in entries of the older format: the defaults
+        if (k == key && (tb == 64 || tb == 128 || tb == 256) && pb > 0 && rows >= 1 && rows <= kMaxTileRows && pass >= 0) {
             t->traceBlock = tb; t->persistentBlocks = pb; t->sortMaterials = so != 0;
+            t->tileRows = rows; t->passSpp = pass;
             found = true;  // the last entry for a key wins
         }
+    }
     return found;
 }
 
@@ -385,7 +393,8 @@ void tune_cache_append(const std::string& key, const WfTuning& t) {
     const char* path = std::getenv("LTS_TUNE_CACHE");
     if (!path || !*path) return;
     std::ofstream out(path, std::ios::app);
-    out << key << " " << t.traceBlock << " " << t.persistentBlocks << " " << t.sortMaterials << "\n";
+    out << key << " " << t.traceBlock << " " << t.persistentBlocks << " " << t.sortMaterials << " " << t.tileRows << " "
+        << t.passSpp << "\n";
 }
 
 // Mean device time of one pass with tuning t (one untimed warm-up, then `reps` passes).
@@ -403,16 +412,18 @@ int time_pass(lts_ctx* c, const WfTuning& t, int maxBounce, int reps, double* ms
     return LTS_OK;
 }
 
-// Coordinate search over the traversal block size, the persistent grid and material sorting, on
-// one pass of the render's own size (the parameters of that pass are uploaded here): the end of
-// every trace launch is a tail whose share depends on the pass size, so smaller trial passes pick
-// other settings than the render needs.
+// Coordinate search over the traversal block size, the persistent grid, material sorting and the
+// band height of the path order, on one pass of the render's own size (the parameters of that pass
+// are uploaded here): the end of every trace launch is a tail whose share depends on the pass size,
+// so smaller trial passes pick other settings than the render needs. Then the pass size itself:
+// half the pass against the whole, compared by time per sample (a smaller pass trades longer tails
+// for less path state in the caches).
 int autotune(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_render_params* p, int pass, WfTuning* best) {
     const int trialSamples = std::max(1, std::min(p->spp, pass));
     int64_t paths = 0;
-    int rc = wf_upload_pass(c, scenes, p, 0, trialSamples, &paths);
-    if (rc != LTS_OK) return rc;
     WfTuning cur = *best;
+    int rc = wf_upload_pass(c, scenes, p, 0, trialSamples, &paths, cur.tileRows);
+    if (rc != LTS_OK) return rc;
     double bestMs = 0;
     if ((rc = time_pass(c, cur, p->max_bounce, 1, &bestMs)) != LTS_OK) return rc;
     auto consider = [&](WfTuning t) -> int {
@@ -437,9 +448,23 @@ int autotune(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_render_p
         WfTuning t = cur; t.sortMaterials = !cur.sortMaterials;
         if ((rc = consider(t)) != LTS_OK) return rc;
     }
-    best->traceBlock = cur.traceBlock;
-    best->persistentBlocks = cur.persistentBlocks;
-    best->sortMaterials = cur.sortMaterials;
+    const WfTuning afterSort = cur;
+    for (int rows : {1, 4, 8, 16}) {
+        if (rows == afterSort.tileRows) continue;
+        WfTuning t = afterSort; t.tileRows = rows;
+        if ((rc = wf_upload_pass(c, scenes, p, 0, trialSamples, &paths, rows)) != LTS_OK) return rc;
+        if ((rc = consider(t)) != LTS_OK) return rc;
+    }
+    cur.passSpp = trialSamples;
+    if (trialSamples >= 2 && p->spp > trialSamples / 2) {
+        // half the pass: the time of a pass of half the samples, per sample, against the whole one's
+        const int half = trialSamples / 2;
+        if ((rc = wf_upload_pass(c, scenes, p, 0, half, &paths, cur.tileRows)) != LTS_OK) return rc;
+        double ms = 0;
+        if ((rc = time_pass(c, cur, p->max_bounce, 1, &ms)) != LTS_OK) return rc;
+        if (ms / half < bestMs / trialSamples) cur.passSpp = half;
+    }
+    *best = cur;
     return LTS_OK;
 }
 
@@ -522,6 +547,8 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
     if (p->persistent_blocks) t.persistentBlocks = p->persistent_blocks;
     if (p->sort_materials >= 0) t.sortMaterials = p->sort_materials != 0;
     if (p->use_graph >= 0) t.useGraph = p->use_graph != 0;
+    if (p->tile_rows) t.tileRows = p->tile_rows;
+    if (p->autotune && !cached && p->spp_per_pass == 0 && !sample_rgb && t.passSpp > 0) pass = std::min(pass, t.passSpp);
     if (p->profile_phases) t.useGraph = 0;  // phase events are recorded between direct launches
     // counting runs launch the counting instantiation of the traversal kernel directly
     unsigned long long* traceCounts = p->count_traversal ? c->counters : nullptr;
@@ -555,7 +582,7 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
             c->nrcPasses++;
         }
         int64_t paths = 0;
-        if ((rc = wf_upload_pass(c, scenes, p, s0, s1, &paths, pc)) != LTS_OK) return rc;
+        if ((rc = wf_upload_pass(c, scenes, p, s0, s1, &paths, t.tileRows, pc)) != LTS_OK) return rc;
         HIPCHK(c, hipEventRecord(c->ev[5], s));
         if ((rc = wf_run(c, t, p->max_bounce, nrc, prof, traceCounts)) != LTS_OK) return rc;
         HIPCHK(c, hipEventRecord(c->ev[6], s));
@@ -611,6 +638,8 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         st->persistent_blocks = t.persistentBlocks;
         st->sort_materials = t.sortMaterials;
         st->use_graph = t.useGraph;
+        st->tile_rows = t.tileRows;
+        st->spp_per_pass = pass;
         st->autotune_ms = tuneMs;
         for (int k = 0; k < 8; k++) st->phase_ms[k] = phase[k];
         if (prof && p->max_bounce > 0) {

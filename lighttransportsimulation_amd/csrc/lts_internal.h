@@ -142,7 +142,7 @@ struct WfParams {
     int32_t trainStride;   // training paths: paths offset + k stride of scene 0 (path index in the pass)
     int32_t trainOffset;
     int32_t trainSlots;    // 0: no training paths
-    int32_t pad;
+    int32_t tileRows;      // path order within a sample: bands of this many rows, walked column by column
 };
 struct WfSceneSlot {
     int64_t pathBase;  // first path of this scene in the pass
@@ -158,12 +158,16 @@ struct WfSceneSlot {
 // slot: the path's sample index into its scene's sample buffer ((sample - s0) * W * H + pixel)
 struct alignas(16) WfTrainRec { float Lb[3]; float beta[3]; float a[3]; int32_t slot; int32_t valid; int32_t pad; };
 
+constexpr int kMaxTileRows = 64;
+
 struct WfTuning {
     int traceBlock = 256;         // block size of the persistent traversal kernels (64, 128, 256)
     int persistentBlocks = 2048;  // grid of the traversal kernels
     int sweepBlocks = 2048;       // grid of the generate / shade / finalize sweeps
     int sortMaterials = 1;        // counting-sort the shade queue by material
     int useGraph = 1;             // capture the pass in a hipGraph and replay it
+    int tileRows = 8;             // band height of the path order (a wave of 64 paths covers tileRows x 64/tileRows pixels)
+    int passSpp = 0;              // samples per pass chosen by the autotuner (0: the automatic pass)
 };
 
 struct NrcQuery;

@@ -22,13 +22,14 @@ namespace {
 
 constexpr int kShadeBlock = 256;
 
-// Pixel of the j-th path of a sample: the image is cut into bands of 8 rows and each band is walked
-// column by column, so 64 consecutive paths (one wave) cover an 8x8 tile and their rays stay close.
-LTS_D int band_pixel(int j, int W, int H) {
-    const int band = j / (8 * W);
-    const int rows = min(8, H - 8 * band);
-    const int jj = j - band * 8 * W;
-    return (8 * band + jj % rows) * W + jj / rows;
+// Pixel of the j-th path of a sample: the image is cut into bands of T rows and each band is walked
+// column by column, so 64 consecutive paths (one wave) cover a T x 64/T tile and their rays stay
+// close (T = 8: an 8x8 tile; T = 1: scanline order). The last band holds the remaining rows.
+LTS_D int band_pixel(int j, int W, int H, int T) {
+    const int band = j / (T * W);
+    const int rows = min(T, H - T * band);
+    const int jj = j - band * T * W;
+    return (T * band + jj % rows) * W + jj / rows;
 }
 
 // Scene of path i of the pass: the last slot whose path range starts at or before i.
@@ -50,7 +51,7 @@ LTS_D void path_origin(const WfParams& P, const WfSceneSlot* slots, long i, int&
     const long npix = (long)sl.width * sl.height;
     scene = s;
     sample = sl.s0 + (int)(local / npix);
-    pixel = band_pixel((int)(local % npix), sl.width, sl.height);
+    pixel = band_pixel((int)(local % npix), sl.width, sl.height, P.tileRows);
 }
 
 // Training paths of the radiance cache: every trainStride-th path of scene 0 from trainOffset; the

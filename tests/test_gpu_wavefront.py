@@ -113,6 +113,26 @@ def test_autotune_keeps_the_image(gpu_ctx, displaced):
     assert (st2["trace_block"], st2["persistent_blocks"], st2["sort_materials"]) == (
         st["trace_block"], st["persistent_blocks"], st["sort_materials"])
     assert _identical(ref, a_rgb) and _identical(ref, b_rgb)
+    assert st["tile_rows"] in (1, 4, 8, 16) and st2["tile_rows"] == st["tile_rows"]
+    assert st["spp_per_pass"] in (2, 4) and st2["spp_per_pass"] == st["spp_per_pass"]
+
+
+@pytest.mark.gpu
+def test_tile_rows_keep_the_image(gpu_ctx, displaced):
+    """The band height of the path order changes which paths share a wave, never a path's result:
+    every band height, including one that does not divide the image height (54 rows) and one above
+    it, gives the default image bit for bit."""
+    d, cfg, _, _ = displaced
+    gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], 96, 54)
+    ref, _, st = gpu_ctx.render(4, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT)
+    assert st["tile_rows"] == 8
+    for t in (1, 4, 5, 16, 64):
+        print(f"[wavefront tile_rows {t}]", flush=True)
+        rgb, _, st = gpu_ctx.render(4, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT, tile_rows=t)
+        assert st["tile_rows"] == t
+        assert _identical(ref, rgb), t
+    with pytest.raises(LtsError):
+        gpu_ctx.render(4, 8, 3, SEED, integrator=INTEGRATOR_WAVEFRONT, tile_rows=65)
 
 
 @pytest.mark.gpu
