@@ -40,8 +40,8 @@ HBM_PEAK_GBS = 8000.0
 # bytes the traversal kernel reads per render-node fetch (a quantized 4-wide node, four 16-byte
 # loads) and per triangle test (three 16-byte loads)
 NODE_BYTES, TRI_BYTES = 64, 48
-# radix passes of the LBVH sort (8-bit digits over the 30-bit Morton codes)
-LBVH_RADIX_PASSES = 4
+# radix passes of the LBVH sort (10-bit digits over the 30-bit Morton codes)
+LBVH_RADIX_PASSES = 3
 
 WORKLOADS = {
     "displaced_1080p": dict(desc="seeded 1M-triangle FBM-displaced sphere on a ground quad, matte/glass/microfacet "
@@ -166,8 +166,9 @@ def lbvh_cpu_baseline(tri) -> dict:
 def lbvh_block(no_cpu: bool) -> dict:
     """Both LBVH workloads in this process: the build alone, repeated on a resident soup."""
     from lighttransportsimulation_amd import Context
-    out = {"definition": "n / device time of the whole build (bounds, Morton codes, radix sort, hierarchy, refit, "
-                         "export layout); achieved = algorithmic bytes per primitive x n / build time",
+    out = {"definition": "n / device time of one build (scene bounds, Morton codes, radix sort, hierarchy and node "
+                         "bounds) on a resident soup, replayed as a captured HIP graph; achieved = algorithmic "
+                         "bytes per primitive x n / build time",
            "radix_passes": LBVH_RADIX_PASSES}
     with Context(0) as ctx:
         for name, n in (("1M", 1 << 20), ("8M", 1 << 23)):

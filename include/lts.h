@@ -198,8 +198,9 @@ int lts_render_tree_info(lts_ctx* ctx, int32_t* kind, int32_t* depth, int32_t* n
 int lts_bvh_build(lts_ctx* ctx, const float* tri_xyz, int32_t n, int32_t mode, lts_bvh_export* out,
                   lts_bvh_timing* timing);
 /* Same build on a soup already resident in device memory (d_tri_xyz is a device pointer, or NULL
- * for the soup uploaded by the last lts_bvh_build with the same n); `repeat` builds are timed back
- * to back and timing holds the per-build mean. */
+ * for the soup uploaded by the last lts_bvh_build with the same n). One build runs with events
+ * between its stages (timing's stage figures); with repeat > 1 the build is then captured as a
+ * HIP graph and replayed `repeat` times back to back, and timing.total_ms is the mean replay. */
 int lts_bvh_build_device(lts_ctx* ctx, const float* d_tri_xyz, int32_t n, int32_t mode, int32_t repeat,
                          lts_bvh_timing* timing);
 /* Copies the arrays of the most recent build to host memory. */
@@ -218,7 +219,7 @@ int lts_bvh_export_last(lts_ctx* ctx, lts_bvh_export* out);
  * stream) and returns when the build is done. Every call on one device shares its buffer: calls
  * that overlap on one device race. LTS_E_ARG beyond LTS_BVH_STATIC_MAX_PRIMS. */
 #define LTS_BVH_STATIC_MAX_PRIMS (1 << 18)
-#define LTS_BVH_STATIC_SCRATCH_BYTES (16u << 20)
+#define LTS_BVH_STATIC_SCRATCH_BYTES (24u << 20)
 int lts_bvh_build_static(const float* d_tri_xyz, int32_t n, int32_t mode, uint32_t* d_codes_sorted, int32_t* d_perm,
                          int32_t* d_left, int32_t* d_right, int32_t* d_parent, float* d_bounds, float* scene_bounds,
                          void* hip_stream);

@@ -1,13 +1,12 @@
 // Karras-2012 LBVH build on gfx950, bit-exact with the reference's construction
 // (acceleration/BVH.cuh:77-452, acceleration/BVH_kernels.cu:8-50):
 //   k_prim_bounds  primitive AABB + centroid inputs, block-reduced scene bounds     (BVH.cuh:92-98)
-//   k_morton       30-bit Morton codes of the normalised centroids                   (BVH.cuh:101-137)
-//   radix sort     stable LSD sort of (code, primitive) pairs, 8-bit digits, 4 passes (BVH.cuh:140-146)
-//                  count -> per-digit scan over tiles -> digit bases -> LDS-ranked scatter
-//   k_karras       interior node i: direction, range end, split                     (BVH.cuh:173-236)
-//   k_refit_l1/l2  leaf-to-root climb, second arriver unions its children            (BVH.cuh:238-262):
-//   /global        subtrees inside 1024-leaf chunks in LDS, then 64K-leaf chunks with LDS arrival
-//                  bits, then the few nodes crossing 64K boundaries with device-scope ordering
+//   k_morton       30-bit Morton codes of the normalised centroids, digit counts     (BVH.cuh:101-137)
+//   k_radix_pass   stable LSD sort of (code, primitive) pairs, 10-bit digits, 3 passes of one kernel
+//                  each (LDS ranking, decoupled look-back over tiles)                (BVH.cuh:140-146)
+//   k_tree_l1/l2   the Karras hierarchy (BVH.cuh:173-236) and its bounds (BVH.cuh:238-262) in one
+//                  bottom-up climb: 512-leaf chunks in LDS, then the chunk roots through global
+//                  arrival slots and records
 //   k_pack_*       traversal layout: both child boxes per node, triangles in leaf order
 // The scene-bound reduction uses a total order on floats (-0 below +0); it equals the reference's
 // float atomics (GeneralHelper.cuh:150-160) on every input without -0.0.
@@ -23,8 +22,14 @@ namespace lts {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kItems = 16;
-constexpr int kTile = kBlock * kItems;  // keys per radix tile
+
+// Build status bits (BvhBuffers::status, 0 = a complete build). Neither can occur on a correct
+// build: a climb is bounded (the tree is at most 56 interior nodes deep for up to 2^26 leaves) and
+// a scatter writes inside the output; a violation is reported instead of hanging or writing out
+// of bounds.
+constexpr int kStatusClimb = 8;   // a climb took more steps than its bound
+constexpr int kStatusSort = 16;   // a radix scatter found inconsistent digit counts
+constexpr int kStatusTree = 32;   // packing found a child index outside the tree
 
 __device__ __forceinline__ uint32_t order_key(float f) {
     uint32_t u = __float_as_uint(f);
@@ -48,10 +53,10 @@ __device__ __forceinline__ void tri_box(const float* t, float* mn, float* mx) {
 __global__ __launch_bounds__(kBlock) void k_prim_bounds(const float* __restrict__ soup, int n, float* __restrict__ primBox,
                                                         uint32_t* __restrict__ partial, int32_t* __restrict__ status) {
     __shared__ uint32_t red[6][kBlock / 64];
-    int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i == 0) *status = 0;  // the build's first kernel: later kernels on the stream see the cleared flags
+    const int i0 = blockIdx.x * kBlock + threadIdx.x;
+    if (i0 == 0) *status = 0;  // the build's first kernel: later kernels on the stream see the cleared flags
     uint32_t k[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
-    if (i < n) {
+    for (int i = i0; i < n; i += gridDim.x * kBlock) {
         float t[9];
 #pragma unroll
         for (int j = 0; j < 9; j++) t[j] = soup[9 * (size_t)i + j];
@@ -61,8 +66,8 @@ __global__ __launch_bounds__(kBlock) void k_prim_bounds(const float* __restrict_
         for (int a = 0; a < 3; a++) {
             primBox[6 * (size_t)i + a] = mn[a];
             primBox[6 * (size_t)i + 3 + a] = mx[a];
-            k[a] = order_key(mn[a]);
-            k[3 + a] = order_key(mx[a]);
+            k[a] = min(k[a], order_key(mn[a]));
+            k[3 + a] = max(k[3 + a], order_key(mx[a]));
         }
     }
 #pragma unroll
@@ -127,45 +132,55 @@ __device__ __forceinline__ uint32_t left_shift3(uint32_t x) {
     return x;
 }
 
-__global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ primBox, int n, const uint32_t* __restrict__ sceneKeys,
-                                                   uint32_t* __restrict__ codes, int32_t* __restrict__ ids) {
-    int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    float gmn[3], gmx[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) { gmn[a] = order_val(sceneKeys[a]); gmx[a] = order_val(sceneKeys[3 + a]); }
-    uint32_t code = 0;
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        float c = primBox[6 * (size_t)i + a] * 0.5f + primBox[6 * (size_t)i + 3 + a] * 0.5f;  // centroid
-        float o = c - gmn[a];
-        if (gmx[a] > gmn[a]) o /= gmx[a] - gmn[a];                                         // Bounds3::offset
-        float s = (float)(1 << 10) * o;
-        code |= left_shift3((uint32_t)s) << a;
-    }
-    codes[i] = code;
-    ids[i] = i;
+// delta (BVH.cuh:173-190): common prefix length of sorted codes i and j; equal codes compare their
+// positions (offset 40 keeps them above every distinct-code value); -1 outside [0, n)
+__device__ __forceinline__ int delta_codes(uint32_t ci, uint32_t cj, int n, int i, int j) {
+    if (i < 0 || i >= n || j < 0 || j >= n) return -1;
+    uint32_t x = ci ^ cj;
+    int extra = 0;
+    if (x == 0) { extra = 40; x = (uint32_t)(i ^ j); }
+    return (x == 0 ? 32 : __clz((int)x)) + extra;
+}
+__device__ __forceinline__ int delta_at(const uint32_t* __restrict__ codes, int n, int i, int j) {
+    if (i < 0 || i >= n || j < 0 || j >= n) return -1;
+    return delta_codes(codes[i], codes[j], n, i, j);
 }
 
 // ---------------------------------------------------------------- radix sort
-__global__ __launch_bounds__(kBlock) void k_radix_count(const uint32_t* __restrict__ keys, int n, int shift, int tiles,
-                                                        uint32_t* __restrict__ counts) {
-    __shared__ uint32_t hist[256];
-    hist[threadIdx.x] = 0;
-    __syncthreads();
-    size_t base = (size_t)blockIdx.x * kTile;
-#pragma unroll
-    for (int j = 0; j < kItems; j++) {
-        size_t idx = base + (size_t)j * kBlock + threadIdx.x;
-        if (idx < (size_t)n) atomicAdd(&hist[(keys[idx] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    counts[(size_t)threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
+// Stable LSD sort of the 30-bit codes in 3 passes of 10-bit digits over 4096-key tiles. A pass is
+// count (a row of 1024 digit counts per tile; the Morton kernel counts the first pass's digits) ->
+// scan (within segments of 32 tiles, then over the segments and the digits) -> scatter. Counts are
+// stored and read as whole rows, so every access is a contiguous 4 KB. The scatter ranks a tile's
+// keys with per-wave digit counters: wave w owns the tile's keys w*1024 .. w*1024+1023 (16 rows of
+// 64) and ranks them in row order, so the sort is stable without a barrier per row.
+constexpr int kSortBits = 10;
+constexpr int kBins = 1 << kSortBits;
+constexpr int kSortPasses = 3;  // 30-bit codes
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortBlock * kSortItems;  // keys per tile
+constexpr int kSortWaves = kSortBlock / 64;
+constexpr int kDigitsPerThread = kBins / kSortBlock;
+static_assert(kDigitsPerThread == 4, "a thread owns one uint4 of a count row");
+constexpr int kSegTiles = 32;  // tiles per scan segment
+constexpr int kScanWindow = 32;  // segment totals a thread loads at once
+
+inline int sort_tiles(int n) { return (n + kSortTile - 1) / kSortTile; }
+inline int sort_segments(int n) { return (sort_tiles(n) + kSegTiles - 1) / kSegTiles; }
+
+// Tile of block b: the 8 XCDs (blocks are dealt to them round-robin) each take a contiguous range
+// of tiles, so the runs that neighbouring tiles write for a digit meet in one XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int tiles) {
+    constexpr int kXcds = 8;
+    const int per = tiles / kXcds, rem = tiles % kXcds, x = b % kXcds;
+    return x * per + min(x, rem) + b / kXcds;
 }
 
-// Block-wide exclusive scan of one value per thread (256 threads); returns the total.
-__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* out, uint32_t* scratch) {
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// Exclusive scan of one value per thread over a block of kThreads threads; returns the total.
+template <int kThreads>
+__device__ uint32_t block_scan(uint32_t v, uint32_t* out, uint32_t* scratch) {
+    constexpr int kWaves = kThreads / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t incl = v;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -175,155 +190,252 @@ __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* out, uint32_t* sc
     if (lane == 63) scratch[wave] = incl;
     __syncthreads();
     uint32_t wbase = 0, total = 0;
-    for (int w = 0; w < kBlock / 64; w++) {
-        if (w < wave) wbase += scratch[w];
-        total += scratch[w];
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+        const uint32_t s = scratch[w];
+        if (w < wave) wbase += s;
+        total += s;
     }
     *out = wbase + incl - v;
     __syncthreads();
     return total;
 }
 
-// One block per digit: exclusive scan of that digit's per-tile counts, total into digitBase.
-__global__ __launch_bounds__(kBlock) void k_radix_scan_tiles(uint32_t* __restrict__ counts, int tiles, uint32_t* __restrict__ digitBase) {
-    __shared__ uint32_t scratch[kBlock / 64];
-    uint32_t* row = counts + (size_t)blockIdx.x * tiles;
-    int per = (tiles + kBlock - 1) / kBlock;
-    int beg = threadIdx.x * per, end = min(beg + per, tiles);
-    uint32_t sum = 0;
-    for (int t = beg; t < end; t++) sum += row[t];
-    uint32_t excl;
-    uint32_t total = block_exclusive_scan(sum, &excl, scratch);
-    for (int t = beg; t < end; t++) { uint32_t c = row[t]; row[t] = excl; excl += c; }
-    if (threadIdx.x == 0) digitBase[blockIdx.x] = total;
+// The tile's digit counts from the LDS histogram to its row counts[tile][*].
+__device__ __forceinline__ void store_tile_counts(const uint32_t* hist, uint32_t* __restrict__ counts) {
+    const int d = kDigitsPerThread * threadIdx.x;
+    reinterpret_cast<uint4*>(counts + (size_t)blockIdx.x * kBins)[threadIdx.x] =
+        make_uint4(hist[d], hist[d + 1], hist[d + 2], hist[d + 3]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_radix_scan_digits(uint32_t* __restrict__ digitBase) {
-    __shared__ uint32_t scratch[kBlock / 64];
-    uint32_t v = digitBase[threadIdx.x], excl;
-    block_exclusive_scan(v, &excl, scratch);
-    digitBase[threadIdx.x] = excl;
-}
-
-// Stable scatter of one 4096-key tile: ranks from wave ballots, tile-local order staged in LDS,
-// then written out in runs per digit.
-__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __restrict__ keysIn, const int32_t* __restrict__ valsIn,
-                                                          uint32_t* __restrict__ keysOut, int32_t* __restrict__ valsOut, int n,
-                                                          int shift, int tiles, const uint32_t* __restrict__ counts,
-                                                          const uint32_t* __restrict__ digitBase) {
-    __shared__ uint32_t sKeys[kTile];
-    __shared__ int32_t sVals[kTile];
-    __shared__ uint32_t waveCnt[kBlock / 64][256];
-    __shared__ uint32_t runTotal[256];
-    __shared__ uint32_t tileStart[256];
-    __shared__ uint32_t globalBase[256];
-    __shared__ uint32_t scratch[kBlock / 64];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const size_t base = (size_t)blockIdx.x * kTile;
-    const int valid = (int)min((size_t)kTile, (size_t)n - base);
-    for (int w = 0; w < kBlock / 64; w++) waveCnt[w][t] = 0;
-    runTotal[t] = 0;
-    globalBase[t] = digitBase[t] + counts[(size_t)t * tiles + blockIdx.x];
-    uint32_t key[kItems];
-    int32_t val[kItems];
-    uint32_t rank[kItems];
+// Morton codes of the centroids (one sort tile per block), with the tile's first-pass digit counts.
+__global__ __launch_bounds__(kSortBlock) void k_morton(const float* __restrict__ primBox, int n, const uint32_t* __restrict__ sceneKeys,
+                                                       uint32_t* __restrict__ codes, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t hist[kBins];
 #pragma unroll
-    for (int j = 0; j < kItems; j++) {
-        size_t idx = base + (size_t)j * kBlock + t;
-        bool ok = idx < (size_t)n;
-        key[j] = ok ? keysIn[idx] : 0xffffffffu;  // tail keys rank last in digit 255 and are never written
-        val[j] = ok ? valsIn[idx] : 0;
+    for (int u = 0; u < kDigitsPerThread; u++) hist[threadIdx.x + u * kSortBlock] = 0;
+    float gmn[3], gmx[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) { gmn[a] = order_val(sceneKeys[a]); gmx[a] = order_val(sceneKeys[3 + a]); }
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int j = 0; j < kSortItems; j++) {
+        const size_t i = base + (size_t)j * kSortBlock + threadIdx.x;
+        if (i >= (size_t)n) break;
+        uint32_t code = 0;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float c = primBox[6 * i + a] * 0.5f + primBox[6 * i + 3 + a] * 0.5f;  // centroid
+            float o = c - gmn[a];
+            if (gmx[a] > gmn[a]) o /= gmx[a] - gmn[a];                         // Bounds3::offset
+            float s = (float)(1 << 10) * o;
+            code |= left_shift3((uint32_t)s) << a;
+        }
+        codes[i] = code;
+        atomicAdd(&hist[code & (kBins - 1)], 1u);
     }
     __syncthreads();
-    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    store_tile_counts(hist, counts);
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_radix_count(const uint32_t* __restrict__ keys, int n, int shift,
+                                                            uint32_t* __restrict__ counts) {
+    __shared__ uint32_t hist[kBins];
 #pragma unroll
-    for (int j = 0; j < kItems; j++) {
-        uint32_t d = (key[j] >> shift) & 255u;
+    for (int u = 0; u < kDigitsPerThread; u++) hist[threadIdx.x + u * kSortBlock] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+    uint32_t k[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        const size_t i = base + (size_t)j * kSortBlock + threadIdx.x;
+        k[j] = i < (size_t)n ? keys[i] : 0xffffffffu;
+    }
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++)
+        if (k[j] != 0xffffffffu) atomicAdd(&hist[(k[j] >> shift) & (kBins - 1)], 1u);
+    __syncthreads();
+    store_tile_counts(hist, counts);
+}
+
+// Block g: exclusive prefix of each digit's counts over the tiles of segment g (in place), and the
+// segment's digit totals.
+__global__ __launch_bounds__(kSortBlock) void k_scan_segments(uint32_t* __restrict__ counts, int tiles, uint32_t* __restrict__ segBase) {
+    const int t0 = blockIdx.x * kSegTiles, m = min(kSegTiles, tiles - t0);
+    uint4* rows = reinterpret_cast<uint4*>(counts + (size_t)t0 * kBins) + threadIdx.x;
+    uint4 c[kSegTiles];
+#pragma unroll
+    for (int i = 0; i < kSegTiles; i++)
+        if (i < m) c[i] = rows[(size_t)i * (kBins / 4)];
+    uint4 run = make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < kSegTiles; i++) {
+        if (i < m) {
+            rows[(size_t)i * (kBins / 4)] = run;
+            run.x += c[i].x; run.y += c[i].y; run.z += c[i].z; run.w += c[i].w;
+        }
+    }
+    reinterpret_cast<uint4*>(segBase + (size_t)blockIdx.x * kBins)[threadIdx.x] = run;
+}
+
+// One block: thread d turns the segment totals of digit d into its base in each segment: the
+// exclusive scan of the digit totals (the digit's base in the pass) plus the totals of the earlier
+// segments (in place).
+__global__ __launch_bounds__(kBins) void k_scan_digits(uint32_t* __restrict__ segBase, int segs) {
+    __shared__ uint32_t scratch[kBins / 64];
+    const int d = threadIdx.x;
+    uint32_t total = 0;
+    for (int g0 = 0; g0 < segs; g0 += kScanWindow) {
+        uint32_t v[kScanWindow];
+#pragma unroll
+        for (int i = 0; i < kScanWindow; i++) v[i] = (g0 + i < segs) ? segBase[(size_t)(g0 + i) * kBins + d] : 0u;
+#pragma unroll
+        for (int i = 0; i < kScanWindow; i++) total += v[i];
+    }
+    uint32_t run;
+    block_scan<kBins>(total, &run, scratch);
+    for (int g0 = 0; g0 < segs; g0 += kScanWindow) {
+        uint32_t v[kScanWindow];
+#pragma unroll
+        for (int i = 0; i < kScanWindow; i++) v[i] = (g0 + i < segs) ? segBase[(size_t)(g0 + i) * kBins + d] : 0u;
+#pragma unroll
+        for (int i = 0; i < kScanWindow; i++) {
+            if (g0 + i < segs) segBase[(size_t)(g0 + i) * kBins + d] = run;
+            run += v[i];
+        }
+    }
+}
+
+// One pass's scatter of a 4096-key tile. kFirst: the values are the keys' input positions.
+template <bool kFirst>
+__global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const uint32_t* __restrict__ keysIn, const int32_t* __restrict__ valsIn,
+                                                              uint32_t* __restrict__ keysOut, int32_t* __restrict__ valsOut, int n,
+                                                              int shift, int tiles, const uint32_t* __restrict__ counts,
+                                                              const uint32_t* __restrict__ segBase, int32_t* __restrict__ status) {
+    __shared__ uint32_t sKeys[kSortTile];
+    __shared__ int32_t sVals[kSortTile];
+    __shared__ uint32_t waveCnt[kSortWaves][kBins];  // per-wave digit counts, then per-wave digit offsets
+    __shared__ uint32_t tileStart[kBins];
+    __shared__ uint32_t globalBase[kBins];
+    __shared__ uint32_t scratch[kSortWaves];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int tile = xcd_tile(blockIdx.x, tiles);
+    for (int q = t; q < kSortWaves * kBins; q += kSortBlock) (&waveCnt[0][0])[q] = 0;
+    {  // the digits' bases in this tile: segment base (with the digit's base in the pass) + prefix in the segment
+        const uint4 sb = reinterpret_cast<const uint4*>(segBase + (size_t)(tile / kSegTiles) * kBins)[t];
+        const uint4 tb = reinterpret_cast<const uint4*>(counts + (size_t)tile * kBins)[t];
+        reinterpret_cast<uint4*>(globalBase)[t] = make_uint4(sb.x + tb.x, sb.y + tb.y, sb.z + tb.z, sb.w + tb.w);
+    }
+    const size_t base = (size_t)tile * kSortTile;
+    const int valid = (int)min((size_t)kSortTile, (size_t)n - base);
+    uint32_t key[kSortItems];
+    int32_t val[kSortItems];
+    uint32_t rank[kSortItems];
+    const size_t wbase = base + (size_t)wave * (64 * kSortItems);
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        const size_t idx = wbase + (size_t)j * 64 + lane;
+        const bool ok = idx < (size_t)n;
+        key[j] = ok ? keysIn[idx] : 0xffffffffu;  // tail keys rank last in digit kBins-1 and are never written
+        if (kFirst) val[j] = (int32_t)idx;
+        else val[j] = ok ? valsIn[idx] : 0;
+    }
+    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t* cnt = waveCnt[wave];
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        const uint32_t d = (key[j] >> shift) & (kBins - 1);
         uint64_t peers = ~0ull;
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bal = __ballot(bit);
+        for (int b = 0; b < kSortBits; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        uint32_t inWave = (uint32_t)__popcll(peers & ltMask);
-        int leader = __ffsll((long long)peers) - 1;
-        if (lane == leader) waveCnt[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        uint32_t pre = runTotal[d];
-        for (int w = 0; w < wave; w++) pre += waveCnt[w][d];
-        rank[j] = pre + inWave;
-        __syncthreads();
-        uint32_t add = 0;
-        for (int w = 0; w < kBlock / 64; w++) { add += waveCnt[w][t]; waveCnt[w][t] = 0; }
-        runTotal[t] += add;
-        __syncthreads();
+        const uint32_t before = (uint32_t)__popcll(peers & ltMask);
+        const uint32_t prev = cnt[d];  // every lane reads before the group's lowest lane writes
+        rank[j] = prev + before;
+        if (before == 0) cnt[d] = prev + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    // per-wave offsets and the tile's local digit starts (tail keys sort after every valid key)
+    uint32_t sum = 0, run[kDigitsPerThread];
+#pragma unroll
+    for (int u = 0; u < kDigitsPerThread; u++) {
+        const int d = kDigitsPerThread * t + u;
+        uint32_t r = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; w++) {
+            const uint32_t c = waveCnt[w][d];
+            waveCnt[w][d] = r;
+            r += c;
+        }
+        run[u] = r;
+        sum += r;
     }
     uint32_t start;
-    block_exclusive_scan(runTotal[t], &start, scratch);
-    tileStart[t] = start;
+    block_scan<kSortBlock>(sum, &start, scratch);
+#pragma unroll
+    for (int u = 0; u < kDigitsPerThread; u++) {
+        tileStart[kDigitsPerThread * t + u] = start;
+        start += run[u];
+    }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kItems; j++) {
-        uint32_t d = (key[j] >> shift) & 255u;
-        uint32_t pos = tileStart[d] + rank[j];
+    for (int j = 0; j < kSortItems; j++) {
+        const uint32_t d = (key[j] >> shift) & (kBins - 1);
+        const uint32_t pos = tileStart[d] + waveCnt[wave][d] + rank[j];
         sKeys[pos] = key[j];
         sVals[pos] = val[j];
     }
     __syncthreads();
-    for (int i = t; i < valid; i += kBlock) {
-        uint32_t k = sKeys[i];
-        uint32_t d = (k >> shift) & 255u;
-        size_t dst = (size_t)globalBase[d] + (i - tileStart[d]);
+    for (int i = t; i < valid; i += kSortBlock) {
+        const uint32_t k = sKeys[i];
+        const uint32_t d = (k >> shift) & (kBins - 1);
+        const size_t dst = (size_t)globalBase[d] + (uint32_t)(i - tileStart[d]);
+        if (dst >= (size_t)n) {  // only inconsistent counts get here
+            atomicOr(status, kStatusSort);
+            continue;
+        }
         keysOut[dst] = k;
         valsOut[dst] = sVals[i];
     }
 }
 
-// ---------------------------------------------------------------- hierarchy
-__device__ __forceinline__ int delta_at(const uint32_t* __restrict__ codes, int n, int i, int j) {
-    if (i < 0 || i >= n || j < 0 || j >= n) return -1;
-    uint32_t x = codes[i] ^ codes[j];
-    int extra = 0;
-    if (x == 0) { extra = 40; x = (uint32_t)(i ^ j); }
-    return (x == 0 ? 32 : __clz((int)x)) + extra;
-}
+// ---------------------------------------------------------------- hierarchy and bounds
+// The Karras tree (BVH.cuh:173-236) is the binary radix tree of the sorted codes (with equal codes
+// told apart by position, delta_at): interior node [l, r] splits where the adjacent delta inside
+// it is smallest, and its parent splits at r (it is a left child) when delta(r, r+1) exceeds
+// delta(l-1, l), else at l-1 (a right child). Karras numbers a left child by its last leaf and a
+// right child by its first (the split g of a node has children g and g+1), the root 0. So the tree
+// and its bounds (BVH.cuh:238-262: the second arrival at a node unions its children's boxes) are
+// built in one bottom-up climb from the leaves: a node arriving at its parent's split records its
+// outer end there; the second arrival learns the parent's range from it, unions the children and
+// climbs on. The result equals the top-down construction and refit bit for bit: same nodes, same
+// numbering, and every box is bUnion(left box, right box).
+//   level 1: one workgroup per 512 sorted leaves; arrival slots, child records and boxes in LDS.
+//            A climb stops when its parent's split lies left of the chunk (one node per chunk,
+//            filed as the chunk's root) or when it arrives first. Unmatched arrivals are written
+//            to the split's global slot, with a record of the waiting node.
+//   level 2: one workgroup per 65536 leaves climbs on from its chunks' roots; it files one root.
+//   level 3: one workgroup climbs from the level-2 roots.
+// Each level's climbs meet each other in LDS and read only what earlier launches wrote, so no
+// climb needs a device-scope fence.
+// Climbs of one workgroup exchange nodes through LDS only. A wave's LDS operations execute in the
+// order it issues them, so a box written before an exchange is in LDS before the exchange is, and a
+// partner reads it only after its own exchange returned; the climbs therefore order their LDS
+// accesses with a compiler barrier instead of a fence (a fence would wait for every outstanding LDS
+// operation at each step of a latency-bound climb).
+#define LTS_LDS_ORDER() __atomic_signal_fence(__ATOMIC_SEQ_CST)
 
-__global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ codes, int n, int32_t* __restrict__ left,
-                                                   int32_t* __restrict__ right, int32_t* __restrict__ parent,
-                                                   int32_t* __restrict__ rangeLo, int32_t* __restrict__ rangeHi) {
-    int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n - 1) return;
-    int d = delta_at(codes, n, i, i + 1) - delta_at(codes, n, i, i - 1);
-    d = d > 0 ? 1 : -1;
-    int deltaMin = delta_at(codes, n, i, i - d);
-    int lMax = 128;
-    while (delta_at(codes, n, i, i + lMax * d) > deltaMin) lMax *= 4;
-    int l = 0;
-    for (int t = lMax / 2; t >= 1; t /= 2)
-        if (delta_at(codes, n, i, i + (l + t) * d) > deltaMin) l += t;
-    int j = i + l * d;
-    int deltaNode = delta_at(codes, n, i, j);
-    int s = 0;
-    for (int t = (l + 1) >> 1; t >= 1; t = (t == 1) ? 0 : ((t + 1) >> 1))  // ceil-halving
-        if (delta_at(codes, n, i, i + (s + t) * d) > deltaNode) s += t;
-    int gamma = i + s * d + min(d, 0);
-    int lo = min(i, j), hi = max(i, j);
-    int lc = (lo == gamma) ? (n - 1) + gamma : gamma;
-    int rc = (hi == gamma + 1) ? (n - 1) + gamma + 1 : gamma + 1;
-    left[i] = lc;
-    right[i] = rc;
-    rangeLo[i] = lo;
-    rangeHi[i] = hi;
-    parent[lc] = i;
-    parent[rc] = i;
-    if (i == 0) parent[0] = -1;
-}
-
-__global__ void k_fill_i32(int32_t* p, int n, int32_t v) {
-    int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) p[i] = v;
-}
+constexpr int kChunk = 512;
+constexpr int kGroup2 = 128;                // level-1 chunks per level-2 group
+constexpr int kChunk2 = kGroup2 * kChunk;   // leaves per level-2 group
+constexpr int kGroup3 = 1024;               // level-2 groups (at most 2^26 / kChunk2)
+static_assert((1 << 26) / kChunk2 <= kGroup3, "level 3 climbs every level-2 root in one workgroup");
+constexpr int kUnset = INT32_MIN;
+constexpr int kMaxClimb = 64;  // steps of a climb above level 1: the tree is at most 56 interior nodes deep
 
 // bUnion(Bounds3, Bounds3) followed by the Bounds3(p1, p2) constructor
 __device__ __forceinline__ void union_box(const float* a, const float* b, float* o) {
@@ -335,19 +447,6 @@ __device__ __forceinline__ void union_box(const float* a, const float* b, float*
     }
 }
 
-constexpr int kChunk = 1024;            // sorted leaves per level-1 workgroup (subtrees refit in LDS)
-constexpr int kChunk2 = 64 * kChunk;     // sorted leaves per level-2 workgroup
-// level-1 roots filed per level-2 chunk: each is a child of a node containing one of the
-// kChunk2 / kChunk + 1 chunk boundaries of that range, and at most 58 nodes contain a boundary
-constexpr int kBucketCap = 2 * (kChunk2 / kChunk + 1) * 58;
-// Build status bits (BvhBuffers::status). The capacities above follow from the depth bound of a
-// Karras tree over at most 2^26 leaves (58 nodes on a root-to-leaf path); a violated bound is
-// reported instead of leaving boxes unrefitted.
-constexpr int kStatusBucket = 1;  // a level-2 chunk received more than kBucketCap level-1 roots
-constexpr int kStatusRoots = 2;   // more level-3 roots than primitives
-constexpr int kStatusLevel3 = 4;  // more level-3 roots than level-3 climbers were launched
-constexpr int kStatusClimb = 8;   // a climb took more than 128 steps
-
 __device__ __forceinline__ void load_box(const float* box, int node, float* o) {
 #pragma unroll
     for (int q = 0; q < 6; q++) o[q] = box[6 * (size_t)node + q];
@@ -357,163 +456,320 @@ __device__ __forceinline__ void store_box(float* box, int node, const float* o) 
     for (int q = 0; q < 6; q++) box[6 * (size_t)node + q] = o[q];
 }
 
-// Level 1: one workgroup per chunk of kChunk sorted leaves. Leaf boxes, the records of the
-// chunk's interior nodes and the boxes of every interior node whose leaf range lies inside the
-// chunk are held in LDS; the leaf-to-root climbs meet through LDS arrival bits. A climb that
-// reaches a node crossing the chunk boundary files the node below it as a level-2 root.
-__global__ __launch_bounds__(kChunk) void k_refit_l1(const float* __restrict__ primBox, const int32_t* __restrict__ perm, int n,
-                                                     int mode, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
-                                                     const int32_t* __restrict__ parent, const int32_t* __restrict__ rangeLo,
-                                                     const int32_t* __restrict__ rangeHi, float* __restrict__ box,
-                                                     int32_t* __restrict__ bucket, int32_t* __restrict__ bucketCount,
-                                                     int32_t* __restrict__ status) {
-    __shared__ float lbox[kChunk * 6];
-    __shared__ float ibox[kChunk * 6];
-    __shared__ int32_t sl[kChunk], sr[kChunk], sp[kChunk];
-    __shared__ uint32_t arrive[kChunk / 32], local[kChunk / 32];
-    const int t = threadIdx.x;
+// Level 1. LDS holds the chunk's nodes under local indices: leaf cs + x at x, interior node cs + x
+// at kChunk + x. Each wave first climbs from its own 64 leaves, stopping where a parent splits
+// left of those leaves (the wave's root); wave 0 then climbs on from the other waves' roots. So
+// most of the climbing runs with full waves, and only the short top of the chunk's subtrees
+// runs on a few lanes.
+__device__ __forceinline__ int local_index(int c, int n, int cs) {
+    return c >= n - 1 ? c - (n - 1) - cs : kChunk + (c - cs);
+}
+
+__global__ __launch_bounds__(kChunk) void k_tree_l1(const uint32_t* __restrict__ codes, const float* __restrict__ primBox,
+                                                    const int32_t* __restrict__ perm, int n, int mode, int32_t* __restrict__ left,
+                                                    int32_t* __restrict__ right, int32_t* __restrict__ parent,
+                                                    float* __restrict__ box, int32_t* __restrict__ slots,
+                                                    ClimbRec* __restrict__ pend, int32_t* __restrict__ chunkRoot,
+                                                    ClimbRec* __restrict__ chunkRec, int32_t* __restrict__ status) {
+    constexpr int kWaves = kChunk / 64;
+    __shared__ __align__(16) float nbox[2 * kChunk * 6];
+    __shared__ int32_t dl[kChunk + 1];  // dl[x] = delta(cs - 1 + x, cs + x)
+    __shared__ int32_t slot[kChunk];    // split cs + x: outer end of a first arrival, or -1
+    __shared__ int32_t sl[kChunk], sr[kChunk];
+    __shared__ int32_t sp[2 * kChunk];  // parent of each local node (kUnset: set by a later level)
+    __shared__ int32_t waveRoot[kWaves];
+    __shared__ int32_t sRoot;
+    __shared__ uint32_t sc[kChunk + 2];  // codes cs - 1 .. cs + kChunk
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int cs = blockIdx.x * kChunk, ce = min(cs + kChunk, n);
-    if (t < kChunk / 32) { arrive[t] = 0; local[t] = 0; }
-    __syncthreads();
     const int k = cs + t;
-    if (k < ce) {
-        const int prim = mode == 0 ? k : perm[k];
+    if (k - 1 >= 0 && k - 1 < n) sc[t] = codes[k - 1];
+    if (t < 2 && cs + kChunk - 1 + t < n) sc[kChunk + t] = codes[cs + kChunk - 1 + t];
+    if (t == 0) sRoot = -1;
+    if (t < kWaves) waveRoot[t] = -1;
+    slot[t] = -1;
+    sl[t] = kUnset;
+    sp[t] = kUnset;
+    sp[kChunk + t] = kUnset;
+    const int m = ce - cs;  // leaves of the chunk
+    if (mode == 0) {  // the chunk's leaf boxes are one contiguous run (8-byte aligned)
+        const float2* src = reinterpret_cast<const float2*>(primBox + 6 * (size_t)cs);
+        for (int i = t; i < 3 * m; i += kChunk) reinterpret_cast<float2*>(nbox)[i] = src[i];
+    } else if (k < ce) {
         float b[6];
-        load_box(primBox, prim, b);
+        load_box(primBox, perm[k], b);
 #pragma unroll
-        for (int q = 0; q < 6; q++) lbox[6 * t + q] = b[q];
-        store_box(box, (n - 1) + k, b);
-    }
-    if (k < ce && k < n - 1) {  // interior node k (an interior node is an end of its own range)
-        sl[t] = left[k]; sr[t] = right[k]; sp[t] = parent[k];
-        if (rangeLo[k] >= cs && rangeHi[k] < ce) atomicOr(&local[t >> 5], 1u << (t & 31));
+        for (int q = 0; q < 6; q++) nbox[6 * t + q] = b[q];
     }
     __syncthreads();
-    if (k < ce && n > 1) {
-        int cur = (n - 1) + k;
-        int node = parent[cur];
-        int step = 0;
-        for (; step < 128; step++) {
-            const int j = node - cs;
-            const bool isLocal = node >= cs && node < ce && node < n - 1 && ((local[j >> 5] >> (j & 31)) & 1u);
-            if (!isLocal) {
-                if (node >= 0) {
-                    int b = cs / kChunk2;
-                    int slot = atomicAdd(&bucketCount[b], 1);
-                    if (slot < kBucketCap) bucket[(size_t)b * kBucketCap + slot] = cur;
-                    else atomicOr(status, kStatusBucket);
-                }
-                break;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            uint32_t old = atomicOr(&arrive[j >> 5], 1u << (j & 31));
-            if (!((old >> (j & 31)) & 1u)) break;  // first arrival
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    dl[t] = delta_codes(sc[t], sc[t + 1], n, k - 1, k);
+    if (t == 0) dl[kChunk] = delta_codes(sc[kChunk], sc[kChunk + 1], n, cs + kChunk - 1, cs + kChunk);
+    __syncthreads();
+    // Climbs [l, r] until its parent splits left of `lo` (returns r: the node is the root of the
+    // leaves from lo) or it arrives first (returns -1). Every step adds a leaf to [l, r].
+    auto climb = [&](int l, int r, int lo) -> int {
+        int dL = dl[l - cs], dR = dl[r + 1 - cs];
+        for (int step = 0; step < kChunk; step++) {
+            if (dL < 0 && dR < 0) return -1;  // the root of the whole tree
+            const bool asLeft = dR > dL;
+            const int p = asLeft ? r : l - 1;
+            if (p < lo) return r;
+            LTS_LDS_ORDER();
+            const int other = atomicExch(&slot[p - cs], asLeft ? l : r);
+            if (other < 0) return -1;  // first arrival
+            LTS_LDS_ORDER();
+            slot[p - cs] = -1;
+            const int L = asLeft ? l : other, R = asLeft ? other : r;
+            const int lc = (L == p) ? (n - 1) + p : p;
+            const int rc = (R == p + 1) ? (n - 1) + p + 1 : p + 1;
+            const int li = local_index(lc, n, cs), ri = local_index(rc, n, cs);
             float a[6], c[6], o[6];
-            const int l = sl[j], r = sr[j];
 #pragma unroll
             for (int q = 0; q < 6; q++) {
-                a[q] = l >= n - 1 ? lbox[6 * (l - (n - 1) - cs) + q] : ibox[6 * (l - cs) + q];
-                c[q] = r >= n - 1 ? lbox[6 * (r - (n - 1) - cs) + q] : ibox[6 * (r - cs) + q];
+                a[q] = nbox[6 * li + q];
+                c[q] = nbox[6 * ri + q];
             }
             union_box(a, c, o);
+            dL = dl[L - cs];
+            dR = dl[R + 1 - cs];
+            const int me = (dL < 0 && dR < 0) ? 0 : (dR > dL ? R : L);
+            const int mi = kChunk + (me - cs);
 #pragma unroll
-            for (int q = 0; q < 6; q++) ibox[6 * j + q] = o[q];
-            if (node == 0) break;
-            cur = node;
-            node = sp[j];
+            for (int q = 0; q < 6; q++) nbox[6 * mi + q] = o[q];
+            sl[me - cs] = lc;
+            sr[me - cs] = rc;
+            sp[li] = me;
+            sp[ri] = me;
+            l = L;
+            r = R;
         }
-        if (step == 128) atomicOr(status, kStatusClimb);
+        atomicOr(status, kStatusClimb);
+        return -1;
+    };
+    const int ws = cs + 64 * wave;  // first leaf of this wave's 64
+    if (k < ce && n > 1) {
+        const int root = climb(k, k, ws);
+        if (root >= 0) {
+            if (wave == 0) sRoot = root;  // parent splits at cs - 1: the chunk's root
+            else waveRoot[wave] = root;
+        }
     }
     __syncthreads();
-    if (k < ce && k < n - 1 && ((local[t >> 5] >> (t & 31)) & 1u)) {
+    if (wave == 0 && lane >= 1 && lane < kWaves && waveRoot[lane] >= 0) {
+        const int root = climb(cs + 64 * lane, waveRoot[lane], cs);
+        if (root >= 0) sRoot = root;
+    }
+    __syncthreads();
+    // boxes as contiguous runs: interior nodes cs .. (those completed above later levels overwrite),
+    // leaves (n - 1) + cs .. (8-byte aligned; in mode 0 the primitive boxes were written there)
+    {
+        const int mi = min(ce, n - 1) - cs;
+        float* dst = box + 6 * (size_t)cs;
+        const float* ib = nbox + 6 * kChunk;
+        const int whole = mi > 0 ? (6 * mi) / 4 : 0;
+        for (int i = t; i < whole; i += kChunk) reinterpret_cast<float4*>(dst)[i] = reinterpret_cast<const float4*>(ib)[i];
+        for (int i = 4 * whole + t; i < 6 * mi; i += kChunk) dst[i] = ib[i];
+        if (mode != 0) {
+            float2* ldst = reinterpret_cast<float2*>(box + 6 * ((size_t)(n - 1) + cs));
+            for (int i = t; i < 3 * m; i += kChunk) ldst[i] = reinterpret_cast<const float2*>(nbox)[i];
+        }
+    }
+    if (k < ce && k < n - 1) {
+        if (sl[t] != kUnset) {
+            left[k] = sl[t];
+            right[k] = sr[t];
+        }
+        if (sp[kChunk + t] != kUnset) parent[k] = sp[kChunk + t];
+        const int v = slot[t];
+        slots[k] = v;
+        if (v >= 0) {  // a first arrival waits at split k: publish its box and the delta at its outer end
+            const bool leftChild = v <= k;
+            const int child = leftChild ? (v == k ? (n - 1) + k : k) : (v == k + 1 ? (n - 1) + k + 1 : k + 1);
+            ClimbRec rec;
+            const float* cb = &nbox[6 * local_index(child, n, cs)];
+#pragma unroll
+            for (int q = 0; q < 6; q++) rec.box[q] = cb[q];
+            rec.a = leftChild ? dl[v - cs] : dl[v + 1 - cs];
+            rec.b = 0;
+            pend[k] = rec;
+        }
+    }
+    if (k < ce && sp[t] != kUnset) parent[(n - 1) + k] = sp[t];
+    if (t == 0) {
+        chunkRoot[blockIdx.x] = sRoot;
+        if (sRoot >= 0) {  // the node [cs, sRoot] (a right child: index cs) with its box and deltas
+            ClimbRec rec;
+            const float* cb = &nbox[6 * (sRoot == cs ? 0 : kChunk)];
+#pragma unroll
+            for (int q = 0; q < 6; q++) rec.box[q] = cb[q];
+            rec.a = dl[0];
+            rec.b = dl[sRoot + 1 - cs];
+            chunkRec[blockIdx.x] = rec;
+        }
+        if (cs == 0) parent[0] = -1;
+    }
+}
+
+// Levels 2 and 3: climbs from the roots handed on by the level below. A climb carries its node's
+// box and the deltas at its ends; a first arrival of an earlier level left its outer end in the
+// split's global slot and its box and outer delta in the split's record, loaded together in one
+// round trip, so a step needs no other load. Arrivals of the same launch meet in an LDS hash table
+// keyed by the split (entries (split + 1) << 32 | climb, claimed by compare-and-swap and never
+// removed); a climb's LDS record holds its node for its final, first arrival. Unmatched entries are
+// published to the global slots and records for the next level.
+__device__ __forceinline__ void store_rec(ClimbRec* dst, const float* b, int a, int c) {
+    ClimbRec r;
+#pragma unroll
+    for (int q = 0; q < 6; q++) r.box[q] = b[q];
+    r.a = a;
+    r.b = c;
+    *dst = r;
+}
+
+// Units of `span` leaves (512-leaf chunks for level 2, 128 of them per group; the groups of level 2
+// for level 3, all in one group). Climb j of a group starts at the root of unit u0 + j (unit u0's
+// root, whose parent splits left of the group, is the group's own root: handed on unless a climb
+// completes a larger one).
+template <int kClimbs, int kHashCap>
+__global__ __launch_bounds__(kClimbs) void k_tree_up(int n, int units, int span, const int32_t* __restrict__ rootEnd,
+                                                     const ClimbRec* __restrict__ rootRec, int32_t* __restrict__ left,
+                                                     int32_t* __restrict__ right, int32_t* __restrict__ parent,
+                                                     float* __restrict__ box, int32_t* __restrict__ slots,
+                                                     ClimbRec* __restrict__ pend, int32_t* __restrict__ outEnd,
+                                                     ClimbRec* __restrict__ outRec, int32_t* __restrict__ status) {
+    __shared__ unsigned long long hash[kHashCap];
+    __shared__ float recBox[kClimbs][6];
+    __shared__ int32_t recEnd[kClimbs], recDelta[kClimbs];
+    __shared__ int32_t sEnd;
+    __shared__ ClimbRec sRec;
+    const int j = threadIdx.x;
+    const int u0 = blockIdx.x * kClimbs;
+    const int cs = u0 * span;
+    for (int i = j; i < kHashCap; i += kClimbs) hash[i] = 0;
+    if (j == 0) {
+        sEnd = cs > 0 ? rootEnd[u0] : -1;
+        if (sEnd >= 0) sRec = rootRec[u0];
+    }
+    __syncthreads();
+    constexpr int kHashBits = __builtin_ctz(kHashCap);
+    const int u = u0 + j;
+    int r = (j >= 1 && u < units) ? rootEnd[u] : -1;
+    if (r >= 0) {
+        int l = u * span;
+        const ClimbRec start = rootRec[u];
         float o[6];
 #pragma unroll
-        for (int q = 0; q < 6; q++) o[q] = ibox[6 * t + q];
-        store_box(box, k, o);
-    }
-}
-
-// Level 2: one workgroup per chunk of kChunk2 leaves climbs from its level-1 roots through the
-// nodes inside the chunk (arrival bits in LDS, boxes in global memory, workgroup-scope fences).
-__global__ __launch_bounds__(1024) void k_refit_l2(int n, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
-                                                   const int32_t* __restrict__ parent, const int32_t* __restrict__ rangeLo,
-                                                   const int32_t* __restrict__ rangeHi, float* box, const int32_t* __restrict__ bucket,
-                                                   const int32_t* __restrict__ bucketCount, int32_t* __restrict__ roots,
-                                                   int32_t* __restrict__ rootCount, int32_t* __restrict__ status) {
-    __shared__ uint32_t arrive[kChunk2 / 32];
-    const int cs = blockIdx.x * kChunk2, ce = min(cs + kChunk2, n);
-    for (int w = threadIdx.x; w < kChunk2 / 32; w += 1024) arrive[w] = 0;
-    __syncthreads();
-    const int cnt = min(bucketCount[blockIdx.x], kBucketCap);
-    for (int e = threadIdx.x; e < cnt; e += 1024) {
-        int cur = bucket[(size_t)blockIdx.x * kBucketCap + e];
-        int node = parent[cur];
-        int step = 0;
-        for (; step < 128 && node >= 0 && node < n - 1; step++) {
-            if (rangeLo[node] < cs || rangeHi[node] >= ce) {
-                int slot = atomicAdd(rootCount, 1);
-                if (slot < n) roots[slot] = cur;
-                else atomicOr(status, kStatusRoots);
+        for (int q = 0; q < 6; q++) o[q] = start.box[q];
+        int dL = start.a, dR = start.b;
+        bool asLeft = dR > dL;
+        int p = asLeft ? r : l - 1;
+        // the slot and record of the next split are loaded before the stores of the node just
+        // completed: on CDNA a load's wait also waits for the stores issued before it
+        int32_t g = p >= cs ? slots[p] : -1;
+        ClimbRec pr = p >= cs ? pend[p] : ClimbRec{};
+        for (int step = 0;; step++) {
+            if (step == kMaxClimb) {
+                atomicOr(status, kStatusClimb);
                 break;
             }
-            const int j = node - cs;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            uint32_t old = atomicOr(&arrive[j >> 5], 1u << (j & 31));
-            if (!((old >> (j & 31)) & 1u)) break;
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            float a[6], c[6], o[6];
-            load_box(box, left[node], a);
-            load_box(box, right[node], c);
-            union_box(a, c, o);
-            store_box(box, node, o);
-            if (node == 0) break;
-            cur = node;
-            node = parent[node];
+            if (p < cs) {  // l == cs: this group's root
+                sEnd = r;
+                store_rec(&sRec, o, dL, dR);
+                break;
+            }
+            float ob[6];
+            int other, od;
+            if (g >= 0) {  // the other child arrived in an earlier level
+                other = g;
+#pragma unroll
+                for (int q = 0; q < 6; q++) ob[q] = pr.box[q];
+                od = pr.a;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 6; q++) recBox[j][q] = o[q];
+                recEnd[j] = asLeft ? l : r;
+                recDelta[j] = asLeft ? dL : dR;
+                LTS_LDS_ORDER();
+                const unsigned long long key = ((unsigned long long)(p + 1) << 32) | (unsigned)j;
+                uint32_t h = ((uint32_t)p * 0x9E3779B1u) >> (32 - kHashBits);
+                int found = -1;
+                bool first = false;
+                for (int probe = 0; probe < kHashCap; probe++) {
+                    const unsigned long long old = atomicCAS(&hash[h], 0ull, key);
+                    if (old == 0ull) { first = true; break; }
+                    if ((old >> 32) == (unsigned long long)(p + 1)) { found = (int)(old & 0xffffffffu); break; }
+                    h = (h + 1) & (kHashCap - 1);
+                }
+                if (first) break;
+                if (found < 0) {  // more arrivals than entries: never reached (kHashCap > kClimbs)
+                    atomicOr(status, kStatusClimb);
+                    break;
+                }
+                LTS_LDS_ORDER();
+                hash[h] = ((unsigned long long)(p + 1) << 32) | 0xffffffffu;  // matched
+#pragma unroll
+                for (int q = 0; q < 6; q++) ob[q] = recBox[found][q];
+                other = recEnd[found];
+                od = recDelta[found];
+            }
+            const int L = asLeft ? l : other, R = asLeft ? other : r;
+            const int lc = (L == p) ? (n - 1) + p : p;
+            const int rc = (R == p + 1) ? (n - 1) + p + 1 : p + 1;
+            float un[6];
+            if (asLeft) union_box(o, ob, un);
+            else union_box(ob, o, un);
+#pragma unroll
+            for (int q = 0; q < 6; q++) o[q] = un[q];
+            if (asLeft) dR = od;
+            else dL = od;
+            const bool root = dL < 0 && dR < 0;
+            const int me = root ? 0 : (dR > dL ? R : L);
+            l = L;
+            r = R;
+            if (!root) {
+                asLeft = dR > dL;
+                p = asLeft ? r : l - 1;
+                if (p >= cs) {
+                    g = slots[p];
+                    pr = pend[p];
+                }
+            }
+            store_box(box, me, o);
+            left[me] = lc;
+            right[me] = rc;
+            parent[lc] = me;
+            parent[rc] = me;
+            if (root) break;
         }
-        if (step == 128) atomicOr(status, kStatusClimb);
     }
-}
-
-// Level 3: nodes whose leaf range crosses a level-2 chunk boundary, climbed from the level-2 roots
-// with device-scope ordering.
-__global__ __launch_bounds__(kBlock) void k_refit_global(int n, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
-                                                         const int32_t* __restrict__ parent, float* box, int32_t* flags,
-                                                         const int32_t* __restrict__ roots, const int32_t* __restrict__ rootCount,
-                                                         int32_t* __restrict__ status) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    const int count = *rootCount;
-    if (t == 0 && count > (int)(gridDim.x * kBlock)) atomicOr(status, kStatusLevel3);
-    if (t >= count || t >= n) return;
-    int node = parent[roots[t]];
-    for (int step = 0;; step++) {
-        if (node < 0 || node >= n - 1) return;
-        if (step == 128) { atomicOr(status, kStatusClimb); return; }
-        __threadfence();
-        int old = __hip_atomic_fetch_add(&flags[node], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) return;
-        __threadfence();
-        float a[6], c[6], o[6];
-        int l = left[node], r = right[node];
-#pragma unroll
-        for (int q = 0; q < 6; q++) {
-            a[q] = __hip_atomic_load(&box[6 * (size_t)l + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            c[q] = __hip_atomic_load(&box[6 * (size_t)r + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    // first arrivals still waiting: published for the next level
+    for (int i = j; i < kHashCap; i += kClimbs) {
+        const unsigned long long e = hash[i];
+        const uint32_t c = (uint32_t)(e & 0xffffffffu);
+        if (e != 0ull && c != 0xffffffffu) {
+            const int p = (int)(e >> 32) - 1;
+            slots[p] = recEnd[c];
+            store_rec(&pend[p], recBox[c], recDelta[c], 0);
         }
-        union_box(a, c, o);
-#pragma unroll
-        for (int q = 0; q < 6; q++) __hip_atomic_store(&box[6 * (size_t)node + q], o[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (node == 0) return;
-        node = parent[node];
+    }
+    if (j == 0 && outEnd) {
+        outEnd[blockIdx.x] = sEnd;
+        if (sEnd >= 0) outRec[blockIdx.x] = sRec;
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_pack_nodes(const float* __restrict__ box, const int32_t* __restrict__ left,
-                                                       const int32_t* __restrict__ right, int n, BvhNode* __restrict__ nodes) {
+                                                       const int32_t* __restrict__ right, int n, BvhNode* __restrict__ nodes,
+                                                       int32_t* __restrict__ status) {
     int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n - 1) return;
     BvhNode nd;
     int l = left[i], r = right[i];
+    if (l < 1 || l > 2 * n - 2 || r < 1 || r > 2 * n - 2) {  // only a broken build gets here
+        atomicOr(status, kStatusTree);
+        l = r = 2 * n - 2;
+    }
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         nd.lmin[a] = box[6 * (size_t)l + a]; nd.lmax[a] = box[6 * (size_t)l + 3 + a];
@@ -537,6 +793,8 @@ __global__ __launch_bounds__(kBlock) void k_pack_tris(const float* __restrict__ 
 }
 
 inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
+// blocks of the scene-bound kernel (a grid-stride loop: few partial bounds to reduce)
+inline int bound_blocks(long n) { return std::min(blocks_for(n), 2048); }
 
 // ---------------------------------------------------------------- 4-wide render BVH
 constexpr int kScanBlock = 1024;
@@ -898,22 +1156,23 @@ static hipError_t dalloc(T** p, size_t count) {
 hipError_t bvh_reserve(BvhBuffers& b, int n) {
     if (n <= b.cap) return hipSuccess;
     bvh_release(b);
-    int tiles = (n + kTile - 1) / kTile;
     hipError_t e = hipSuccess;
 #define TRY(x) do { e = (x); if (e != hipSuccess) { bvh_release(b); return e; } } while (0)
     TRY(dalloc(&b.primBox, 6 * (size_t)n));
     TRY(dalloc(&b.box, 6 * (size_t)(2 * n - 1)));
     TRY(dalloc(&b.keysA, n)); TRY(dalloc(&b.keysB, n));
     TRY(dalloc(&b.valsA, n)); TRY(dalloc(&b.valsB, n));
-    TRY(dalloc(&b.counts, 256 * (size_t)tiles));
-    TRY(dalloc(&b.digitBase, 256));
+    TRY(dalloc(&b.counts, (size_t)kBins * sort_tiles(n)));
+    TRY(dalloc(&b.segBase, (size_t)kBins * sort_segments(n)));
     TRY(dalloc(&b.left, n)); TRY(dalloc(&b.right, n));
     TRY(dalloc(&b.parent, 2 * (size_t)n));
     TRY(dalloc(&b.flags, n));
-    TRY(dalloc(&b.rangeLo, n)); TRY(dalloc(&b.rangeHi, n));
-    TRY(dalloc(&b.roots, n));
-    TRY(dalloc(&b.rootCount, (n + kChunk2 - 1) / kChunk2 + 1));
-    TRY(dalloc(&b.bucket, (size_t)((n + kChunk2 - 1) / kChunk2) * kBucketCap));
+    TRY(dalloc(&b.slots, n));
+    TRY(dalloc(&b.chunkRoot, (n + kChunk - 1) / kChunk));
+    TRY(dalloc(&b.pend, n));
+    TRY(dalloc(&b.chunkRec, (n + kChunk - 1) / kChunk));
+    TRY(dalloc(&b.root2, (n + kChunk2 - 1) / kChunk2));
+    TRY(dalloc(&b.rec2, (n + kChunk2 - 1) / kChunk2));
     TRY(dalloc(&b.partial, 6 * (size_t)blocks_for(n)));
     TRY(dalloc(&b.sceneKeys, 8));
     TRY(dalloc(&b.status, 1));
@@ -1038,7 +1297,7 @@ hipError_t ploc_enqueue(BvhBuffers& b, hipStream_t s, int maxDepth, int* depthOu
     TRY(dalloc(&keep, n)); TRY(dalloc(&merge, n));
     TRY(dalloc(&sums, 2 * (size_t)nbMax + 2)); TRY(dalloc(&totals, 4));
     TRY(hipHostMalloc((void**)&host, 4 * sizeof(int32_t), hipHostMallocDefault));
-    k_ploc_init<<<blocks_for(n), kBlock, 0, s>>>(b.primBox, b.valsA, n, box, clA, parent);
+    k_ploc_init<<<blocks_for(n), kBlock, 0, s>>>(b.lastPrimBox, b.valsA, n, box, clA, parent);
     int m = n, created = 0;
     for (int round = 0; m > 1; round++) {
         if (round > n) { cleanup(); return hipErrorUnknown; }  // unreachable: every round merges
@@ -1065,7 +1324,7 @@ hipError_t ploc_enqueue(BvhBuffers& b, hipStream_t s, int maxDepth, int* depthOu
     *depthOut = host[2];
     if (host[2] <= maxDepth) {
         // adopt the tree: the traversal layout and the parents the 4-wide collapse climbs
-        k_pack_nodes<<<blocks_for(n - 1), kBlock, 0, s>>>(box, left, right, n, b.nodes);
+        k_pack_nodes<<<blocks_for(n - 1), kBlock, 0, s>>>(box, left, right, n, b.nodes, b.status);
         TRY(hipMemcpyAsync(b.parent, parent, (size_t)(2 * n - 1) * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
         TRY(hipGetLastError());
         TRY(hipStreamSynchronize(s));
@@ -1076,8 +1335,8 @@ hipError_t ploc_enqueue(BvhBuffers& b, hipStream_t s, int maxDepth, int* depthOu
 }
 
 void bvh_release(BvhBuffers& b) {
-    void* ps[] = {b.primBox, b.box, b.keysA, b.keysB, b.valsA, b.valsB, b.counts, b.digitBase, b.left, b.right,
-                  b.parent, b.flags, b.sceneKeys, b.nodes, b.tris, b.rangeLo, b.rangeHi, b.roots, b.rootCount, b.bucket, b.partial,
+    void* ps[] = {b.primBox, b.box, b.keysA, b.keysB, b.valsA, b.valsB, b.counts, b.segBase,
+                  b.left, b.right, b.parent, b.flags, b.sceneKeys, b.nodes, b.tris, b.slots, b.pend, b.chunkRoot, b.chunkRec, b.root2, b.rec2, b.partial,
                   b.nodes4, b.index4, b.scanTotals, b.status};
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -1087,49 +1346,51 @@ void bvh_release(BvhBuffers& b) {
 hipError_t bvh_enqueue(BvhBuffers& b, const float* soup, int n, int mode, bool pack, hipStream_t s, BvhStageEvents* ev) {
     if (n < 1 || n > b.cap) return hipErrorInvalidValue;
     b.n = n;
-    b.tiles = (n + kTile - 1) / kTile;
+    b.tiles = sort_tiles(n);
     if (ev) (void)hipEventRecord(ev->e[0], s);
-    k_prim_bounds<<<blocks_for(n), kBlock, 0, s>>>(soup, n, b.primBox, b.partial, b.status);
-    k_reduce_scene_keys<<<1, 1024, 0, s>>>(b.partial, blocks_for(n), b.sceneKeys);
-    k_morton<<<blocks_for(n), kBlock, 0, s>>>(b.primBox, n, b.sceneKeys, b.keysA, b.valsA);
+    // primitive boxes: in mode 0 leaf k holds primitive k, so they go straight to the leaf section
+    // of the node bounds
+    float* pb = mode == 0 ? b.box + 6 * (size_t)(n - 1) : b.primBox;
+    b.lastPrimBox = pb;
+    const int boundBlocks = bound_blocks(n);
+    k_prim_bounds<<<boundBlocks, kBlock, 0, s>>>(soup, n, pb, b.partial, b.status);
+    k_reduce_scene_keys<<<1, 1024, 0, s>>>(b.partial, boundBlocks, b.sceneKeys);
+    k_morton<<<b.tiles, kSortBlock, 0, s>>>(pb, n, b.sceneKeys, b.keysB, b.counts);
     if (ev) (void)hipEventRecord(ev->e[1], s);
-    uint32_t* kin = b.keysA; uint32_t* kout = b.keysB;
-    int32_t* vin = b.valsA; int32_t* vout = b.valsB;
-    for (int pass = 0; pass < 4; pass++) {
-        int shift = 8 * pass;
-        k_radix_count<<<b.tiles, kBlock, 0, s>>>(kin, n, shift, b.tiles, b.counts);
-        k_radix_scan_tiles<<<256, kBlock, 0, s>>>(b.counts, b.tiles, b.digitBase);
-        k_radix_scan_digits<<<1, kBlock, 0, s>>>(b.digitBase);
-        k_radix_scatter<<<b.tiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, shift, b.tiles, b.counts, b.digitBase);
-        uint32_t* tk = kin; kin = kout; kout = tk;
-        int32_t* tv = vin; vin = vout; vout = tv;
+    // B -> A (values: input positions), A -> B, B -> A: the sorted pairs end in keysA / valsA
+    uint32_t* kin = b.keysB; uint32_t* kout = b.keysA;
+    int32_t* vin = nullptr; int32_t* vout = b.valsA;
+    const int segs = sort_segments(n);
+    for (int pass = 0; pass < kSortPasses; pass++) {
+        const int shift = kSortBits * pass;
+        if (pass > 0) k_radix_count<<<b.tiles, kSortBlock, 0, s>>>(kin, n, shift, b.counts);
+        k_scan_segments<<<segs, kSortBlock, 0, s>>>(b.counts, b.tiles, b.segBase);
+        k_scan_digits<<<1, kBins, 0, s>>>(b.segBase, segs);
+        if (pass == 0)
+            k_radix_scatter<true><<<b.tiles, kSortBlock, 0, s>>>(kin, vin, kout, vout, n, shift, b.tiles, b.counts,
+                                                                 b.segBase, b.status);
+        else
+            k_radix_scatter<false><<<b.tiles, kSortBlock, 0, s>>>(kin, vin, kout, vout, n, shift, b.tiles, b.counts,
+                                                                  b.segBase, b.status);
+        std::swap(kin, kout);
+        vin = vout;
+        vout = (vout == b.valsA) ? b.valsB : b.valsA;
     }
-    // after 4 passes the sorted arrays are back in keysA / valsA
     if (ev) (void)hipEventRecord(ev->e[2], s);
-    if (n > 1) {
-        k_fill_i32<<<blocks_for(n - 1), kBlock, 0, s>>>(b.flags, n - 1, 0);
-        k_fill_i32<<<blocks_for(2 * n - 1), kBlock, 0, s>>>(b.parent, 2 * n - 1, -1);
-        k_karras<<<blocks_for(n - 1), kBlock, 0, s>>>(b.keysA, n, b.left, b.right, b.parent, b.rangeLo, b.rangeHi);
-    } else {
-        k_fill_i32<<<1, kBlock, 0, s>>>(b.parent, 1, -1);
-    }
-    const int chunks = (n + kChunk - 1) / kChunk, chunks2 = (n + kChunk2 - 1) / kChunk2;
-    k_fill_i32<<<blocks_for(chunks2 + 1), kBlock, 0, s>>>(b.rootCount, chunks2 + 1, 0);  // level-3 count + buckets
-    k_refit_l1<<<chunks, kChunk, 0, s>>>(b.primBox, b.valsA, n, mode, b.left, b.right, b.parent, b.rangeLo, b.rangeHi, b.box,
-                                         b.bucket, b.rootCount + 1, b.status);
+    const int chunks = (n + kChunk - 1) / kChunk;
+    k_tree_l1<<<chunks, kChunk, 0, s>>>(b.keysA, pb, b.valsA, n, mode, b.left, b.right, b.parent, b.box, b.slots,
+                                        b.pend, b.chunkRoot, b.chunkRec, b.status);
+    const int chunks2 = (n + kChunk2 - 1) / kChunk2;
     if (chunks > 1)
-        k_refit_l2<<<chunks2, 1024, 0, s>>>(n, b.left, b.right, b.parent, b.rangeLo, b.rangeHi, b.box, b.bucket,
-                                            b.rootCount + 1, b.roots, b.rootCount, b.status);
-    if (chunks2 > 1) {
-        // level-3 roots: at most two per level-2-crossing node, and each of the chunks2 - 1
-        // boundaries is crossed by at most 58 nodes
-        const long bound = std::min<long>((long)n, 2L * chunks2 * 58);
-        k_refit_global<<<blocks_for(bound), kBlock, 0, s>>>(n, b.left, b.right, b.parent, b.box, b.flags, b.roots, b.rootCount,
-                                                            b.status);
-    }
+        k_tree_up<kGroup2, 2 * kGroup2><<<chunks2, kGroup2, 0, s>>>(n, chunks, kChunk, b.chunkRoot, b.chunkRec, b.left, b.right,
+                                                                   b.parent, b.box, b.slots, b.pend, b.root2, b.rec2,
+                                                                   b.status);
+    if (chunks2 > 1)
+        k_tree_up<kGroup3, 2 * kGroup3><<<1, kGroup3, 0, s>>>(n, chunks2, kChunk2, b.root2, b.rec2, b.left, b.right, b.parent,
+                                                             b.box, b.slots, b.pend, nullptr, nullptr, b.status);
     if (ev) (void)hipEventRecord(ev->e[3], s);
     if (pack) {
-        if (n > 1) k_pack_nodes<<<blocks_for(n - 1), kBlock, 0, s>>>(b.box, b.left, b.right, n, b.nodes);
+        if (n > 1) k_pack_nodes<<<blocks_for(n - 1), kBlock, 0, s>>>(b.box, b.left, b.right, n, b.nodes, b.status);
         k_pack_tris<<<blocks_for(n), kBlock, 0, s>>>(soup, b.valsA, n, b.tris);
     }
     if (ev) (void)hipEventRecord(ev->e[4], s);
@@ -1156,7 +1417,6 @@ size_t carve(size_t& at, size_t bytes) {
 
 // The scratch layout of an n-primitive build; `base` null measures it (the returned size).
 size_t static_layout(BvhBuffers& b, unsigned char* base, int n) {
-    const int tiles = (n + kTile - 1) / kTile, chunks2 = (n + kChunk2 - 1) / kChunk2;
     size_t at = 0;
     auto take = [&](auto*& ptr, size_t count) {
         const size_t o = carve(at, count * sizeof(*ptr));
@@ -1165,14 +1425,14 @@ size_t static_layout(BvhBuffers& b, unsigned char* base, int n) {
     take(b.primBox, 6 * (size_t)n);
     take(b.keysB, n);
     take(b.valsB, n);
-    take(b.counts, 256 * (size_t)tiles);
-    take(b.digitBase, 256);
-    take(b.flags, n);
-    take(b.rangeLo, n);
-    take(b.rangeHi, n);
-    take(b.roots, n);
-    take(b.rootCount, chunks2 + 1);
-    take(b.bucket, (size_t)chunks2 * kBucketCap);
+    take(b.counts, (size_t)kBins * sort_tiles(n));
+    take(b.segBase, (size_t)kBins * sort_segments(n));
+    take(b.slots, n);
+    take(b.chunkRoot, (n + kChunk - 1) / kChunk);
+    take(b.pend, n);
+    take(b.chunkRec, (n + kChunk - 1) / kChunk);
+    take(b.root2, (n + kChunk2 - 1) / kChunk2);
+    take(b.rec2, (n + kChunk2 - 1) / kChunk2);
     take(b.partial, 6 * (size_t)blocks_for(n));
     take(b.sceneKeys, 8);
     take(b.status, 1);
@@ -1182,10 +1442,11 @@ size_t static_layout(BvhBuffers& b, unsigned char* base, int n) {
 }  // namespace
 
 // The largest build (kStaticMaxPrims) must fit the buffer.
-static_assert(kStaticScratchBytes >= 256 * 20 + 6 * 4 * (size_t)kStaticMaxPrims + 8 * 4 * (size_t)kStaticMaxPrims +
-                                         256 * 4 * ((kStaticMaxPrims + kTile - 1) / kTile) + 4 * 256 +
-                                         4 * ((kStaticMaxPrims + kChunk2 - 1) / kChunk2 + 1) +
-                                         4 * (size_t)((kStaticMaxPrims + kChunk2 - 1) / kChunk2) * kBucketCap +
+static_assert(kStaticScratchBytes >= 256 * 16 + (4 * (6 + 1 + 1 + 1) + sizeof(ClimbRec)) * (size_t)kStaticMaxPrims +
+                                         4 * (size_t)kBins * ((kStaticMaxPrims + kSortTile - 1) / kSortTile) +
+                                         4 * (size_t)kBins * ((kStaticMaxPrims + kSortTile * kSegTiles - 1) / (kSortTile * kSegTiles)) +
+                                         (4 + sizeof(ClimbRec)) * ((kStaticMaxPrims + kChunk - 1) / kChunk) +
+                                         (4 + sizeof(ClimbRec)) * ((kStaticMaxPrims + kChunk2 - 1) / kChunk2) +
                                          4 * 6 * ((kStaticMaxPrims + kBlock - 1) / kBlock) + 4 * 8 + 4,
               "static LBVH scratch too small for kStaticMaxPrims");
 

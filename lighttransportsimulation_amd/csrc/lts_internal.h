@@ -17,31 +17,40 @@ inline uint64_t next_buffer_generation() {
     return ++gen;
 }
 
+// A node handed between the levels of the bottom-up hierarchy: its box and two deltas (a chunk
+// root: the deltas at both ends; a first arrival waiting at a split: the delta at its outer end).
+struct alignas(32) ClimbRec {
+    float box[6];
+    int32_t a, b;
+};
+
 // Device buffers of one LBVH build (sized for `cap` primitives; reused across builds).
 struct BvhBuffers {
     int cap = 0;
     int n = 0;
     int tiles = 0;
-    float* primBox = nullptr;     // 6n    primitive bounds
+    float* primBox = nullptr;     // 6n    primitive bounds (mode 1 builds)
+    const float* lastPrimBox = nullptr;  // where the last build put the primitive bounds
     float* box = nullptr;         // 6(2n-1) node bounds, unified index
     uint32_t* keysA = nullptr;    // n
     uint32_t* keysB = nullptr;
     int32_t* valsA = nullptr;
     int32_t* valsB = nullptr;
-    uint32_t* counts = nullptr;   // 256 * tiles
-    uint32_t* digitBase = nullptr;// 256
+    uint32_t* counts = nullptr;   // tiles x 1024: a radix pass's digit counts per tile, then prefixes in the segment
+    uint32_t* segBase = nullptr;  // segments x 1024: digit totals per 16-tile segment, then the digits' bases
     int32_t* left = nullptr;      // n-1
     int32_t* right = nullptr;     // n-1
     int32_t* parent = nullptr;    // 2n-1
-    int32_t* flags = nullptr;     // n-1
-    int32_t* rangeLo = nullptr;   // n-1 first sorted leaf under interior i
-    int32_t* rangeHi = nullptr;   // n-1 last sorted leaf under interior i
-    int32_t* roots = nullptr;     // phase-1 subtree roots handed to the global refit
-    int32_t* rootCount = nullptr;  // [0] level-3 roots, [1 + c] level-1 roots filed for 64K-leaf chunk c
-    int32_t* bucket = nullptr;     // level-1 roots per 64K-leaf chunk
+    int32_t* flags = nullptr;     // n-1 (4-wide collapse)
+    int32_t* slots = nullptr;     // n-1 arrival slot per split of the bottom-up hierarchy
+    ClimbRec* pend = nullptr;     // n-1: the node waiting at a split (written where a slot is set)
+    int32_t* chunkRoot = nullptr; // per 512-leaf chunk: last leaf of the node its climbs handed on, or -1
+    ClimbRec* chunkRec = nullptr; // per 512-leaf chunk: that node
+    int32_t* root2 = nullptr;     // the same per 65536-leaf group (level 2 to level 3)
+    ClimbRec* rec2 = nullptr;
     uint32_t* partial = nullptr;   // per-block scene-bound keys
     uint32_t* sceneKeys = nullptr;// 6 ordered keys of the scene bounds
-    int32_t* status = nullptr;    // refit capacity violations of the last build (bits; 0 = none)
+    int32_t* status = nullptr;    // failed bounded waits of the last build (bits; 0 = none)
     BvhNode* nodes = nullptr;     // n-1 (or 1 for n == 1)
     TriLeaf* tris = nullptr;      // n
     BvhQNode* nodes4 = nullptr;   // 4-wide collapse for rendering (allocated by bvh4_enqueue)
@@ -72,7 +81,7 @@ struct BvhStageEvents {
 // Build without allocation (lts_bvh_build_static): all temporaries in one static device buffer of
 // kStaticScratchBytes per device, enough for kStaticMaxPrims primitives.
 constexpr int kStaticMaxPrims = 1 << 18;
-constexpr size_t kStaticScratchBytes = (size_t)16 << 20;
+constexpr size_t kStaticScratchBytes = (size_t)24 << 20;
 constexpr int kStaticMaxDevices = 64;
 // The buffer's device address on `device` (the current device), looked up once per device.
 hipError_t bvh_static_scratch(int device, void** ptr);

@@ -118,6 +118,30 @@ def test_autotune_keeps_the_image(gpu_ctx, displaced):
 
 
 @pytest.mark.gpu
+def test_autotuned_tuning_is_not_slower(gpu_ctx, tmp_path):
+    """The tuning the autotuner keeps renders a 480x270 16-spp frame of a 160k-triangle displaced
+    sphere at least about as fast as the default tuning (median render time of 7 frames each; the
+    autotuner re-times its choice against the default and keeps the default unless the choice wins,
+    so only timing noise separates them)."""
+    cfg = scenegen.gen_displaced_sphere(str(tmp_path), width=480, height=270, spp=16, n_lat=200, n_lon=400,
+                                        image_size=256)
+    gpu_ctx.load_scene(str(tmp_path), cfg["scene"], cfg["subject"], 480, 270)
+    kw = dict(integrator=INTEGRATOR_WAVEFRONT, want_image=False)
+    print("[wavefront autotune speed]", flush=True)
+    _, _, tuned = gpu_ctx.render(16, 8, 3, SEED, autotune=1, **kw)
+    pick = {k: tuned[k] for k in ("trace_block", "persistent_blocks", "sort_materials", "tile_rows")}
+    gpu_ctx.render(16, 8, 3, SEED, **kw)  # graph capture of the default tuning
+
+    def median_ms(**extra):
+        return float(np.median([gpu_ctx.render(16, 8, 3, SEED, **kw, **extra)[2]["render_ms"] for _ in range(7)]))
+
+    default_ms = median_ms()
+    tuned_ms = median_ms(autotune=1)
+    print(f"default {default_ms:.3f} ms, tuned {tuned_ms:.3f} ms, choice {pick}", flush=True)
+    assert tuned_ms <= 1.10 * default_ms
+
+
+@pytest.mark.gpu
 def test_tile_rows_keep_the_image(gpu_ctx, displaced):
     """The band height of the path order changes which paths share a wave, never a path's result:
     every band height, including one that does not divide the image height (54 rows) and one above

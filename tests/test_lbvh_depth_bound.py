@@ -1,7 +1,8 @@
 """The depth bound the device refit relies on, checked on adversarial inputs with the oracle (no GPU).
 
-The refit provides for at most 58 nodes on a root-to-leaf path of the Karras tree (kBucketCap and
-the level-3 climber count in bvh.hip; a violation sets a status bit and the build fails). The bound
+The bottom-up hierarchy bounds every climb above its first level at 64 steps (kMaxClimb in bvh.hip;
+a violation sets a status bit and the build fails), so it relies on at most 56 interior nodes on a
+root-to-leaf path of the Karras tree. The bound
 follows from delta (BVH.cuh:173-190) strictly increasing down any path: distinct 30-bit codes give
 delta in [2, 31] and equal codes 40 + clz(i ^ j) in [46, 71] for n <= 2^26, so at most 30 + 26 = 56
 interior nodes lie on a path. Here a comb of single-bit Morton codes (every prefix length used)
