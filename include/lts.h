@@ -206,6 +206,16 @@ int lts_bvh_build_device(lts_ctx* ctx, const float* d_tri_xyz, int32_t n, int32_
 /* Copies the arrays of the most recent build to host memory. */
 int lts_bvh_export_last(lts_ctx* ctx, lts_bvh_export* out);
 
+/* Replaces the world-transform loops of the TriangleMesh constructor (shapes/Triangle.cu:26-41):
+ * transforms n_points points and n_normals normals (3 floats each, host memory) in place on the
+ * context's GPU. `subject` holds the scene file's subject transform (core/SceneLoader.cuh:33-193):
+ * translation (3), rotation axis (3), angle in radians (taken modulo 2 pi), uniform scale. Points
+ * map to s q p q* + t with q = (cos a/2, axis sin a/2); normals to normalize(q* n q), normalised
+ * with the reference's fast inverse square root. lts_load_scene applies the same kernel to every
+ * mesh (identity for the scene's own meshes). */
+int lts_mesh_transform(lts_ctx* ctx, const float* subject, float* points, int32_t n_points, float* normals,
+                       int32_t n_normals);
+
 /* No counterpart in the reference: the LBVH build for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Builds the
  * tree lts_bvh_build_device builds (mode 0 or 1) for a device soup of n triangles, 1 <= n <=

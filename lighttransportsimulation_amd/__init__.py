@@ -25,7 +25,7 @@ EXPORTED_SYMBOLS = (
     "lts_write_image", "lts_device_copy_benchmark", "lts_render_batch", "lts_nrc_configure", "lts_nrc_set_weights",
     "lts_nrc_get_weights", "lts_nrc_infer", "lts_nrc_train_step", "lts_set_render_tree", "lts_render_tree_info",
     "lts_obj_to_bin", "lts_film_save", "lts_film_load", "lts_film_resolve", "lts_set_texture_filter",
-    "lts_load_image", "lts_bvh_build_static",
+    "lts_load_image", "lts_bvh_build_static", "lts_mesh_transform",
 )
 
 TEXTURE_REFERENCE = 0
@@ -116,6 +116,7 @@ def load_library():
     L.lts_obj_to_bin.argtypes = [C.c_char_p, C.POINTER(i32)]
     L.lts_load_image.argtypes = [C.c_char_p, C.POINTER(i32), C.POINTER(i32), vp, C.c_int64]
     L.lts_bvh_build_static.argtypes = [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.lts_mesh_transform.argtypes = [vp, vp, vp, i32, vp, i32]
     L.lts_film_save.argtypes = [vp, C.c_char_p]
     L.lts_film_load.argtypes = [vp, C.c_char_p]
     L.lts_film_resolve.argtypes = [vp, vp]
@@ -237,6 +238,17 @@ class Context:
                                                   repeat, C.byref(t)),
                     "lts_bvh_build_device")
         return dict(total_ms=t.total_ms, sort_ms=t.sort_ms, bounds_ms=t.bounds_ms, tree_ms=t.tree_ms)
+
+    def mesh_transform(self, subject, points, normals):
+        """World transform of a mesh on the GPU (include/lts.h: lts_mesh_transform). `subject` is the
+        scene file's 8 subject tokens (translation, axis, angle in radians, scale); returns the
+        transformed (points, normals) as new [n][3] float32 arrays."""
+        sub = np.ascontiguousarray(subject, np.float32).reshape(8)
+        pts = np.array(points, np.float32, copy=True).reshape(-1, 3)
+        nrm = np.array(normals, np.float32, copy=True).reshape(-1, 3)
+        self._check(self.lib.lts_mesh_transform(self.h, _ptr(sub), _ptr(pts), pts.shape[0], _ptr(nrm), nrm.shape[0]),
+                    "lts_mesh_transform")
+        return pts, nrm
 
     def bvh_export_last(self, n: int) -> dict:
         arrs = bvh_arrays(n)

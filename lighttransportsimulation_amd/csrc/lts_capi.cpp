@@ -105,6 +105,8 @@ struct lts_ctx {
     size_t nrcIoCap = 0;        // records
     int32_t* nrcCount = nullptr;  // device record count for lts_nrc_infer
     uint64_t nrcPasses = 0;       // cached passes rendered (moves the training-pixel offset)
+    float* xfBuf = nullptr;       // points and normals of a mesh being transformed
+    size_t xfCap = 0;             // floats
 };
 
 namespace {
@@ -808,7 +810,7 @@ void lts_destroy(lts_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     free_scene(c->scene);
     bvh_release(c->build);
-    void* ps[] = {c->buildSoup, c->sampleL, c->film, c->rgb, c->counters, c->perm};
+    void* ps[] = {c->buildSoup, c->sampleL, c->film, c->rgb, c->counters, c->perm, c->xfBuf};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev)
@@ -851,6 +853,31 @@ int lts_read_app_params(const char* path, int32_t* w, int32_t* h, int32_t* spp, 
     return LTS_OK;
 }
 
+// Transforms nP points and nN normals (host arrays, 3 floats each) in place on c's device.
+static hipError_t transform_on_device(lts_ctx* c, const MeshXf& xf, float* P, size_t nP, float* N, size_t nN) {
+    const size_t need = 3 * (nP + nN);
+    if (need == 0) return hipSuccess;
+    if (nP > (size_t)INT32_MAX || nN > (size_t)INT32_MAX) return hipErrorInvalidValue;
+    if (need > c->xfCap) {
+        if (c->xfBuf) (void)hipFree(c->xfBuf);
+        c->xfBuf = nullptr;
+        c->xfCap = 0;
+        hipError_t e = hipMalloc((void**)&c->xfBuf, need * sizeof(float));
+        if (e != hipSuccess) return e;
+        c->xfCap = need;
+    }
+    float* dP = c->xfBuf;
+    float* dN = c->xfBuf + 3 * nP;
+    hipError_t e = hipSuccess;
+    if (nP) e = hipMemcpyAsync(dP, P, 3 * nP * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && nN) e = hipMemcpyAsync(dN, N, 3 * nN * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = mesh_transform(xf, dP, (int)nP, dN, (int)nN, c->stream);
+    if (e == hipSuccess && nP) e = hipMemcpyAsync(P, dP, 3 * nP * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && nN) e = hipMemcpyAsync(N, dN, 3 * nN * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e;
+}
+
 static int load_scene_impl(lts_ctx* c, const char* dir, const char* scene, const char* subject, int32_t width, int32_t height) {
     if (!c || !dir || !scene) return fail(c, LTS_E_ARG, "null argument");
     HIPCHK(c, hipSetDevice(c->device));
@@ -859,7 +886,14 @@ static int load_scene_impl(lts_ctx* c, const char* dir, const char* scene, const
     std::string err;
     std::string d(dir);
     if (!d.empty() && d.back() != '/') d += '/';
-    if (!load_scene_files(d, scene, subject ? subject : "", width, height, S.host, err, c->textureFilter == LTS_TEXTURE_FILTERED))
+    // every mesh's points and normals go through its world transform on the GPU
+    auto transform = [c](const MeshXf& xf, std::vector<float>& P, std::vector<float>& N, std::string& e) {
+        const hipError_t r = transform_on_device(c, xf, P.data(), P.size() / 3, N.data(), N.size() / 3);
+        if (r != hipSuccess) e = std::string("mesh transform: ") + hipGetErrorString(r);
+        return r == hipSuccess;
+    };
+    if (!load_scene_files(transform, d, scene, subject ? subject : "", width, height, S.host, err,
+                          c->textureFilter == LTS_TEXTURE_FILTERED))
         return fail(c, LTS_E_IO, err);
     const HostScene& H = S.host;
     const int n = (int)H.shade.size();
@@ -1352,6 +1386,16 @@ int lts_bvh_build(lts_ctx* c, const float* tri, int32_t n, int32_t mode, lts_bvh
 
 int lts_bvh_build_device(lts_ctx* c, const float* dtri, int32_t n, int32_t mode, int32_t repeat, lts_bvh_timing* t) {
     return guarded(c, [&] { return bvh_build_device_impl(c, dtri, n, mode, repeat, t); });
+}
+
+int lts_mesh_transform(lts_ctx* c, const float* subject, float* points, int32_t n_points, float* normals, int32_t n_normals) {
+    return guarded(c, [&] {
+        if (!c || !subject || n_points < 0 || n_normals < 0 || (n_points > 0 && !points) || (n_normals > 0 && !normals))
+            return fail(c, LTS_E_ARG, "bad mesh transform arguments");
+        HIPCHK(c, hipSetDevice(c->device));
+        HIPCHK(c, transform_on_device(c, mesh_xf_from_tokens(subject), points, (size_t)n_points, normals, (size_t)n_normals));
+        return LTS_OK;
+    });
 }
 
 int lts_bvh_export_last(lts_ctx* c, lts_bvh_export* out) {

@@ -6,6 +6,7 @@
 #include <atomic>
 
 #include "lts_scene.h"
+#include "mesh_xf.h"
 
 namespace lts {
 
@@ -98,6 +99,9 @@ hipError_t bvh_enqueue_static(void* scratch, const float* soup, int n, int mode,
 // When pack is set, also writes nodes / tris for the traversal kernels (mode 1 layout).
 hipError_t bvh_enqueue(BvhBuffers& b, const float* soup, int n, int mode, bool pack, hipStream_t stream,
                        BvhStageEvents* ev);
+
+// Mesh world transform (mesh.hip): points and normals (3 floats each, device memory) in place.
+hipError_t mesh_transform(const MeshXf& xf, float* P, int nP, float* N, int nN, hipStream_t stream);
 
 // Render launches (render.hip).
 struct RenderLaunch {

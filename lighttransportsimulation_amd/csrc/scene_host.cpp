@@ -48,25 +48,6 @@ float rsqrt_quake(float f) {
 }
 F3 unit(F3 a) { return mul(a, rsqrt_quake(dot3(a, a))); }
 
-// Transform with translation, (unnormalised) quaternion rotation and uniform scale.
-struct Q { float w, x, y, z; };
-Q qm(Q a, Q b) {
-    return Q{a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
-             a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x, a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w};
-}
-struct Xf {
-    F3 t{0, 0, 0};
-    Q r{1, 0, 0, 0};
-    float s = 1;
-    F3 point(F3 p) const {
-        Q o = qm(qm(r, Q{0.0f, p.x, p.y, p.z}), Q{r.w, -r.x, -r.y, -r.z});
-        return add(mul(f3(o.x, o.y, o.z), s), t);
-    }
-    F3 normal(F3 n) const {  // conjugate first, as Transform::operator()(Normal3)
-        Q o = qm(qm(Q{r.w, -r.x, -r.y, -r.z}, Q{0.0f, n.x, n.y, n.z}), r);
-        return unit(f3(o.x, o.y, o.z));
-    }
-};
 
 struct Tokens {
     std::ifstream in;
@@ -94,8 +75,9 @@ struct MeshData {
 };
 
 // parseMesh (core/FileIO.cuh:212-231): <stem>.bin when it exists, else <stem>.obj, which is
-// converted and written to <stem>.bin for the next load (when the directory is writable).
-bool read_mesh(const std::string& stem, const Xf& xf, MeshData& m, std::string& err) {
+// converted and written to <stem>.bin for the next load (when the directory is writable). Points
+// and normals go through `transform` (the mesh's world transform, applied on the GPU).
+bool read_mesh(const std::string& stem, const MeshXf& xf, const MeshTransformFn& transform, MeshData& m, std::string& err) {
     std::vector<char> b;
     if (!read_all(stem + ".bin", b)) {
         int nTri = 0;
@@ -111,16 +93,16 @@ bool read_mesh(const std::string& stem, const Xf& xf, MeshData& m, std::string& 
     size_t need = 16 + 12ull * m.nP + 8ull * m.nUV + 12ull * m.nN + 36ull * m.nTri;
     if (b.size() < need) { err = "truncated mesh " + stem; return false; }
     const char* at = b.data() + 16;
-    std::vector<float> raw(3ull * m.nP);
-    std::memcpy(raw.data(), at, 12ull * m.nP); at += 12ull * m.nP;
-    m.P.resize(m.nP);
-    for (int i = 0; i < m.nP; i++) m.P[i] = xf.point(f3(raw[3 * i], raw[3 * i + 1], raw[3 * i + 2]));
+    std::vector<float> rawP(3ull * m.nP), rawN(3ull * m.nN);
+    std::memcpy(rawP.data(), at, 12ull * m.nP); at += 12ull * m.nP;
     m.UV.resize(2ull * m.nUV);
     std::memcpy(m.UV.data(), at, 8ull * m.nUV); at += 8ull * m.nUV;
-    raw.resize(3ull * m.nN);
-    std::memcpy(raw.data(), at, 12ull * m.nN); at += 12ull * m.nN;
+    std::memcpy(rawN.data(), at, 12ull * m.nN); at += 12ull * m.nN;
+    if (!transform(xf, rawP, rawN, err)) return false;
+    m.P.resize(m.nP);
+    for (int i = 0; i < m.nP; i++) m.P[i] = f3(rawP[3 * i], rawP[3 * i + 1], rawP[3 * i + 2]);
     m.N.resize(m.nN);
-    for (int i = 0; i < m.nN; i++) m.N[i] = xf.normal(f3(raw[3 * i], raw[3 * i + 1], raw[3 * i + 2]));
+    for (int i = 0; i < m.nN; i++) m.N[i] = f3(rawN[3 * i], rawN[3 * i + 1], rawN[3 * i + 2]);
     m.idx.resize(9ull * m.nTri);
     std::memcpy(m.idx.data(), at, 36ull * m.nTri);
     for (size_t k = 0; k < m.idx.size(); k++) {
@@ -332,7 +314,8 @@ void gaussian_filter_table(float* out) {
         }
 }
 
-bool load_scene_files(const std::string& dir, const std::string& sceneName, const std::string& subjectName, int W,
+bool load_scene_files(const MeshTransformFn& transform, const std::string& dir, const std::string& sceneName,
+                      const std::string& subjectName, int W,
                       int H, HostScene& S, std::string& err, bool filteredTextures) {
     if (W < 2 || H < 2) { err = "film must be at least 2x2"; return false; }
     Tokens sc, sb;
@@ -345,16 +328,12 @@ bool load_scene_files(const std::string& dir, const std::string& sceneName, cons
     int subjectMeshes = 0, areaLightMeshes = 0;
     bool hasInfinite = false;
     float cam[11];
-    Xf subjectXf;
+    MeshXf subjectXf;
     try {
         for (float& c : cam) c = sc.f();
         float tr[8];
         for (float& v : tr) v = sc.f();
-        float th = std::fmod(tr[6], 2 * kPi);
-        float sn = (float)std::sin((double)(th / 2));
-        subjectXf.t = f3(tr[0], tr[1], tr[2]);
-        subjectXf.r = Q{(float)std::cos((double)(th / 2)), tr[3] * sn, tr[4] * sn, tr[5] * sn};
-        subjectXf.s = tr[7];
+        subjectXf = mesh_xf_from_tokens(tr);
         // images
         int subjectImages = hasSubject ? sb.i() : 0;
         int nImg = subjectImages + sc.i();
@@ -437,8 +416,8 @@ bool load_scene_files(const std::string& dir, const std::string& sceneName, cons
     // triangles in global order
     for (int m = 0; m < nMesh; m++) {
         MeshData md;
-        Xf identity;
-        if (!read_mesh(meshFiles[m], m < subjectMeshes ? subjectXf : identity, md, err)) return false;
+        const MeshXf identity;
+        if (!read_mesh(meshFiles[m], m < subjectMeshes ? subjectXf : identity, transform, md, err)) return false;
         const bool emissive = m >= nMesh - areaLightMeshes;
         const int le = nMesh - m - 1;  // Le order reversed (SceneLoader_kernels.cu:76-78)
         for (int t = 0; t < md.nTri; t++) {

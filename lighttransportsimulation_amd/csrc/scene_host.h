@@ -1,10 +1,12 @@
 // Host-side scene ingest: reads the reference's input files and produces the flat arrays of
 // lts_scene.h, ready to upload.
 #pragma once
+#include <functional>
 #include <string>
 #include <vector>
 
 #include "lts_scene.h"
+#include "mesh_xf.h"
 
 namespace lts {
 
@@ -25,10 +27,16 @@ struct HostScene {
     int infiniteLight = -1;
 };
 
+// A mesh's world transform applied to its points and normals (3 floats each) in place; false with
+// `err` set on failure. The loader's transform runs on the GPU (mesh_transform).
+using MeshTransformFn = std::function<bool(const MeshXf&, std::vector<float>& points, std::vector<float>& normals,
+                                           std::string& err)>;
+
 // parseScene (core/SceneLoader.cuh:195-360). Returns false with `err` set on failure.
 // filteredTextures: build the filtered mip pyramids instead of the reference's.
-bool load_scene_files(const std::string& dir, const std::string& scene, const std::string& subject, int width,
-                      int height, HostScene& out, std::string& err, bool filteredTextures = false);
+bool load_scene_files(const MeshTransformFn& transform, const std::string& dir, const std::string& scene,
+                      const std::string& subject, int width, int height, HostScene& out, std::string& err,
+                      bool filteredTextures = false);
 
 // OBJ import (core/FileIO.cuh:68-158, parseMeshFile with binMeshWrite): parses <stem>.obj and
 // returns the bytes of the equivalent <stem>.bin (header nTri, nP, nUV, nN; points, uvs, normals;
