@@ -39,6 +39,9 @@ def _compile(src: str) -> tuple[str, int, str]:
     cmd = [HIPCC] + COMMON
     if src.endswith(".hip"):
         cmd += ["-x", "hip"] + DEVICE
+    if src == "nrc.hip":
+        # MFMA results in VGPRs: the network's activations feed the next layer's operands directly
+        cmd += ["-mllvm", "-amdgpu-mfma-vgpr-form"]
     else:
         cmd += ["-D__HIP_PLATFORM_AMD__"]
     cmd += ["-c", path, "-o", _obj(src)]

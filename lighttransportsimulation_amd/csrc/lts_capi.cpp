@@ -1237,7 +1237,7 @@ static int nrc_infer_impl(lts_ctx* c, const float* raw, int32_t n, float* out) {
     float* dOut = c->nrcIo + (size_t)n * kNrcRaw;
     HIPCHK(c, hipMemcpyAsync(dRaw, raw, (size_t)n * kNrcRaw * sizeof(float), hipMemcpyHostToDevice, s));
     HIPCHK(c, hipMemcpyAsync(c->nrcCount, &n, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    HIPCHK(c, nrc_infer(c->nrc, dRaw, c->nrcCount, n, dOut, nullptr, nullptr, s));
+    HIPCHK(c, nrc_infer(c->nrc, dRaw, c->nrcCount, n, dOut, s));
     HIPCHK(c, hipMemcpyAsync(out, dOut, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     return LTS_OK;

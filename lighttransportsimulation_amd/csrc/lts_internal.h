@@ -211,9 +211,8 @@ struct WfBuffers {
     // cached integrator (allocated by wf_reserve_cache)
     size_t cacheCap = 0;          // paths per pass covered by the query arrays
     int trainCap = 0;             // training-path slots
-    float* qRaw = nullptr;        // [cacheCap][16] query record of each querying path
-    NrcQuery* qMeta = nullptr;    // [cacheCap]
-    int32_t* qQueue = nullptr;    // [cacheCap] querying paths
+    float* qRaw = nullptr;        // [cacheCap][16] query records, compact in append order
+    NrcQuery* qMeta = nullptr;    // [cacheCap] the queries' throughput, reflectance, radiance and sample slot
     WfTrainRec* tRec = nullptr;   // [trainCap][kWfMaxBounce]
     float* tRaw = nullptr;        // [trainCap][kWfMaxBounce][16]
     int32_t* tCount = nullptr;    // per-block counts / offsets of the batch compaction
@@ -259,7 +258,9 @@ constexpr int kNrcBatchAlign = 1024;            // batch capacity multiple (4 sp
 constexpr int kNrcMaxSteps = 64;                // Adam steps per training call
 constexpr int kNrcTrainBlocks = 256;
 
-struct NrcQuery { float beta[3]; float a[3]; int32_t slot; int32_t pad; };  // slot: sample index into sampleL
+// A query of the cached pass: the path's throughput, the reflectance at the query vertex, the
+// radiance it gathered before it (L) and its sample slot in sampleL; 48 bytes.
+struct NrcQuery { float beta[3]; float a[3]; float L[3]; int32_t slot; int32_t pad[2]; };
 struct NrcState {
     int32_t t;        // Adam steps taken
     int32_t nTotal;   // samples collected in trainRaw / trainTarget for the next training call
@@ -299,11 +300,12 @@ hipError_t nrc_reserve(NrcBuffers& b, int batchCap, int steps, int chunks);
 void nrc_release(NrcBuffers& b);
 // master -> MFMA fragments (after weights are set)
 hipError_t nrc_pack(const NrcBuffers& b, hipStream_t stream);
-// Evaluates min(*nDev, nCap) raw records (record s, or record index[s] when index is given). With
-// meta == nullptr, out receives 3 floats per record; otherwise sampleL[3 slot + c] +=
-// beta_c * max(y_c, 0) * a_c for each query.
-hipError_t nrc_infer(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out,
-                     const NrcQuery* meta, float* sampleL, hipStream_t stream, const int32_t* index = nullptr);
+// Evaluates min(*nDev, nCap) raw records; out receives 3 floats per record.
+hipError_t nrc_infer(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out, hipStream_t stream);
+// The cached pass's queries (compact records raw / meta): evaluates them and writes each query's
+// sample, sampleL[3 slot + c] = L_c + beta_c * max(y_c, 0) * a_c.
+hipError_t nrc_infer_queries(const NrcBuffers& b, const float* raw, const NrcQuery* meta, const int32_t* nDev, int nCap,
+                             float* sampleL, hipStream_t stream);
 // `steps` Adam steps over the state->nTotal samples in trainRaw / trainTarget: step k trains on batch
 // (k mod ceil(nTotal / batchCap)), so steps beyond the collected batches make further passes.
 hipError_t nrc_train(const NrcBuffers& b, hipStream_t stream, int steps);

@@ -76,6 +76,59 @@ LTS_D float nrc_feature_block(const float* __restrict__ raw, int m, int g) {
     return g < 2 ? raw[12 + g] : 1.0f;
 }
 
+// The fields of a raw record one lane reads (raw[0..7], raw[8 + g], raw[12 + g]), so the next tile's
+// can be loaded while the current one is evaluated.
+struct RawLane { float4 a, b; float c, d; };
+
+LTS_D RawLane load_raw_lane(const float* __restrict__ raw, int g) {
+    RawLane r;
+    const float4* q = reinterpret_cast<const float4*>(raw);
+    r.a = q[0];
+    r.b = q[1];
+    r.c = raw[8 + g];
+    r.d = g < 2 ? raw[12 + g] : 1.0f;
+    return r;
+}
+
+// nrc_feature_block on the lane's fields (the same operations on the same values).
+LTS_D float nrc_feature_regs(const RawLane& r, int m, int g) {
+    if (m < 9) {
+        const int d = (4 * m) / 12, f = (4 * m) % 12 + g;
+        const float x = ldexpf(d == 0 ? r.a.x : d == 1 ? r.a.y : r.a.z, f - 1);
+        const float fr = x - floorf(x);
+        return fabsf(2.0f * fr - 1.0f);
+    }
+    if (m < 13) {
+        const int k = m - 9;
+        return one_blob(k == 0 ? r.a.w : k == 1 ? r.b.x : k == 2 ? r.b.y : r.b.z, g);
+    }
+    if (m == 13) return one_blob(r.b.w, g);
+    if (m == 14) return r.c;
+    return r.d;
+}
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// The next layer's B fragment from two accumulator tiles: ReLU as a signed integer max with 0 on
+// the float bits (negative values and -0 have the sign bit set and become +0; for every input but
+// NaN this is max(x, 0)), then rounding to T in pairs. One integer op per value and one packed
+// conversion per two, where a float max canonicalises its input first (two ops per value) and
+// single conversions need a permute to pack.
+template <typename T>
+LTS_D typename Vec8<T>::type relu_fragment(const f32x4& lo, const f32x4& hi) {
+    typedef T t2 __attribute__((ext_vector_type(2)));
+    typedef T t4 __attribute__((ext_vector_type(4)));
+    const f32x4 a = __builtin_bit_cast(f32x4, __builtin_elementwise_max(__builtin_bit_cast(i32x4, lo), (i32x4)0));
+    const f32x4 b = __builtin_bit_cast(f32x4, __builtin_elementwise_max(__builtin_bit_cast(i32x4, hi), (i32x4)0));
+    const t2 p0 = __builtin_convertvector((f32x2{a[0], a[1]}), t2);
+    const t2 p1 = __builtin_convertvector((f32x2{a[2], a[3]}), t2);
+    const t2 p2 = __builtin_convertvector((f32x2{b[0], b[1]}), t2);
+    const t2 p3 = __builtin_convertvector((f32x2{b[2], b[3]}), t2);
+    const t4 q0 = __builtin_shufflevector(p0, p1, 0, 1, 2, 3), q1 = __builtin_shufflevector(p2, p3, 0, 1, 2, 3);
+    return __builtin_shufflevector(q0, q1, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
 template <typename T>
 LTS_D typename Vec8<T>::type lds_frag(const uint4* w, int row) {
     const uint4 q = w[row];
@@ -110,9 +163,7 @@ LTS_D f32x4 forward_tile(const uint4* wf, const float* __restrict__ raw, bool va
             for (int ks = 0; ks < 2; ks++) acc[mt] = mfma(lds_frag<T>(wf, l * 512 + (mt * 2 + ks) * 64 + lane), h[l][ks], acc[mt]);
         }
 #pragma unroll
-        for (int ks = 0; ks < 2; ks++)
-#pragma unroll
-            for (int j = 0; j < 8; j++) h[l + 1][ks][j] = (T)fmaxf(acc[2 * ks + (j >> 2)][j & 3], 0.0f);
+        for (int ks = 0; ks < 2; ks++) h[l + 1][ks] = relu_fragment<T>(acc[2 * ks], acc[2 * ks + 1]);
     }
     if (Reload) asm volatile("" ::: "memory");
     f32x4 y = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -124,7 +175,7 @@ LTS_D f32x4 forward_tile(const uint4* wf, const float* __restrict__ raw, bool va
 // Forward pass of NT 16-sample tiles at once (independent MFMA chains interleaved); only the current
 // layer's activations are kept.
 template <typename T, int NT>
-LTS_D void forward_tiles(const uint4* wf, const float* const (&raw)[NT], const bool (&valid)[NT], int lane, f32x4 (&y)[NT]) {
+LTS_D void forward_tiles(const uint4* wf, const RawLane (&raw)[NT], const bool (&valid)[NT], int lane, f32x4 (&y)[NT]) {
     const int g = lane >> 4;
     typename Vec8<T>::type h[NT][2];
 #pragma unroll
@@ -132,7 +183,7 @@ LTS_D void forward_tiles(const uint4* wf, const float* const (&raw)[NT], const b
 #pragma unroll
         for (int ks = 0; ks < 2; ks++)
 #pragma unroll
-            for (int j = 0; j < 8; j++) h[u][ks][j] = valid[u] ? (T)nrc_feature_block(raw[u], 8 * ks + j, g) : (T)0.0f;
+            for (int j = 0; j < 8; j++) h[u][ks][j] = valid[u] ? (T)nrc_feature_regs(raw[u], 8 * ks + j, g) : (T)0.0f;
 #pragma unroll
     for (int l = 0; l < 5; l++) {
         f32x4 acc[NT][4];
@@ -151,9 +202,7 @@ LTS_D void forward_tiles(const uint4* wf, const float* const (&raw)[NT], const b
 #pragma unroll
         for (int u = 0; u < NT; u++)
 #pragma unroll
-            for (int ks = 0; ks < 2; ks++)
-#pragma unroll
-                for (int j = 0; j < 8; j++) h[u][ks][j] = (T)fmaxf(acc[u][2 * ks + (j >> 2)][j & 3], 0.0f);
+            for (int ks = 0; ks < 2; ks++) h[u][ks] = relu_fragment<T>(acc[u][2 * ks], acc[u][2 * ks + 1]);
     }
 #pragma unroll
     for (int u = 0; u < NT; u++) y[u] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -165,41 +214,57 @@ LTS_D void forward_tiles(const uint4* wf, const float* const (&raw)[NT], const b
     }
 }
 
-constexpr int kInferTiles = 2;  // 16-sample tiles per wave iteration
+constexpr int kInferTiles = 1;  // 16-sample tiles per wave iteration
 
+// Inference over records 0 .. n-1 in sequence. Each wave takes a contiguous run of tiles, so its
+// loads walk memory in order. Without meta, output c of record s goes to out[3 s + c]; with meta
+// (the cached pass's queries), sampleL[3 slot + c] = L_c + beta_c * max(y_c, 0) * a_c.
 template <typename T>
 __global__ __launch_bounds__(256) void k_nrc_infer(const uint16_t* __restrict__ fwd, const float* __restrict__ raw,
                                                    const int32_t* __restrict__ nDev, int nCap, float* __restrict__ out,
-                                                   const NrcQuery* __restrict__ meta, float* __restrict__ sampleL,
-                                                   const int32_t* __restrict__ index) {
+                                                   const NrcQuery* __restrict__ meta, float* __restrict__ sampleL) {
     __shared__ uint4 wf[kFwdVec];
     stage(wf, fwd, kFwdVec);
     __syncthreads();
     const int n = min(*nDev, nCap);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int groups = (n + 16 * kInferTiles - 1) / (16 * kInferTiles);
-    for (int t = blockIdx.x * 4 + wave; t < groups; t += gridDim.x * 4) {
-        int s[kInferTiles], rec[kInferTiles];
+    const int waves = gridDim.x * 4, w = blockIdx.x * 4 + wave;
+    const int per = (groups + waves - 1) / waves;
+    const int g0 = w * per, g1 = min(groups, g0 + per);
+    const int g = lane >> 4;
+    // tile t's record fields are loaded during tile t - 1
+    RawLane next[kInferTiles];
+    auto fetch = [&](int t) {
+#pragma unroll
+        for (int u = 0; u < kInferTiles; u++) {
+            const int su = (t * kInferTiles + u) * 16 + (lane & 15);
+            next[u] = load_raw_lane(raw + (size_t)(su < n ? su : 0) * kNrcRaw, g);
+        }
+    };
+    if (g0 < g1) fetch(g0);
+    for (int t = g0; t < g1; t++) {
+        int s[kInferTiles];
         bool valid[kInferTiles];
-        const float* rp[kInferTiles];
+        RawLane cur[kInferTiles];
 #pragma unroll
         for (int u = 0; u < kInferTiles; u++) {
             s[u] = (t * kInferTiles + u) * 16 + (lane & 15);
             valid[u] = s[u] < n;
-            rec[u] = valid[u] ? (index ? index[s[u]] : s[u]) : 0;
-            rp[u] = raw + (size_t)rec[u] * kNrcRaw;
+            cur[u] = next[u];
         }
+        if (t + 1 < g1) fetch(t + 1);
         f32x4 y[kInferTiles];
-        forward_tiles<T, kInferTiles>(wf, rp, valid, lane, y);
+        forward_tiles<T, kInferTiles>(wf, cur, valid, lane, y);
 #pragma unroll
         for (int u = 0; u < kInferTiles; u++) {
             if (lane >= 16 || !valid[u]) continue;
             if (meta) {
-                const NrcQuery q = meta[rec[u]];
+                const NrcQuery q = meta[s[u]];
                 float* o = sampleL + 3 * (size_t)q.slot;
-                o[0] += q.beta[0] * fmaxf(y[u][0], 0.0f) * q.a[0];
-                o[1] += q.beta[1] * fmaxf(y[u][1], 0.0f) * q.a[1];
-                o[2] += q.beta[2] * fmaxf(y[u][2], 0.0f) * q.a[2];
+                o[0] = q.L[0] + q.beta[0] * fmaxf(y[u][0], 0.0f) * q.a[0];
+                o[1] = q.L[1] + q.beta[1] * fmaxf(y[u][1], 0.0f) * q.a[1];
+                o[2] = q.L[2] + q.beta[2] * fmaxf(y[u][2], 0.0f) * q.a[2];
             } else {
                 out[3 * (size_t)s[u] + 0] = y[u][0];
                 out[3 * (size_t)s[u] + 1] = y[u][1];
@@ -465,12 +530,12 @@ __global__ __launch_bounds__(256) void k_nrc_pack(const float* __restrict__ mast
 
 template <typename T>
 hipError_t infer_t(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out, const NrcQuery* meta,
-                   float* sampleL, const int32_t* index, hipStream_t s) {
+                   float* sampleL, hipStream_t s) {
     // two blocks per CU are resident (LDS weights, 220 VGPRs): a grid of that size stages the
     // weights once per block and loops over the tiles
     const int groups = (nCap + 16 * kInferTiles - 1) / (16 * kInferTiles);
     const int blocks = std::max(1, std::min((groups + 3) / 4, 512));
-    k_nrc_infer<T><<<blocks, 256, 0, s>>>(b.fwd, raw, nDev, nCap, out, meta, sampleL, index);
+    k_nrc_infer<T><<<blocks, 256, 0, s>>>(b.fwd, raw, nDev, nCap, out, meta, sampleL);
     return hipGetLastError();
 }
 
@@ -504,11 +569,17 @@ hipError_t nrc_pack(const NrcBuffers& b, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t nrc_infer(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out, const NrcQuery* meta,
-                     float* sampleL, hipStream_t s, const int32_t* index) {
+hipError_t nrc_infer(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out, hipStream_t s) {
     if (nCap <= 0) return hipSuccess;
-    return b.precision == 1 ? infer_t<_Float16>(b, raw, nDev, nCap, out, meta, sampleL, index, s)
-                            : infer_t<__bf16>(b, raw, nDev, nCap, out, meta, sampleL, index, s);
+    return b.precision == 1 ? infer_t<_Float16>(b, raw, nDev, nCap, out, nullptr, nullptr, s)
+                            : infer_t<__bf16>(b, raw, nDev, nCap, out, nullptr, nullptr, s);
+}
+
+hipError_t nrc_infer_queries(const NrcBuffers& b, const float* raw, const NrcQuery* meta, const int32_t* nDev, int nCap,
+                             float* sampleL, hipStream_t s) {
+    if (nCap <= 0) return hipSuccess;
+    return b.precision == 1 ? infer_t<_Float16>(b, raw, nDev, nCap, nullptr, meta, sampleL, s)
+                            : infer_t<__bf16>(b, raw, nDev, nCap, nullptr, meta, sampleL, s);
 }
 
 hipError_t nrc_train(const NrcBuffers& b, hipStream_t s, int steps) {

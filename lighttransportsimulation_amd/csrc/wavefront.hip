@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                                                           WfShadow* __restrict__ shadows, int32_t* __restrict__ shadowCount,
                                                           WfMis* __restrict__ misRays, int32_t* __restrict__ misCount,
                                                           float* __restrict__ qRaw, NrcQuery* __restrict__ qMeta,
-                                                          int32_t* __restrict__ qQueue, int32_t* __restrict__ qCount,
+                                                          int32_t* __restrict__ qCount,
                                                           WfTrainRec* __restrict__ tRec, float* __restrict__ tRaw,
                                                           int32_t* __restrict__ doneQueue, int32_t* __restrict__ doneCount,
                                                           int32_t* __restrict__ shadowQueue, int32_t* __restrict__ misQueue) {
@@ -242,6 +242,10 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
     // view points at its context's one table)
     __shared__ int perm[512];
     __shared__ int32_t appendLds[2][5 * (kShadeBlock / 64) + 5];
+    // a query's raw record (4 float4) and its NrcQuery (3 float4), held by the thread that makes it
+    // until the round's append gives the query its compact slot
+    constexpr int kStage = 7;
+    __shared__ float4 qStage[kCached ? kShadeBlock * kStage : 1];
     for (int i = threadIdx.x; i < 512; i += kShadeBlock) perm[i] = scenes[0].perm[i];
     __syncthreads();
     const int stride = gridDim.x * kShadeBlock;
@@ -342,12 +346,13 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                         tRec[r] = tr;
                     } else if (bounce >= queryBounce) {
                         // the cache stands for all radiance scattered here
-                        nrc_raw_record(si, bsdf, sl, qRaw + 16 * (size_t)pi);
+                        nrc_raw_record(si, bsdf, sl, reinterpret_cast<float*>(&qStage[kStage * threadIdx.x]));
                         NrcQuery q;
                         q.beta[0] = beta.r; q.beta[1] = beta.g; q.beta[2] = beta.b;
                         q.a[0] = bsdf.R.r; q.a[1] = bsdf.R.g; q.a[2] = bsdf.R.b;
-                        q.slot = slot; q.pad = 0;
-                        qMeta[pi] = q;
+                        q.L[0] = q.L[1] = q.L[2] = 0.0f;  // set where the path ends, below
+                        q.slot = slot; q.pad[0] = q.pad[1] = 0;
+                        *reinterpret_cast<NrcQuery*>(&qStage[kStage * threadIdx.x + 4]) = q;
                         wantQuery = true;
                         alive = false;
                     }
@@ -450,6 +455,10 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                 // the path ends here: its sample is final now, or once its light sample is traced
                 if (ne.used) {
                     wantDone = true;
+                } else if (cached && wantQuery) {
+                    // the sample is written by inference: L plus the cache's estimate
+                    NrcQuery* q = reinterpret_cast<NrcQuery*>(&qStage[kStage * threadIdx.x + 4]);
+                    q->L[0] = L.r; q->L[1] = L.g; q->L[2] = L.b;
                 } else {
                     float* o = sl.sampleL + 3 * ((size_t)(sample - sl.s0) * sl.width * sl.height + pixel);
                     o[0] = L.r; o[1] = L.g; o[2] = L.b;
@@ -480,10 +489,20 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
             lds[kQueues * kWaves + q] = run ? atomicAdd(counter[q], run) : 0;
         }
         __syncthreads();
-        int32_t* const out[kQueues] = {shadowQueue, misQueue, nextQueue, doneQueue, qQueue};
+        int32_t* const out[kQueues - 1] = {shadowQueue, misQueue, nextQueue, doneQueue};
 #pragma unroll
-        for (int q = 0; q < kQueues; q++)
+        for (int q = 0; q < kQueues - 1; q++)
             if (want[q]) out[q][lds[kQueues * kWaves + q] + lds[q * kWaves + wave] + (int)__popcll(mask[q] & below)] = pi;
+        if (cached && want[4]) {
+            // queries are stored compactly in append order, so inference reads them in sequence
+            const size_t at = (size_t)(lds[kQueues * kWaves + 4] + lds[4 * kWaves + wave] + (int)__popcll(mask[4] & below));
+            float4* r = reinterpret_cast<float4*>(qRaw + 16 * at);
+#pragma unroll
+            for (int j = 0; j < 4; j++) r[j] = qStage[kStage * threadIdx.x + j];
+            float4* m = reinterpret_cast<float4*>(qMeta + at);
+#pragma unroll
+            for (int j = 0; j < 3; j++) m[j] = qStage[kStage * threadIdx.x + 4 + j];
+        }
     }
 }
 
@@ -746,7 +765,7 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
                                 : (bounce == 0 ? k_wf_shade<false, true> : k_wf_shade<false, false>);
             shade<<<gridAll, kShadeBlock, 0, s>>>(b.scenes, b.params, b.slots, bounce, b.paths, b.rays, b.hits, b.nee, shadeQueue,
                                                   cnt, b.queue[q ^ 1], nxt, b.shadows, cnt + 1, b.mis, cnt + 2,
-                                                  cached ? b.qRaw : nullptr, b.qMeta, b.qQueue, qCount, b.tRec, b.tRaw,
+                                                  cached ? b.qRaw : nullptr, b.qMeta, qCount, b.tRec, b.tRaw,
                                                   b.done, cnt + 3, b.shadowQ, b.misQ);
         }
         mark(8 * bounce + 2);
@@ -758,7 +777,7 @@ hipError_t wf_enqueue_pass(const WfBuffers& b, const WfTuning& t, int maxBounce,
     }
     if (cached) {
         mark(8 * kWfMaxBounce + 0);
-        hipError_t e = nrc_infer(*nrc, b.qRaw, qCount, (int)b.cap, nullptr, b.qMeta, b.sampleBase, s, b.qQueue);
+        hipError_t e = nrc_infer_queries(*nrc, b.qRaw, b.qMeta, qCount, (int)b.cap, b.sampleBase, s);
         if (e != hipSuccess) return e;
         mark(8 * kWfMaxBounce + 1);
         const int nb = (b.trainCap * maxBounce + kCollectPerBlock - 1) / kCollectPerBlock;
@@ -815,16 +834,15 @@ hipError_t wf_reserve(WfBuffers& b, size_t cap, int maxScenes) {
 
 hipError_t wf_reserve_cache(WfBuffers& b, size_t cap, int trainCap) {
     if (cap <= b.cacheCap && trainCap <= b.trainCap) return hipSuccess;
-    void* ps[] = {b.qRaw, b.qMeta, b.qQueue, b.tRec, b.tRaw, b.tCount};
+    void* ps[] = {b.qRaw, b.qMeta, b.tRec, b.tRaw, b.tCount};
     for (void* p : ps)
         if (p) (void)hipFree(p);
-    b.qRaw = nullptr; b.qMeta = nullptr; b.qQueue = nullptr; b.tRec = nullptr; b.tRaw = nullptr; b.tCount = nullptr;
+    b.qRaw = nullptr; b.qMeta = nullptr; b.tRec = nullptr; b.tRaw = nullptr; b.tCount = nullptr;
     b.cacheCap = 0; b.trainCap = 0;
     hipError_t e = hipSuccess;
 #define CTRY(x) do { e = (x); if (e != hipSuccess) return e; } while (0)
     CTRY(wf_alloc(&b.qRaw, cap * kNrcRaw));
     CTRY(wf_alloc(&b.qMeta, cap));
-    CTRY(wf_alloc(&b.qQueue, cap));
     CTRY(wf_alloc(&b.tRec, (size_t)trainCap * kWfMaxBounce));
     CTRY(wf_alloc(&b.tRaw, (size_t)trainCap * kWfMaxBounce * kNrcRaw));
     CTRY(wf_alloc(&b.tCount, ((size_t)trainCap * kWfMaxBounce + kCollectPerBlock - 1) / kCollectPerBlock));
@@ -836,7 +854,7 @@ hipError_t wf_reserve_cache(WfBuffers& b, size_t cap, int trainCap) {
 }
 
 void wf_release(WfBuffers& b) {
-    void* cs[] = {b.qRaw, b.qMeta, b.qQueue, b.tRec, b.tRaw, b.tCount};
+    void* cs[] = {b.qRaw, b.qMeta, b.tRec, b.tRaw, b.tCount};
     for (void* p : cs)
         if (p) (void)hipFree(p);
     void* ps[] = {b.scenes, b.params, b.slots, b.paths, b.rays, b.hits, b.nee, b.queue[0], b.queue[1], b.sorted, b.done, b.shadowQ, b.misQ, b.bins, b.shadows, b.mis, b.counts};
