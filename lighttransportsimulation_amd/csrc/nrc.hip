@@ -186,6 +186,8 @@ LTS_D void forward_tiles(const uint4* wf, const RawLane (&raw)[NT], const bool (
             for (int j = 0; j < 8; j++) h[u][ks][j] = valid[u] ? (T)nrc_feature_regs(raw[u], 8 * ks + j, g) : (T)0.0f;
 #pragma unroll
     for (int l = 0; l < 5; l++) {
+        // a compiler barrier per layer: only this layer's weight fragments are live in registers
+        asm volatile("" ::: "memory");
         f32x4 acc[NT][4];
 #pragma unroll
         for (int mt = 0; mt < 4; mt++)
@@ -214,13 +216,14 @@ LTS_D void forward_tiles(const uint4* wf, const RawLane (&raw)[NT], const bool (
     }
 }
 
-constexpr int kInferTiles = 1;  // 16-sample tiles per wave iteration
+constexpr int kInferTiles = 2;   // 16-sample tiles per wave iteration (each weight fragment read serves both)
+constexpr int kInferBlock = 512;  // 8 waves share one LDS copy of the weights
 
 // Inference over records 0 .. n-1 in sequence. Each wave takes a contiguous run of tiles, so its
 // loads walk memory in order. Without meta, output c of record s goes to out[3 s + c]; with meta
 // (the cached pass's queries), sampleL[3 slot + c] = L_c + beta_c * max(y_c, 0) * a_c.
 template <typename T>
-__global__ __launch_bounds__(256) void k_nrc_infer(const uint16_t* __restrict__ fwd, const float* __restrict__ raw,
+__global__ __launch_bounds__(kInferBlock) void k_nrc_infer(const uint16_t* __restrict__ fwd, const float* __restrict__ raw,
                                                    const int32_t* __restrict__ nDev, int nCap, float* __restrict__ out,
                                                    const NrcQuery* __restrict__ meta, float* __restrict__ sampleL) {
     __shared__ uint4 wf[kFwdVec];
@@ -229,7 +232,7 @@ __global__ __launch_bounds__(256) void k_nrc_infer(const uint16_t* __restrict__ 
     const int n = min(*nDev, nCap);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int groups = (n + 16 * kInferTiles - 1) / (16 * kInferTiles);
-    const int waves = gridDim.x * 4, w = blockIdx.x * 4 + wave;
+    const int waves = gridDim.x * (kInferBlock / 64), w = blockIdx.x * (kInferBlock / 64) + wave;
     const int per = (groups + waves - 1) / waves;
     const int g0 = w * per, g1 = min(groups, g0 + per);
     const int g = lane >> 4;
@@ -531,11 +534,12 @@ __global__ __launch_bounds__(256) void k_nrc_pack(const float* __restrict__ mast
 template <typename T>
 hipError_t infer_t(const NrcBuffers& b, const float* raw, const int32_t* nDev, int nCap, float* out, const NrcQuery* meta,
                    float* sampleL, hipStream_t s) {
-    // two blocks per CU are resident (LDS weights, 220 VGPRs): a grid of that size stages the
-    // weights once per block and loops over the tiles
+    // two blocks per CU are resident (114 VGPRs: 4 waves per SIMD): a grid of that size stages the
+    // weights once per block and loops over the tiles, with no second round of blocks
+    constexpr int kWaves = kInferBlock / 64;
     const int groups = (nCap + 16 * kInferTiles - 1) / (16 * kInferTiles);
-    const int blocks = std::max(1, std::min((groups + 3) / 4, 512));
-    k_nrc_infer<T><<<blocks, 256, 0, s>>>(b.fwd, raw, nDev, nCap, out, meta, sampleL);
+    const int blocks = std::max(1, std::min((groups + kWaves - 1) / kWaves, 2 * 256));
+    k_nrc_infer<T><<<blocks, kInferBlock, 0, s>>>(b.fwd, raw, nDev, nCap, out, meta, sampleL);
     return hipGetLastError();
 }
 
