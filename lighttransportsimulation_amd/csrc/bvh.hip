@@ -313,9 +313,13 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const uint32_t* __
                                                               uint32_t* __restrict__ keysOut, int32_t* __restrict__ valsOut, int n,
                                                               int shift, int tiles, const uint32_t* __restrict__ counts,
                                                               const uint32_t* __restrict__ segBase, int32_t* __restrict__ status) {
-    __shared__ uint32_t sKeys[kSortTile];
-    __shared__ int32_t sVals[kSortTile];
-    __shared__ uint32_t waveCnt[kSortWaves][kBins];  // per-wave digit counts, then per-wave digit offsets
+    // The staged tile (keys, then values) overlays the per-wave digit counters: each key's position
+    // is in registers before the tile is staged, so 40 KB of LDS let 3 workgroups share a CU.
+    __shared__ uint32_t pool[2 * kSortTile];
+    static_assert(kSortWaves * kBins <= 2 * kSortTile, "the digit counters fit the staging area");
+    uint32_t (*waveCnt)[kBins] = reinterpret_cast<uint32_t (*)[kBins]>(pool);  // per-wave digit counts, then offsets
+    uint32_t* sKeys = pool;
+    int32_t* sVals = reinterpret_cast<int32_t*>(pool + kSortTile);
     __shared__ uint32_t tileStart[kBins];
     __shared__ uint32_t globalBase[kBins];
     __shared__ uint32_t scratch[kSortWaves];
@@ -385,9 +389,13 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const uint32_t* __
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
         const uint32_t d = (key[j] >> shift) & (kBins - 1);
-        const uint32_t pos = tileStart[d] + waveCnt[wave][d] + rank[j];
-        sKeys[pos] = key[j];
-        sVals[pos] = val[j];
+        rank[j] += tileStart[d] + waveCnt[wave][d];  // the key's position in the staged tile
+    }
+    __syncthreads();  // every counter is read before the staging overwrites them
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        sKeys[rank[j]] = key[j];
+        sVals[rank[j]] = val[j];
     }
     __syncthreads();
     for (int i = t; i < valid; i += kSortBlock) {
