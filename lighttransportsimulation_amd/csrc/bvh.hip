@@ -2,15 +2,16 @@
 // (acceleration/BVH.cuh:77-452, acceleration/BVH_kernels.cu:8-50):
 //   k_prim_bounds  primitive AABB + centroid inputs, block-reduced scene bounds     (BVH.cuh:92-98)
 //   k_morton       30-bit Morton codes of the normalised centroids, digit counts     (BVH.cuh:101-137)
-//   k_radix_pass   stable LSD sort of (code, primitive) pairs, 10-bit digits, 3 passes of one kernel
-//                  each (LDS ranking, decoupled look-back over tiles)                (BVH.cuh:140-146)
-//   k_tree_l1/l2   the Karras hierarchy (BVH.cuh:173-236) and its bounds (BVH.cuh:238-262) in one
+//   k_radix_*      stable LSD sort of (code, primitive) pairs, 10-bit digits, 3 passes of tile digit
+//   k_scan_*       counts, a two-level scan and an LDS-ranked scatter                 (BVH.cuh:140-146)
+//   k_tree_l1/up   the Karras hierarchy (BVH.cuh:173-236) and its bounds (BVH.cuh:238-262) in one
 //                  bottom-up climb: 512-leaf chunks in LDS, then the chunk roots through global
 //                  arrival slots and records
 //   k_pack_*       traversal layout: both child boxes per node, triangles in leaf order
 // The scene-bound reduction uses a total order on floats (-0 below +0); it equals the reference's
 // float atomics (GeneralHelper.cuh:150-160) on every input without -0.0.
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <type_traits>
 
@@ -422,10 +423,11 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const uint32_t* __
 // outer end there; the second arrival learns the parent's range from it, unions the children and
 // climbs on. The result equals the top-down construction and refit bit for bit: same nodes, same
 // numbering, and every box is bUnion(left box, right box).
-//   level 1: one workgroup per 512 sorted leaves; arrival slots, child records and boxes in LDS.
-//            A climb stops when its parent's split lies left of the chunk (one node per chunk,
-//            filed as the chunk's root) or when it arrives first. Unmatched arrivals are written
-//            to the split's global slot, with a record of the waiting node.
+//   level 1: 512 sorted leaves per chunk, a persistent workgroup per resident slot; arrival slots,
+//            child records and boxes in LDS. A climb stops when its parent's split lies left of
+//            the chunk (one node per chunk, filed as the chunk's root) or when it arrives first.
+//            Unmatched arrivals are written to the split's global slot, with a record of the
+//            waiting node.
 //   level 2: one workgroup per 65536 leaves climbs on from its chunks' roots; it files one root.
 //   level 3: one workgroup climbs from the level-2 roots.
 // Each level's climbs meet each other in LDS and read only what earlier launches wrote, so no
@@ -445,13 +447,28 @@ static_assert((1 << 26) / kChunk2 <= kGroup3, "level 3 climbs every level-2 root
 constexpr int kUnset = INT32_MIN;
 constexpr int kMaxClimb = 64;  // steps of a climb above level 1: the tree is at most 56 interior nodes deep
 
+// v_min_f32 / v_max_f32 as fminf / fmaxf execute them, without the canonicalising v_max x, x the
+// compiler puts before each operand it cannot prove canonical (IEEE mode). Canonicalising changes
+// only signalling NaNs (denormals are kept), and every box value here is the result of a min or a
+// max (the primitive boxes of tri_box included) or a copy of one, which is canonical.
+__device__ __forceinline__ float box_min(float a, float b) {
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float box_max(float a, float b) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // bUnion(Bounds3, Bounds3) followed by the Bounds3(p1, p2) constructor
 __device__ __forceinline__ void union_box(const float* a, const float* b, float* o) {
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        float lo = fminf(a[k], b[k]), hi = fmaxf(a[3 + k], b[3 + k]);
-        o[k] = fminf(lo, hi);
-        o[3 + k] = fmaxf(lo, hi);
+        float lo = box_min(a[k], b[k]), hi = box_max(a[3 + k], b[3 + k]);
+        o[k] = box_min(lo, hi);
+        o[3 + k] = box_max(lo, hi);
     }
 }
 
@@ -465,156 +482,239 @@ __device__ __forceinline__ void store_box(float* box, int node, const float* o) 
 }
 
 // Level 1. LDS holds the chunk's nodes under local indices: leaf cs + x at x, interior node cs + x
-// at kChunk + x. Each wave first climbs from its own 64 leaves, stopping where a parent splits
-// left of those leaves (the wave's root); wave 0 then climbs on from the other waves' roots. So
-// most of the climbing runs with full waves, and only the short top of the chunk's subtrees
-// runs on a few lanes.
+// at kChunk + x. The climbs advance in rounds over a dense work list: in a round every node that
+// is complete and not yet parented takes one step (one thread each), and the nodes completed by
+// second arrivals form the next round's list, so a lane whose climb has ended never idles in a
+// wave while other lanes of the wave climb on. A work item carries the node's range and the deltas
+// at both of its ends, and a first arrival leaves its outer end and the delta there in the split's
+// slot, so a step reads only its item, the slot and the two child boxes. Every box a round reads
+// was written before the barrier that opened the round (a node's box is written when the node is
+// completed); the exchanges at the splits only decide which arrival completes the parent.
 __device__ __forceinline__ int local_index(int c, int n, int cs) {
     return c >= n - 1 ? c - (n - 1) - cs : kChunk + (c - cs);
 }
 
-__global__ __launch_bounds__(kChunk) void k_tree_l1(const uint32_t* __restrict__ codes, const float* __restrict__ primBox,
+// A workgroup barrier that orders LDS only: global loads and stores stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+constexpr int kTreeWaves = kChunk / 64;
+constexpr uint32_t kNone16 = 0xffffu;
+static_assert(kChunk <= 512, "level-1 items pack chunk-local ends in 9 bits");
+
+// work item: l | r << 9 | (dL + 1) << 18 | (dR + 1) << 25 (chunk-local ends, end deltas -1 .. 71)
+__device__ __forceinline__ uint32_t pack_item(int l, int r, int dL, int dR) {
+    return (uint32_t)l | ((uint32_t)r << 9) | ((uint32_t)(dL + 1) << 18) | ((uint32_t)(dR + 1) << 25);
+}
+
+// Level-1 inputs of one thread for a chunk, loaded a chunk ahead: the codes around leaf cs + t
+// and a third of the chunk's leaf boxes (mode 0: float2 runs t, t + kChunk, t + 2 kChunk of the
+// contiguous leaf section; otherwise the box of leaf cs + t).
+struct TreeL1Fetch {
+    uint32_t cPrev, c, cNext;
+    float2 b[3];
+};
+
+__device__ __forceinline__ void tree_l1_fetch(TreeL1Fetch& f, const uint32_t* __restrict__ codes,
+                                              const float* __restrict__ primBox, const int32_t* __restrict__ perm,
+                                              int n, int mode, int chunk, int t) {
+    // every load is unconditional, at clamped in-range addresses (lanes past the end reload the last
+    // entries), so no branch joins the loaded registers and nothing waits for them until they are used
+    const int cs = chunk * kChunk, m = min(kChunk, n - cs), k = cs + min(t, m - 1);
+    int pk = 0;
+    if (mode != 0) pk = perm[k];  // first: the box loads wait for it, the code loads need not
+    f.c = codes[k];
+    f.cPrev = codes[max(k - 1, 0)];
+    f.cNext = codes[min(k + 1, n - 1)];
+    const float2* src = reinterpret_cast<const float2*>(primBox) + 3 * (size_t)cs;
+    const float2* own = reinterpret_cast<const float2*>(primBox) + 3 * (size_t)pk;
+#pragma unroll
+    for (int j = 0; j < 3; j++) f.b[j] = mode == 0 ? src[min(t + j * kChunk, 3 * m - 1)] : own[j];
+}
+
+// Persistent: workgroup b builds chunks b, b + gridDim.x, ..., loading each chunk's inputs while
+// the previous chunk climbs, so the climbs are not held up by global-load latency. The climbs are
+// latency-bound, so residency is what counts: at most 64 SGPRs (a few spilled to VGPR lanes) let
+// four workgroups share a CU, where the 87 the compiler picks by itself allow three (measured).
+__global__ __launch_bounds__(kChunk) __attribute__((amdgpu_num_sgpr(64))) void k_tree_l1(const uint32_t* __restrict__ codes, const float* __restrict__ primBox,
                                                     const int32_t* __restrict__ perm, int n, int mode, int32_t* __restrict__ left,
                                                     int32_t* __restrict__ right, int32_t* __restrict__ parent,
                                                     float* __restrict__ box, int32_t* __restrict__ slots,
                                                     ClimbRec* __restrict__ pend, int32_t* __restrict__ chunkRoot,
                                                     ClimbRec* __restrict__ chunkRec, int32_t* __restrict__ status) {
-    constexpr int kWaves = kChunk / 64;
     __shared__ __align__(16) float nbox[2 * kChunk * 6];
-    __shared__ int32_t dl[kChunk + 1];  // dl[x] = delta(cs - 1 + x, cs + x)
-    __shared__ int32_t slot[kChunk];    // split cs + x: outer end of a first arrival, or -1
-    __shared__ int32_t sl[kChunk], sr[kChunk];
-    __shared__ int32_t sp[2 * kChunk];  // parent of each local node (kUnset: set by a later level)
-    __shared__ int32_t waveRoot[kWaves];
-    __shared__ int32_t sRoot;
-    __shared__ uint32_t sc[kChunk + 2];  // codes cs - 1 .. cs + kChunk
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int cs = blockIdx.x * kChunk, ce = min(cs + kChunk, n);
-    const int k = cs + t;
-    if (k - 1 >= 0 && k - 1 < n) sc[t] = codes[k - 1];
-    if (t < 2 && cs + kChunk - 1 + t < n) sc[kChunk + t] = codes[cs + kChunk - 1 + t];
-    if (t == 0) sRoot = -1;
-    if (t < kWaves) waveRoot[t] = -1;
-    slot[t] = -1;
-    sl[t] = kUnset;
-    sp[t] = kUnset;
-    sp[kChunk + t] = kUnset;
-    const int m = ce - cs;  // leaves of the chunk
-    if (mode == 0) {  // the chunk's leaf boxes are one contiguous run (8-byte aligned)
-        const float2* src = reinterpret_cast<const float2*>(primBox + 6 * (size_t)cs);
-        for (int i = t; i < 3 * m; i += kChunk) reinterpret_cast<float2*>(nbox)[i] = src[i];
-    } else if (k < ce) {
-        float b[6];
-        load_box(primBox, perm[k], b);
+    // split cs + x: -1, or a first arrival's outer end - cs | (delta at that end + 1) << 9
+    __shared__ int32_t slot[kChunk];
+    // interior node cs + x completed here: its split - cs, bit 9 when its left child is the leaf at
+    // the split, bit 10 when its right child is the leaf after it (else the interior nodes there)
+    __shared__ uint16_t ssplit[kChunk];
+    __shared__ uint16_t sp[2 * kChunk];  // parent of each local node - cs (kNone16: set by a later level)
+    __shared__ uint32_t work[2][kChunk];  // round r's items in list r & 1
+    __shared__ int32_t workCount[3];      // round r's items: counter r % 3
+    __shared__ int32_t sRoot[3];  // the chunk's root: its right end - cs (or -1) and its end deltas
+    const int t = threadIdx.x, lane = t & 63;
+    const int chunks = (n + kChunk - 1) / kChunk;
+    TreeL1Fetch f;
+    if ((int)blockIdx.x < chunks) tree_l1_fetch(f, codes, primBox, perm, n, mode, blockIdx.x, t);
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): every chunk starts with its inputs in registers
+    for (int chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {
+        const int cs = chunk * kChunk, ce = min(cs + kChunk, n);
+        const int k = cs + t;
+        const int m = ce - cs;  // leaves of the chunk
+        int dL = -1, dR = -1;   // deltas at both ends of leaf k
+        if (k < ce) {
+            if (k >= 1) dL = delta_codes(f.cPrev, f.c, n, k - 1, k);
+            if (k + 1 < n) dR = delta_codes(f.c, f.cNext, n, k, k + 1);
+        }
+        if (t == 0) sRoot[0] = -1;
+        if (t < 3) workCount[t] = 0;
+        slot[t] = -1;
+        ssplit[t] = kNone16;
+        sp[t] = kNone16;
+        sp[kChunk + t] = kNone16;
+        if (mode == 0) {
 #pragma unroll
-        for (int q = 0; q < 6; q++) nbox[6 * t + q] = b[q];
-    }
-    __syncthreads();
-    dl[t] = delta_codes(sc[t], sc[t + 1], n, k - 1, k);
-    if (t == 0) dl[kChunk] = delta_codes(sc[kChunk], sc[kChunk + 1], n, cs + kChunk - 1, cs + kChunk);
-    __syncthreads();
-    // Climbs [l, r] until its parent splits left of `lo` (returns r: the node is the root of the
-    // leaves from lo) or it arrives first (returns -1). Every step adds a leaf to [l, r].
-    auto climb = [&](int l, int r, int lo) -> int {
-        int dL = dl[l - cs], dR = dl[r + 1 - cs];
-        for (int step = 0; step < kChunk; step++) {
-            if (dL < 0 && dR < 0) return -1;  // the root of the whole tree
-            const bool asLeft = dR > dL;
-            const int p = asLeft ? r : l - 1;
-            if (p < lo) return r;
-            LTS_LDS_ORDER();
-            const int other = atomicExch(&slot[p - cs], asLeft ? l : r);
-            if (other < 0) return -1;  // first arrival
-            LTS_LDS_ORDER();
-            slot[p - cs] = -1;
-            const int L = asLeft ? l : other, R = asLeft ? other : r;
-            const int lc = (L == p) ? (n - 1) + p : p;
-            const int rc = (R == p + 1) ? (n - 1) + p + 1 : p + 1;
-            const int li = local_index(lc, n, cs), ri = local_index(rc, n, cs);
-            float a[6], c[6], o[6];
-#pragma unroll
-            for (int q = 0; q < 6; q++) {
-                a[q] = nbox[6 * li + q];
-                c[q] = nbox[6 * ri + q];
+            for (int j = 0; j < 3; j++) {
+                const int i = t + j * kChunk;
+                if (i < 3 * m) reinterpret_cast<float2*>(nbox)[i] = f.b[j];
             }
-            union_box(a, c, o);
-            dL = dl[L - cs];
-            dR = dl[R + 1 - cs];
-            const int me = (dL < 0 && dR < 0) ? 0 : (dR > dL ? R : L);
-            const int mi = kChunk + (me - cs);
+        } else if (k < ce) {
 #pragma unroll
-            for (int q = 0; q < 6; q++) nbox[6 * mi + q] = o[q];
-            sl[me - cs] = lc;
-            sr[me - cs] = rc;
-            sp[li] = me;
-            sp[ri] = me;
-            l = L;
-            r = R;
+            for (int j = 0; j < 3; j++) reinterpret_cast<float2*>(nbox)[3 * t + j] = f.b[j];
         }
-        atomicOr(status, kStatusClimb);
-        return -1;
-    };
-    const int ws = cs + 64 * wave;  // first leaf of this wave's 64
-    if (k < ce && n > 1) {
-        const int root = climb(k, k, ws);
-        if (root >= 0) {
-            if (wave == 0) sRoot = root;  // parent splits at cs - 1: the chunk's root
-            else waveRoot[wave] = root;
+        tree_l1_fetch(f, codes, primBox, perm, n, mode, min(chunk + (int)gridDim.x, chunks - 1), t);
+        lds_barrier();
+        // round r reads list r & 1 and appends to list (r + 1) & 1 through counter (r + 1) % 3, while
+        // counter (r + 2) % 3, last read before the barrier that opened round r, is cleared
+        int count = n > 1 ? m : 0;
+        for (int round = 0; count > 0; round++) {
+            if (round == kChunk) {  // every round completes at least one node: never reached
+                if (t == 0) atomicOr(status, kStatusClimb);
+                break;
+            }
+            bool next = false;
+            uint32_t item = 0;
+            if (t < count) {
+                int l = t, r = t, a = dL, b = dR;
+                if (round > 0) {
+                    const uint32_t w = work[round & 1][t];
+                    l = (int)(w & 511u);
+                    r = (int)((w >> 9) & 511u);
+                    a = (int)((w >> 18) & 127u) - 1;
+                    b = (int)(w >> 25) - 1;
+                }
+                if (a >= 0 || b >= 0) {  // not the root of the whole tree
+                    const bool asLeft = b > a;
+                    const int p = asLeft ? r : l - 1;
+                    if (p < 0) {  // its parent splits left of the chunk: the chunk's root (one node)
+                        sRoot[0] = r;
+                        sRoot[1] = a;
+                        sRoot[2] = b;
+                    } else {
+                        const int mine = asLeft ? (l | ((a + 1) << 9)) : (r | ((b + 1) << 9));
+                        const int other = atomicExch(&slot[p], mine);
+                        if (other >= 0) {  // second arrival: complete the parent
+                            slot[p] = -1;
+                            const int oe = other & 511, od = (other >> 9) - 1;
+                            const int L = asLeft ? l : oe, R = asLeft ? oe : r;
+                            const int pL = asLeft ? a : od, pR = asLeft ? od : b;
+                            const bool leafL = L == p, leafR = R == p + 1;
+                            const int li = leafL ? p : kChunk + p, ri = leafR ? p + 1 : kChunk + p + 1;
+                            float x[6], y[6], o[6];
+    #pragma unroll
+                            for (int q = 0; q < 6; q++) {
+                                x[q] = nbox[6 * li + q];
+                                y[q] = nbox[6 * ri + q];
+                            }
+                            union_box(x, y, o);
+                            // Karras numbering; local 0 is the whole tree's root only in chunk 0
+                            const int me = (pL < 0 && pR < 0) ? 0 : (pR > pL ? R : L);
+    #pragma unroll
+                            for (int q = 0; q < 6; q++) nbox[6 * (kChunk + me) + q] = o[q];
+                            ssplit[me] = (uint16_t)(p | (leafL ? 512 : 0) | (leafR ? 1024 : 0));
+                            sp[li] = (uint16_t)me;
+                            sp[ri] = (uint16_t)me;
+                            item = pack_item(L, R, pL, pR);
+                            next = true;
+                        }
+                    }
+                }
+            }
+            // the completed parents form the next list: one LDS atomic per wave reserves its entries
+            const unsigned long long ballot = __ballot(next);
+            if (ballot) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(&workCount[(round + 1) % 3], (int)__popcll(ballot));
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (next) work[(round + 1) & 1][base + (int)__popcll(ballot & ((1ull << lane) - 1ull))] = item;
+            }
+            if (t == 0) workCount[(round + 2) % 3] = 0;
+            lds_barrier();
+            count = workCount[(round + 1) % 3];
         }
-    }
-    __syncthreads();
-    if (wave == 0 && lane >= 1 && lane < kWaves && waveRoot[lane] >= 0) {
-        const int root = climb(cs + 64 * lane, waveRoot[lane], cs);
-        if (root >= 0) sRoot = root;
-    }
-    __syncthreads();
-    // boxes as contiguous runs: interior nodes cs .. (those completed above later levels overwrite),
-    // leaves (n - 1) + cs .. (8-byte aligned; in mode 0 the primitive boxes were written there)
-    {
-        const int mi = min(ce, n - 1) - cs;
-        float* dst = box + 6 * (size_t)cs;
-        const float* ib = nbox + 6 * kChunk;
-        const int whole = mi > 0 ? (6 * mi) / 4 : 0;
-        for (int i = t; i < whole; i += kChunk) reinterpret_cast<float4*>(dst)[i] = reinterpret_cast<const float4*>(ib)[i];
-        for (int i = 4 * whole + t; i < 6 * mi; i += kChunk) dst[i] = ib[i];
-        if (mode != 0) {
-            float2* ldst = reinterpret_cast<float2*>(box + 6 * ((size_t)(n - 1) + cs));
-            for (int i = t; i < 3 * m; i += kChunk) ldst[i] = reinterpret_cast<const float2*>(nbox)[i];
+        // the next chunk's inputs arrived during the climbs: wait for them here, before the stores
+        // below, so that the wait at the next chunk does not also wait for those stores
+        __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+        // boxes as contiguous runs: interior nodes cs .. (those completed above later levels overwrite),
+        // leaves (n - 1) + cs .. (8-byte aligned; in mode 0 the primitive boxes were written there)
+        {
+            const int mi = min(ce, n - 1) - cs;
+            float* dst = box + 6 * (size_t)cs;
+            const float* ib = nbox + 6 * kChunk;
+            const int whole = mi > 0 ? (6 * mi) / 4 : 0;
+            for (int i = t; i < whole; i += kChunk) reinterpret_cast<float4*>(dst)[i] = reinterpret_cast<const float4*>(ib)[i];
+            for (int i = 4 * whole + t; i < 6 * mi; i += kChunk) dst[i] = ib[i];
+            if (mode != 0) {
+                float2* ldst = reinterpret_cast<float2*>(box + 6 * ((size_t)(n - 1) + cs));
+                for (int i = t; i < 3 * m; i += kChunk) ldst[i] = reinterpret_cast<const float2*>(nbox)[i];
+            }
         }
-    }
-    if (k < ce && k < n - 1) {
-        if (sl[t] != kUnset) {
-            left[k] = sl[t];
-            right[k] = sr[t];
+        if (k < ce && k < n - 1) {
+            const uint32_t sv = ssplit[t];
+            if (sv != kNone16) {
+                const int g = cs + (int)(sv & 511u);
+                left[k] = (sv & 512u) ? (n - 1) + g : g;
+                right[k] = (sv & 1024u) ? (n - 1) + g + 1 : g + 1;
+            }
+            const uint32_t pv = sp[kChunk + t];
+            if (pv != kNone16) parent[k] = cs + (int)pv;
+            const int w = slot[t];
+            const int v = w >= 0 ? cs + (w & 511) : -1;
+            slots[k] = v;
+            if (v >= 0) {  // a first arrival waits at split k: publish its box and the delta at its outer end
+                const bool leftChild = v <= k;
+                const int child = leftChild ? (v == k ? (n - 1) + k : k) : (v == k + 1 ? (n - 1) + k + 1 : k + 1);
+                ClimbRec rec;
+                const float* cb = &nbox[6 * local_index(child, n, cs)];
+    #pragma unroll
+                for (int q = 0; q < 6; q++) rec.box[q] = cb[q];
+                rec.a = (w >> 9) - 1;
+                rec.b = 0;
+                pend[k] = rec;
+            }
         }
-        if (sp[kChunk + t] != kUnset) parent[k] = sp[kChunk + t];
-        const int v = slot[t];
-        slots[k] = v;
-        if (v >= 0) {  // a first arrival waits at split k: publish its box and the delta at its outer end
-            const bool leftChild = v <= k;
-            const int child = leftChild ? (v == k ? (n - 1) + k : k) : (v == k + 1 ? (n - 1) + k + 1 : k + 1);
-            ClimbRec rec;
-            const float* cb = &nbox[6 * local_index(child, n, cs)];
-#pragma unroll
-            for (int q = 0; q < 6; q++) rec.box[q] = cb[q];
-            rec.a = leftChild ? dl[v - cs] : dl[v + 1 - cs];
-            rec.b = 0;
-            pend[k] = rec;
+        if (k < ce) {
+            const uint32_t pv = sp[t];
+            if (pv != kNone16) parent[(n - 1) + k] = cs + (int)pv;
         }
-    }
-    if (k < ce && sp[t] != kUnset) parent[(n - 1) + k] = sp[t];
-    if (t == 0) {
-        chunkRoot[blockIdx.x] = sRoot;
-        if (sRoot >= 0) {  // the node [cs, sRoot] (a right child: index cs) with its box and deltas
-            ClimbRec rec;
-            const float* cb = &nbox[6 * (sRoot == cs ? 0 : kChunk)];
-#pragma unroll
-            for (int q = 0; q < 6; q++) rec.box[q] = cb[q];
-            rec.a = dl[0];
-            rec.b = dl[sRoot + 1 - cs];
-            chunkRec[blockIdx.x] = rec;
+        if (t == 0) {
+            const int rr = sRoot[0];
+            chunkRoot[chunk] = rr >= 0 ? cs + rr : -1;
+            if (rr >= 0) {  // the node [cs, cs + rr] (a right child: index cs) with its box and end deltas
+                ClimbRec rec;
+                const float* cb = &nbox[6 * (rr == 0 ? 0 : kChunk)];
+    #pragma unroll
+                for (int q = 0; q < 6; q++) rec.box[q] = cb[q];
+                rec.a = sRoot[1];
+                rec.b = sRoot[2];
+                chunkRec[chunk] = rec;
+            }
+            if (cs == 0) parent[0] = -1;
         }
-        if (cs == 0) parent[0] = -1;
+        lds_barrier();  // the chunk's LDS is read before the next chunk overwrites it
     }
 }
 
@@ -801,6 +901,22 @@ __global__ __launch_bounds__(kBlock) void k_pack_tris(const float* __restrict__ 
 }
 
 inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+// Level-1 workgroups resident on the current device at once (the persistent grid), from the
+// occupancy calculator; kept per device after the first query.
+static int tree_l1_grid() {
+    static std::atomic<int> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    int g = cached[dev].load(std::memory_order_relaxed);
+    if (g > 0) return g;
+    int perCu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_tree_l1, kChunk, 0) != hipSuccess || perCu < 1) perCu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    g = perCu * cus;
+    cached[dev].store(g, std::memory_order_relaxed);
+    return g;
+}
 // blocks of the scene-bound kernel (a grid-stride loop: few partial bounds to reduce)
 inline int bound_blocks(long n) { return std::min(blocks_for(n), 2048); }
 
@@ -1386,7 +1502,7 @@ hipError_t bvh_enqueue(BvhBuffers& b, const float* soup, int n, int mode, bool p
     }
     if (ev) (void)hipEventRecord(ev->e[2], s);
     const int chunks = (n + kChunk - 1) / kChunk;
-    k_tree_l1<<<chunks, kChunk, 0, s>>>(b.keysA, pb, b.valsA, n, mode, b.left, b.right, b.parent, b.box, b.slots,
+    k_tree_l1<<<std::min(chunks, tree_l1_grid()), kChunk, 0, s>>>(b.keysA, pb, b.valsA, n, mode, b.left, b.right, b.parent, b.box, b.slots,
                                         b.pend, b.chunkRoot, b.chunkRec, b.status);
     const int chunks2 = (n + kChunk2 - 1) / kChunk2;
     if (chunks > 1)
