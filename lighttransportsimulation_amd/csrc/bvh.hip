@@ -590,6 +590,7 @@ __global__ __launch_bounds__(kChunk) __attribute__((amdgpu_num_sgpr(64))) void k
         // round r reads list r & 1 and appends to list (r + 1) & 1 through counter (r + 1) % 3, while
         // counter (r + 2) % 3, last read before the barrier that opened round r, is cleared
         int count = n > 1 ? m : 0;
+        uint32_t w = 0;  // the thread's item of the round (rounds >= 1)
         for (int round = 0; count > 0; round++) {
             if (round == kChunk) {  // every round completes at least one node: never reached
                 if (t == 0) atomicOr(status, kStatusClimb);
@@ -600,7 +601,6 @@ __global__ __launch_bounds__(kChunk) __attribute__((amdgpu_num_sgpr(64))) void k
             if (t < count) {
                 int l = t, r = t, a = dL, b = dR;
                 if (round > 0) {
-                    const uint32_t w = work[round & 1][t];
                     l = (int)(w & 511u);
                     r = (int)((w >> 9) & 511u);
                     a = (int)((w >> 18) & 127u) - 1;
@@ -654,6 +654,7 @@ __global__ __launch_bounds__(kChunk) __attribute__((amdgpu_num_sgpr(64))) void k
             if (t == 0) workCount[(round + 2) % 3] = 0;
             lds_barrier();
             count = workCount[(round + 1) % 3];
+            w = work[(round + 1) & 1][t];  // read with the count that bounds it: one LDS round trip
         }
         // the next chunk's inputs arrived during the climbs: wait for them here, before the stores
         // below, so that the wait at the next chunk does not also wait for those stores
