@@ -590,7 +590,8 @@ int render_wavefront(lts_ctx* c, const std::vector<lts_ctx*>& scenes, const lts_
         const int s1 = std::min(b1, s0 + pass);
         PassCache pc;
         if (cached) {
-            // training paths: every stride-th path of the pass from an offset that moves from pass to pass
+            // training paths: every stride-th (sample, pixel) of scene 0 in the pass, in sample-major
+            // row-major order, from an offset that moves from pass to pass
             const size_t paths0 = npix0 * (size_t)(s1 - s0);
             pc.queryBounce = c->nrcPasses < (uint64_t)c->nrcCfg.warmup_passes ? (1 << 30) : c->nrcCfg.query_bounce;
             pc.trainStride = (int)((paths0 + trainPaths - 1) / trainPaths);

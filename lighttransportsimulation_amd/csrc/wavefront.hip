@@ -54,12 +54,16 @@ LTS_D void path_origin(const WfParams& P, const WfSceneSlot* slots, long i, int&
     pixel = band_pixel((int)(local % npix), sl.width, sl.height, P.tileRows);
 }
 
-// Training paths of the radiance cache: every trainStride-th path of scene 0 from trainOffset; the
-// slot of path k, or -1.
-LTS_D int train_slot(const WfParams& P, int scene, long k) {
-    if (P.trainSlots > 0 && scene == 0 && k >= P.trainOffset) {
-        const long d = k - P.trainOffset;
-        if (d % P.trainStride == 0 && d / P.trainStride < P.trainSlots) return (int)(d / P.trainStride);
+// Training paths of the radiance cache: in scene 0, every trainStride-th (sample, pixel) pair of
+// the pass in sample-major, row-major order from trainOffset, so that the set and its slot order
+// do not depend on the order in which the pass walks its pixels (tileRows); the slot, or -1.
+LTS_D int train_slot(const WfParams& P, const WfSceneSlot& sl, int scene, int pixel, int sample) {
+    if (P.trainSlots > 0 && scene == 0) {
+        const long c = (long)(sample - sl.s0) * ((long)sl.width * sl.height) + pixel;
+        if (c >= P.trainOffset) {
+            const long d = c - P.trainOffset;
+            if (d % P.trainStride == 0 && d / P.trainStride < P.trainSlots) return (int)(d / P.trainStride);
+        }
     }
     return -1;
 }
@@ -80,7 +84,7 @@ __global__ __launch_bounds__(256) void k_wf_generate(const SceneView* __restrict
         const WfSceneSlot& sl = slots[sc];
         const int px = pixel % sv.cam.width, py = pixel / sv.cam.width;
         CamRays cr = camera_sample(sv, px, py, sample, sl.spp, sl.stratified, sl.seed);
-        const int ts = train_slot(P, sc, k);
+        const int ts = train_slot(P, sl, sc, pixel, sample);
         if (ts >= 0)
             for (int b = 0; b < P.maxBounce; b++) tRec[(size_t)ts * kWfMaxBounce + b].valid = 0;
         WfRay r;
@@ -277,10 +281,10 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_wf_shade(const SceneView* __
                 sample = (int)(p.sampleFlags >> kWfFlagBits);
                 pflags = (int)(p.sampleFlags & ((1u << kWfFlagBits) - 1));
             }
-            const int trainSlot = cached ? train_slot(P, scene, pi) : -1;
             const WfHit hh = hits[pi];
             const SceneView& sv = scenes[scene];
             const WfSceneSlot& sl = slots[scene];
+            const int trainSlot = cached ? train_slot(P, sl, scene, pixel, sample) : -1;
             Ray r;
             CamRays cam;
             if (kFirst) {

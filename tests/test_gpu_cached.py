@@ -95,3 +95,22 @@ def test_reallocated_cache_buffers_recapture_the_pass_graph(gpu_ctx, cornell):
         results.append(img)
     # same configuration, same seed: the same image
     assert np.array_equal(results[0].view(np.uint32), results[3].view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_cached_image_does_not_depend_on_the_tuning(gpu_ctx, cornell):
+    """The cache's training paths are chosen by (sample, pixel), so the images of a cache trained
+    and queried under different tunings (pixel walk, traversal blocks, material sort) are the same
+    bit for bit; so are two runs of one tuning."""
+    d, cfg = cornell
+    gpu_ctx.load_scene(d, cfg["scene"], cfg["subject"], W, H)
+    images = []
+    for tune in (dict(tile_rows=1), dict(tile_rows=8), dict(tile_rows=8), dict(tile_rows=4, trace_block=64, sort_materials=0)):
+        print(f"[cached tuning {tune}]", flush=True)
+        gpu_ctx.nrc_configure(warmup_passes=2, seed=11, train_steps=2)
+        for f in range(3):
+            img, _, st = gpu_ctx.render(8, 5, 3, SEED + f, integrator=INTEGRATOR_CACHED, spp_per_pass=2, **tune)
+        assert st["nrc_queries"] > 0 and st["nrc_steps"] > 0
+        images.append(img)
+    for img in images[1:]:
+        assert np.array_equal(img.view(np.uint32), images[0].view(np.uint32))
