@@ -27,9 +27,17 @@ struct SurfacePoint {
     int leaf;  // BVH leaf of the triangle, which indexes its ShadeTri record
 };
 
+// The scene's arrays live in device (global) memory. Kernels that read a SceneView from memory see
+// its pointers as generic ones, whose flat loads also count against the LDS wait counter (so a wait
+// for a traversal-stack read waits for node loads too); the traversal's loads go through global
+// pointers instead.
+typedef float GlobalF4 __attribute__((ext_vector_type(4)));
+typedef unsigned GlobalU4 __attribute__((ext_vector_type(4)));
+LTS_D GlobalF4 load_global_f4(const void* p, int i) { return ((const __attribute__((address_space(1))) GlobalF4*)p)[i]; }
+LTS_D GlobalU4 load_global_u4(const void* p, int i) { return ((const __attribute__((address_space(1))) GlobalU4*)p)[i]; }
+
 LTS_D void load_tri(const TriLeaf* __restrict__ tris, int k, V3& p0, V3& p1, V3& p2, int& prim) {
-    const float4* q = reinterpret_cast<const float4*>(tris + k);
-    float4 a = q[0], b = q[1], c = q[2];
+    const GlobalF4 a = load_global_f4(tris + k, 0), b = load_global_f4(tris + k, 1), c = load_global_f4(tris + k, 2);
     p0 = v3(a.x, a.y, a.z);
     p1 = v3(a.w, b.x, b.y);
     p2 = v3(b.z, b.w, c.x);
@@ -150,8 +158,8 @@ LTS_D TraceStack lds_stack(int* shared, int tid, int blockSize) { return TraceSt
 // the exact arithmetic passes still passes, so the test stays conservative.
 LTS_D void node4_hits(const BvhQNode* nodes4, int cur, const Ray& r, V3 inv, int nx, int ny, int nz, float (&t)[4],
                       int (&c)[4], int& count) {
-    const uint4* q = reinterpret_cast<const uint4*>(nodes4 + cur);
-    const uint4 h = q[0], ch = q[1], lo = q[2], hi = q[3];
+    const GlobalU4 h = load_global_u4(nodes4 + cur, 0), ch = load_global_u4(nodes4 + cur, 1), lo = load_global_u4(nodes4 + cur, 2),
+                   hi = load_global_u4(nodes4 + cur, 3);
     const float ox = __uint_as_float(h.x), oy = __uint_as_float(h.y), oz = __uint_as_float(h.z);
     const int ex = (int)(int8_t)(h.w & 0xffu), ey = (int)(int8_t)((h.w >> 8) & 0xffu), ez = (int)(int8_t)((h.w >> 16) & 0xffu);
     count = (int)(h.w >> 24);
