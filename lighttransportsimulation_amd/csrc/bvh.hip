@@ -501,7 +501,6 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-constexpr int kTreeWaves = kChunk / 64;
 constexpr uint32_t kNone16 = 0xffffu;
 static_assert(kChunk <= 512, "level-1 items pack chunk-local ends in 9 bits");
 
@@ -513,9 +512,14 @@ __device__ __forceinline__ uint32_t pack_item(int l, int r, int dL, int dR) {
 // Level-1 inputs of one thread for a chunk, loaded a chunk ahead: the codes around leaf cs + t
 // and a third of the chunk's leaf boxes (mode 0: float2 runs t, t + kChunk, t + 2 kChunk of the
 // contiguous leaf section; otherwise the box of leaf cs + t).
+constexpr int kTreeThreads = kChunk / 2;  // a thread per two leaves: five chunks share a CU's LDS
+
+// Level-1 inputs of one thread for a chunk, loaded a chunk ahead: the codes around leaves cs + t and
+// cs + t + kTreeThreads, and a third of the chunk's leaf boxes (mode 0: float2 entries t + j
+// kTreeThreads of the contiguous leaf section; otherwise the boxes of the thread's two leaves).
 struct TreeL1Fetch {
-    uint32_t cPrev, c, cNext;
-    float2 b[3];
+    uint32_t cPrev[2], c[2], cNext[2];
+    float2 b[6];
 };
 
 __device__ __forceinline__ void tree_l1_fetch(TreeL1Fetch& f, const uint32_t* __restrict__ codes,
@@ -523,28 +527,36 @@ __device__ __forceinline__ void tree_l1_fetch(TreeL1Fetch& f, const uint32_t* __
                                               int n, int mode, int chunk, int t) {
     // every load is unconditional, at clamped in-range addresses (lanes past the end reload the last
     // entries), so no branch joins the loaded registers and nothing waits for them until they are used
-    const int cs = chunk * kChunk, m = min(kChunk, n - cs), k = cs + min(t, m - 1);
-    int pk = 0;
-    if (mode != 0) pk = perm[k];  // first: the box loads wait for it, the code loads need not
-    f.c = codes[k];
-    f.cPrev = codes[max(k - 1, 0)];
-    f.cNext = codes[min(k + 1, n - 1)];
-    const float2* src = reinterpret_cast<const float2*>(primBox) + 3 * (size_t)cs;
-    const float2* own = reinterpret_cast<const float2*>(primBox) + 3 * (size_t)pk;
+    const int cs = chunk * kChunk, m = min(kChunk, n - cs);
+    int pk[2] = {0, 0};
 #pragma unroll
-    for (int j = 0; j < 3; j++) f.b[j] = mode == 0 ? src[min(t + j * kChunk, 3 * m - 1)] : own[j];
+    for (int h = 0; h < 2; h++)
+        if (mode != 0) pk[h] = perm[cs + min(t + h * kTreeThreads, m - 1)];  // first: the box loads wait for them
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int k = cs + min(t + h * kTreeThreads, m - 1);
+        f.c[h] = codes[k];
+        f.cPrev[h] = codes[max(k - 1, 0)];
+        f.cNext[h] = codes[min(k + 1, n - 1)];
+    }
+    const float2* src = reinterpret_cast<const float2*>(primBox) + 3 * (size_t)cs;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        const float2* own = reinterpret_cast<const float2*>(primBox) + 3 * (size_t)pk[j / 3];
+        f.b[j] = mode == 0 ? src[min(t + j * kTreeThreads, 3 * m - 1)] : own[j % 3];
+    }
 }
 
 // Persistent: workgroup b builds chunks b, b + gridDim.x, ..., loading each chunk's inputs while
 // the previous chunk climbs, so the climbs are not held up by global-load latency. The climbs are
-// latency-bound, so residency is what counts: at most 64 SGPRs (a few spilled to VGPR lanes) let
-// four workgroups share a CU, where the 87 the compiler picks by itself allow three (measured).
-__global__ __launch_bounds__(kChunk) __attribute__((amdgpu_num_sgpr(64))) void k_tree_l1(const uint32_t* __restrict__ codes, const float* __restrict__ primBox,
-                                                    const int32_t* __restrict__ perm, int n, int mode, int32_t* __restrict__ left,
-                                                    int32_t* __restrict__ right, int32_t* __restrict__ parent,
-                                                    float* __restrict__ box, int32_t* __restrict__ slots,
-                                                    ClimbRec* __restrict__ pend, int32_t* __restrict__ chunkRoot,
-                                                    ClimbRec* __restrict__ chunkRec, int32_t* __restrict__ status) {
+// latency-bound, so residency is what counts: 256 threads per 512-leaf chunk (round 0 takes the
+// leaves in two halves; no later round has more than 256 nodes) let five chunks share a CU's LDS,
+// and at most 64 SGPRs keep the register files from allowing fewer.
+__global__ __launch_bounds__(kTreeThreads) __attribute__((amdgpu_num_sgpr(64))) void k_tree_l1(
+    const uint32_t* __restrict__ codes, const float* __restrict__ primBox, const int32_t* __restrict__ perm, int n, int mode,
+    int32_t* __restrict__ left, int32_t* __restrict__ right, int32_t* __restrict__ parent, float* __restrict__ box,
+    int32_t* __restrict__ slots, ClimbRec* __restrict__ pend, int32_t* __restrict__ chunkRoot, ClimbRec* __restrict__ chunkRec,
+    int32_t* __restrict__ status) {
     __shared__ __align__(16) float nbox[2 * kChunk * 6];
     // split cs + x: -1, or a first arrival's outer end - cs | (delta at that end + 1) << 9
     __shared__ int32_t slot[kChunk];
@@ -552,109 +564,127 @@ __global__ __launch_bounds__(kChunk) __attribute__((amdgpu_num_sgpr(64))) void k
     // the split, bit 10 when its right child is the leaf after it (else the interior nodes there)
     __shared__ uint16_t ssplit[kChunk];
     __shared__ uint16_t sp[2 * kChunk];  // parent of each local node - cs (kNone16: set by a later level)
-    __shared__ uint32_t work[2][kChunk];  // round r's items in list r & 1
-    __shared__ int32_t workCount[3];      // round r's items: counter r % 3
-    __shared__ int32_t sRoot[3];  // the chunk's root: its right end - cs (or -1) and its end deltas
+    // round r >= 1 reads list r & 1. A round's items are the nodes the round before completed: at
+    // most half the leaves after round 0, and never more than the items of the round before.
+    __shared__ uint32_t work[2][kTreeThreads];
+    __shared__ int32_t workCount[3];  // items appended for round r: counter r % 3
+    __shared__ int32_t sRoot[3];      // the chunk's root: its right end - cs (or -1) and its end deltas
     const int t = threadIdx.x, lane = t & 63;
     const int chunks = (n + kChunk - 1) / kChunk;
     TreeL1Fetch f;
     if ((int)blockIdx.x < chunks) tree_l1_fetch(f, codes, primBox, perm, n, mode, blockIdx.x, t);
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): every chunk starts with its inputs in registers
+    // one climb step of node [l, r] (chunk-local, end deltas a, b): true, with the completed
+    // parent's item, for a second arrival; files the chunk's root
+    auto step = [&](int l, int r, int a, int b, uint32_t& item) -> bool {
+        if (a < 0 && b < 0) return false;  // the root of the whole tree
+        const bool asLeft = b > a;
+        const int p = asLeft ? r : l - 1;
+        if (p < 0) {  // its parent splits left of the chunk: the chunk's root (one node)
+            sRoot[0] = r;
+            sRoot[1] = a;
+            sRoot[2] = b;
+            return false;
+        }
+        const int mine = asLeft ? (l | ((a + 1) << 9)) : (r | ((b + 1) << 9));
+        const int other = atomicExch(&slot[p], mine);
+        if (other < 0) return false;  // first arrival
+        slot[p] = -1;
+        const int oe = other & 511, od = (other >> 9) - 1;
+        const int L = asLeft ? l : oe, R = asLeft ? oe : r;
+        const int pL = asLeft ? a : od, pR = asLeft ? od : b;
+        const bool leafL = L == p, leafR = R == p + 1;
+        const int li = leafL ? p : kChunk + p, ri = leafR ? p + 1 : kChunk + p + 1;
+        float x[6], y[6], o[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            x[q] = nbox[6 * li + q];
+            y[q] = nbox[6 * ri + q];
+        }
+        union_box(x, y, o);
+        // Karras numbering; local 0 is the whole tree's root only in chunk 0
+        const int me = (pL < 0 && pR < 0) ? 0 : (pR > pL ? R : L);
+#pragma unroll
+        for (int q = 0; q < 6; q++) nbox[6 * (kChunk + me) + q] = o[q];
+        ssplit[me] = (uint16_t)(p | (leafL ? 512 : 0) | (leafR ? 1024 : 0));
+        sp[li] = (uint16_t)me;
+        sp[ri] = (uint16_t)me;
+        item = pack_item(L, R, pL, pR);
+        return true;
+    };
+    // the completed parents of the lanes with `next` join list `list`: one LDS atomic per wave
+    auto append = [&](int list, int counter, bool next, uint32_t item) {
+        const unsigned long long ballot = __ballot(next);
+        if (ballot) {
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&workCount[counter], (int)__popcll(ballot));
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (next) work[list][base + (int)__popcll(ballot & ((1ull << lane) - 1ull))] = item;
+        }
+    };
     for (int chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {
         const int cs = chunk * kChunk, ce = min(cs + kChunk, n);
-        const int k = cs + t;
         const int m = ce - cs;  // leaves of the chunk
-        int dL = -1, dR = -1;   // deltas at both ends of leaf k
-        if (k < ce) {
-            if (k >= 1) dL = delta_codes(f.cPrev, f.c, n, k - 1, k);
-            if (k + 1 < n) dR = delta_codes(f.c, f.cNext, n, k, k + 1);
+        int dL[2] = {-1, -1}, dR[2] = {-1, -1};  // deltas at both ends of the thread's two leaves
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int k = cs + t + h * kTreeThreads;
+            if (k < ce) {
+                if (k >= 1) dL[h] = delta_codes(f.cPrev[h], f.c[h], n, k - 1, k);
+                if (k + 1 < n) dR[h] = delta_codes(f.c[h], f.cNext[h], n, k, k + 1);
+            }
         }
         if (t == 0) sRoot[0] = -1;
         if (t < 3) workCount[t] = 0;
-        slot[t] = -1;
-        ssplit[t] = kNone16;
-        sp[t] = kNone16;
-        sp[kChunk + t] = kNone16;
-        if (mode == 0) {
 #pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const int i = t + j * kChunk;
+        for (int h = 0; h < 2; h++) {
+            slot[t + h * kTreeThreads] = -1;
+            ssplit[t + h * kTreeThreads] = kNone16;
+        }
+#pragma unroll
+        for (int h = 0; h < 4; h++) sp[t + h * kTreeThreads] = kNone16;
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            if (mode == 0) {
+                const int i = t + j * kTreeThreads;
                 if (i < 3 * m) reinterpret_cast<float2*>(nbox)[i] = f.b[j];
+            } else {
+                const int x = t + (j / 3) * kTreeThreads;
+                if (x < m) reinterpret_cast<float2*>(nbox)[3 * x + j % 3] = f.b[j];
             }
-        } else if (k < ce) {
-#pragma unroll
-            for (int j = 0; j < 3; j++) reinterpret_cast<float2*>(nbox)[3 * t + j] = f.b[j];
         }
         tree_l1_fetch(f, codes, primBox, perm, n, mode, min(chunk + (int)gridDim.x, chunks - 1), t);
         lds_barrier();
-        // round r reads list r & 1 and appends to list (r + 1) & 1 through counter (r + 1) % 3, while
-        // counter (r + 2) % 3, last read before the barrier that opened round r, is cleared
-        int count = n > 1 ? m : 0;
-        uint32_t w = 0;  // the thread's item of the round (rounds >= 1)
-        for (int round = 0; count > 0; round++) {
-            if (round == kChunk) {  // every round completes at least one node: never reached
-                if (t == 0) atomicOr(status, kStatusClimb);
-                break;
+        if (n > 1) {
+            // round 0: every leaf, in two halves
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int x = t + h * kTreeThreads;
+                uint32_t item = 0;
+                const bool next = x < m && step(x, x, dL[h], dR[h], item);
+                append(1, 1, next, item);
             }
-            bool next = false;
-            uint32_t item = 0;
-            if (t < count) {
-                int l = t, r = t, a = dL, b = dR;
-                if (round > 0) {
-                    l = (int)(w & 511u);
-                    r = (int)((w >> 9) & 511u);
-                    a = (int)((w >> 18) & 127u) - 1;
-                    b = (int)(w >> 25) - 1;
-                }
-                if (a >= 0 || b >= 0) {  // not the root of the whole tree
-                    const bool asLeft = b > a;
-                    const int p = asLeft ? r : l - 1;
-                    if (p < 0) {  // its parent splits left of the chunk: the chunk's root (one node)
-                        sRoot[0] = r;
-                        sRoot[1] = a;
-                        sRoot[2] = b;
-                    } else {
-                        const int mine = asLeft ? (l | ((a + 1) << 9)) : (r | ((b + 1) << 9));
-                        const int other = atomicExch(&slot[p], mine);
-                        if (other >= 0) {  // second arrival: complete the parent
-                            slot[p] = -1;
-                            const int oe = other & 511, od = (other >> 9) - 1;
-                            const int L = asLeft ? l : oe, R = asLeft ? oe : r;
-                            const int pL = asLeft ? a : od, pR = asLeft ? od : b;
-                            const bool leafL = L == p, leafR = R == p + 1;
-                            const int li = leafL ? p : kChunk + p, ri = leafR ? p + 1 : kChunk + p + 1;
-                            float x[6], y[6], o[6];
-    #pragma unroll
-                            for (int q = 0; q < 6; q++) {
-                                x[q] = nbox[6 * li + q];
-                                y[q] = nbox[6 * ri + q];
-                            }
-                            union_box(x, y, o);
-                            // Karras numbering; local 0 is the whole tree's root only in chunk 0
-                            const int me = (pL < 0 && pR < 0) ? 0 : (pR > pL ? R : L);
-    #pragma unroll
-                            for (int q = 0; q < 6; q++) nbox[6 * (kChunk + me) + q] = o[q];
-                            ssplit[me] = (uint16_t)(p | (leafL ? 512 : 0) | (leafR ? 1024 : 0));
-                            sp[li] = (uint16_t)me;
-                            sp[ri] = (uint16_t)me;
-                            item = pack_item(L, R, pL, pR);
-                            next = true;
-                        }
-                    }
-                }
-            }
-            // the completed parents form the next list: one LDS atomic per wave reserves its entries
-            const unsigned long long ballot = __ballot(next);
-            if (ballot) {
-                int base = 0;
-                if (lane == 0) base = atomicAdd(&workCount[(round + 1) % 3], (int)__popcll(ballot));
-                base = __builtin_amdgcn_readfirstlane(base);
-                if (next) work[(round + 1) & 1][base + (int)__popcll(ballot & ((1ull << lane) - 1ull))] = item;
-            }
-            if (t == 0) workCount[(round + 2) % 3] = 0;
+            if (t == 0) workCount[2] = 0;
             lds_barrier();
-            count = workCount[(round + 1) % 3];
-            w = work[(round + 1) & 1][t];  // read with the count that bounds it: one LDS round trip
+            // round r >= 1 reads list r & 1 (its count in counter r % 3) and appends to list
+            // (r + 1) & 1 through counter (r + 1) % 3, while counter (r + 2) % 3, last read before the
+            // barrier that opened round r, is cleared
+            int count = workCount[1];
+            uint32_t w = work[1][t];  // read with the count that bounds it: one LDS round trip
+            for (int round = 1; count > 0; round++) {
+                if (round == kChunk) {  // every round completes at least one node: never reached
+                    if (t == 0) atomicOr(status, kStatusClimb);
+                    break;
+                }
+                uint32_t item = 0;
+                const bool next = t < count && step((int)(w & 511u), (int)((w >> 9) & 511u), (int)((w >> 18) & 127u) - 1,
+                                                    (int)(w >> 25) - 1, item);
+                append((round + 1) & 1, (round + 1) % 3, next, item);
+                if (t == 0) workCount[(round + 2) % 3] = 0;
+                lds_barrier();
+                count = workCount[(round + 1) % 3];
+                w = work[(round + 1) & 1][t];
+            }
         }
         // the next chunk's inputs arrived during the climbs: wait for them here, before the stores
         // below, so that the wait at the next chunk does not also wait for those stores
@@ -666,39 +696,40 @@ __global__ __launch_bounds__(kChunk) __attribute__((amdgpu_num_sgpr(64))) void k
             float* dst = box + 6 * (size_t)cs;
             const float* ib = nbox + 6 * kChunk;
             const int whole = mi > 0 ? (6 * mi) / 4 : 0;
-            for (int i = t; i < whole; i += kChunk) reinterpret_cast<float4*>(dst)[i] = reinterpret_cast<const float4*>(ib)[i];
-            for (int i = 4 * whole + t; i < 6 * mi; i += kChunk) dst[i] = ib[i];
+            for (int i = t; i < whole; i += kTreeThreads) reinterpret_cast<float4*>(dst)[i] = reinterpret_cast<const float4*>(ib)[i];
+            for (int i = 4 * whole + t; i < 6 * mi; i += kTreeThreads) dst[i] = ib[i];
             if (mode != 0) {
                 float2* ldst = reinterpret_cast<float2*>(box + 6 * ((size_t)(n - 1) + cs));
-                for (int i = t; i < 3 * m; i += kChunk) ldst[i] = reinterpret_cast<const float2*>(nbox)[i];
+                for (int i = t; i < 3 * m; i += kTreeThreads) ldst[i] = reinterpret_cast<const float2*>(nbox)[i];
             }
         }
-        if (k < ce && k < n - 1) {
-            const uint32_t sv = ssplit[t];
-            if (sv != kNone16) {
-                const int g = cs + (int)(sv & 511u);
-                left[k] = (sv & 512u) ? (n - 1) + g : g;
-                right[k] = (sv & 1024u) ? (n - 1) + g + 1 : g + 1;
+        for (int x = t; x < m; x += kTreeThreads) {
+            const int k = cs + x;
+            if (k < n - 1) {
+                const uint32_t sv = ssplit[x];
+                if (sv != kNone16) {
+                    const int g = cs + (int)(sv & 511u);
+                    left[k] = (sv & 512u) ? (n - 1) + g : g;
+                    right[k] = (sv & 1024u) ? (n - 1) + g + 1 : g + 1;
+                }
+                const uint32_t pv = sp[kChunk + x];
+                if (pv != kNone16) parent[k] = cs + (int)pv;
+                const int sw = slot[x];
+                const int v = sw >= 0 ? cs + (sw & 511) : -1;
+                slots[k] = v;
+                if (v >= 0) {  // a first arrival waits at split k: publish its box and the delta at its outer end
+                    const bool leftChild = v <= k;
+                    const int child = leftChild ? (v == k ? (n - 1) + k : k) : (v == k + 1 ? (n - 1) + k + 1 : k + 1);
+                    ClimbRec rec;
+                    const float* cb = &nbox[6 * local_index(child, n, cs)];
+#pragma unroll
+                    for (int q = 0; q < 6; q++) rec.box[q] = cb[q];
+                    rec.a = (sw >> 9) - 1;
+                    rec.b = 0;
+                    pend[k] = rec;
+                }
             }
-            const uint32_t pv = sp[kChunk + t];
-            if (pv != kNone16) parent[k] = cs + (int)pv;
-            const int w = slot[t];
-            const int v = w >= 0 ? cs + (w & 511) : -1;
-            slots[k] = v;
-            if (v >= 0) {  // a first arrival waits at split k: publish its box and the delta at its outer end
-                const bool leftChild = v <= k;
-                const int child = leftChild ? (v == k ? (n - 1) + k : k) : (v == k + 1 ? (n - 1) + k + 1 : k + 1);
-                ClimbRec rec;
-                const float* cb = &nbox[6 * local_index(child, n, cs)];
-    #pragma unroll
-                for (int q = 0; q < 6; q++) rec.box[q] = cb[q];
-                rec.a = (w >> 9) - 1;
-                rec.b = 0;
-                pend[k] = rec;
-            }
-        }
-        if (k < ce) {
-            const uint32_t pv = sp[t];
+            const uint32_t pv = sp[x];
             if (pv != kNone16) parent[(n - 1) + k] = cs + (int)pv;
         }
         if (t == 0) {
@@ -707,7 +738,7 @@ __global__ __launch_bounds__(kChunk) __attribute__((amdgpu_num_sgpr(64))) void k
             if (rr >= 0) {  // the node [cs, cs + rr] (a right child: index cs) with its box and end deltas
                 ClimbRec rec;
                 const float* cb = &nbox[6 * (rr == 0 ? 0 : kChunk)];
-    #pragma unroll
+#pragma unroll
                 for (int q = 0; q < 6; q++) rec.box[q] = cb[q];
                 rec.a = sRoot[1];
                 rec.b = sRoot[2];
@@ -912,7 +943,7 @@ static int tree_l1_grid() {
     int g = cached[dev].load(std::memory_order_relaxed);
     if (g > 0) return g;
     int perCu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_tree_l1, kChunk, 0) != hipSuccess || perCu < 1) perCu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_tree_l1, kTreeThreads, 0) != hipSuccess || perCu < 1) perCu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
     g = perCu * cus;
     cached[dev].store(g, std::memory_order_relaxed);
@@ -1503,7 +1534,7 @@ hipError_t bvh_enqueue(BvhBuffers& b, const float* soup, int n, int mode, bool p
     }
     if (ev) (void)hipEventRecord(ev->e[2], s);
     const int chunks = (n + kChunk - 1) / kChunk;
-    k_tree_l1<<<std::min(chunks, tree_l1_grid()), kChunk, 0, s>>>(b.keysA, pb, b.valsA, n, mode, b.left, b.right, b.parent, b.box, b.slots,
+    k_tree_l1<<<std::min(chunks, tree_l1_grid()), kTreeThreads, 0, s>>>(b.keysA, pb, b.valsA, n, mode, b.left, b.right, b.parent, b.box, b.slots,
                                         b.pend, b.chunkRoot, b.chunkRec, b.status);
     const int chunks2 = (n + kChunk2 - 1) / kChunk2;
     if (chunks > 1)
