@@ -282,12 +282,20 @@ __global__ __launch_bounds__(kSortBlock) void k_scan_segments(uint32_t* __restri
 
 // One block: thread d turns the segment totals of digit d into its base in each segment: the
 // exclusive scan of the digit totals (the digit's base in the pass) plus the totals of the earlier
-// segments (in place).
+// segments (in place). The first kScanHeld segments (every segment up to 2^23 keys) are read once,
+// all loads in flight together, and kept in registers for the second sweep; later ones are read
+// again in windows.
+constexpr int kScanHeld = 2 * kScanWindow;
 __global__ __launch_bounds__(kBins) void k_scan_digits(uint32_t* __restrict__ segBase, int segs) {
     __shared__ uint32_t scratch[kBins / 64];
     const int d = threadIdx.x;
+    uint32_t held[kScanHeld];
     uint32_t total = 0;
-    for (int g0 = 0; g0 < segs; g0 += kScanWindow) {
+#pragma unroll
+    for (int i = 0; i < kScanHeld; i++) held[i] = i < segs ? segBase[(size_t)i * kBins + d] : 0u;
+#pragma unroll
+    for (int i = 0; i < kScanHeld; i++) total += held[i];
+    for (int g0 = kScanHeld; g0 < segs; g0 += kScanWindow) {
         uint32_t v[kScanWindow];
 #pragma unroll
         for (int i = 0; i < kScanWindow; i++) v[i] = (g0 + i < segs) ? segBase[(size_t)(g0 + i) * kBins + d] : 0u;
@@ -296,7 +304,12 @@ __global__ __launch_bounds__(kBins) void k_scan_digits(uint32_t* __restrict__ se
     }
     uint32_t run;
     block_scan<kBins>(total, &run, scratch);
-    for (int g0 = 0; g0 < segs; g0 += kScanWindow) {
+#pragma unroll
+    for (int i = 0; i < kScanHeld; i++) {
+        if (i < segs) segBase[(size_t)i * kBins + d] = run;
+        run += held[i];
+    }
+    for (int g0 = kScanHeld; g0 < segs; g0 += kScanWindow) {
         uint32_t v[kScanWindow];
 #pragma unroll
         for (int i = 0; i < kScanWindow; i++) v[i] = (g0 + i < segs) ? segBase[(size_t)(g0 + i) * kBins + d] : 0u;
