@@ -147,6 +147,7 @@ typedef struct lts_stats {
 /* Context. */
 int lts_create(lts_ctx** out, int hip_device);
 void lts_destroy(lts_ctx* ctx);
+/* With ctx NULL: the message of the calling thread's last failed call that takes no context. */
 const char* lts_last_error(const lts_ctx* ctx);
 const char* lts_version(void);
 
@@ -300,9 +301,9 @@ int lts_device_copy_benchmark(lts_ctx* ctx, uint64_t bytes, int32_t repeat, doub
  * rgb rows are top first, as lts_render returns them. */
 int lts_write_image(const char* path, const float* rgb, int32_t width, int32_t height, int32_t format);
 
-/* Replaces loadImageFile_s (core/ImageI.cuh:11-37): decodes an 8-bit image file (PNG or binary PPM)
- * to linear RGB with pow(c / 255, 2.2), bottom row first, as the
- * texture loader does. *width and *height receive the size; rgb_out (may be NULL to query the size)
+/* Replaces loadImageFile_s (core/ImageI.cuh:11-37): decodes an 8-bit image file (PNG, binary PPM
+ * or baseline JPEG, the last to the bytes of the reference's stb_image 2.27) to linear RGB with
+ * pow(c / 255, 2.2), bottom row first, as the texture loader does. *width and *height receive the size; rgb_out (may be NULL to query the size)
  * receives width * height * 3 floats when cap_floats is large enough, else LTS_E_ARG. LTS_E_IO for
  * a missing, malformed or unsupported file. No GPU use. */
 int lts_load_image(const char* path, int32_t* width, int32_t* height, float* rgb_out, int64_t cap_floats);

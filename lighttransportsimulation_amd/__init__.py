@@ -377,9 +377,10 @@ def obj_to_bin(mesh_stem: str) -> int:
     """Converts <mesh_stem>.obj to <mesh_stem>.bin (include/lts.h: lts_obj_to_bin); returns the
     triangle count."""
     n = C.c_int32()
-    rc = load_library().lts_obj_to_bin(mesh_stem.encode(), C.byref(n))
+    L = load_library()
+    rc = L.lts_obj_to_bin(mesh_stem.encode(), C.byref(n))
     if rc != 0:
-        raise LtsError(f"lts_obj_to_bin({mesh_stem}) failed ({rc})")
+        raise LtsError(f"lts_obj_to_bin({mesh_stem}) failed ({rc}): {L.lts_last_error(None).decode()}")
     return n.value
 
 
@@ -398,7 +399,7 @@ def load_image(path: str) -> np.ndarray:
     w, h = C.c_int32(), C.c_int32()
     rc = L.lts_load_image(path.encode(), C.byref(w), C.byref(h), None, 0)
     if rc != 0:
-        raise LtsError(f"lts_load_image({path}) failed ({rc})")
+        raise LtsError(f"lts_load_image({path}) failed ({rc}): {L.lts_last_error(None).decode()}")
     out = np.zeros((h.value, w.value, 3), np.float32)
     rc = L.lts_load_image(path.encode(), C.byref(w), C.byref(h), _ptr(out), out.size)
     if rc != 0:

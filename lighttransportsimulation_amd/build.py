@@ -27,7 +27,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-fhip-fp32-correctly-rounded-divide-sqrt", 
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 SOURCES = ["bvh.hip", "render.hip", "wavefront.hip", "nrc.hip", "mesh.hip", "lts_capi.cpp", "scene_host.cpp",
-           "image_io.cpp"]
+           "image_io.cpp", "jpeg.cpp"]
 
 
 def _obj(src: str) -> str:

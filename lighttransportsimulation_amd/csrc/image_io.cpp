@@ -1,4 +1,4 @@
-// Image input (8-bit PNG / binary PPM -> linear RGB) and output (PFM, P3 PPM).
+// Image input (8-bit PNG / binary PPM / baseline JPEG -> linear RGB) and output (PFM, P3 PPM).
 //   input:  the reference decodes with stb_image and converts 8-bit values with
 //           pow(c / 255, 2.2) (core/ImageI.cuh:11-37, extern/stb_image.h:1828)
 //   output: core/ImageO.cuh:16-78
@@ -20,11 +20,6 @@ static bool slurp(const std::string& path, std::vector<uint8_t>& buf) {
     buf.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
     return true;
 }
-
-// Largest image accepted: 32768 texels a side and 2^28 texels (a 16384^2 image), so that the
-// decoded bytes and the pyramid stay far below the sizes where allocation sizes could overflow.
-constexpr uint32_t kMaxImageSide = 1u << 15;
-constexpr uint64_t kMaxImageTexels = 1ull << 28;
 
 static inline uint32_t rd_be32(const uint8_t* p) {
     return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | (uint32_t)p[3];
@@ -132,8 +127,10 @@ static bool ppm_to_rgb8(const std::vector<uint8_t>& b, int& w, int& h, std::vect
 bool load_image_linear(const std::string& path, int& w, int& h, std::vector<float>& rgb, std::string& err) {
     std::vector<uint8_t> file, px;
     if (!slurp(path, file)) { err = "cannot open image " + path; return false; }
-    bool ok = (file.size() > 2 && file[0] == 'P' && file[1] == '6') ? ppm_to_rgb8(file, w, h, px, err)
-                                                                    : png_to_rgb8(file, w, h, px, err);
+    bool ok;
+    if (file.size() > 2 && file[0] == 'P' && file[1] == '6') ok = ppm_to_rgb8(file, w, h, px, err);
+    else if (file.size() > 2 && file[0] == 0xFF && file[1] == 0xD8) ok = jpeg_to_rgb8(file, w, h, px, err);
+    else ok = png_to_rgb8(file, w, h, px, err);
     if (!ok) { err = path + ": " + err; return false; }
     float lin[256];
     for (int c = 0; c < 256; c++) lin[c] = (float)(std::pow((double)(c / 255.0f), (double)2.2f) * (double)1.0f);

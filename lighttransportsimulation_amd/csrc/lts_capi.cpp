@@ -111,8 +111,12 @@ struct lts_ctx {
 
 namespace {
 
+// Message of the last failed call made without a context, per host thread.
+thread_local std::string g_noCtxErr;
+
 int fail(lts_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
+    else g_noCtxErr = msg;
     return code;
 }
 
@@ -830,7 +834,7 @@ void lts_destroy(lts_ctx* c) {
     delete c;
 }
 
-const char* lts_last_error(const lts_ctx* c) { return c ? c->err.c_str() : "no context"; }
+const char* lts_last_error(const lts_ctx* c) { return c ? c->err.c_str() : g_noCtxErr.c_str(); }
 
 int lts_read_app_params(const char* path, int32_t* w, int32_t* h, int32_t* spp, int32_t* bounce, int32_t* rr,
                         int32_t* fmt, char* output, char* scene, char* subject, int32_t cap) {
@@ -1355,7 +1359,7 @@ static int load_image_impl(const char* path, int32_t* w, int32_t* h, float* rgb_
     int iw = 0, ih = 0;
     std::vector<float> rgb;
     std::string err;
-    if (!load_image_linear(path, iw, ih, rgb, err)) return LTS_E_IO;
+    if (!load_image_linear(path, iw, ih, rgb, err)) return fail(nullptr, LTS_E_IO, err);
     *w = iw; *h = ih;
     if (!rgb_out) return LTS_OK;
     if (cap < (int64_t)rgb.size()) return LTS_E_ARG;
@@ -1368,9 +1372,10 @@ static int obj_to_bin_impl(const char* stem, int32_t* nTris) {
     std::vector<char> bin;
     std::string err;
     int n = 0;
-    if (!obj_mesh_bytes(stem, bin, &n, err)) return LTS_E_IO;
+    if (!obj_mesh_bytes(stem, bin, &n, err)) return fail(nullptr, LTS_E_IO, err);
     std::ofstream out(std::string(stem) + ".bin", std::ios::binary);
-    if (!out || !out.write(bin.data(), (std::streamsize)bin.size())) return LTS_E_IO;
+    if (!out || !out.write(bin.data(), (std::streamsize)bin.size()))
+        return fail(nullptr, LTS_E_IO, std::string("cannot write ") + stem + ".bin");
     if (nTris) *nTris = n;
     return LTS_OK;
 }

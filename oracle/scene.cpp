@@ -6,9 +6,11 @@
 //   material / texture construction ........ core/SceneLoader_kernels.cu:8-50, SceneLoader.cuh:362-391
 //   lights (one per emissive triangle) ..... SceneLoader_kernels.cu:52-111 (fixes: global light order)
 //   images + mip pyramid .................... core/ImageI.cuh:11-37, textures/Mipmap_kernel.cu:9-73
+//                                             (JPEG through the reference's stb_image, see decodeJPEG)
 //   camera + Gaussian filter table ......... rendering/Camera.cu:9-43, rendering/Filter.cuh:32-99
 #include "scene.h"
 #include "oracle.h"
+#include <dlfcn.h>
 #include <zlib.h>
 #include <cstdio>
 #include <fstream>
@@ -103,12 +105,41 @@ static bool decodePPM(const std::vector<unsigned char>& b, int& w, int& h, std::
     return true;
 }
 
+// JPEG: the reference's own decoder, stb_image 2.27 (extern/stb_image.h), compiled by this
+// directory's Makefile into _ref/libstb_image.so beside _build/, and called through
+// stbi_load_from_memory with 3 components requested, as stbi_loadf does (ImageI.cuh:14).
+static bool decodeJPEG(const std::vector<unsigned char>& b, int& w, int& h, std::vector<unsigned char>& rgb, std::string& err) {
+    typedef unsigned char* (*LoadFn)(const unsigned char*, int, int*, int*, int*, int);
+    typedef void (*FreeFn)(void*);
+    static LoadFn load = nullptr;
+    static FreeFn release = nullptr;
+    if (!load) {
+        Dl_info self;
+        if (!dladdr((void*)&decodeJPEG, &self) || !self.dli_fname) { err = "cannot locate the oracle library"; return false; }
+        std::string dir(self.dli_fname);
+        dir = dir.substr(0, dir.find_last_of('/'));
+        void* lib = dlopen((dir + "/../_ref/libstb_image.so").c_str(), RTLD_NOW | RTLD_LOCAL);
+        if (!lib) { err = "JPEG needs oracle/_ref/libstb_image.so (built from the reference's extern/stb_image.h)"; return false; }
+        release = (FreeFn)dlsym(lib, "stbi_image_free");
+        load = (LoadFn)dlsym(lib, "stbi_load_from_memory");
+        if (!load || !release) { err = "stb_image symbols missing"; load = nullptr; return false; }
+    }
+    int comp = 0;
+    unsigned char* px = load(b.data(), (int)b.size(), &w, &h, &comp, 3);
+    if (!px) { err = "stb_image could not decode the JPEG"; return false; }
+    rgb.assign(px, px + (size_t)w * h * 3);
+    release(px);
+    return true;
+}
+
 // stbi_loadf + loadImageFile_s: 8-bit -> linear float via pow(c/255, 2.2) (extern/stb_image.h:1828),
 // rows flipped so that row 0 is the bottom of the image (ImageI.cuh:26-35).
 static bool loadImage(const std::string& path, int& w, int& h, std::vector<Spec>& img, std::string& err) {
     std::vector<unsigned char> buf, rgb;
     if (!readFile(path, buf)) { err = "cannot open image " + path; return false; }
-    bool ok = (buf.size() > 1 && buf[0] == 'P') ? decodePPM(buf, w, h, rgb, err) : decodePNG(buf, w, h, rgb, err);
+    bool ok = (buf.size() > 1 && buf[0] == 'P')           ? decodePPM(buf, w, h, rgb, err)
+              : (buf.size() > 1 && buf[0] == 0xFF && buf[1] == 0xD8) ? decodeJPEG(buf, w, h, rgb, err)
+                                                                      : decodePNG(buf, w, h, rgb, err);
     if (!ok) { err = path + ": " + err; return false; }
     float lut[256];
     for (int c = 0; c < 256; c++) lut[c] = (float)(std::pow((double)(c / 255.0f), (double)2.2f) * (double)1.0f);

@@ -5,7 +5,9 @@ the GPU against the oracle, sample by sample, with the app.params settings the r
 Scenes: platforms/bigBox (area light box + infinite light) and kernel_box_scene/kernel (a 64^2 PNG
 texture, Black wrap), with the diamond (glass, eta 2.419), holder and empty subjects; the empty
 scene (sky only) with the diamond; the folding box with its donut (an Oren-Nayar subject and 64
-emissive triangles). The behelit and chicken subjects use the mirror material R
+emissive triangles). The reference's JPEG texture (res/blackDragon.jpg, decoded by the oracle with
+the reference's own stb_image) appears twice: on the kernel box in place of its PNG, and through
+blackDragon.subject's image and material lines on the diamond mesh. The behelit and chicken subjects use the mirror material R
 (Material.cuh:78-95) but their meshes are not in the reference tree, so a mirror subject on the
 diamond mesh (behelit's material line) stands in for them.
 """
@@ -29,6 +31,8 @@ CASES = [
     ("res/empty", "res/diamond"),
     ("res/foldingbox/folding", "res/foldingbox/donut"),
     ("res/platforms/bigBox", "res/mirror_diamond"),
+    ("res/kernel_box_scene/kernel_jpeg", "res/diamond"),
+    ("res/platforms/bigBox", "res/blackDragon_diamond"),
 ]
 
 
@@ -39,6 +43,19 @@ def project(tmp_path_factory):
     # behelit.subject's material (R c 0.9 0.02 0.02 1) on the diamond mesh
     with open(os.path.join(d, "res", "mirror_diamond.subject"), "w") as f:
         f.write("0\n1\tres/diamond\n1\nR\tc\t0.9 0.02 0.02\t1\n")
+    # the kernel box with the reference's JPEG texture (res/blackDragon.jpg) in place of its PNG
+    with open(os.path.join(GOLDEN, "res", "kernel_box_scene", "kernel.scene")) as f:
+        kernel = f.read()
+    assert "res/kernel_box_scene/kernel.png" in kernel
+    with open(os.path.join(d, "res", "kernel_box_scene", "kernel_jpeg.scene"), "w") as f:
+        f.write(kernel.replace("res/kernel_box_scene/kernel.png", "res/blackDragon.jpg"))
+    # blackDragon.subject (its JPEG texture and 'L i 0 0.5' material) on the diamond mesh: the
+    # dragon mesh (2 MB) is not among the fixtures
+    with open(os.path.join(GOLDEN, "res", "blackDragon.subject")) as f:
+        dragon = f.read()
+    assert "res/blackDragon\n" in dragon
+    with open(os.path.join(d, "res", "blackDragon_diamond.subject"), "w") as f:
+        f.write(dragon.replace("res/blackDragon\n", "res/diamond\n"))
     return str(d) + "/"
 
 
