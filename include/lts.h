@@ -53,6 +53,12 @@ typedef struct lts_bvh_timing {
 #define LTS_INTEGRATOR_MEGAKERNEL 0
 #define LTS_INTEGRATOR_WAVEFRONT 1
 #define LTS_INTEGRATOR_CACHED 2
+/* Bidirectional path tracing (replaces BidirectionalPathIntegrator,
+ * intergrators/BidirectionalPathIntegrator.cuh:1-665, which the reference disables at
+ * LightTransportSimulation.cu:116-121): every strategy with a camera subpath of at least two
+ * vertices, balance-heuristic MIS, max_bounce at most 8, no Russian roulette (rr_start unused).
+ * Same film and sample layout as the megakernel. */
+#define LTS_INTEGRATOR_BDPT 3
 
 typedef struct lts_render_params {
     int32_t spp;          /* samples per pixel, 1 .. 2^29                          */

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(PKG_DIR, "liblts.so")
 INTEGRATOR_MEGAKERNEL = 0
 INTEGRATOR_WAVEFRONT = 1
 INTEGRATOR_CACHED = 2
+INTEGRATOR_BDPT = 3
 
 # every symbol declared in include/lts.h
 EXPORTED_SYMBOLS = (

@@ -26,7 +26,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "
 DEVICE = [f"--offload-arch={ARCH}", "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["bvh.hip", "render.hip", "wavefront.hip", "nrc.hip", "mesh.hip", "lts_capi.cpp", "scene_host.cpp",
+SOURCES = ["bvh.hip", "render.hip", "bdpt.hip", "wavefront.hip", "nrc.hip", "mesh.hip", "lts_capi.cpp", "scene_host.cpp",
            "image_io.cpp", "jpeg.cpp"]
 
 

@@ -221,6 +221,9 @@ int check_render_params(lts_ctx* c, const lts_render_params* p) {
             return fail(c, LTS_E_ARG, "trace_block must be 64, 128 or 256");
         if (p->persistent_blocks < 0 || p->persistent_blocks > (1 << 20)) return fail(c, LTS_E_ARG, "bad persistent_blocks");
         if (p->tile_rows < 0 || p->tile_rows > kMaxTileRows) return fail(c, LTS_E_ARG, "tile_rows must be 0 .. 64");
+    } else if (p->integrator == LTS_INTEGRATOR_BDPT) {
+        if (p->max_bounce > kBdptMaxBounce) return fail(c, LTS_E_ARG, "bidirectional max_bounce above 8");
+        if (p->count_traversal) return fail(c, LTS_E_ARG, "count_traversal is a megakernel option");
     } else if (p->integrator != LTS_INTEGRATOR_MEGAKERNEL) {
         return fail(c, LTS_E_ARG, "unsupported integrator");
     }
@@ -1099,7 +1102,7 @@ static int render_impl(lts_ctx* c, const lts_render_params* p, float* rgb_out, f
         rl.s0 = s0;
         rl.s1 = std::min(b1, s0 + pass);
         HIPCHK(c, hipEventRecord(c->ev[5], s));
-        HIPCHK(c, launch_megakernel(sv, rl, s));
+        HIPCHK(c, p->integrator == LTS_INTEGRATOR_BDPT ? launch_bdpt(sv, rl, s) : launch_megakernel(sv, rl, s));
         HIPCHK(c, hipEventRecord(c->ev[6], s));
         HIPCHK(c, launch_film_gather(sv, rl, s));
         HIPCHK(c, hipEventRecord(c->ev[7], s));

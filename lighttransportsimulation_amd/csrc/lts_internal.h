@@ -116,6 +116,9 @@ struct RenderLaunch {
     unsigned long long* counters;  // optional [2]: node visits, triangle tests
 };
 hipError_t launch_megakernel(const SceneView& sv, const RenderLaunch& rl, hipStream_t stream);
+// Bidirectional path tracing (bdpt.hip): the same launch, sample layout and film as the megakernel.
+constexpr int kBdptMaxBounce = 8;
+hipError_t launch_bdpt(const SceneView& sv, const RenderLaunch& rl, hipStream_t stream);
 hipError_t launch_film_gather(const SceneView& sv, const RenderLaunch& rl, hipStream_t stream);
 hipError_t launch_film_clear(float* film, int npix, hipStream_t stream);
 // tris[k].mat = shade[k].mat for the n leaves of a loaded scene (shade is in leaf order).

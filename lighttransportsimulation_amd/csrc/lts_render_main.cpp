@@ -1,6 +1,6 @@
 // lts_render: the reference executable's flow (src/LightTransportSimulation.cu:23-146) over the C ABI.
 //
-//   lts_render [projectDir] [--seed N] [--integrator mega|wavefront|cached]
+//   lts_render [projectDir] [--seed N] [--integrator mega|wavefront|cached|bdpt]
 //
 // Reads <projectDir>/app.params (positional: W H / spp / bounces / roulette start / format /
 // output / scene / subject, core/FileIO.cuh:234-257), loads the scene and subject relative to
@@ -26,7 +26,10 @@ int main(int argc, char** argv) {
         if (!std::strcmp(argv[i], "--seed") && i + 1 < argc) seed = (unsigned)std::strtoul(argv[++i], nullptr, 0);
         else if (!std::strcmp(argv[i], "--integrator") && i + 1 < argc) {
             std::string v = argv[++i];
-            integrator = v == "wavefront" ? LTS_INTEGRATOR_WAVEFRONT : v == "cached" ? LTS_INTEGRATOR_CACHED : LTS_INTEGRATOR_MEGAKERNEL;
+            integrator = v == "wavefront" ? LTS_INTEGRATOR_WAVEFRONT
+                         : v == "cached"  ? LTS_INTEGRATOR_CACHED
+                         : v == "bdpt"    ? LTS_INTEGRATOR_BDPT
+                                          : LTS_INTEGRATOR_MEGAKERNEL;
         } else dir = argv[i];
     }
     if (!dir.empty() && dir.back() != '/') dir += '/';

@@ -2,8 +2,9 @@
 the GPU against the oracle, sample by sample, with the app.params settings the reference ships
 (10 samples, 5 bounces, roulette from 2; app.params:1-5).
 
-Scenes: platforms/bigBox (area light box + infinite light) and kernel_box_scene/kernel (a 64^2 PNG
-texture, Black wrap), with the diamond (glass, eta 2.419), holder and empty subjects; the empty
+Scenes: platforms/bigBox (area light box + infinite light), platforms/plane (the dome mesh under
+the infinite light only) and kernel_box_scene/kernel (a 64^2 PNG texture, Black wrap), with the
+diamond (glass, eta 2.419), holder and empty subjects; the empty
 scene (sky only) with the diamond; the folding box with its donut (an Oren-Nayar subject and 64
 emissive triangles). The reference's JPEG texture (res/blackDragon.jpg, decoded by the oracle with
 the reference's own stb_image) appears twice: on the kernel box in place of its PNG, and through
@@ -31,6 +32,7 @@ CASES = [
     ("res/empty", "res/diamond"),
     ("res/foldingbox/folding", "res/foldingbox/donut"),
     ("res/platforms/bigBox", "res/mirror_diamond"),
+    ("res/platforms/plane", "res/diamond"),
     ("res/kernel_box_scene/kernel_jpeg", "res/diamond"),
     ("res/platforms/bigBox", "res/blackDragon_diamond"),
 ]
