@@ -28,6 +28,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import tempfile
 import time
@@ -164,22 +165,30 @@ def lbvh_cpu_baseline(tri) -> dict:
 
 
 def lbvh_block(no_cpu: bool) -> dict:
-    """Both LBVH workloads in this process: the build alone, repeated on a resident soup."""
-    from lighttransportsimulation_amd import Context
+    """Both LBVH workloads, the build alone repeated on a resident soup, each measured by a child
+    process of its own (this script with --workload lbvh_1m / lbvh_8m, 3 warm-up and 20 timed
+    builds), i.e. in a fresh HIP context as the standalone LBVH lines are."""
     out = {"definition": "n / device time of one build (scene bounds, Morton codes, radix sort, hierarchy and node "
                          "bounds) on a resident soup, replayed as a captured HIP graph; achieved = algorithmic "
                          "bytes per primitive x n / build time",
-           "radix_passes": LBVH_RADIX_PASSES}
-    with Context(0) as ctx:
-        for name, n in (("1M", 1 << 20), ("8M", 1 << 23)):
-            m = lbvh_measure(ctx, n, 3, 20)
-            out[name] = {"primitives": n, "mprims_per_s": round(m["value"], 1), "ms_per_build": round(m["ms"], 4),
-                         "stage_ms": {k: round(v, 4) for k, v in m["stages"].items()},
-                         "achieved_GBps": round(m["achieved"], 1), "hbm_frac": round(m["achieved"] / HBM_PEAK_GBS, 4),
-                         "algorithmic_bytes_per_prim": m["bytes_per_prim"]}
-            if not no_cpu and name == "1M":
-                out[name]["cpu_baseline"] = lbvh_cpu_baseline(m["tri"])
-            progress(f"lbvh {name}: {m['value']:.0f} Mprims/s")
+           "radix_passes": LBVH_RADIX_PASSES,
+           "process": "one child process per workload: bench.py --workload lbvh_1m / lbvh_8m --steps 20 --warmup 3"}
+    for name, workload in (("1M", "lbvh_1m"), ("8M", "lbvh_8m")):
+        cmd = [sys.executable, os.path.abspath(__file__), "--workload", workload, "--steps", "20", "--warmup", "3"]
+        if no_cpu or name == "8M":
+            cmd.append("--no-cpu")
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        if p.returncode != 0:
+            raise RuntimeError(f"{' '.join(cmd)} failed ({p.returncode}): {p.stderr.strip()[-2000:]}")
+        line = json.loads(p.stdout.strip().splitlines()[-1])
+        rf = line["roofline"]
+        out[name] = {"primitives": line["config"]["primitives"], "mprims_per_s": round(line["value"], 1),
+                     "ms_per_build": line["ms_per_step"], "stage_ms": line["config"]["stage_ms"],
+                     "achieved_GBps": rf["achieved"], "hbm_frac": rf["frac"],
+                     "algorithmic_bytes_per_prim": rf["algorithmic_bytes_per_prim"]}
+        if "cpu_baseline" in line:
+            out[name]["cpu_baseline"] = line["cpu_baseline"]
+        progress(f"lbvh {name}: {line['value']:.0f} Mprims/s")
     return out
 
 
@@ -472,7 +481,8 @@ def main() -> int:
     if WORKLOADS[args.workload]["gen"] == "batch":
         print(json.dumps(run_batch(args)), flush=True)
         return 0
-    from lighttransportsimulation_amd import INTEGRATOR_CACHED, INTEGRATOR_MEGAKERNEL, INTEGRATOR_WAVEFRONT, Context
+    from lighttransportsimulation_amd import (INTEGRATOR_BDPT, INTEGRATOR_CACHED, INTEGRATOR_MEGAKERNEL,
+                                          INTEGRATOR_WAVEFRONT, Context)
     if args.cached_frame:
         with tempfile.TemporaryDirectory() as d:
             cfg = make_workload(args.workload, d)
@@ -526,7 +536,7 @@ def main() -> int:
             progress(f"{args.steps} timed frames: {wall / args.steps * 1e3:.1f} ms per frame")
             if wavefront and tuning is None:
                 tuning = {k: st[k] for k in TUNING_KEYS}
-            graph_rate = mega = tree = counts = None
+            graph_rate = mega = tree = counts = bdpt = None
             if wavefront:
                 # the same frames replayed as captured graphs, without the phase events
                 gk = dict(kw, autotune=0, profile_phases=0, trace_block=tuning["trace_block"],
@@ -544,6 +554,16 @@ def main() -> int:
                 for _ in range(2):
                     ctx.render(**mk)
                 mega = W * H * spp * 2 / (time.perf_counter() - t1) / 1e6
+                # bidirectional path tracing on the same frame at 4 spp (a different estimator: every
+                # connection strategy per sample, so far more rays per sample; for scale only)
+                bd_spp = 4
+                bk = dict(kw, integrator=INTEGRATOR_BDPT, autotune=0, profile_phases=0, spp=bd_spp, spp_per_pass=0)
+                ctx.render(**bk)
+                t1 = time.perf_counter()
+                _, _, bst = ctx.render(**bk)
+                bdpt = {"spp": bd_spp, "max_bounce": cfg["max_bounce"],
+                        "msamples_per_s": round(W * H * bd_spp / (time.perf_counter() - t1) / 1e6, 3),
+                        "kernel_ms_per_frame": round(bst["render_ms"], 3)}
                 # the traversal kernel's own work on the same frame: render-node fetches and triangle
                 # tests of every ray kind, counted by its counting instantiation (an untimed run)
                 tree = ctx.render_tree_info()
@@ -570,6 +590,8 @@ def main() -> int:
             out["config"]["tuning"] = tuning
         if mega:
             out["config"]["megakernel_msamples_per_s"] = round(mega, 3)
+        if bdpt:
+            out["config"]["bdpt"] = bdpt
         if tree:
             out["config"]["render_tree"] = tree
         if graph_rate:
