@@ -457,6 +457,8 @@ def main() -> int:
     ap.add_argument("--spp-per-pass", type=int, default=0, help="samples per pixel per pass (0: the library's choice)")
     ap.add_argument("--no-quality", action="store_true", help="skip the cached-vs-plain equal-time comparison")
     ap.add_argument("--no-extras", action="store_true", help="skip the lbvh, batch64 and reference_scene blocks")
+    ap.add_argument("--bdpt", action="store_true",
+                    help="also time the bidirectional integrator on the workload's frame (4 spp)")
     ap.add_argument("--batch-scenes", type=int, default=0, help="batch64: number of scenes (default 64)")
     ap.add_argument("--cached-frame", action="store_true",
                     help="render steady-state frames of the workload with the cached integrator and print their "
@@ -554,16 +556,18 @@ def main() -> int:
                 for _ in range(2):
                     ctx.render(**mk)
                 mega = W * H * spp * 2 / (time.perf_counter() - t1) / 1e6
-                # bidirectional path tracing on the same frame at 4 spp (a different estimator: every
-                # connection strategy per sample, so far more rays per sample; for scale only)
-                bd_spp = 4
-                bk = dict(kw, integrator=INTEGRATOR_BDPT, autotune=0, profile_phases=0, spp=bd_spp, spp_per_pass=0)
-                ctx.render(**bk)
-                t1 = time.perf_counter()
-                _, _, bst = ctx.render(**bk)
-                bdpt = {"spp": bd_spp, "max_bounce": cfg["max_bounce"],
-                        "msamples_per_s": round(W * H * bd_spp / (time.perf_counter() - t1) / 1e6, 3),
-                        "kernel_ms_per_frame": round(bst["render_ms"], 3)}
+                if args.bdpt:
+                    # bidirectional path tracing on the same frame at 4 spp (a different estimator:
+                    # every connection strategy per sample, so far more rays per sample; for scale)
+                    bd_spp = 4
+                    bk = dict(kw, integrator=INTEGRATOR_BDPT, autotune=0, profile_phases=0, spp=bd_spp,
+                              spp_per_pass=0)
+                    ctx.render(**bk)
+                    t1 = time.perf_counter()
+                    _, _, bst = ctx.render(**bk)
+                    bdpt = {"spp": bd_spp, "max_bounce": cfg["max_bounce"],
+                            "msamples_per_s": round(W * H * bd_spp / (time.perf_counter() - t1) / 1e6, 3),
+                            "kernel_ms_per_frame": round(bst["render_ms"], 3)}
                 # the traversal kernel's own work on the same frame: render-node fetches and triangle
                 # tests of every ray kind, counted by its counting instantiation (an untimed run)
                 tree = ctx.render_tree_info()
