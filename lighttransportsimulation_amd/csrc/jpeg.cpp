@@ -363,7 +363,9 @@ private:
     void decode_block(BitReader& br, Component& c, int16_t* blk) {
         const int t = br.decode(dc[c.td]);
         if (t > 15) fail("bad JPEG DC code");
-        c.pred += t ? br.receive_extend(t) : 0;
+        // the prediction wraps modulo 2^32 rather than overflowing (only hostile files get there; the
+        // dequantised value keeps its low 16 bits either way)
+        c.pred = (int32_t)((uint32_t)c.pred + (uint32_t)(t ? br.receive_extend(t) : 0));
         const uint16_t* q = qt[c.tq];
         std::memset(blk, 0, 64 * sizeof(int16_t));
         blk[0] = (int16_t)(uint16_t)((int64_t)c.pred * q[0]);
