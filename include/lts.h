@@ -198,6 +198,13 @@ int lts_set_texture_filter(lts_ctx* ctx, int32_t mode);
  * the even-depth collapse), its depth in edges and its 4-wide node count. */
 int lts_render_tree_info(lts_ctx* ctx, int32_t* kind, int32_t* depth, int32_t* nodes4);
 
+/* Replaces Scene::intersect / BVH::intersect (acceleration/BVH.cuh:365-398) as a query: the closest
+ * hit of each of n rays (rays[6 i .. 6 i + 5] = origin, direction, world space, tMax infinite) in the
+ * loaded scene, by the renderer's own traversal of its render tree. prim[i] receives the hit
+ * triangle's primitive number (the order of the scene's meshes, as lts_scene_info counts them) or
+ * -1, t[i] the distance in units of the direction's length (infinity for a miss). */
+int lts_trace_rays(lts_ctx* ctx, const float* rays, int32_t n, int32_t* prim, float* t);
+
 /* Replaces CreateBVHTree (acceleration/BVH.cuh:411-452) for a triangle soup of n triangles
  * (9 floats each, host memory). mode 0 reproduces the reference's leaf assignment (leaf k holds
  * primitive k, BVH.cuh:242); mode 1 assigns leaf k to the k-th primitive in Morton order.

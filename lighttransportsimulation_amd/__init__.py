@@ -26,7 +26,7 @@ EXPORTED_SYMBOLS = (
     "lts_write_image", "lts_device_copy_benchmark", "lts_render_batch", "lts_nrc_configure", "lts_nrc_set_weights",
     "lts_nrc_get_weights", "lts_nrc_infer", "lts_nrc_train_step", "lts_set_render_tree", "lts_render_tree_info",
     "lts_obj_to_bin", "lts_film_save", "lts_film_load", "lts_film_resolve", "lts_set_texture_filter",
-    "lts_load_image", "lts_bvh_build_static", "lts_mesh_transform",
+    "lts_load_image", "lts_bvh_build_static", "lts_mesh_transform", "lts_trace_rays",
 )
 
 TEXTURE_REFERENCE = 0
@@ -108,6 +108,7 @@ def load_library():
     L.lts_set_render_tree.argtypes = [vp, i32]
     L.lts_set_texture_filter.argtypes = [vp, i32]
     L.lts_render_tree_info.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
+    L.lts_trace_rays.argtypes = [vp, vp, i32, vp, vp]
     L.lts_bvh_build.argtypes = [vp, vp, i32, i32, C.POINTER(BvhExport), C.POINTER(BvhTiming)]
     L.lts_bvh_build_device.argtypes = [vp, vp, i32, i32, i32, C.POINTER(BvhTiming)]
     L.lts_bvh_export_last.argtypes = [vp, C.POINTER(BvhExport)]
@@ -212,6 +213,18 @@ class Context:
         """Render tree of the scenes loaded from now on: TREE_PLOC (default) or TREE_KARRAS, with
         TREE_COLLAPSE_EVEN or-ed in for the even-depth 4-wide collapse instead of the greedy one."""
         self._check(self.lib.lts_set_render_tree(self.h, kind), "lts_set_render_tree")
+
+    def trace_rays(self, origins: np.ndarray, directions: np.ndarray):
+        """Closest hits of rays in the loaded scene (include/lts.h: lts_trace_rays): returns
+        (prim [n] int32, -1 for a miss; t [n] float32)."""
+        o = np.asarray(origins, np.float32).reshape(-1, 3)
+        d = np.asarray(directions, np.float32).reshape(-1, 3)
+        rays = np.ascontiguousarray(np.concatenate([o, d], axis=1))
+        n = rays.shape[0]
+        prim = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        self._check(self.lib.lts_trace_rays(self.h, _ptr(rays), n, _ptr(prim), _ptr(t)), "lts_trace_rays")
+        return prim, t
 
     def render_tree_info(self) -> dict:
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()

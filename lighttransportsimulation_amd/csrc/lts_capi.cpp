@@ -994,6 +994,33 @@ int lts_set_texture_filter(lts_ctx* c, int32_t mode) {
     return LTS_OK;
 }
 
+static int trace_rays_impl(lts_ctx* c, const float* rays, int32_t n, int32_t* prim, float* t) {
+    if (!c || !rays || !prim || !t || n < 0 || n > (1 << 26)) return fail(c, LTS_E_ARG, "bad trace_rays arguments");
+    if (!c->scene.loaded) return fail(c, LTS_E_STATE, "no scene loaded");
+    if (n == 0) return LTS_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    float* dr = nullptr;
+    float* dt = nullptr;
+    int32_t* dp = nullptr;
+    hipError_t e = hipMalloc((void**)&dr, 6 * (size_t)n * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&dt, (size_t)n * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&dp, (size_t)n * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMemcpyAsync(dr, rays, 6 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_trace_rays(c->scene.view, dr, n, dp, dt, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(prim, dp, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(t, dt, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (dr) (void)hipFree(dr);
+    if (dt) (void)hipFree(dt);
+    if (dp) (void)hipFree(dp);
+    HIPCHK(c, e);
+    return LTS_OK;
+}
+
+int lts_trace_rays(lts_ctx* c, const float* rays, int32_t n, int32_t* prim, float* t) {
+    return guarded(c, [&] { return trace_rays_impl(c, rays, n, prim, t); });
+}
+
 int lts_render_tree_info(lts_ctx* c, int32_t* kind, int32_t* depth, int32_t* nodes4) {
     if (!c) return LTS_E_ARG;
     if (!c->scene.loaded) return fail(c, LTS_E_STATE, "no scene loaded");

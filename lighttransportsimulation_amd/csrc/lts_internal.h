@@ -121,6 +121,7 @@ constexpr int kBdptMaxBounce = 8;
 hipError_t launch_bdpt(const SceneView& sv, const RenderLaunch& rl, hipStream_t stream);
 hipError_t launch_film_gather(const SceneView& sv, const RenderLaunch& rl, hipStream_t stream);
 hipError_t launch_film_clear(float* film, int npix, hipStream_t stream);
+hipError_t launch_trace_rays(const SceneView& sv, const float* rays, int n, int32_t* prim, float* t, hipStream_t stream);
 // tris[k].mat = shade[k].mat for the n leaves of a loaded scene (shade is in leaf order).
 hipError_t launch_tri_materials(TriLeaf* tris, const ShadeTri* shade, int n, hipStream_t stream);
 hipError_t launch_get_render(const SceneView& sv, const float* film, float* rgb, hipStream_t stream);

@@ -272,13 +272,24 @@ LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned lon
 // The triangle test of an any-hit traversal: Triangle::simpleIntersect for shadow rays, or the
 // closest-hit test (Triangle::intersect) when the question is whether leaf `leaf` would win the
 // closest hit against leaf `tieLeaf` at distance r.tMax (nearer, or as near with a lower index).
+//
+// Triangle::simpleIntersect accepts points up to SHADOW_EPS outside the triangle's edges, and the
+// reference reaches a triangle only through its leaf's box test (BVH.cuh:343-348: the leaf node's
+// bounds, the primitive's own bounds, by Bounds3::simpleIntersect with the IEEE inverse direction).
+// The render tree's quantized boxes are wider than that leaf box, so a shadow ray passing just
+// outside a triangle's edge and outside its exact bounds would meet the enlarged test here but not
+// in the reference; the leaf-box test comes first, as there. (The closest-hit test accepts only
+// points inside the triangle shrunk by SHADOW_EPS, which lie inside its bounds.)
 template <bool kHitTest>
-LTS_D bool leaf_blocks(V3 p0, V3 p1, V3 p2, const Ray& r, int leaf, int tieLeaf) {
+LTS_D bool leaf_blocks(V3 p0, V3 p1, V3 p2, const Ray& r, int leaf, int tieLeaf, V3 invExact, int ex, int ey, int ez) {
     if constexpr (kHitTest) {
         float t, u, v, id;
         return tri_hit_closest(p0, p1, p2, r, leaf, tieLeaf, t, u, v, id);
     } else {
-        return tri_occludes(p0, p1, p2, r);
+        const float mn[3] = {fminf(fminf(p0.x, p1.x), p2.x), fminf(fminf(p0.y, p1.y), p2.y), fminf(fminf(p0.z, p1.z), p2.z)};
+        const float mx[3] = {fmaxf(fmaxf(p0.x, p1.x), p2.x), fmaxf(fmaxf(p0.y, p1.y), p2.y), fmaxf(fmaxf(p0.z, p1.z), p2.z)};
+        float tEnter;
+        return box_hit(mn, mx, r, invExact, ex, ey, ez, tEnter) && tri_occludes(p0, p1, p2, r);
     }
 }
 
@@ -290,11 +301,14 @@ LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts, TraceTall
     const int n = sv.nPrim;
     const V3 inv = ray_inverse(r.d);
     int nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+    // the reference's inverse direction (BVH.cuh:336) for its leaf-box test
+    const V3 invE = kHitTest ? inv : v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    const int ex = invE.x < 0, ey = invE.y < 0, ez = invE.z < 0;
     if (n == 1) {
         V3 p0, p1, p2; int prim;
         load_tri(sv.tris, 0, p0, p1, p2, prim);
         if (tally) tally->tris++;
-        return leaf_blocks<kHitTest>(p0, p1, p2, r, 0, tieLeaf);
+        return leaf_blocks<kHitTest>(p0, p1, p2, r, 0, tieLeaf, invE, ex, ey, ez);
     }
     int spill[kStack - kShortStack];
     int sp = 0;
@@ -314,7 +328,7 @@ LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts, TraceTall
                     V3 p0, p1, p2; int prim;
                     load_tri(sv.tris, leaf, p0, p1, p2, prim);
                     if (tally) tally->tris++;
-                    if (leaf_blocks<kHitTest>(p0, p1, p2, r, leaf, tieLeaf)) return true;
+                    if (leaf_blocks<kHitTest>(p0, p1, p2, r, leaf, tieLeaf, invE, ex, ey, ez)) return true;
                 } else if (sp < kStack) {
                     if (sp < kShortStack) ts.lds[sp * ts.stride] = c[k];
                     else spill[sp - kShortStack] = c[k];

@@ -199,6 +199,33 @@ hipError_t launch_tri_materials(TriLeaf* tris, const ShadeTri* shade, int n, hip
     return hipGetLastError();
 }
 
+// Closest hits of caller-given rays: the renderer's own traversal (trace_closest), one thread per
+// ray. rays[6 i ..] = origin, direction; a miss gives prim -1.
+__global__ __launch_bounds__(256) void k_trace_rays(SceneView sv, const float* __restrict__ rays, int n,
+                                                    int32_t* __restrict__ prim, float* __restrict__ tOut) {
+    __shared__ int stk[kShortStack * 256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    Ray r;
+    r.o = v3(rays[6 * (size_t)i], rays[6 * (size_t)i + 1], rays[6 * (size_t)i + 2]);
+    r.d = v3(rays[6 * (size_t)i + 3], rays[6 * (size_t)i + 4], rays[6 * (size_t)i + 5]);
+    r.tMax = INFINITY;
+    const Hit h = trace_closest(sv, r, lds_stack(stk, threadIdx.x, 256), nullptr, nullptr);
+    int p = -1;
+    if (h.leaf >= 0) {
+        V3 p0, p1, p2;
+        load_tri(sv.tris, h.leaf, p0, p1, p2, p);
+    }
+    prim[i] = p;
+    tOut[i] = h.leaf >= 0 ? h.t : INFINITY;
+}
+
+hipError_t launch_trace_rays(const SceneView& sv, const float* rays, int n, int32_t* prim, float* t, hipStream_t stream) {
+    if (n < 1) return hipSuccess;
+    k_trace_rays<<<(n + 255) / 256, 256, 0, stream>>>(sv, rays, n, prim, t);
+    return hipGetLastError();
+}
+
 hipError_t launch_film_clear(float* film, int npix, hipStream_t stream) {
     k_film_clear<<<(npix + 255) / 256, 256, 0, stream>>>(reinterpret_cast<float4*>(film), npix);
     return hipGetLastError();
