@@ -269,66 +269,65 @@ LTS_D Hit trace_closest(const SceneView& sv, Ray& r, TraceStack ts, unsigned lon
     return h;
 }
 
-// The triangle test of an any-hit traversal: Triangle::simpleIntersect for shadow rays, or the
-// closest-hit test (Triangle::intersect) when the question is whether leaf `leaf` would win the
-// closest hit against leaf `tieLeaf` at distance r.tMax (nearer, or as near with a lower index).
-//
-// Triangle::simpleIntersect accepts points up to SHADOW_EPS outside the triangle's edges, and the
-// reference reaches a triangle only through its leaf's box test (BVH.cuh:343-348: the leaf node's
-// bounds, the primitive's own bounds, by Bounds3::simpleIntersect with the IEEE inverse direction).
-// The render tree's quantized boxes are wider than that leaf box, so a shadow ray passing just
-// outside a triangle's edge and outside its exact bounds would meet the enlarged test here but not
-// in the reference; the leaf-box test comes first, as there. (The closest-hit test accepts only
-// points inside the triangle shrunk by SHADOW_EPS, which lie inside its bounds.)
-template <bool kHitTest>
-LTS_D bool leaf_blocks(V3 p0, V3 p1, V3 p2, const Ray& r, int leaf, int tieLeaf, V3 invExact, int ex, int ey, int ez) {
+// The reference's leaf-box test for a shadow ray (its inverse direction, BVH.cuh:336, and the
+// triangle's bounds), out of line: it runs once per occluded shadow ray.
+static __device__ __attribute__((noinline)) bool leaf_box_passes(V3 p0, V3 p1, V3 p2, Ray r) {
+    const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    const float mn[3] = {fminf(fminf(p0.x, p1.x), p2.x), fminf(fminf(p0.y, p1.y), p2.y), fminf(fminf(p0.z, p1.z), p2.z)};
+    const float mx[3] = {fmaxf(fmaxf(p0.x, p1.x), p2.x), fmaxf(fmaxf(p0.y, p1.y), p2.y), fmaxf(fmaxf(p0.z, p1.z), p2.z)};
+    float tEnter;
+    return box_hit(mn, mx, r, inv, inv.x < 0, inv.y < 0, inv.z < 0, tEnter);
+}
+
+// The triangle test of an any-hit traversal: Triangle::simpleIntersect for shadow rays (with
+// kLeafBox, only when the leaf-box test passes too), or the closest-hit test (Triangle::intersect)
+// when the question is whether leaf `leaf` would win the closest hit against leaf `tieLeaf` at
+// distance r.tMax (nearer, or as near with a lower index).
+template <bool kHitTest, bool kLeafBox>
+LTS_D bool leaf_blocks(V3 p0, V3 p1, V3 p2, const Ray& r, int leaf, int tieLeaf) {
     if constexpr (kHitTest) {
         float t, u, v, id;
         return tri_hit_closest(p0, p1, p2, r, leaf, tieLeaf, t, u, v, id);
+    } else if constexpr (kLeafBox) {
+        return tri_occludes(p0, p1, p2, r) && leaf_box_passes(p0, p1, p2, r);
     } else {
-        const float mn[3] = {fminf(fminf(p0.x, p1.x), p2.x), fminf(fminf(p0.y, p1.y), p2.y), fminf(fminf(p0.z, p1.z), p2.z)};
-        const float mx[3] = {fmaxf(fmaxf(p0.x, p1.x), p2.x), fmaxf(fmaxf(p0.y, p1.y), p2.y), fmaxf(fmaxf(p0.z, p1.z), p2.z)};
-        float tEnter;
-        return box_hit(mn, mx, r, invExact, ex, ey, ez, tEnter) && tri_occludes(p0, p1, p2, r);
+        return tri_occludes(p0, p1, p2, r);
     }
 }
 
 // Any hit (BVH::simpleIntersect, acceleration/BVH.cuh:331-363) over the 4-wide tree: leaf children
-// are tested as soon as their box is hit, interior children are stacked.
-// kHitTest: the closest-hit question of an MIS ray, whether any leaf beats tieLeaf at r.tMax.
-template <bool kHitTest = false>
-LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts, TraceTally* tally = nullptr, int tieLeaf = -1) {
-    const int n = sv.nPrim;
+// are tested as soon as their box is hit, interior children are stacked. Returns the first leaf
+// whose test passes, or -1.
+template <bool kHitTest, bool kLeafBox>
+LTS_D int any_hit_leaf(int n, const BvhQNode* nodes4, const TriLeaf* tris, const Ray& r, TraceStack ts, TraceTally* tally,
+                       int tieLeaf) {
     const V3 inv = ray_inverse(r.d);
     int nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
-    // the reference's inverse direction (BVH.cuh:336) for its leaf-box test
-    const V3 invE = kHitTest ? inv : v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
-    const int ex = invE.x < 0, ey = invE.y < 0, ez = invE.z < 0;
     if (n == 1) {
         V3 p0, p1, p2; int prim;
-        load_tri(sv.tris, 0, p0, p1, p2, prim);
+        load_tri(tris, 0, p0, p1, p2, prim);
         if (tally) tally->tris++;
-        return leaf_blocks<kHitTest>(p0, p1, p2, r, 0, tieLeaf, invE, ex, ey, ez);
+        return leaf_blocks<kHitTest, kLeafBox>(p0, p1, p2, r, 0, tieLeaf) ? 0 : -1;
     }
     int spill[kStack - kShortStack];
     int sp = 0;
     int cur = 0;
     for (unsigned steps = 0; steps < 2u * (unsigned)n + 64u; steps++) {
-        if (cur < 0 || cur >= n - 1) return false;
+        if (cur < 0 || cur >= n - 1) return -1;
         float t[4];
         int c[4], count;
-        node4_hits(sv.nodes4, cur, r, inv, nx, ny, nz, t, c, count);
+        node4_hits(nodes4, cur, r, inv, nx, ny, nz, t, c, count);
         if (tally) tally->nodes++;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (t[k] < INFINITY) {
                 if (c[k] < 0) {
                     const int leaf = ~c[k];
-                    if (leaf >= n) return false;
+                    if (leaf >= n) return -1;
                     V3 p0, p1, p2; int prim;
-                    load_tri(sv.tris, leaf, p0, p1, p2, prim);
+                    load_tri(tris, leaf, p0, p1, p2, prim);
                     if (tally) tally->tris++;
-                    if (leaf_blocks<kHitTest>(p0, p1, p2, r, leaf, tieLeaf, invE, ex, ey, ez)) return true;
+                    if (leaf_blocks<kHitTest, kLeafBox>(p0, p1, p2, r, leaf, tieLeaf)) return leaf;
                 } else if (sp < kStack) {
                     if (sp < kShortStack) ts.lds[sp * ts.stride] = c[k];
                     else spill[sp - kShortStack] = c[k];
@@ -336,11 +335,34 @@ LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts, TraceTall
                 }
             }
         }
-        if (sp == 0) return false;
+        if (sp == 0) return -1;
         --sp;
         cur = sp < kShortStack ? ts.lds[sp * ts.stride] : spill[sp - kShortStack];
     }
-    return false;
+    return -1;
+}
+
+// kHitTest: the closest-hit question of an MIS ray, whether any leaf beats tieLeaf at r.tMax.
+// Otherwise a shadow ray. Triangle::simpleIntersect accepts points up to SHADOW_EPS outside the
+// triangle's edges, and the reference reaches a triangle only through its leaf's box test
+// (BVH.cuh:343-348: the leaf node's bounds, the primitive's own bounds). The render tree's quantized
+// boxes are wider than that leaf box, so a triangle the enlarged test accepts is also put to the
+// leaf-box test; when that fails (a ray passing just outside a triangle's edge and its bounds, rare),
+// the ray is traversed again with the leaf-box test on every triangle hit. The common path keeps the
+// plain loop. (The closest-hit test accepts only points inside the triangle shrunk by SHADOW_EPS,
+// which lie inside its bounds.)
+template <bool kHitTest = false>
+LTS_D bool trace_any(const SceneView& sv, const Ray& r, TraceStack ts, TraceTally* tally = nullptr, int tieLeaf = -1) {
+    if constexpr (kHitTest) {
+        return any_hit_leaf<true, false>(sv.nPrim, sv.nodes4, sv.tris, r, ts, tally, tieLeaf) >= 0;
+    } else {
+        const int leaf = any_hit_leaf<false, false>(sv.nPrim, sv.nodes4, sv.tris, r, ts, tally, -1);
+        if (leaf < 0) return false;
+        V3 p0, p1, p2; int prim;
+        load_tri(sv.tris, leaf, p0, p1, p2, prim);
+        if (leaf_box_passes(p0, p1, p2, r)) return true;
+        return any_hit_leaf<false, true>(sv.nPrim, sv.nodes4, sv.tris, r, ts, tally, -1) >= 0;
+    }
 }
 
 // Triangle::intersect interaction part + SurfaceInteraction constructor / setShaddingGeometry

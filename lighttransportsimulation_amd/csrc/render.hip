@@ -76,10 +76,10 @@ __global__ __launch_bounds__(256) void k_tri_materials(TriLeaf* __restrict__ tri
     if (k < n) tris[k].mat = shade[k].mat;
 }
 
-// Latency-bound: four waves per SIMD (at most 128 VGPRs; the compiler spills the rest of the path
-// state to scratch) measured 755 Msamples/s on the 1080p workload against 470 at the two waves its
-// natural 219 VGPRs allow; five and six waves spill more and lose.
-__global__ __launch_bounds__(256, 4) void k_megakernel(SceneView sv, RenderLaunch rl) {
+// Latency-bound: five waves per SIMD (the compiler spills the rest of the path state to scratch)
+// measured 696 Msamples/s on the 1080p workload against 679 at four, 647 at three and 470 at the two
+// waves its natural register count allows.
+__global__ __launch_bounds__(256, 5) void k_megakernel(SceneView sv, RenderLaunch rl) {
     __shared__ int stk[kShortStack * 256];
     __shared__ int perm[512];  // the procedural textures' permutation lookups chain three deep
     for (int i = threadIdx.x; i < 512; i += 256) perm[i] = sv.perm[i];
