@@ -4,7 +4,8 @@
 //
 // Reads <projectDir>/app.params (positional: W H / spp / bounces / roulette start / format /
 // output / scene / subject, core/FileIO.cuh:234-257), loads the scene and subject relative to
-// projectDir, renders with the plain path tracer and writes <projectDir>/<output>.pfm (format 1)
+// projectDir, renders (the plain path tracer unless --integrator names another; an unknown name
+// exits with status 2 before any device call) and writes <projectDir>/<output>.pfm (format 1)
 // or a P3 PPM to <projectDir>/<output>.hdr (format 0), as the reference names it. The scene and
 // render stages print the reference's "(1) Scene creation" / "(2) Ray tracing" timing lines.
 // The reference seeds with time(0) (LightTransportSimulation.cu:104); here the seed defaults to 0
@@ -22,14 +23,22 @@ int main(int argc, char** argv) {
     std::string dir = ".";
     unsigned seed = 0;
     int integrator = LTS_INTEGRATOR_MEGAKERNEL;
+    const char* technique = "Path Tracing";
     for (int i = 1; i < argc; i++) {
         if (!std::strcmp(argv[i], "--seed") && i + 1 < argc) seed = (unsigned)std::strtoul(argv[++i], nullptr, 0);
         else if (!std::strcmp(argv[i], "--integrator") && i + 1 < argc) {
-            std::string v = argv[++i];
-            integrator = v == "wavefront" ? LTS_INTEGRATOR_WAVEFRONT
-                         : v == "cached"  ? LTS_INTEGRATOR_CACHED
-                         : v == "bdpt"    ? LTS_INTEGRATOR_BDPT
-                                          : LTS_INTEGRATOR_MEGAKERNEL;
+            const std::string v = argv[++i];
+            if (v == "mega") integrator = LTS_INTEGRATOR_MEGAKERNEL;
+            else if (v == "wavefront") integrator = LTS_INTEGRATOR_WAVEFRONT;
+            else if (v == "cached") integrator = LTS_INTEGRATOR_CACHED;
+            else if (v == "bdpt") integrator = LTS_INTEGRATOR_BDPT;
+            else {
+                std::fprintf(stderr, "unknown integrator '%s' (mega, wavefront, cached or bdpt)\n", v.c_str());
+                return 2;
+            }
+            technique = integrator == LTS_INTEGRATOR_BDPT     ? "Bidirectional Path Tracing"
+                        : integrator == LTS_INTEGRATOR_CACHED ? "Path Tracing with a learned radiance cache"
+                                                              : "Path Tracing";
         } else dir = argv[i];
     }
     if (!dir.empty() && dir.back() != '/') dir += '/';
@@ -40,8 +49,8 @@ int main(int argc, char** argv) {
         return 1;
     }
     std::printf("---%s---\n", lts_version());
-    std::printf("===RENDERING PARAMETERS===\n\tTechnique: Path Tracing\n\tImage resolution: %dx%d\n"
-                "\tSamples per pixel: %d\n\tMaximum bounces: %d\n", W, H, spp, bounce);
+    std::printf("===RENDERING PARAMETERS===\n\tTechnique: %s\n\tImage resolution: %dx%d\n"
+                "\tSamples per pixel: %d\n\tMaximum bounces: %d\n", technique, W, H, spp, bounce);
     lts_ctx* ctx = nullptr;
     if (lts_create(&ctx, 0) != LTS_OK) {
         std::fprintf(stderr, "no HIP device\n");

@@ -5,14 +5,15 @@ import subprocess
 import numpy as np
 import pytest
 
-from lighttransportsimulation_amd import INTEGRATOR_MEGAKERNEL, INTEGRATOR_WAVEFRONT, PKG_DIR, scenegen
+from lighttransportsimulation_amd import INTEGRATOR_BDPT, INTEGRATOR_MEGAKERNEL, INTEGRATOR_WAVEFRONT, PKG_DIR, scenegen
 from test_capi_symbols import _read_pfm
 
 CLI = os.path.join(PKG_DIR, "lts_render")
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("integrator,flag", [(INTEGRATOR_MEGAKERNEL, "mega"), (INTEGRATOR_WAVEFRONT, "wavefront")])
+@pytest.mark.parametrize("integrator,flag", [(INTEGRATOR_MEGAKERNEL, "mega"), (INTEGRATOR_WAVEFRONT, "wavefront"),
+                                             (INTEGRATOR_BDPT, "bdpt")])
 def test_cli_writes_the_library_image(gpu_ctx, tmp_path, integrator, flag):
     print(f"[cli {flag}]", flush=True)
     cfg = scenegen.gen_cornell_sphere(str(tmp_path), width=48, height=32, spp=4, subdiv=3)
@@ -20,7 +21,18 @@ def test_cli_writes_the_library_image(gpu_ctx, tmp_path, integrator, flag):
                          timeout=120)
     assert out.returncode == 0, out.stderr
     assert "(1) Scene creation finished" in out.stdout and "(2) Ray tracing finished" in out.stdout
+    technique = "Bidirectional Path Tracing" if flag == "bdpt" else "Path Tracing"
+    assert f"\tTechnique: {technique}\n" in out.stdout
     img = _read_pfm(str(tmp_path / "c1_out.pfm"))
     gpu_ctx.load_scene(str(tmp_path), cfg["scene"], cfg["subject"], 48, 32)
     rgb, _, _ = gpu_ctx.render(4, 5, 3, 77, integrator=integrator)
     assert np.array_equal(img, rgb)
+
+
+def test_cli_rejects_an_unknown_integrator_before_any_device_call(tmp_path):
+    """Runs without a GPU: the name is checked before app.params is read or a device is opened."""
+    if not os.path.exists(CLI):
+        pytest.fail(f"{CLI} missing: run __graft_entry__.build()")
+    out = subprocess.run([CLI, str(tmp_path), "--integrator", "photon"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 2
+    assert "unknown integrator 'photon'" in out.stderr
